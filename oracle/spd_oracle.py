@@ -1,0 +1,916 @@
+"""CPU oracle: an FP64 NumPy restatement of the reference SD-ADER time step.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP path: only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import it.  The product package ``spd_amd`` never imports,
+links or calls anything under ``oracle/``.
+
+It restates, in whole-array NumPy like the reference, the path of
+``SDADER_Simulator.perform_update`` (``src/sdader_simulator.py:361-378``) for
+``update in {"SD", "FV"}`` with and without the a-posteriori fallback
+(``FB``), including the reference's quirks:
+
+* Q1  the MUSCL-Hancock pressure predictor is zeroed (``src/muscl.py:173``);
+* Q2  FV ghost layers at reflective/outflow sides are *not* refreshed by the
+      trouble-flag fill (``src/fv_simulator.py:235-246``,
+      ``src/trouble_detection.py:73-76``): the ghost troubles hold the density
+      of the last full primitive fill;
+* Q3  the smooth-extrema branch is gated by a global ``any``
+      (``src/trouble_detection.py:41``);
+* Q5  the edge rule of ``compute_min`` (``src/trouble_detection.py:108-110``);
+* Q6  the 2-D/3-D blending stencils (``src/trouble_detection.py:139-169``);
+* Q7  minmod as ``clip(R/L, 0, 1) * L`` (``src/muscl.py:12-35``);
+* Q8  HLLC wave speeds (``src/riemann_solver.py:67-145``);
+* Q19 the Hancock predictor's transverse loop runs over the *unrolled*
+      ``vels[1:]`` (``src/muscl.py:171-172``).
+
+The persistent ghost-store arrays and the persistent ``M_fv`` work array are
+kept as state, exactly so that Q2 reproduces.  Every function cites the
+reference lines it follows.  Arithmetic order follows the reference where it
+is cheap to do so; the contractions use ``numpy.einsum`` with the reference's
+subscripts (summation order of ``einsum`` itself is not pinned).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# ---------------------------------------------------------------------------
+# Polynomial operators (src/polynomials.py)
+# ---------------------------------------------------------------------------
+
+
+def gl_nodes(p):
+    """p-point Gauss-Legendre nodes/weights on [0, 1] (``src/polynomials.py:5-17``)."""
+    if p == 0:
+        return np.ndarray(0), np.ndarray(0)
+    x, w = np.polynomial.legendre.leggauss(p)
+    return (x + 1.0) * 0.5, w * 0.5
+
+
+def solution_pts(p):
+    """Gauss-Chebyshev solution points on [0, 1] (``src/polynomials.py:19-31``)."""
+    i = np.arange(p + 1)
+    return (1.0 - np.cos((2 * i + 1) / (2 * (p + 1)) * np.pi)) / 2.0
+
+
+def flux_pts(p):
+    """{0, GL(p), 1} (``src/polynomials.py:33-44``)."""
+    x, _ = gl_nodes(p)
+    return np.hstack((0.0, x, 1.0))
+
+
+def lagrange(x_to, x_from):
+    """L[a, b] = l_b(x_to[a]) for the Lagrange basis on ``x_from``
+    (``src/polynomials.py:46-65``)."""
+    x_to = np.asarray(x_to, dtype=np.float64)
+    x_from = np.asarray(x_from, dtype=np.float64)
+    n = x_from.size
+    out = np.ones((x_to.size, n))
+    for b in range(n):
+        num = np.ones(x_to.size)
+        den = 1.0
+        for k in range(n):
+            if k == b:
+                continue
+            num = num * (x_to - x_from[k])
+            den = den * (x_from[b] - x_from[k])
+        out[:, b] = num / den
+    return out
+
+
+def lagrange_prime(x_to, x_from):
+    """D[a, b] = l_b'(x_to[a]) (``src/polynomials.py:67-88``)."""
+    x_to = np.asarray(x_to, dtype=np.float64)
+    x_from = np.asarray(x_from, dtype=np.float64)
+    n = x_from.size
+    out = np.zeros((x_to.size, n))
+    for b in range(n):
+        den = 1.0
+        for k in range(n):
+            if k != b:
+                den = den * (x_from[b] - x_from[k])
+        acc = np.zeros(x_to.size)
+        for l in range(n):
+            if l == b:
+                continue
+            term = np.ones(x_to.size)
+            for k in range(n):
+                if k == b or k == l:
+                    continue
+                term = term * (x_to - x_from[k])
+            acc = acc + term
+        out[:, b] = acc / den
+    return out
+
+
+def subcell_mean_matrix(x_sp, x_fp):
+    """M[k, b]: mean over [x_fp[k], x_fp[k+1]] of l_b (``src/polynomials.py:90-107``)."""
+    n = x_sp.size - 1
+    if n == 0:
+        return np.ones((1, 1))
+    xq, wq = gl_nodes(n)
+    ncv = x_fp.size - 1
+    out = np.zeros((ncv, x_sp.size))
+    for k in range(ncv):
+        pts = x_fp[k] + (x_fp[k + 1] - x_fp[k]) * xq
+        out[k] = wq @ lagrange(pts, x_sp)
+    return out
+
+
+def ader_time_matrix(x_t, w_t):
+    """K[i, j] = l_i(1) l_j(1) - w_j l_i'(x_j) (``src/polynomials.py:109-122``)."""
+    end = lagrange(np.array([1.0]), x_t).ravel()
+    d = lagrange_prime(x_t, x_t)          # d[a, b] = l_b'(x_a)
+    return end[None, :] * end[:, None] - d.T * w_t[None, :]
+
+
+class Operators:
+    """All per-order operator matrices (``src/sd_simulator.py:44-52``,
+    ``src/sdader_simulator.py:40-46``)."""
+
+    def __init__(self, p, m):
+        self.p, self.m = p, m
+        self.sp = solution_pts(p)
+        self.fp = flux_pts(p)
+        self.sp_to_fp = lagrange(self.fp, self.sp)
+        self.fp_to_sp = lagrange(self.sp, self.fp)
+        self.dfp_to_sp = lagrange_prime(self.sp, self.fp)
+        self.sp_to_cv = subcell_mean_matrix(self.sp, self.fp)
+        self.cv_to_sp = np.linalg.inv(self.sp_to_cv)
+        self.x_tp, self.w_tp = gl_nodes(m + 1)
+        K = ader_time_matrix(self.x_tp, self.w_tp)
+        inv = np.linalg.inv(K)
+        self.invader = inv * self.w_tp[None, :]
+
+
+# ---------------------------------------------------------------------------
+# Hydrodynamics (src/hydro.py) -- variable 0 is density, vels are momenta /
+# velocities, P is energy / pressure.  ``vels`` may be rolled (normal first).
+# ---------------------------------------------------------------------------
+
+
+def cs2(P, rho, gamma, min_c2):
+    """``src/hydro.py:3-25``: the where-clamp maps NaN to ``min_c2`` (Q9)."""
+    c2 = gamma * P / rho
+    return np.where(c2 > min_c2, c2, min_c2)
+
+
+def prims(U, vels, P, gamma):
+    """``src/hydro.py:48-86``."""
+    W = U.copy()
+    K = W[0] * 0
+    for v in vels:
+        W[v] = U[v] / U[0]
+        K = K + W[v] ** 2
+    K = K * (0.5 * U[0])
+    W[P] = (gamma - 1) * (U[P] - K)
+    return W
+
+
+def cons(W, vels, P, gamma):
+    """``src/hydro.py:88-125``."""
+    U = W.copy()
+    K = W[0] * 0
+    for v in vels:
+        U[v] = W[v] * U[0]
+        K = K + W[v] ** 2
+    K = K * (0.5 * U[0])
+    U[P] = W[P] / (gamma - 1) + K
+    return U
+
+
+def euler_flux(W, vels, P, gamma):
+    """``src/hydro.py:127-170``: the normal component is ``vels[0]``; the kinetic
+    sum runs over ``vels[::-1]`` (Q14)."""
+    F = W.copy()
+    K = W[0] * 0
+    v1 = vels[0]
+    m = None
+    for v in vels[::-1]:
+        m = W[0] * W[v]
+        K = K + m * W[v]
+        F[v] = m * W[v1]
+    E = W[P] / (gamma - 1) + 0.5 * K
+    F[0] = m
+    F[v1] = m * W[v1] + W[P]
+    F[P] = W[v1] * (E + W[P])
+    return F
+
+
+def riemann_llf(WL, WR, UL, UR, vels, P, gamma, min_c2):
+    """``src/riemann_solver.py:31-65``."""
+    v1 = vels[0]
+    FL = euler_flux(WL, vels, P, gamma)
+    FR = euler_flux(WR, vels, P, gamma)
+    cL = np.sqrt(cs2(WL[P], WL[0], gamma, min_c2)) + np.abs(WL[v1])
+    cR = np.sqrt(cs2(WR[P], WR[0], gamma, min_c2)) + np.abs(WR[v1])
+    cmax = np.where(cL > cR, cL, cR)[None]
+    return 0.5 * (FR + FL) - 0.5 * cmax * (UR - UL)
+
+
+def riemann_hllc(WL, WR, UL, UR, vels, P, gamma, min_c2):
+    """``src/riemann_solver.py:67-145`` (wave speeds Q8)."""
+    v1 = vels[0]
+    cL = np.sqrt(cs2(WL[P], WL[0], gamma, min_c2)) + np.abs(WL[v1])
+    cR = np.sqrt(cs2(WR[P], WR[0], gamma, min_c2)) + np.abs(WR[v1])
+    cmax = np.where(cL > cR, cL, cR)
+    vL = WL[v1]
+    vR = WR[v1]
+    sL = np.where(vL < vR, vL, vR) - cmax
+    sR = np.where(vL > vR, vL, vR) + cmax
+    rcL = WL[0] * (vL - sL)
+    rcR = WR[0] * (sR - vR)
+    vs = (rcR * vR + rcL * vL + (WL[P] - WR[P])) / (rcR + rcL)
+    Ps = (rcR * WL[P] + rcL * WR[P] + rcL * rcR * (vL - vR)) / (rcR + rcL)
+    rsL = WL[0] * (sL - vL) / (sL - vs)
+    esL = ((sL - vL) * UL[P] - WL[P] * vL + Ps * vs) / (sL - vs)
+    rsR = WR[0] * (sR - vR) / (sR - vs)
+    esR = ((sR - vR) * UR[P] - WR[P] * vR + Ps * vs) / (sR - vs)
+    r = np.where(sL > 0, WL[0], np.where(vs > 0, rsL, np.where(sR > 0, rsR, WR[0])))
+    v = np.where(sL > 0, WL[v1], np.where(vs > 0, vs, np.where(sR > 0, vs, WR[v1])))
+    p = np.where(sL > 0, WL[P], np.where(vs > 0, Ps, np.where(sR > 0, Ps, WR[P])))
+    e = np.where(sL > 0, UL[P], np.where(vs > 0, esL, np.where(sR > 0, esR, UR[P])))
+    F = WL.copy()
+    F[0] = r * v
+    F[v1] = F[0] * v + p
+    F[P] = v * (e + p)
+    for vt in vels[1:]:
+        F[vt] = F[0] * np.where(vs > 0, WL[vt], WR[vt])
+    return F
+
+
+def riemann(name, ML, MR, vels, P, gamma, min_c2, prims_in):
+    """``src/riemann_solver.py:10-28``: converts to the other variable set first."""
+    if prims_in:
+        WL, WR = ML, MR
+        UL, UR = cons(WL, vels, P, gamma), cons(WR, vels, P, gamma)
+    else:
+        UL, UR = ML, MR
+        WL, WR = prims(UL, vels, P, gamma), prims(UR, vels, P, gamma)
+    f = {"llf": riemann_llf, "hllc": riemann_hllc}[name]
+    return f(WL, WR, UL, UR, vels, P, gamma, min_c2)
+
+
+# ---------------------------------------------------------------------------
+# Array helpers
+# ---------------------------------------------------------------------------
+
+
+def _sl(nd_total, axis, s):
+    """Index tuple selecting ``s`` along ``axis`` (counted from the front)."""
+    idx = [slice(None)] * nd_total
+    idx[axis] = s
+    return tuple(idx)
+
+
+def _contract(A, M, axis):
+    """out[..., f, ...] = sum_s M[f, s] A[..., s, ...] along ``axis``
+    (``src/transforms.py:3-25``)."""
+    A = np.moveaxis(A, axis, -1)
+    out = np.einsum("fs,...s->...f", M, A)
+    return np.moveaxis(out, -1, axis)
+
+
+# ---------------------------------------------------------------------------
+# The simulator restatement
+# ---------------------------------------------------------------------------
+
+BC_KINDS = ("periodic", "gradfree", "reflective")
+
+
+class OracleSimulator:
+    """FP64 NumPy restatement of ``SDADER_Simulator`` for the supported options.
+
+    Layouts follow the reference: element layout ``(V, [Nz,] [Ny,] Nx, [nz,] [ny,] nx)``
+    and finite-volume layout ``(V, [Nz*nz,] [Ny*ny,] Nx*nx)``; dimension index 0
+    is x (the last array axis).
+    """
+
+    def __init__(self, init_fct, p=1, m=-1, N=(32, 32), xlim=(0, 1), ylim=(0, 1),
+                 zlim=(0, 1), gamma=1.4, cfl_coeff=0.8, min_c2=1e-10,
+                 BC=(("periodic", "periodic"),) * 3, update="SD", FB=False,
+                 riemann_solver_sd="llf", riemann_solver_fv="llf",
+                 slope_limiter="minmod", tolerance=1e-5, SED=True, NAD="",
+                 PAD=True, blending=True, min_rho=1e-10, max_rho=1e10,
+                 min_P=1e-10, godunov=False, Nghe=1, Nghc=2, noise=None,
+                 **unused):
+        if m == -1:
+            m = p
+        if slope_limiter != "minmod":
+            raise NotImplementedError(slope_limiter)
+        self.p, self.m = p, m
+        self.ndim = nd = len(N)
+        self.N = list(N)
+        self.n = p + 1
+        self.nf = p + 2
+        self.T = m + 1
+        self.gamma, self.cfl, self.min_c2 = gamma, cfl_coeff, min_c2
+        self.update, self.FB = update, FB
+        self.rs_sd, self.rs_fv = riemann_solver_sd, riemann_solver_fv
+        self.tol, self.SED, self.NAD, self.PAD = tolerance, SED, NAD, PAD
+        self.blending, self.godunov = blending, godunov
+        self.min_rho, self.max_rho, self.min_P = min_rho, max_rho, min_P
+        self.Nghe, self.ngh = Nghe, Nghc
+        self.BC = [tuple(BC[a]) for a in range(nd)]
+        for a in range(nd):
+            for side in self.BC[a]:
+                if side not in BC_KINDS:
+                    raise NotImplementedError(side)
+        self.lim = [tuple(map(float, (xlim, ylim, zlim)[a])) for a in range(nd)]
+        self.len = [self.lim[a][1] - self.lim[a][0] for a in range(nd)]
+        self.h = [self.len[a] / self.N[a] for a in range(nd)]
+        self.h_min = 1e10
+        for a in range(nd):
+            self.h_min = min(self.h_min, self.h[a])
+        self.vels = np.arange(nd) + 1
+        self.P = nd + 1
+        self.V = nd + 2
+        self.ops = Operators(p, m)
+        self._geometry()
+        self.time = 0.0
+        self.n_step = 0
+        self.captures = None          # list -> record FV substep internals
+        self._post_init(init_fct, noise)
+        self._init_fv_state()
+        self.compute_dt()
+
+    # -- geometry -----------------------------------------------------------
+    def _geometry(self):
+        """Subcell face positions, centers and widths with ghost mirroring
+        (``src/sdader_simulator.py:61-92``, Q15)."""
+        g = self.ngh
+        self.faces, self.centers, self.h_fp, self.h_cv = [], [], [], []
+        for a in range(self.ndim):
+            N, n, h = self.N[a], self.n, self.h[a]
+            fpos = np.ndarray(N * n + 2 * g + 1)
+            fpos[g:-g] = h * np.hstack((np.arange(N).repeat(n)
+                                        + np.tile(self.ops.fp[:-1], N), N))
+            fpos[:g] = -fpos[g + 1:2 * g + 1][::-1]
+            fpos[-g:] = fpos[-(g + 1)] + fpos[g + 1:2 * g + 1]
+            cv = 0.5 * (fpos[1:] + fpos[:-1])
+            self.faces.append(fpos)
+            self.centers.append(cv)
+            self.h_fp.append(fpos[1:] - fpos[:-1])
+            self.h_cv.append(cv[1:] - cv[:-1])
+
+    def _bshape(self, a, arr, lead=1):
+        """Broadcast a 1-D per-dimension array along dimension ``a`` of an
+        FV-layout array with ``lead`` leading axes."""
+        shape = [1] * (lead + self.ndim)
+        shape[lead + self.ndim - 1 - a] = arr.size
+        return arr.reshape(shape)
+
+    # -- layouts --------------------------------------------------------------
+    def to_fv(self, M):
+        """Element layout -> FV layout (``src/sd_simulator.py:104-115``)."""
+        nd = self.ndim
+        lead = M.ndim - 2 * nd
+        E = M.shape[lead:lead + nd]
+        L = M.shape[lead + nd:]
+        perm = list(range(lead))
+        for k in range(nd):
+            perm += [lead + k, lead + nd + k]
+        shape = list(M.shape[:lead]) + [E[k] * L[k] for k in range(nd)]
+        return np.transpose(M, perm).reshape(shape)
+
+    def to_sd(self, M):
+        """FV layout -> element layout (``src/sd_simulator.py:117-131``)."""
+        nd = self.ndim
+        lead = M.ndim - nd
+        shape = list(M.shape[:lead])
+        for k in range(nd):
+            a = nd - 1 - k
+            shape += [self.N[a], self.n]
+        X = M.reshape(shape)
+        perm = list(range(lead)) + [lead + 2 * k for k in range(nd)] + \
+            [lead + 2 * k + 1 for k in range(nd)]
+        return np.transpose(X, perm)
+
+    def _full(self, M, A):
+        """Tensor-product operator on all local axes (``src/transforms.py:27-45``)."""
+        nd = self.ndim
+        if nd == 1:
+            return np.einsum("il,uxl->uxi", A, M)
+        if nd == 2:
+            return np.einsum("jm,il,uyxml->uyxji", A, A, M)
+        return np.einsum("kn,jm,il,uzyxnml->uzyxkji", A, A, A, M)
+
+    def sp_to_cv(self, M):
+        return self._full(M, self.ops.sp_to_cv)
+
+    def cv_to_sp(self, M):
+        return self._full(M, self.ops.cv_to_sp)
+
+    def prims(self, U, vels=None):
+        return prims(U, self.vels if vels is None else vels, self.P, self.gamma)
+
+    def cons(self, W, vels=None):
+        return cons(W, self.vels if vels is None else vels, self.P, self.gamma)
+
+    # -- initial condition ----------------------------------------------------
+    def _post_init(self, init_fct, noise):
+        """Subcell means of the primitive IC with 2-point GL per dimension, then
+        ``W_sp = cv_to_sp(W_cv)``, ``U_sp = cons(W_sp)``, ``U_cv = cons(W_cv)``
+        (``src/sd_simulator.py:59-90``, ``src/polynomials.py:124-148``, Q11)."""
+        nd = self.ndim
+        g = self.Nghe
+        xq, wq = gl_nodes(2)
+        # per-dimension subcell lower bounds and widths on the element grid
+        lows, wids = [], []
+        for a in range(nd):
+            Ng = self.N[a] + 2 * g
+            length = self.len[a] + 2 * g * self.h[a]
+            mesh = self.lim[a][0] + (np.arange(Ng)[:, None] + self.ops.fp[None, :]) \
+                * length / Ng - self.h[a]
+            mesh = mesh[g:-g]                       # active elements
+            lows.append(mesh[:, :-1])
+            wids.append(np.diff(mesh, axis=1))
+        # coordinates at quadrature points: shape (nd, E..., L..., Q...)
+        shapeE = [self.N[a] for a in range(nd)][::-1]
+        full_shape = shapeE + [self.n] * nd + [2] * nd
+        pts = np.empty([nd] + full_shape)
+        for a in range(nd):
+            k = nd - 1 - a                          # array position of dim a
+            lo = lows[a]
+            wd = wids[a]
+            shp = [1] * (3 * nd)
+            shp[k] = self.N[a]
+            shp[nd + k] = self.n
+            qs = [1] * (3 * nd)
+            qs[2 * nd + k] = 2
+            pts[a] = np.broadcast_to(lo.reshape(shp) + wd.reshape(shp) * xq.reshape(qs),
+                                     full_shape)
+        W = np.empty([self.V] + shapeE + [self.n] * nd)
+        for var in range(self.V):
+            f = np.asarray(init_fct(pts, var), dtype=np.float64) * np.ones(full_shape)
+            # weighted sum over the quadrature axes (the last nd axes)
+            args = [f, list(range(3 * nd))]
+            for k in range(nd):
+                args += [wq, [2 * nd + k]]
+            args.append(list(range(2 * nd)))
+            W[var] = np.einsum(*args)
+        if noise is not None:
+            W = W + noise
+        self.W_init_cv = W.copy()
+        self.W_cv = W.copy()
+        W_sp = self.cv_to_sp(self.W_cv)
+        self.U_sp = self.cons(W_sp)
+        self.U_cv = self.cons(self.W_cv)
+
+    # -- dt -------------------------------------------------------------------
+    def compute_dt(self):
+        """``src/sd_simulator.py:214-227`` on the subcell-mean primitives (Q10)."""
+        W = self.W_cv
+        c = np.sqrt(cs2(W[self.P], W[0], self.gamma, self.min_c2)) * self.ndim
+        for v in self.vels:
+            c = c + np.abs(W[v])
+        cmax = np.max(c)
+        h = self.h_min / (self.p + 1)
+        self.dt = self.cfl * (h / cmax)
+        return self.dt
+
+    # -- SD predictor -----------------------------------------------------------
+    def _local_axis(self, a, lead):
+        return lead + 2 * self.ndim - 1 - a
+
+    def _elem_axis(self, a, lead):
+        return lead + self.ndim - 1 - a
+
+    def solve_faces(self, M):
+        """Per dimension: interpolate to flux points, Euler flux, element-face
+        Riemann problems with boundary traces, overwrite the end flux points
+        (``src/sdader_simulator.py:213-240``, ``src/sd_boundary.py:7-76``, Q16, Q17)."""
+        nd = self.ndim
+        lead = 2                                     # (V, T)
+        Ffp = []
+        for a in range(nd):
+            vels = np.roll(self.vels, -a)
+            la = self._local_axis(a, lead)
+            ea = self._elem_axis(a, lead)
+            Mfp = _contract(M, self.ops.sp_to_fp, la)
+            F = euler_flux(self.prims(Mfp, vels), vels, self.P, self.gamma)
+            left = np.take(Mfp, 0, axis=la)          # element-local fp 0
+            right = np.take(Mfp, self.nf - 1, axis=la)
+            eal = ea                     # element axis is unchanged by the take (la > ea)
+            Na = self.N[a]
+            bl, br = self.BC[a]
+            if bl == "periodic":
+                gL = np.take(right, [Na - 1], axis=eal)
+            else:
+                gL = np.take(left, [0], axis=eal).copy()
+                if bl == "reflective":
+                    gL[a + 1] *= -1
+            if br == "periodic":
+                gR = np.take(left, [0], axis=eal)
+            else:
+                gR = np.take(right, [Na - 1], axis=eal).copy()
+                if br == "reflective":
+                    gR[a + 1] *= -1
+            ML = np.concatenate([gL, right], axis=eal)
+            MR = np.concatenate([left, gR], axis=eal)
+            Fr = riemann(self.rs_sd, ML, MR, vels, self.P, self.gamma, self.min_c2,
+                         prims_in=False)
+            lo = np.take(Fr, np.arange(Na), axis=eal)
+            hi = np.take(Fr, np.arange(1, Na + 1), axis=eal)
+            # F moved so that the local axis of ``a`` is last; removing it leaves
+            # the axis order of the face arrays.
+            F = np.moveaxis(F, la, -1).copy()
+            F[..., 0] = lo
+            F[..., -1] = hi
+            Ffp.append(np.moveaxis(F, -1, la))
+        return Ffp
+
+    def ader_dudt(self, Ffp):
+        """``src/sdader_simulator.py:148-158``."""
+        lead = 2
+        d = None
+        for a in range(self.ndim):
+            term = _contract(Ffp[a], self.ops.dfp_to_sp, self._local_axis(a, lead)) / self.h[a]
+            d = term if d is None else d + term
+        return d
+
+    def ader_predictor(self):
+        """Global Picard iteration (``src/sdader_simulator.py:160-200``)."""
+        Uad = np.broadcast_to(self.U_sp[:, None], (self.V, self.T) + self.U_sp.shape[1:]).copy()
+        Ffp = None
+        for it in range(self.m + 1):
+            Ffp = self.solve_faces(Uad)
+            if it < self.m:
+                dudt = self.ader_dudt(Ffp)
+                Uad = np.einsum("np,up...->un...", self.ops.invader, dudt) * self.dt
+                Uad = self.U_sp[:, None] - Uad
+        self.U_ader = Uad
+        return Ffp
+
+    # -- FV corrector -------------------------------------------------------------
+    def _init_fv_state(self):
+        nd, g = self.ndim, self.ngh
+        Ns = [self.N[a] * self.n for a in range(nd)]
+        self.Ns = Ns
+        self.M_fv = np.zeros([self.V] + [s + 2 * g for s in Ns[::-1]])
+        self.BC_fv = []
+        for a in range(nd):
+            shp = [2, self.V] + [(Ns[b] + 2 * g if b != a else g) for b in range(nd)][::-1]
+            self.BC_fv.append(np.zeros(shp))
+
+    def _active(self):
+        g = self.ngh
+        return (slice(None),) + (slice(g, -g),) * self.ndim
+
+    def fv_fill(self, Mfv, all_sides=True):
+        """Ghost layers of ``Mfv`` dimension by dimension (x first): periodic
+        wrap, or a non-mirrored copy of the first/last ``ngh`` active layers
+        (reflective: normal velocity negated).  With ``all_sides=False`` the
+        non-periodic stores are skipped and the previous ghost store is applied
+        again (Q2).  ``src/fv_simulator.py:224-273``."""
+        g = self.ngh
+        nt = Mfv.ndim
+        for a in range(self.ndim):
+            ax = nt - 1 - a
+            inner = (_sl(nt, ax, slice(-2 * g, -g)), _sl(nt, ax, slice(g, 2 * g)))
+            store = self.BC_fv[a]
+            for side in (0, 1):
+                kind = self.BC[a][side]
+                if kind == "periodic":
+                    store[side] = Mfv[inner[side]]
+                elif all_sides:
+                    store[side] = Mfv[inner[1 - side]]
+                    if kind == "reflective":
+                        store[side][a + 1] *= -1
+            Mfv[_sl(nt, ax, slice(None, g))] = store[0]
+            Mfv[_sl(nt, ax, slice(-g, None))] = store[1]
+
+    def _face_dx(self, a):
+        g = self.ngh
+        f = self.faces[a]
+        return f[g + 1:-g] - f[g:-(g + 1)]
+
+    def fv_apply(self, U_cv, Ff, dt):
+        """``U_new = U_cv - dt * sum_a (F[f+1]-F[f])/dx`` (``src/fv_simulator.py:189-205``)."""
+        nt = U_cv.ndim
+        dU = U_cv * 0
+        for a in range(self.ndim):
+            ax = nt - 1 - a
+            dx = self._bshape(a, self._face_dx(a))
+            dU = dU + (Ff[a][_sl(nt, ax, slice(1, None))] - Ff[a][_sl(nt, ax, slice(None, -1))]) / dx
+        return U_cv - dU * dt
+
+    def ho_faces(self, Favg, i):
+        """Scatter the averaged high-order fluxes of time node ``i`` into the FV
+        face arrays; the shared element faces are taken once
+        (``src/sdader_simulator.py:310-323``)."""
+        out = []
+        nd = self.ndim
+        for a in range(nd):
+            F = Favg[a][:, i]                         # (V, E..., L...)
+            la = 1 + 2 * nd - 1 - a
+            ea = 1 + nd - 1 - a
+            body = np.take(F, np.arange(self.n), axis=la)
+            fv = self.to_fv(body)                     # (V, Ns...) along a: N*n
+            # the last face: last element, last flux point (size-1 axes kept)
+            last = np.take(np.take(F, [self.N[a] - 1], axis=ea), [self.nf - 1], axis=la)
+            out.append(np.concatenate([fv, self.to_fv(last)], axis=1 + nd - 1 - a))
+        return out
+
+    # -- trouble detection (src/trouble_detection.py) -------------------------------
+    @staticmethod
+    def _ext(W, ax, f):
+        """3-point extremum along ``ax`` (``src/trouble_detection.py:89-102``)."""
+        nt = W.ndim
+        Wf = W.copy()
+        Wf[_sl(nt, ax, slice(None, -1))] = f(W[_sl(nt, ax, slice(None, -1))],
+                                            W[_sl(nt, ax, slice(1, None))])
+        Wf[_sl(nt, ax, slice(1, None))] = f(Wf[_sl(nt, ax, slice(1, None))],
+                                           Wf[_sl(nt, ax, slice(None, -1))])
+        return Wf
+
+    def _smooth_extrema(self, U, a):
+        """Smooth-extrema relaxation alpha along dimension ``a``
+        (``src/trouble_detection.py:104-137``, Q5)."""
+        nt = U.ndim
+        ax = nt - 1 - a
+        c = self._bshape(a, self.centers[a])
+
+        def S(arr, lo, hi):
+            return arr[_sl(nt, ax, slice(lo, hi))]
+
+        dU = (S(U, 2, None) - S(U, None, -2)) / (S(c, 2, None) - S(c, None, -2))
+        c1 = S(c, 1, -1)
+        d2U = (S(dU, 2, None) - S(dU, None, -2)) / (S(c1, 2, None) - S(c1, None, -2))
+        dv = 0.5 * S(self._bshape(a, self.h_fp[a]), 2, -2) * d2U
+        vL = S(dU, None, -2) - S(dU, 1, -1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            aL = -np.where(dv < 0, np.where(vL > 0, vL, 0), np.where(vL < 0, vL, 0)) / dv
+            aL = np.where(np.abs(dv) <= 0, 1, aL)
+            aL = np.where(aL < 1, aL, 1)
+            vR = S(dU, 2, None) - S(dU, 1, -1)
+            aR = np.where(dv > 0, np.where(vR > 0, vR, 0), np.where(vR < 0, vR, 0)) / dv
+            aR = np.where(np.abs(dv) <= 0, 1, aR)
+            aR = np.where(aR < 1, aR, 1)
+        A = np.where(aR < aL, aR, aL)
+        out = aL.copy()
+        out[_sl(nt, ax, slice(None, -1))] = np.minimum(A[_sl(nt, ax, slice(None, -1))],
+                                                      A[_sl(nt, ax, slice(1, None))])
+        out[_sl(nt, ax, slice(1, None))] = np.minimum(A[_sl(nt, ax, slice(None, -1))],
+                                                     out[_sl(nt, ax, slice(1, None))])
+        return out
+
+    def _blend(self, trouble, theta):
+        """theta rings (``src/trouble_detection.py:139-169``, Q6)."""
+        nd = self.ndim
+        nt = trouble.ndim
+        a_ = slice(None, -1)
+        b_ = slice(1, None)
+        for ax in range(nt - 1, -1, -1):           # x first
+            lo = _sl(nt, ax, a_)
+            hi = _sl(nt, ax, b_)
+            theta[lo] = np.maximum(0.75 * trouble[hi], theta[lo])
+            theta[hi] = np.maximum(0.75 * trouble[lo], theta[hi])
+        cuts = [(a_, a_), (a_, b_), (b_, a_), (b_, b_)]
+        if nd == 2:
+            for i in range(4):
+                theta[cuts[i]] = np.maximum(0.5 * trouble[cuts[3 - i]], theta[cuts[i]])
+        elif nd == 3:
+            for i in range(4):
+                for idim in range(3):
+                    s1 = _roll3((slice(None),) + cuts[i], idim)
+                    s2 = _roll3((slice(None),) + cuts[(i - 1) % 4], idim)
+                    theta[s1] = np.maximum(0.5 * trouble[s2], theta[s1])
+            c1 = [(x, y, z) for x in (a_, b_) for y in (a_, b_) for z in (a_, b_)]
+            c2 = [(x, y, z) for x in (b_, a_) for y in (b_, a_) for z in (b_, a_)]
+            for i in range(4):
+                theta[c1[i]] = np.maximum(0.375 * trouble[c2[i]], theta[c1[i]])
+        for ax in range(nt - 1, -1, -1):
+            lo = _sl(nt, ax, a_)
+            hi = _sl(nt, ax, b_)
+            theta[lo] = np.maximum(0.25 * (theta[hi] > 0), theta[lo])
+            theta[hi] = np.maximum(0.25 * (theta[lo] > 0), theta[hi])
+
+    def detect_troubles(self, U_cv, U_new):
+        """NAD + SED + PAD flags, ghost fill, blending and face coefficients
+        (``src/trouble_detection.py:5-87``)."""
+        nd, g = self.ndim, self.ngh
+        act = self._active()
+        Mfv = self.M_fv
+        W_old = self.prims(U_cv)
+        Mfv[act] = W_old
+        W_new = self.prims(U_new)
+        self.fv_fill(Mfv, True)
+        nt = Mfv.ndim
+        Wmax = Mfv.copy()
+        Wmin = Mfv.copy()
+        for a in range(nd):
+            ax = nt - 1 - a
+            Wmax = np.maximum(self._ext(Mfv, ax, np.maximum), Wmax)
+            Wmin = np.minimum(self._ext(Mfv, ax, np.minimum), Wmin)
+        Wmax = Wmax[act]
+        Wmin = Wmin[act]
+        if self.p > 0:
+            if self.NAD == "delta":
+                eps = self.tol * (Wmax - Wmin)
+                Wmin = Wmin - eps
+                Wmax = Wmax + eps
+            else:
+                Wmin = Wmin - np.abs(Wmin) * self.tol
+                Wmax = Wmax + np.abs(Wmax) * self.tol
+        pt = np.where(W_new >= Wmin, 0, 1)
+        pt = np.where(W_new <= Wmax, pt, 1)
+        # distance of each decision to its threshold, relative to the local
+        # magnitude (used by the tests to exempt ties from flag parity)
+        with np.errstate(invalid="ignore"):
+            scale = np.maximum(np.maximum(np.abs(Wmax), np.abs(Wmin)), 1e-300)
+            nad_m = np.minimum(np.abs(W_new - Wmin), np.abs(Wmax - W_new)) / scale
+        margin = np.minimum(nad_m[0], nad_m[self.P])
+        self.sed_ran = bool(np.any(pt) and self.p > 1 and self.SED)
+        if self.sed_ran:
+            Mfv[act] = W_new
+            self.fv_fill(Mfv, True)
+            alpha = W_new * 0 + 1
+            for a in range(nd):
+                other = [slice(None)] * (1 + nd)
+                for b in range(nd):
+                    if b != a:
+                        other[nd - b] = slice(g, -g)
+                an = self._smooth_extrema(Mfv, a)[tuple(other)]
+                alpha = np.where(an < alpha, an, alpha)
+            pt = pt * np.where(alpha < 1, 1, 0)
+            with np.errstate(invalid="ignore"):
+                am = np.abs(1.0 - alpha)
+            margin = np.minimum(margin, np.minimum(am[0], am[self.P]))
+        tr = np.maximum(pt[0], pt[self.P])[None]
+        if self.PAD:
+            tr = np.where(W_new[0] >= self.min_rho, tr, 1)
+            tr = np.where(W_new[0] <= self.max_rho, tr, 1)
+            tr = np.where(W_new[self.P] >= self.min_P, tr, 1)
+        if self.PAD:
+            with np.errstate(invalid="ignore"):
+                pm = np.minimum(np.abs(W_new[0] - self.min_rho) / max(self.min_rho, 1e-300),
+                                np.abs(W_new[self.P] - self.min_P) / max(self.min_P, 1e-300))
+            margin = np.minimum(margin, pm)
+        self.troubles = tr
+        self.margin = np.where(np.isnan(margin), 0.0, margin)[None]
+        Mfv[...] = 0
+        Mfv[act] = tr
+        self.fv_fill(Mfv, False)
+        trouble = Mfv[0]
+        theta = trouble.copy()
+        if self.blending:
+            self._blend(trouble, theta)
+        self.theta = theta[None]
+        aff = []
+        for a in range(nd):
+            ax = nd - 1 - a
+            s1 = [slice(g, -g)] * nd
+            s2 = [slice(g, -g)] * nd
+            s1[ax] = slice(g - 1, -g)
+            s2[ax] = slice(g, -(g - 1))
+            aff.append(np.maximum(theta[tuple(s1)], theta[tuple(s2)])[None])
+        self.aff = aff
+        return aff
+
+    def muscl_hancock(self, W_cv, dt):
+        """MUSCL-Hancock fallback fluxes on the ghosted primitive state
+        (``src/fv_simulator.py:167-175``, ``src/muscl.py:61-75,129-221``,
+        Q1, Q7, Q19)."""
+        nd, g = self.ndim, self.ngh
+        Mfv = self.M_fv
+        Mfv[...] = 0
+        Mfv[self._active()] = W_cv
+        self.fv_fill(Mfv, True)
+        nt = Mfv.ndim
+        S, grads = {}, {}
+        with np.errstate(divide="ignore", invalid="ignore"):
+            for a in range(nd):
+                ax = nt - 1 - a
+                hcv = self._bshape(a, self.h_cv[a])
+                hfp = self._bshape(a, self.h_fp[a])
+                dM = (Mfv[_sl(nt, ax, slice(1, None))] - Mfv[_sl(nt, ax, slice(None, -1))]) / hcv
+                L = dM[_sl(nt, ax, slice(None, -1))]
+                R = dM[_sl(nt, ax, slice(1, None))]
+                r = R / L
+                r = np.where(r < 1, r, 1)
+                dMh = np.where(r > 0, r, 0) * L
+                S[a] = 0.5 * dMh * hfp[_sl(nt, ax, slice(1, -1))]
+                crop = [slice(None)] * nt
+                for b in range(nd):
+                    if b != a:
+                        crop[nt - 1 - b] = slice(1, -1)
+                grads[a] = dMh[tuple(crop)]
+        inner = (slice(None),) + (slice(1, -1),) * nd
+        W = Mfv[inner]
+        dtW = np.zeros_like(W)
+        P = self.P
+        vels = self.vels
+        for a in range(nd):
+            v = vels[a]
+            dW = grads[a]
+            dtW[0] -= (W[v] * dW[0] + W[0] * dW[v])
+            dtW[P] -= (W[v] * dW[P] + self.gamma * W[P] * dW[v])
+            dtW[v] -= (W[v] * dW[v] + dW[P] / W[0])
+            for v2 in vels[1:]:
+                dtW[v2] -= W[v] * dW[v2]
+        dtW[P] *= 0
+        Mfv[inner] += 0.5 * dtW * dt
+        F = []
+        for a in range(nd):
+            ax = nt - 1 - a
+            sR = [slice(None)] + [slice(g, -g)] * nd
+            sL = list(sR)
+            tR = list(sR)
+            tL = list(sR)
+            sR[ax] = slice(g, -1)
+            tR[ax] = slice(1, None)
+            sL[ax] = slice(1, -g)
+            tL[ax] = slice(None, -1)
+            MR = Mfv[tuple(sR)] - S[a][tuple(tR)]
+            ML = Mfv[tuple(sL)] + S[a][tuple(tL)]
+            F.append(riemann(self.rs_fv, ML, MR, np.roll(vels, -a), P, self.gamma,
+                             self.min_c2, prims_in=True))
+        return F
+
+    def fv_update(self, Ffp):
+        """T sequential FV substeps with HO fluxes of time node i and, with FB,
+        detection + MUSCL-Hancock + blending (``src/sdader_simulator.py:288-349``)."""
+        nd = self.ndim
+        U_cv = self.to_fv(self.sp_to_cv(self.U_sp))
+        Favg = []
+        for a in range(nd):
+            F = Ffp[a]
+            for b in range(nd):
+                if b != a:
+                    F = _contract(F, self.ops.sp_to_cv, self._local_axis(b, 2))
+            Favg.append(F)
+        for i in range(self.T):
+            W_cv = self.prims(U_cv)
+            dti = self.dt * self.ops.w_tp[i]
+            Fho = self.ho_faces(Favg, i)
+            U_new = self.fv_apply(U_cv, Fho, dti)
+            if self.FB:
+                aff = self.detect_troubles(U_cv, U_new)
+                Ffb = self.muscl_hancock(W_cv, dti)
+                if self.captures is not None:
+                    rec = {"troubles": self.troubles.astype(np.float64),
+                           "margin": self.margin.copy(),
+                           "theta": self.theta.copy(), "Ucand": U_new.copy(),
+                           "Wold": W_cv.copy(), "sed_ran": self.sed_ran}
+                    for a in range(nd):
+                        d = "xyz"[a]
+                        rec[f"aff_{d}"] = aff[a].copy()
+                        rec[f"FHO_{d}"] = Fho[a].copy()
+                        rec[f"FFB_{d}"] = Ffb[a].copy()
+                    self.captures.append(rec)
+                Fb = []
+                for a in range(nd):
+                    th = 1 if self.godunov else aff[a]
+                    Fb.append(th * Ffb[a] + (1 - th) * Fho[a])
+                U_new = self.fv_apply(U_cv, Fb, dti)
+            U_cv = U_new
+        self.U_cv = self.to_sd(U_cv)
+        self.U_sp = self.cv_to_sp(self.U_cv)
+
+    def ader_update(self, Ffp):
+        """SD-mode corrector (``src/sdader_simulator.py:202-208``)."""
+        dudt = self.ader_dudt(Ffp)
+        self.U_sp = self.U_sp - np.einsum("t,ut...->u...", self.ops.w_tp, dudt) * self.dt
+
+    def perform_update(self):
+        """One step (``src/sdader_simulator.py:361-378``)."""
+        self.n_step += 1
+        Ffp = self.ader_predictor()
+        if self.update == "SD":
+            self.ader_update(Ffp)
+        else:
+            self.fv_update(Ffp)
+        self.W_cv = self.prims(self.U_cv)
+        self.time += self.dt
+        return True
+
+    def end_sim(self):
+        """``src/sdader_simulator.py:385-392`` without the print."""
+        self.U_cv = self.sp_to_cv(self.U_sp)
+        self.W_cv = self.prims(self.U_cv)
+
+    def perform_iterations(self, n):
+        """``src/sdader_simulator.py:394-399``."""
+        for _ in range(n):
+            self.compute_dt()
+            self.perform_update()
+        self.end_sim()
+
+    def perform_time_evolution(self, t_end):
+        """``src/sdader_simulator.py:401-413``."""
+        while self.time < t_end:
+            self.compute_dt()
+            if self.time + self.dt >= t_end:
+                self.dt = t_end - self.time
+            if self.dt < 1e-14:
+                break
+            self.perform_update()
+        self.end_sim()
+
+
+def _roll3(t, k):
+    """``tuple(np.roll(np.array(t, dtype=object), -k))`` for a 3-tuple of slices."""
+    t = list(t)
+    k %= 3
+    return tuple(t[k:] + t[:k])

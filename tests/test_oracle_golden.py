@@ -1,0 +1,77 @@
+"""The oracle against the reference's own FP64 output (golden fixtures written
+by ``tests/golden/make_golden.py`` from the reference).
+
+Bar: every state/flux/coefficient array within 1e-12 norm-wise relative
+difference (momenta share one scale, see ``fixtures.relerr_fields``); the
+troubled-cell flags identical except where the oracle's decision lies within
+1e-10 (relative) of its threshold.
+"""
+import numpy as np
+import pytest
+
+from fixtures import (golden_names, load, make_ic, noise_of, relerr, relerr_fields,
+                      sim_kwargs, flag_mismatch)
+from oracle.spd_oracle import OracleSimulator
+
+TOL = 1e-12
+NAMES = golden_names()
+
+
+def build(fx):
+    cfg = fx["cfg"]
+    return OracleSimulator(make_ic(cfg["ic"]), noise=noise_of(cfg), **sim_kwargs(cfg))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_ic_and_dt(name):
+    fx = load(name)
+    s = build(fx)
+    assert relerr_fields(s.W_init_cv, fx["W_init_cv"]) < TOL
+    assert relerr_fields(s.U_sp, fx["U_sp0"]) < TOL
+    assert abs(s.dt - fx["dt0"]) <= TOL * fx["dt0"]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_steps_and_substeps(name):
+    fx = load(name)
+    s = build(fx)
+    nsteps = int(fx["nsteps"])
+    ties = 0
+    for k in range(1, nsteps + 1):
+        s.captures = []
+        s.compute_dt()
+        assert abs(s.dt - fx[f"dt_{k}"]) <= TOL * fx[f"dt_{k}"], k
+        s.perform_update()
+        if f"U_sp_{k}" in fx:
+            assert relerr_fields(s.U_sp, fx[f"U_sp_{k}"]) < TOL, k
+        if f"U_cv_{k}" in fx:
+            assert relerr_fields(s.U_cv, fx[f"U_cv_{k}"]) < TOL, k
+            assert relerr_fields(s.W_cv, fx[f"W_cv_{k}"]) < TOL, k
+        for i, rec in enumerate(s.captures):
+            pre = f"cap{k}_{i}_"
+            keys = [kk[len(pre):] for kk in fx if kk.startswith(pre)]
+            if not keys:
+                continue
+            bad, ndiff = flag_mismatch(rec["troubles"], fx[pre + "troubles"], rec["margin"])
+            assert bad == 0, (k, i, bad)
+            ties += ndiff
+            for key in keys:
+                if key == "troubles":
+                    continue
+                if ndiff and key in ("theta", "aff_x", "aff_y", "aff_z"):
+                    continue          # follow from a tie-broken flag
+                got = np.asarray(rec[key]).reshape(fx[pre + key].shape)
+                err = relerr_fields(got, fx[pre + key]) if key.startswith(("F", "U", "W")) \
+                    else relerr(got, fx[pre + key])
+                assert err < TOL, (k, i, key, err)
+
+
+@pytest.mark.parametrize("name", [n for n in NAMES if "final_U_sp" in load(n)])
+def test_time_evolution(name):
+    fx = load(name)
+    s = build(fx)
+    s.perform_time_evolution(fx["cfg"]["t_end"])
+    assert s.n_step == int(fx["final_n_step"])
+    assert abs(s.time - float(fx["final_time"])) < 1e-14
+    assert relerr_fields(s.U_sp, fx["final_U_sp"]) < TOL
+    assert relerr_fields(s.W_cv, fx["final_W_cv"]) < TOL
