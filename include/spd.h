@@ -1,0 +1,152 @@
+/*
+ * spd.h -- C ABI of the MI355X-native SD-ADER time step (libspd_hip.so).
+ *
+ * One context describes one simulation (dimension, order, mesh, options) and
+ * owns every scratch buffer of the step.  The caller owns the state arrays;
+ * all of them are FP64 device pointers in the reference's layouts:
+ *
+ *   element layout  (V, [Nz,] [Ny,] Nx, [nz,] [ny,] nx)   U_sp, U_cv, W_cv
+ *
+ * with V = ndim + 2 variables ordered (rho, v_x[, v_y[, v_z]], P) for
+ * primitives and (rho, m_x[, m_y[, m_z]], E) for conservatives, and
+ * n = p + 1 solution points per dimension.  All calls are asynchronous on the
+ * given HIP stream; nothing in spd_step or spd_compute_dt synchronises the
+ * host.  Functions return 0 on success or a negative SPD_E* code, and
+ * spd_last_error() describes the last failure of the calling thread.  A
+ * context is not re-entrant: calls that overlap on one context race.
+ *
+ * Each entry point cites the reference interface it replaces
+ * (paths relative to the reference repository jpalafou/spd).
+ */
+#ifndef SPD_H
+#define SPD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPD_ABI_VERSION 1
+
+enum spd_status {
+  SPD_OK = 0,
+  SPD_E_INVALID = -1,       /* invalid or unsupported configuration / argument */
+  SPD_E_UNSUPPORTED = -2,   /* option the reference has but this build does not */
+  SPD_E_HIP = -3,           /* HIP runtime error (allocation, launch) */
+  SPD_E_NOMEM = -4
+};
+
+enum spd_riemann { SPD_RS_LLF = 0, SPD_RS_HLLC = 1 };
+enum spd_bc { SPD_BC_PERIODIC = 0, SPD_BC_GRADFREE = 1, SPD_BC_REFLECTIVE = 2 };
+enum spd_update { SPD_UPDATE_SD = 0, SPD_UPDATE_FV = 1 };
+enum spd_nad { SPD_NAD_RELATIVE = 0, SPD_NAD_DELTA = 1 };
+
+/* Constructor keywords of SDADER_Simulator (src/sdader_simulator.py:15-27,
+ * src/sd_simulator.py:20-26, src/fv_simulator.py:11-17, src/simulator.py:10-36)
+ * as a plain struct.  Unsupported reference options (viscosity, potential,
+ * well-balancing, isothermal, "ic"/"eq"/"pressure" sides, moncen, lhllc,
+ * predictor=False) have no field; the Python facade rejects them. */
+typedef struct spd_config {
+  int ndim;               /* 1, 2 or 3 */
+  int N[3];               /* elements per dimension (x, y, z) */
+  int p;                  /* polynomial degree (solution points p+1) */
+  int m;                  /* ADER order - 1; must equal p in this build */
+  double lim[3][2];       /* domain bounds per dimension */
+  double gamma;
+  double cfl_coeff;
+  double min_c2;
+  int update;             /* spd_update */
+  int FB;                 /* a-posteriori fallback on/off */
+  int riemann_sd;         /* spd_riemann of the element faces */
+  int riemann_fv;         /* spd_riemann of the fallback faces */
+  int bc[3][2];           /* spd_bc per dimension and side (left, right) */
+  double tolerance;       /* NAD tolerance */
+  int NAD;                /* spd_nad */
+  int SED;
+  int PAD;
+  int blending;
+  double min_rho, max_rho, min_P;
+  int godunov;
+} spd_config;
+
+typedef struct spd_ctx spd_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* spd_last_error(void);
+
+/* ABI version of the library (SPD_ABI_VERSION) and sizeof(spd_config), for
+ * bindings to check their struct mirror. */
+int spd_abi_version(void);
+size_t spd_config_size(void);
+
+/* Host-only (no GPU needed): the operator matrices of order (p, m) in the
+ * layout of spd_get_operators, and the ghosted subcell geometry of one
+ * dimension with N elements of width h -- face positions (N(p+1)+5), centres
+ * (N(p+1)+4), widths (N(p+1)+4), centre spacings (N(p+1)+3), concatenated
+ * (src/polynomials.py:5-122, src/sdader_simulator.py:40-92).  Return the
+ * number of doubles written or a negative code. */
+int spd_host_operators(int p, int m, double* host_out, int len);
+int spd_host_geometry(int N, double h, int p, double* host_out, int len);
+
+/* Build the operator matrices (src/polynomials.py:5-122,
+ * src/sd_simulator.py:44-52, src/sdader_simulator.py:40-46), the subcell
+ * geometry (src/sdader_simulator.py:61-92) and allocate the step's scratch on
+ * the current HIP device.  Replaces the constructor of SDADER_Simulator. */
+int spd_ctx_create(const spd_config* cfg, spd_ctx** out);
+int spd_ctx_destroy(spd_ctx* ctx);
+
+/* Bytes of device scratch the context holds. */
+size_t spd_ctx_scratch_bytes(const spd_ctx* ctx);
+
+/* Copy the host operator matrices: sp_to_fp (nf x n), dfp_to_sp (n x nf),
+ * sp_to_cv (n x n), cv_to_sp (n x n), invader (T x T), w_tp (T), x_tp (T),
+ * row-major, concatenated in that order.  `len` is the capacity in doubles;
+ * returns the number of doubles written or a negative code. */
+int spd_get_operators(const spd_ctx* ctx, double* host_out, int len);
+
+/* dt = cfl * (h_min/(p+1)) / max(ndim*c_s + sum|v|) over the subcell-mean
+ * primitives (src/sd_simulator.py:214-227).  With W_cv != NULL the maximum is
+ * taken over that array (element layout); with W_cv == NULL the maximum the
+ * last spd_step recorded for its new state is used (the value the reference
+ * computes from its post-step W_cv).  Writes dt to dt_dev. */
+int spd_compute_dt(spd_ctx* ctx, const double* W_cv, double* dt_dev, void* stream);
+
+/* One step: ADER predictor + SD or FV(/FB) corrector
+ * (SDADER_Simulator.perform_update, src/sdader_simulator.py:361-378).
+ * U_sp is updated in place.  If `snapshot` is non-zero, U_cv and W_cv receive
+ * the post-step subcell means and primitives in element layout (the values
+ * perform_update leaves in dm.U_cv / dm.W_cv); otherwise they are not touched
+ * and may be NULL.  dt_dev points at the step's dt on the device. */
+int spd_step(spd_ctx* ctx, double* U_sp, double* U_cv, double* W_cv, const double* dt_dev,
+             int snapshot, void* stream);
+
+/* Element-layout tensor-product transforms (src/transforms.py:27-45 via
+ * SD_Simulator.compute_cv_from_sp / compute_sp_from_cv, src/sd_simulator.py:184-188). */
+int spd_sp_to_cv(spd_ctx* ctx, const double* in, double* out, void* stream);
+int spd_cv_to_sp(spd_ctx* ctx, const double* in, double* out, void* stream);
+
+/* Pointwise conservative -> primitive on `count` points of a (V, count) array
+ * (src/hydro.py:48-86). */
+int spd_cons_to_prims(spd_ctx* ctx, const double* U, double* W, long long count, void* stream);
+
+/* Pointwise primitive -> conservative on `count` points of a (V, count) array
+ * (src/hydro.py:88-125). */
+int spd_prims_to_cons(spd_ctx* ctx, const double* W, double* U, long long count, void* stream);
+
+/* Debug tap for flag parity: during the next spd_step, after the trouble
+ * detection of FV substep `substep`, copy the troubled-cell flags
+ * (Ns_z x Ns_y x Ns_x doubles), the blended coefficients theta on the ghosted
+ * grid ((Ns+4)^ndim doubles) and, per dimension, the face coefficients
+ * (the reference's dm.troubles, dm.theta and dm.affected_faces_{x,y,z},
+ * src/trouble_detection.py:5-87) into the given device buffers.  Pass
+ * substep = -1 to disable, or SPD_TAP_ALL to record every substep (the buffers
+ * then hold m+1 consecutive blocks).  Any buffer may be NULL. */
+#define SPD_TAP_ALL (-2)
+int spd_set_debug_tap(spd_ctx* ctx, int substep, double* troubles, double* theta,
+                      double* aff_x, double* aff_y, double* aff_z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPD_H */

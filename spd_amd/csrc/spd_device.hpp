@@ -1,0 +1,260 @@
+// Device-side building blocks of the SD-ADER step: pointwise Euler algebra,
+// the two Riemann solvers, NaN-propagating extrema and the index arithmetic
+// of the element and finite-volume layouts.  Everything here is FP64.
+//
+// Variable order (both sets): 0 = density, 1..NDIM = momentum / velocity
+// components (x first), NDIM+1 = total energy / pressure.  A "rolled" velocity
+// list for dimension `a` starts with the normal component a+1 and continues
+// cyclically, which is `np.roll(vels, -a)` in the reference
+// (src/sdader_simulator.py:219, src/fv_simulator.py:149).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spd {
+
+enum : int { RS_LLF = 0, RS_HLLC = 1 };
+enum : int { BC_PERIODIC = 0, BC_GRADFREE = 1, BC_REFLECTIVE = 2 };
+
+constexpr int NGH = 2;  // finite-volume ghost layers (Nghc in the reference)
+
+template <int NDIM> struct Dims {
+  static constexpr int V = NDIM + 2;
+  static constexpr int P = NDIM + 1;  // index of energy / pressure
+};
+
+template <int B, int E> struct IPow { static constexpr int v = B * IPow<B, E - 1>::v; };
+template <int B> struct IPow<B, 0> { static constexpr int v = 1; };
+
+// numpy.maximum / numpy.minimum: NaN in either operand propagates.
+__device__ __forceinline__ double nmax(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return a > b ? a : b;
+}
+__device__ __forceinline__ double nmin(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return a < b ? a : b;
+}
+
+// Rolled velocity index k (0-based position in the rolled list) for dim a.
+template <int NDIM> __device__ __forceinline__ constexpr int rolled(int a, int k) {
+  return 1 + (a + k) % NDIM;
+}
+
+// conservative -> primitive (src/hydro.py:48-86), rolled list of dim a.
+template <int NDIM>
+__device__ __forceinline__ void prims(const double* U, double* W, int a, double gamma) {
+  constexpr int P = Dims<NDIM>::P;
+  W[0] = U[0];
+  double K = U[0] * 0.0;
+#pragma unroll
+  for (int k = 0; k < NDIM; ++k) {
+    const int v = rolled<NDIM>(a, k);
+    W[v] = U[v] / U[0];
+    K = K + W[v] * W[v];
+  }
+  K = K * (0.5 * U[0]);
+  W[P] = (gamma - 1.0) * (U[P] - K);
+}
+
+// primitive -> conservative (src/hydro.py:88-125), rolled list of dim a.
+template <int NDIM>
+__device__ __forceinline__ void cons(const double* W, double* U, int a, double gamma) {
+  constexpr int P = Dims<NDIM>::P;
+  U[0] = W[0];
+  double K = W[0] * 0.0;
+#pragma unroll
+  for (int k = 0; k < NDIM; ++k) {
+    const int v = rolled<NDIM>(a, k);
+    U[v] = W[v] * U[0];
+    K = K + W[v] * W[v];
+  }
+  K = K * (0.5 * U[0]);
+  U[P] = W[P] / (gamma - 1.0) + K;
+}
+
+// Euler flux along dim a from primitives (src/hydro.py:127-170): the kinetic
+// sum runs over the rolled list in reverse, so the normal component is last.
+template <int NDIM>
+__device__ __forceinline__ void flux(const double* W, double* F, int a, double gamma) {
+  constexpr int P = Dims<NDIM>::P;
+  const int v1 = a + 1;
+  double K = W[0] * 0.0;
+  double m = 0.0;
+#pragma unroll
+  for (int k = NDIM - 1; k >= 0; --k) {
+    const int v = rolled<NDIM>(a, k);
+    m = W[0] * W[v];
+    K = K + m * W[v];
+    F[v] = m * W[v1];
+  }
+  const double E = W[P] / (gamma - 1.0) + 0.5 * K;
+  F[0] = m;
+  F[v1] = m * W[v1] + W[P];
+  F[P] = W[v1] * (E + W[P]);
+}
+
+// Sound speed with the where-clamp of src/hydro.py:3-46 (NaN -> min_c2).
+__device__ __forceinline__ double sound(double P, double rho, double gamma, double min_c2) {
+  double c2 = gamma * P / rho;
+  c2 = (c2 > min_c2) ? c2 : min_c2;
+  return sqrt(c2);
+}
+
+// Local Lax-Friedrichs flux (src/riemann_solver.py:31-65).
+template <int NDIM>
+__device__ __forceinline__ void llf(const double* WL, const double* WR, const double* UL,
+                                    const double* UR, double* F, int a, double gamma,
+                                    double min_c2) {
+  constexpr int V = Dims<NDIM>::V;
+  constexpr int P = Dims<NDIM>::P;
+  const int v1 = a + 1;
+  double FL[V], FR[V];
+  flux<NDIM>(WL, FL, a, gamma);
+  flux<NDIM>(WR, FR, a, gamma);
+  const double cL = sound(WL[P], WL[0], gamma, min_c2) + fabs(WL[v1]);
+  const double cR = sound(WR[P], WR[0], gamma, min_c2) + fabs(WR[v1]);
+  const double cmax = (cL > cR) ? cL : cR;
+#pragma unroll
+  for (int v = 0; v < V; ++v) F[v] = 0.5 * (FR[v] + FL[v]) - 0.5 * cmax * (UR[v] - UL[v]);
+}
+
+// HLLC flux in the Godunov-state form of src/riemann_solver.py:67-145.  Only
+// the selected star state is evaluated; its formula is the reference's.
+template <int NDIM>
+__device__ __forceinline__ void hllc(const double* WL, const double* WR, const double* UL,
+                                     const double* UR, double* F, int a, double gamma,
+                                     double min_c2) {
+  constexpr int P = Dims<NDIM>::P;
+  const int v1 = a + 1;
+  const double cL = sound(WL[P], WL[0], gamma, min_c2) + fabs(WL[v1]);
+  const double cR = sound(WR[P], WR[0], gamma, min_c2) + fabs(WR[v1]);
+  const double cmax = (cL > cR) ? cL : cR;
+  const double vL = WL[v1], vR = WR[v1];
+  const double sL = ((vL < vR) ? vL : vR) - cmax;
+  const double sR = ((vL > vR) ? vL : vR) + cmax;
+  const double rcL = WL[0] * (vL - sL);
+  const double rcR = WR[0] * (sR - vR);
+  const double vs = (rcR * vR + rcL * vL + (WL[P] - WR[P])) / (rcR + rcL);
+  const double Ps = (rcR * WL[P] + rcL * WR[P] + rcL * rcR * (vL - vR)) / (rcR + rcL);
+  double r, v, p, e;
+  if (sL > 0.0) {
+    r = WL[0]; v = vL; p = WL[P]; e = UL[P];
+  } else if (vs > 0.0) {
+    r = WL[0] * (sL - vL) / (sL - vs);
+    e = ((sL - vL) * UL[P] - WL[P] * vL + Ps * vs) / (sL - vs);
+    v = vs; p = Ps;
+  } else if (sR > 0.0) {
+    r = WR[0] * (sR - vR) / (sR - vs);
+    e = ((sR - vR) * UR[P] - WR[P] * vR + Ps * vs) / (sR - vs);
+    v = vs; p = Ps;
+  } else {
+    r = WR[0]; v = vR; p = WR[P]; e = UR[P];
+  }
+  F[0] = r * v;
+  F[v1] = F[0] * v + p;
+  F[P] = v * (e + p);
+#pragma unroll
+  for (int k = 1; k < NDIM; ++k) {
+    const int vt = rolled<NDIM>(a, k);
+    F[vt] = F[0] * ((vs > 0.0) ? WL[vt] : WR[vt]);
+  }
+}
+
+template <int NDIM, int RS>
+__device__ __forceinline__ void riemann(const double* WL, const double* WR, const double* UL,
+                                        const double* UR, double* F, int a, double gamma,
+                                        double min_c2) {
+  if constexpr (RS == RS_HLLC)
+    hllc<NDIM>(WL, WR, UL, UR, F, a, gamma, min_c2);
+  else
+    llf<NDIM>(WL, WR, UL, UR, F, a, gamma, min_c2);
+}
+
+// ---------------------------------------------------------------------------
+// Problem description shared by all kernels (passed by value).
+// ---------------------------------------------------------------------------
+struct Geom {
+  int N[3];       // elements per dimension (x, y, z); 1 for absent dims
+  int Ns[3];      // subcells per dimension = N * (p+1)
+  int Ng[3];      // ghosted subcells per dimension = Ns + 2*NGH
+  int bc[3][2];   // boundary kind per dimension and side
+  double h[3];    // element widths
+  // subcell geometry of the ghosted grid, per dimension
+  const double* faces[3];    // Ns + 5 face positions
+  const double* centers[3];  // Ns + 4 cell centres
+  const double* hfp[3];      // Ns + 4 cell widths
+  const double* hcv[3];      // Ns + 3 centre spacings
+  long long NE;   // number of elements
+  long long Nc;   // number of active subcells
+  long long Ngc;  // number of ghosted subcells
+};
+
+struct Params {
+  double gamma, min_c2;
+  double tol, min_rho, max_rho, min_P;
+  int p;
+  int nad_delta, sed, pad, blending, godunov;
+};
+
+// Map a ghosted-grid coordinate (active range [0, Ns)) to the active cell whose
+// value the finite-volume ghost fill copies there (src/fv_simulator.py:224-273):
+// periodic wrap, or the non-mirrored copy of the first / last NGH active layers.
+// Returns the mapped coordinate and sets *flip when a reflective side negates
+// the normal velocity.
+__device__ __forceinline__ int ghost_map(int c, int Ns, const int* bc, bool* flip) {
+  if (c < 0) {
+    if (bc[0] == BC_PERIODIC) return c + Ns;
+    if (bc[0] == BC_REFLECTIVE) *flip = !*flip;
+    return c + NGH;
+  }
+  if (c >= Ns) {
+    if (bc[1] == BC_PERIODIC) return c - Ns;
+    if (bc[1] == BC_REFLECTIVE) *flip = !*flip;
+    return c - NGH;
+  }
+  return c;
+}
+
+// FV-layout linear index of an active cell (x fastest).
+template <int NDIM>
+__device__ __forceinline__ long long cell_index(const Geom& g, const int* c) {
+  long long idx = 0;
+#pragma unroll
+  for (int a = NDIM - 1; a >= 0; --a) idx = idx * g.Ns[a] + c[a];
+  return idx;
+}
+
+// Primitive state of the ghost-filled array at a ghosted coordinate `c`
+// (active range [0, Ns) per dimension; ghosts in [-NGH, 0) and [Ns, Ns+NGH)).
+// The value is the one left by the fill x first, then y, then z: every ghost
+// coordinate is mapped independently and each reflective crossing negates its
+// own velocity component.
+template <int NDIM>
+__device__ __forceinline__ void load_W(const double* __restrict__ U, const Geom& g, const int* c,
+                                       double* W, double gamma) {
+  constexpr int V = Dims<NDIM>::V;
+  int m[3];
+  bool flip[3] = {false, false, false};
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) m[a] = ghost_map(c[a], g.Ns[a], g.bc[a], &flip[a]);
+  const long long idx = cell_index<NDIM>(g, m);
+  double u[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) u[v] = U[v * g.Nc + idx];
+  prims<NDIM>(u, W, 0, gamma);
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a)
+    if (flip[a]) W[a + 1] = -W[a + 1];
+}
+
+// Atomic max of a non-negative double held as its IEEE bit pattern (the
+// ordering of non-negative doubles, +inf and positive NaN matches the
+// unsigned ordering of their bits, so np.max semantics carry over).
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* addr, double v) {
+  atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+}  // namespace spd

@@ -1,0 +1,1678 @@
+// MI355X (gfx950) kernels of the SD-ADER step and the C ABI of libspd_hip.so.
+//
+// Data layouts (all FP64):
+//   element layout   (V, E, L)        E = elements (x fastest), L = (p+1)^d points
+//   predictor state  (T, V, E, L)     T = m+1 time nodes
+//   element faces    (T, V, N_a+1, E_perp, L_perp) per dimension a
+//   FV cells         (V, Ns_z, Ns_y, Ns_x)
+//   FV faces         (V, ...) with Ns_a+1 faces along dimension a
+//   ghosted FV grid  (Ns_z+4, Ns_y+4, Ns_x+4)  (theta)
+//
+// One step (SDADER_Simulator.perform_update, src/sdader_simulator.py:361-378):
+//   predictor  m+1 x [element-face Riemann (k_face_sd) + update (k_pred)]
+//   corrector  SD : k_pred in SD-update mode
+//              FV : sp->cv (k_sp2cv), then T substeps of
+//                   [HO face fluxes (k_hoflux), candidate (k_apply),
+//                    detection (k_detect), theta (k_theta1, k_ring),
+//                    fallback + blend (k_fbflux), apply (k_apply)],
+//                   then cv->sp with the dt reduction (k_cv2sp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/spd.h"
+#include "spd_device.hpp"
+#include "spd_host_ops.hpp"
+
+namespace spd {
+
+template <int P> struct Ops {
+  static constexpr int n = P + 1, nf = P + 2, T = P + 1;
+  double s2f[nf][n];  // sp_to_fp
+  double d2s[n][nf];  // dfp_to_sp
+  double s2c[n][n];   // sp_to_cv
+  double c2s[n][n];   // cv_to_sp
+  double inv[T][T];   // invader
+  double w[T];        // w_tp
+};
+
+// ---------------------------------------------------------------------------
+// index helpers
+// ---------------------------------------------------------------------------
+template <int NDIM>
+__device__ __forceinline__ void elem_coords(const Geom& g, long long e, int* c) {
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    c[a] = (int)(e % g.N[a]);
+    e /= g.N[a];
+  }
+}
+template <int NDIM>
+__device__ __forceinline__ long long elem_index(const Geom& g, const int* c) {
+  long long e = 0;
+#pragma unroll
+  for (int a = NDIM - 1; a >= 0; --a) e = e * g.N[a] + c[a];
+  return e;
+}
+// element coordinates from (perpendicular element index, coordinate along a)
+template <int NDIM>
+__device__ __forceinline__ void elem_from_perp(const Geom& g, int a, long long ep, int ea, int* c) {
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    if (b == a) continue;
+    c[b] = (int)(ep % g.N[b]);
+    ep /= g.N[b];
+  }
+  c[a] = ea;
+}
+template <int NDIM>
+__device__ __forceinline__ long long perp_index(const Geom& g, int a, const int* c) {
+  long long r = 0;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b) {
+    if (b == a) continue;
+    r = r * g.N[b] + c[b];
+  }
+  return r;
+}
+// local point index of the line `lp` (local coordinates other than a) at
+// position 0 along a; the stride along a is n^a.
+template <int NDIM, int n>
+__device__ __forceinline__ int line_base(int a, int lp) {
+  int c[3] = {0, 0, 0};
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    if (b == a) continue;
+    c[b] = lp % n;
+    lp /= n;
+  }
+  int l = 0;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b) l = l * n + c[b];
+  return l;
+}
+template <int n> __device__ __forceinline__ int stride_of(int a) {
+  int s = 1;
+  for (int b = 0; b < a; ++b) s *= n;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Predictor: Riemann problems at element faces for every time node
+// (src/sdader_simulator.py:213-240, src/sd_boundary.py:7-76).
+// One thread per (t, face, perpendicular element, perpendicular line).
+// ---------------------------------------------------------------------------
+template <int NDIM, int P, int RS>
+__global__ void __launch_bounds__(256)
+k_face_sd(Ops<P> op, Geom g, Params prm, int a, const double* __restrict__ src, int Tsrc,
+          double* __restrict__ Fface) {
+  constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
+  constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
+  const int Na = g.N[a];
+  const long long NEp = g.NE / Na;
+  const long long per_t = (long long)(Na + 1) * NEp * NLP;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= per_t * Tsrc) return;
+  const int lp = (int)(tid % NLP);
+  long long r = tid / NLP;
+  const long long ep = r % NEp;
+  r /= NEp;
+  const int f = (int)(r % (Na + 1));
+  const int t = (int)(r / (Na + 1));
+  const int lb = line_base<NDIM, n>(a, lp);
+  const int sa = stride_of<n>(a);
+  const double* base = src + (long long)t * V * g.NE * NL;
+
+  auto trace = [&](int ea, int fp, double* q) {
+    int c[3];
+    elem_from_perp<NDIM>(g, a, ep, ea, c);
+    const long long e = elem_index<NDIM>(g, c);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const double* u = base + ((long long)v * g.NE + e) * NL + lb;
+      double acc = 0.0;
+#pragma unroll
+      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[s * sa];
+      q[v] = acc;
+    }
+  };
+  double qL[V], qR[V];
+  if (f > 0) trace(f - 1, nf - 1, qL);
+  if (f < Na) trace(f, 0, qR);
+  if (f == 0) {
+    const int k = g.bc[a][0];
+    if (k == BC_PERIODIC) {
+      trace(Na - 1, nf - 1, qL);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) qL[v] = qR[v];
+      if (k == BC_REFLECTIVE) qL[a + 1] = -qL[a + 1];
+    }
+  }
+  if (f == Na) {
+    const int k = g.bc[a][1];
+    if (k == BC_PERIODIC) {
+      trace(0, 0, qR);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) qR[v] = qL[v];
+      if (k == BC_REFLECTIVE) qR[a + 1] = -qR[a + 1];
+    }
+  }
+  double WL[V], WR[V], F[V];
+  prims<NDIM>(qL, WL, a, prm.gamma);
+  prims<NDIM>(qR, WR, a, prm.gamma);
+  riemann<NDIM, RS>(WL, WR, qL, qR, F, a, prm.gamma, prm.min_c2);
+  const long long plane = NEp * NLP;
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+    Fface[((long long)(t * V + v) * (Na + 1) + f) * plane + ep * NLP + lp] = F[v];
+}
+
+// ---------------------------------------------------------------------------
+// Predictor update.  A block owns EPB elements and all T time nodes.  Phase
+// "dim a": one thread per (element, t, line along a) interpolates the line to
+// the flux points, evaluates the Euler flux at the interior ones, takes the
+// element-face Riemann fluxes at the ends and differentiates back
+// (src/sdader_simulator.py:148-158, src/sd_simulator.py:190-206); the
+// derivative accumulates in LDS.  Update phase:
+//   mode 0 (Picard, src/sdader_simulator.py:187-200):
+//      Uad[n] = U_sp - dt * sum_p invader[n,p] dudt[p]
+//   mode 1 (SD corrector, src/sdader_simulator.py:202-208):
+//      U_sp  -= dt * sum_t w_t dudt[t]
+// With Tsrc == 1 the source is U_sp for every time node (first iteration)
+// and only t = 0 is evaluated.
+// ---------------------------------------------------------------------------
+template <int NDIM, int P>
+__global__ void __launch_bounds__(256)
+k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
+       const double* __restrict__ Ff0, const double* __restrict__ Ff1,
+       const double* __restrict__ Ff2, const double* Usp, const double* __restrict__ dt_dev,
+       double* out, int mode, int EPB) {
+  constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2, T = P + 1;
+  constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
+  extern __shared__ double lds[];  // [EPB][T][V][NL]
+  const int el = threadIdx.x / (T * NLP);
+  const int rem = threadIdx.x % (T * NLP);
+  const int t = rem / NLP;
+  const int lp = rem % NLP;
+  const long long e = (long long)blockIdx.x * EPB + el;
+  const bool active = (el < EPB) && (e < g.NE) && (t < Tsrc);
+  const double* Ff[3] = {Ff0, Ff1, Ff2};
+  int c[3] = {0, 0, 0};
+  if (e < g.NE) elem_coords<NDIM>(g, e, c);
+
+#pragma unroll 1
+  for (int a = 0; a < NDIM; ++a) {
+    if (active) {
+      const int lb = line_base<NDIM, n>(a, lp);
+      const int sa = stride_of<n>(a);
+      double u[V][n];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const double* s0 = src + ((long long)(t * V + v) * g.NE + e) * NL + lb;
+#pragma unroll
+        for (int s = 0; s < n; ++s) u[v][s] = s0[s * sa];
+      }
+      double F[nf][V];
+      const int Na = g.N[a];
+      const long long NEp = g.NE / Na;
+      const long long ep = perp_index<NDIM>(g, a, c);
+      const long long plane = NEp * NLP;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const double* fb = Ff[a] + (long long)(t * V + v) * (Na + 1) * plane + ep * NLP + lp;
+        F[0][v] = fb[(long long)c[a] * plane];
+        F[nf - 1][v] = fb[(long long)(c[a] + 1) * plane];
+      }
+#pragma unroll
+      for (int fp = 1; fp < nf - 1; ++fp) {
+        double M[V], W[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          double acc = 0.0;
+#pragma unroll
+          for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[v][s];
+          M[v] = acc;
+        }
+        prims<NDIM>(M, W, a, prm.gamma);
+        flux<NDIM>(W, F[fp], a, prm.gamma);
+      }
+      const double ha = g.h[a];
+      double* d0 = lds + ((long long)(el * T + t) * V) * NL + lb;
+#pragma unroll
+      for (int s = 0; s < n; ++s) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          double acc = 0.0;
+#pragma unroll
+          for (int fp = 0; fp < nf; ++fp) acc += op.d2s[s][fp] * F[fp][v];
+          const double d = acc / ha;
+          double* dst = d0 + v * NL + s * sa;
+          if (a == 0)
+            *dst = d;
+          else
+            *dst = *dst + d;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  const double dt = *dt_dev;
+  const int total = EPB * V * NL;
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int le = idx / (V * NL);
+    const int v = (idx / NL) % V;
+    const int l = idx % NL;
+    const long long ee = (long long)blockIdx.x * EPB + le;
+    if (ee >= g.NE) continue;
+    const long long off = ((long long)v * g.NE + ee) * NL + l;
+    const double u0 = Usp[off];
+    const double* dd = lds + ((long long)le * T * V + v) * NL + l;  // + p*V*NL
+    if (mode == 0) {
+#pragma unroll
+      for (int tn = 0; tn < T; ++tn) {
+        double acc = 0.0;
+#pragma unroll
+        for (int p = 0; p < T; ++p) {
+          const int ps = (p < Tsrc) ? p : (Tsrc - 1);
+          acc += op.inv[tn][p] * dd[(long long)ps * V * NL];
+        }
+        out[((long long)(tn * V + v) * g.NE + ee) * NL + l] = u0 - acc * dt;
+      }
+    } else {
+      double acc = 0.0;
+#pragma unroll
+      for (int p = 0; p < T; ++p) acc += op.w[p] * dd[(long long)p * V * NL];
+      out[off] = u0 - acc * dt;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// High-order fluxes of time node t on the FV faces of dimension a: flux-point
+// values (end points from the element-face Riemann solve), averaged over the
+// subcell extent in the transverse dimensions (src/sdader_simulator.py:288-323,
+// src/sd_simulator.py:208-212).  One thread per (element, flux point).
+// ---------------------------------------------------------------------------
+template <int NDIM, int P>
+__global__ void __launch_bounds__(256)
+k_hoflux(Ops<P> op, Geom g, Params prm, int a, int t, const double* __restrict__ Uad,
+         const double* __restrict__ Fface, double* __restrict__ Fho) {
+  constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
+  constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= g.NE * nf) return;
+  const int f = (int)(tid % nf);
+  const long long e = tid / nf;
+  int c[3] = {0, 0, 0};
+  elem_coords<NDIM>(g, e, c);
+  const int Na = g.N[a];
+  if (f == nf - 1 && c[a] != Na - 1) return;
+  const long long NEp = g.NE / Na;
+  const long long ep = perp_index<NDIM>(g, a, c);
+  const long long plane = NEp * NLP;
+  const int sa = stride_of<n>(a);
+  double Fl[NLP][V];
+#pragma unroll
+  for (int lp = 0; lp < NLP; ++lp) {
+    if (f == 0 || f == nf - 1) {
+      const int fa = c[a] + (f == 0 ? 0 : 1);
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        Fl[lp][v] = Fface[((long long)(t * V + v) * (Na + 1) + fa) * plane + ep * NLP + lp];
+    } else {
+      const int lb = line_base<NDIM, n>(a, lp);
+      double M[V], W[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const double* u = Uad + ((long long)(t * V + v) * g.NE + e) * NL + lb;
+        double acc = 0.0;
+#pragma unroll
+        for (int s = 0; s < n; ++s) acc += op.s2f[f][s] * u[s * sa];
+        M[v] = acc;
+      }
+      prims<NDIM>(M, W, a, prm.gamma);
+      flux<NDIM>(W, Fl[lp], a, prm.gamma);
+    }
+  }
+  // transverse subcell averages, lower dimension first
+  double Fa[NLP][V];
+  if constexpr (NDIM == 1) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) Fa[0][v] = Fl[0][v];
+  } else if constexpr (NDIM == 2) {
+#pragma unroll
+    for (int q = 0; q < n; ++q)
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        double acc = 0.0;
+#pragma unroll
+        for (int s = 0; s < n; ++s) acc += op.s2c[q][s] * Fl[s][v];
+        Fa[q][v] = acc;
+      }
+  } else {
+    double G[NLP][V];
+#pragma unroll
+    for (int c2 = 0; c2 < n; ++c2)
+#pragma unroll
+      for (int q1 = 0; q1 < n; ++q1)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          double acc = 0.0;
+#pragma unroll
+          for (int s = 0; s < n; ++s) acc += op.s2c[q1][s] * Fl[s + n * c2][v];
+          G[q1 + n * c2][v] = acc;
+        }
+#pragma unroll
+    for (int q2 = 0; q2 < n; ++q2)
+#pragma unroll
+      for (int q1 = 0; q1 < n; ++q1)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          double acc = 0.0;
+#pragma unroll
+          for (int s = 0; s < n; ++s) acc += op.s2c[q2][s] * G[q1 + n * s][v];
+          Fa[q1 + n * q2][v] = acc;
+        }
+  }
+  // scatter into the FV face array of dimension a
+  long long ext[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) ext[b] = (b < NDIM) ? (g.Ns[b] + (b == a ? 1 : 0)) : 1;
+  long long nface = 1;
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) nface *= ext[b];
+#pragma unroll
+  for (int lp = 0; lp < NLP; ++lp) {
+    int q[3] = {0, 0, 0};
+    int r = lp;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      if (b == a) continue;
+      q[b] = r % n;
+      r /= n;
+    }
+    long long idx = 0;
+#pragma unroll
+    for (int b = NDIM - 1; b >= 0; --b) {
+      const long long cb = (b == a) ? (long long)c[a] * n + f : (long long)c[b] * n + q[b];
+      idx = idx * ext[b] + cb;
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) Fho[v * nface + idx] = Fa[lp][v];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element-layout tensor-product transform (sp <-> cv) of a (V, E, L) array,
+// one thread per (element, variable, output point); the sum runs over all
+// input points with the product of the 1-D factors (src/transforms.py:27-45).
+// out_fv != 0 writes the FV layout instead of the element layout.
+// ---------------------------------------------------------------------------
+template <int NDIM, int P>
+__global__ void __launch_bounds__(256)
+k_tensor(Ops<P> op, Geom g, int to_cv, const double* __restrict__ in, double* __restrict__ out,
+         int out_fv) {
+  constexpr int V = Dims<NDIM>::V, n = P + 1;
+  constexpr int NL = IPow<n, NDIM>::v;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= g.NE * V * NL) return;
+  const int lo = (int)(tid % NL);
+  const long long r = tid / NL;
+  const long long e = r % g.NE;
+  const int v = (int)(r / g.NE);
+  const double(*M)[n] = to_cv ? op.s2c : op.c2s;
+  int qo[3] = {0, 0, 0};
+  {
+    int x = lo;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      qo[b] = x % n;
+      x /= n;
+    }
+  }
+  const double* u = in + ((long long)v * g.NE + e) * NL;
+  double acc = 0.0;
+  for (int li = 0; li < NL; ++li) {
+    int x = li;
+    int qi[3];
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      qi[b] = x % n;
+      x /= n;
+    }
+    double w = 1.0;
+    // product in the order of the einsum operands (outermost dimension first)
+#pragma unroll
+    for (int b = NDIM - 1; b >= 0; --b) w = (b == NDIM - 1) ? M[qo[b]][qi[b]] : w * M[qo[b]][qi[b]];
+    acc += w * u[li];
+  }
+  if (!out_fv) {
+    out[((long long)v * g.NE + e) * NL + lo] = acc;
+  } else {
+    int c[3] = {0, 0, 0};
+    elem_coords<NDIM>(g, e, c);
+    int cc[3];
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) cc[b] = c[b] * n + qo[b];
+    out[(long long)v * g.Nc + cell_index<NDIM>(g, cc)] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FV update U_out = U_in - dt_i * sum_a (F_a[f+1] - F_a[f]) / dx_a
+// (src/fv_simulator.py:189-205); in place when U_out == U_in.
+// ---------------------------------------------------------------------------
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_apply(Geom g, const double* Uin, const double* __restrict__ F0, const double* __restrict__ F1,
+        const double* __restrict__ F2, const double* __restrict__ dt_dev, double w_i,
+        double* Uout) {
+  constexpr int V = Dims<NDIM>::V;
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= g.Nc) return;
+  int c[3] = {0, 0, 0};
+  {
+    long long x = cell;
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+      c[a] = (int)(x % g.Ns[a]);
+      x /= g.Ns[a];
+    }
+  }
+  const double* F[3] = {F0, F1, F2};
+  long long fi[3], fs[3], nf[3];
+  double dx[3];
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    long long idx = 0, stride = 1, sa = 0, tot = 1;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      const long long ext = g.Ns[b] + (b == a ? 1 : 0);
+      if (b == a) sa = stride;
+      idx += c[b] * stride;
+      stride *= ext;
+      tot *= ext;
+    }
+    fi[a] = idx;
+    fs[a] = sa;
+    nf[a] = tot;
+    dx[a] = g.hfp[a][c[a] + NGH];
+  }
+  const double dti = (*dt_dev) * w_i;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const double u = Uin[v * g.Nc + cell];
+    double d = u * 0.0;
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+      const double* Fa = F[a] + v * nf[a] + fi[a];
+      d = d + (Fa[fs[a]] - Fa[0]) / dx[a];
+    }
+    Uout[v * g.Nc + cell] = u - d * dti;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Trouble detection per active subcell (src/trouble_detection.py:5-71):
+// NAD against the plus-stencil extrema of the ghost-filled old primitives,
+// smooth-extrema relaxation (p > 1), PAD bounds.  Sets *any when any variable
+// of any cell is a NAD candidate (the reference's global gate, Q3).
+// ---------------------------------------------------------------------------
+template <int NDIM>
+__device__ __forceinline__ double sed_alpha(const double* __restrict__ Unew, const Geom& g,
+                                            const int* c, int a, int var, double gamma) {
+  constexpr int V = Dims<NDIM>::V;
+  const int Ns = g.Ns[a];
+  const int ca = c[a];
+  const double* cen = g.centers[a] + NGH;  // cen[k] = centre of active-relative cell k
+  const double* hf = g.hfp[a] + NGH;
+  double w[7];
+#pragma unroll
+  for (int k = -3; k <= 3; ++k) {
+    const int x = ca + k;
+    if (x >= -NGH && x < Ns + NGH) {
+      int cc[3] = {c[0], c[1], c[2]};
+      cc[a] = x;
+      double W[V];
+      load_W<NDIM>(Unew, g, cc, W, gamma);
+      w[k + 3] = W[var];
+    } else {
+      w[k + 3] = 0.0;
+    }
+  }
+  double dU[5];  // relative -2..2
+#pragma unroll
+  for (int k = -2; k <= 2; ++k) {
+    const int x = ca + k;
+    if (x >= -1 && x <= Ns)
+      dU[k + 2] = (w[k + 4] - w[k + 2]) / (cen[x + 1] - cen[x - 1]);
+    else
+      dU[k + 2] = 0.0;
+  }
+  double A[3], aLc = 1.0;
+#pragma unroll
+  for (int k = -1; k <= 1; ++k) {
+    const int x = ca + k;
+    if (x < 0 || x >= Ns) {
+      A[k + 1] = 0.0;
+      continue;
+    }
+    const double d2U = (dU[k + 3] - dU[k + 1]) / (cen[x + 1] - cen[x - 1]);
+    const double dv = 0.5 * hf[x] * d2U;
+    const double vL = dU[k + 1] - dU[k + 2];
+    const double vR = dU[k + 3] - dU[k + 2];
+    double aL = -((dv < 0.0) ? ((vL > 0.0) ? vL : 0.0) : ((vL < 0.0) ? vL : 0.0)) / dv;
+    aL = (fabs(dv) <= 0.0) ? 1.0 : aL;
+    aL = (aL < 1.0) ? aL : 1.0;
+    double aR = ((dv > 0.0) ? ((vR > 0.0) ? vR : 0.0) : ((vR < 0.0) ? vR : 0.0)) / dv;
+    aR = (fabs(dv) <= 0.0) ? 1.0 : aR;
+    aR = (aR < 1.0) ? aR : 1.0;
+    A[k + 1] = (aR < aL) ? aR : aL;
+    if (k == 0) aLc = aL;
+  }
+  // compute_min with the reference's edge rules (Q5)
+  if (Ns == 1) return aLc;
+  if (ca == 0) return nmin(A[1], A[2]);
+  if (ca == Ns - 1) return nmin(A[0], aLc);
+  return nmin(A[0], nmin(A[1], A[2]));
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_detect(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ Unew,
+         uint8_t* __restrict__ flags, int* __restrict__ any_flag) {
+  constexpr int V = Dims<NDIM>::V, P = Dims<NDIM>::P;
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool cand = false;
+  if (cell < g.Nc) {
+    int c[3] = {0, 0, 0};
+    {
+      long long x = cell;
+#pragma unroll
+      for (int a = 0; a < NDIM; ++a) {
+        c[a] = (int)(x % g.Ns[a]);
+        x /= g.Ns[a];
+      }
+    }
+    double Wn[V], Wo[V], mx[V], mn[V];
+    load_W<NDIM>(Unew, g, c, Wn, prm.gamma);
+    load_W<NDIM>(Ucv, g, c, Wo, prm.gamma);
+#pragma unroll
+    for (int v = 0; v < V; ++v) mx[v] = mn[v] = Wo[v];
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a)
+#pragma unroll
+      for (int s = -1; s <= 1; s += 2) {
+        int cc[3] = {c[0], c[1], c[2]};
+        cc[a] += s;
+        double Wb[V];
+        load_W<NDIM>(Ucv, g, cc, Wb, prm.gamma);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          mx[v] = nmax(mx[v], Wb[v]);
+          mn[v] = nmin(mn[v], Wb[v]);
+        }
+      }
+    bool pt[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double lo = mn[v], hi = mx[v];
+      if (prm.p > 0) {
+        if (prm.nad_delta) {
+          const double eps = prm.tol * (hi - lo);
+          lo = lo - eps;
+          hi = hi + eps;
+        } else {
+          lo = lo - fabs(lo) * prm.tol;
+          hi = hi + fabs(hi) * prm.tol;
+        }
+      }
+      pt[v] = !(Wn[v] >= lo) || !(Wn[v] <= hi);
+      cand = cand || pt[v];
+    }
+    if (prm.p > 1 && prm.sed) {
+      const int vars[2] = {0, P};
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int var = vars[k];
+        if (!pt[var]) continue;
+        double alpha = 1.0;
+#pragma unroll
+        for (int a = 0; a < NDIM; ++a) {
+          const double an = sed_alpha<NDIM>(Unew, g, c, a, var, prm.gamma);
+          alpha = (an < alpha) ? an : alpha;
+        }
+        pt[var] = pt[var] && (alpha < 1.0);
+      }
+    }
+    bool tr = pt[0] || pt[P];
+    if (prm.pad) {
+      if (!(Wn[0] >= prm.min_rho) || !(Wn[0] <= prm.max_rho) || !(Wn[P] >= prm.min_P)) tr = true;
+    }
+    flags[cell] = tr ? 1 : 0;
+  }
+  if (__any(cand) && (threadIdx.x % warpSize) == 0) atomicOr(any_flag, 1);
+}
+
+// Trouble value on the ghosted grid after the flag fill that refreshes only
+// periodic sides (src/trouble_detection.py:73-76, src/fv_simulator.py:224-273,
+// Q2): a ghost of a non-periodic side keeps the density of the last full
+// primitive fill (the new state if the smooth-extrema branch ran, else the old).
+template <int NDIM>
+__device__ __forceinline__ double trouble_at(const Geom& g, const uint8_t* __restrict__ flags,
+                                             const double* __restrict__ Ulast, const int* cg) {
+  int m[3] = {cg[0], cg[1], cg[2]};
+#pragma unroll
+  for (int a = NDIM - 1; a >= 0; --a) {
+    const int Ns = g.Ns[a];
+    if (m[a] < 0 || m[a] >= Ns) {
+      const int side = (m[a] < 0) ? 0 : 1;
+      if (g.bc[a][side] != BC_PERIODIC) {
+        m[a] = (side == 0) ? m[a] + NGH : m[a] - NGH;
+#pragma unroll
+        for (int b = a - 1; b >= 0; --b) {
+          bool fl = false;
+          m[b] = ghost_map(m[b], g.Ns[b], g.bc[b], &fl);
+        }
+        return Ulast[cell_index<NDIM>(g, m)];
+      }
+      m[a] = (side == 0) ? m[a] + Ns : m[a] - Ns;
+    }
+  }
+  return (double)flags[cell_index<NDIM>(g, m)];
+}
+
+// Blending stencil: (target slice per dim: 0 full, 1 = [:-1], 2 = [1:]),
+// source offset per dim, weight (src/trouble_detection.py:139-169, Q6).
+struct BlendRule {
+  signed char tgt[3];
+  signed char off[3];
+  double w;
+};
+// 2-D diagonals, dims ordered (x, y)
+__constant__ BlendRule kRules2[4] = {
+    {{1, 1, 0}, {+1, +1, 0}, 0.5},
+    {{2, 1, 0}, {-1, +1, 0}, 0.5},
+    {{1, 2, 0}, {+1, -1, 0}, 0.5},
+    {{2, 2, 0}, {-1, -1, 0}, 0.5},
+};
+// 3-D: per plane (y,x), (z,y), (x,z) the four updates of the reference's loop
+// (two diagonals, two axis neighbours), then the four corners with dz = +1.
+__constant__ BlendRule kRules3[16] = {
+    // plane idim=0: A=y, B=x
+    {{1, 1, 0}, {+1, +1, 0}, 0.5},
+    {{2, 1, 0}, {-1, 0, 0}, 0.5},
+    {{1, 2, 0}, {+1, -1, 0}, 0.5},
+    {{2, 2, 0}, {-1, 0, 0}, 0.5},
+    // plane idim=1: A=z, B=y
+    {{0, 1, 1}, {0, +1, +1}, 0.5},
+    {{0, 2, 1}, {0, -1, 0}, 0.5},
+    {{0, 1, 2}, {0, +1, -1}, 0.5},
+    {{0, 2, 2}, {0, -1, 0}, 0.5},
+    // plane idim=2: A=x, B=z
+    {{1, 0, 1}, {+1, 0, +1}, 0.5},
+    {{1, 0, 2}, {0, 0, -1}, 0.5},
+    {{2, 0, 1}, {-1, 0, +1}, 0.5},
+    {{2, 0, 2}, {0, 0, -1}, 0.5},
+    // corners
+    {{1, 1, 1}, {+1, +1, +1}, 0.375},
+    {{2, 1, 1}, {-1, +1, +1}, 0.375},
+    {{1, 2, 1}, {+1, -1, +1}, 0.375},
+    {{2, 2, 1}, {-1, -1, +1}, 0.375},
+};
+
+__device__ __forceinline__ bool in_slice(int gx, int Ng, int kind) {
+  return kind == 0 || (kind == 1 && gx <= Ng - 2) || (kind == 2 && gx >= 1);
+}
+
+// theta before the last ring: trouble maximum-blended with its neighbours;
+// without blending, theta = trouble (and k_ring is skipped).
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_theta1(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* __restrict__ Ucv,
+         const double* __restrict__ Unew, const int* __restrict__ any_flag,
+         double* __restrict__ theta1) {
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= g.Ngc) return;
+  int gx[3] = {0, 0, 0};
+  {
+    long long x = tid;
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+      gx[a] = (int)(x % g.Ng[a]);
+      x /= g.Ng[a];
+    }
+  }
+  const bool gate = (*any_flag != 0) && prm.p > 1 && prm.sed;
+  const double* Ulast = gate ? Unew : Ucv;  // density is variable 0 of both sets
+  int c[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) c[a] = gx[a] - NGH;
+  double th = trouble_at<NDIM>(g, flags, Ulast, c);
+  if (prm.blending) {
+    // first neighbours
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a)
+#pragma unroll
+      for (int s = -1; s <= 1; s += 2) {
+        const int x = gx[a] + s;
+        if (x < 0 || x >= g.Ng[a]) continue;
+        int cc[3] = {c[0], c[1], c[2]};
+        cc[a] += s;
+        th = nmax(0.75 * trouble_at<NDIM>(g, flags, Ulast, cc), th);
+      }
+    if constexpr (NDIM >= 2) {
+      const BlendRule* rules = (NDIM == 2) ? kRules2 : kRules3;
+      const int nr = (NDIM == 2) ? 4 : 16;
+      for (int k = 0; k < nr; ++k) {
+        const BlendRule R = rules[k];
+        bool ok = true;
+#pragma unroll
+        for (int a = 0; a < NDIM; ++a) ok = ok && in_slice(gx[a], g.Ng[a], R.tgt[a]);
+        if (!ok) continue;
+        int cc[3] = {c[0], c[1], c[2]};
+#pragma unroll
+        for (int a = 0; a < NDIM; ++a) cc[a] += R.off[a];
+        th = nmax(R.w * trouble_at<NDIM>(g, flags, Ulast, cc), th);
+      }
+    }
+  }
+  theta1[tid] = th;
+}
+
+// One pass of the final 0.25 ring along a line of the ghosted grid
+// (src/trouble_detection.py:166-169): two sequential vectorised updates.
+__device__ __forceinline__ double ring1(int i, int n, double lo, double mid, double hi) {
+  const double base = (i <= n - 2) ? nmax(0.25 * (hi > 0.0 ? 1.0 : 0.0), mid) : mid;
+  if (i < 1) return base;
+  const double prev = nmax(0.25 * (mid > 0.0 ? 1.0 : 0.0), lo);
+  return nmax(0.25 * (prev > 0.0 ? 1.0 : 0.0), base);
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_ring(Geom g, const double* __restrict__ t1, double* __restrict__ theta) {
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= g.Ngc) return;
+  int gx[3] = {0, 0, 0};
+  {
+    long long x = tid;
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+      gx[a] = (int)(x % g.Ng[a]);
+      x /= g.Ng[a];
+    }
+  }
+  const int nx = g.Ng[0], ny = g.Ng[1], nz = g.Ng[2];
+  auto at = [&](int i, int j, int k) -> double {
+    return t1[((long long)k * ny + j) * nx + i];
+  };
+  auto rx = [&](int j, int k) -> double {
+    const int i = gx[0];
+    const double lo = (i >= 1) ? at(i - 1, j, k) : 0.0;
+    const double hi = (i <= nx - 2) ? at(i + 1, j, k) : 0.0;
+    return ring1(i, nx, lo, at(i, j, k), hi);
+  };
+  double out;
+  if constexpr (NDIM == 1) {
+    out = rx(0, 0);
+  } else if constexpr (NDIM == 2) {
+    const int j = gx[1];
+    const double lo = (j >= 1) ? rx(j - 1, 0) : 0.0;
+    const double hi = (j <= ny - 2) ? rx(j + 1, 0) : 0.0;
+    out = ring1(j, ny, lo, rx(j, 0), hi);
+  } else {
+    auto ry = [&](int k) -> double {
+      const int j = gx[1];
+      const double lo = (j >= 1) ? rx(j - 1, k) : 0.0;
+      const double hi = (j <= ny - 2) ? rx(j + 1, k) : 0.0;
+      return ring1(j, ny, lo, rx(j, k), hi);
+    };
+    const int k = gx[2];
+    const double lo = (k >= 1) ? ry(k - 1) : 0.0;
+    const double hi = (k <= nz - 2) ? ry(k + 1) : 0.0;
+    out = ring1(k, nz, lo, ry(k), hi);
+  }
+  theta[tid] = out;
+}
+
+// ---------------------------------------------------------------------------
+// MUSCL-Hancock fallback flux on the FV faces of dimension a whose blending
+// coefficient max(theta_L, theta_R) is non-zero, blended into the HO flux in
+// place (src/muscl.py:12-35,61-75,129-221, src/fv_simulator.py:88-175,
+// src/sdader_simulator.py:325-331).
+// ---------------------------------------------------------------------------
+template <int NDIM>
+__device__ __forceinline__ void hancock_state(const double* __restrict__ Ucv, const Geom& g,
+                                              const Params& prm, const int* cc, int a, double dti,
+                                              double* Mp, double* Sa) {
+  constexpr int V = Dims<NDIM>::V, P = Dims<NDIM>::P;
+  double W[V];
+  load_W<NDIM>(Ucv, g, cc, W, prm.gamma);
+  double dMh[NDIM][V];
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    int cp[3] = {cc[0], cc[1], cc[2]}, cm[3] = {cc[0], cc[1], cc[2]};
+    cp[b] += 1;
+    cm[b] -= 1;
+    double Wp[V], Wm[V];
+    load_W<NDIM>(Ucv, g, cp, Wp, prm.gamma);
+    load_W<NDIM>(Ucv, g, cm, Wm, prm.gamma);
+    const int G = cc[b] + NGH;
+    const double hl = g.hcv[b][G - 1], hr = g.hcv[b][G];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const double L = (W[v] - Wm[v]) / hl;
+      const double R = (Wp[v] - W[v]) / hr;
+      double r = R / L;
+      r = (r < 1.0) ? r : 1.0;
+      dMh[b][v] = ((r > 0.0) ? r : 0.0) * L;
+    }
+  }
+  // Hancock half step with the reference's unrolled transverse loop (Q19) and
+  // zeroed pressure row (Q1)
+  double dtW[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) dtW[v] = 0.0;
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    const int vel = b + 1;
+    const double* dW = dMh[b];
+    dtW[0] = dtW[0] - (W[vel] * dW[0] + W[0] * dW[vel]);
+    dtW[P] = dtW[P] - (W[vel] * dW[P] + prm.gamma * W[P] * dW[vel]);
+    dtW[vel] = dtW[vel] - (W[vel] * dW[vel] + dW[P] / W[0]);
+#pragma unroll
+    for (int v2 = 2; v2 <= NDIM; ++v2) dtW[v2] = dtW[v2] - W[vel] * dW[v2];
+  }
+  dtW[P] = dtW[P] * 0.0;
+  const double hf = g.hfp[a][cc[a] + NGH];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    Mp[v] = W[v] + 0.5 * dtW[v] * dti;
+    Sa[v] = 0.5 * dMh[a][v] * hf;
+  }
+}
+
+template <int NDIM, int RS>
+__global__ void __launch_bounds__(256)
+k_fbflux(Geom g, Params prm, int a, const double* __restrict__ Ucv, const double* __restrict__ theta,
+         const double* __restrict__ dt_dev, double w_i, double* __restrict__ Fho) {
+  constexpr int V = Dims<NDIM>::V;
+  long long ext[3] = {1, 1, 1};
+  long long nface = 1;
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    ext[b] = g.Ns[b] + (b == a ? 1 : 0);
+    nface *= ext[b];
+  }
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nface) return;
+  int c[3] = {0, 0, 0};
+  {
+    long long x = tid;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      c[b] = (int)(x % ext[b]);
+      x /= ext[b];
+    }
+  }
+  // theta at the two cells (ghosted indices)
+  long long gl = 0, gr = 0;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b) {
+    const int xl = c[b] + NGH - (b == a ? 1 : 0);
+    const int xr = c[b] + NGH;
+    gl = gl * g.Ng[b] + xl;
+    gr = gr * g.Ng[b] + xr;
+  }
+  double th = nmax(theta[gl], theta[gr]);
+  if (prm.godunov) th = 1.0;
+  if (th == 0.0) return;
+  const double dti = (*dt_dev) * w_i;
+  int cL[3] = {c[0], c[1], c[2]}, cR[3] = {c[0], c[1], c[2]};
+  cL[a] -= 1;
+  double MpL[V], SL[V], MpR[V], SR[V];
+  hancock_state<NDIM>(Ucv, g, prm, cL, a, dti, MpL, SL);
+  hancock_state<NDIM>(Ucv, g, prm, cR, a, dti, MpR, SR);
+  double WL[V], WR[V], UL[V], UR[V], F[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    WL[v] = MpL[v] + SL[v];
+    WR[v] = MpR[v] - SR[v];
+  }
+  cons<NDIM>(WL, UL, a, prm.gamma);
+  cons<NDIM>(WR, UR, a, prm.gamma);
+  riemann<NDIM, RS>(WL, WR, UL, UR, F, a, prm.gamma, prm.min_c2);
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    double* dst = Fho + v * nface + tid;
+    *dst = th * F[v] + (1.0 - th) * (*dst);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// End of the FV corrector: U_sp = cv_to_sp(U_cv) (src/sdader_simulator.py:300-307),
+// optional snapshot of U_cv / W_cv in element layout, and the maximum wave
+// speed of the new subcell-mean state for the next dt
+// (src/sd_simulator.py:214-227).  One thread per (element, point).
+// ---------------------------------------------------------------------------
+template <int NDIM, int P>
+__global__ void __launch_bounds__(256)
+k_cv2sp(Ops<P> op, Geom g, Params prm, const double* __restrict__ Ucv, double* __restrict__ Usp,
+        double* __restrict__ Ucv_sd, double* __restrict__ Wcv_sd, unsigned long long* cmax) {
+  constexpr int V = Dims<NDIM>::V, P_ = Dims<NDIM>::P, n = P + 1;
+  constexpr int NL = IPow<n, NDIM>::v;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double cw = 0.0;
+  if (tid < g.NE * NL) {
+    const int lo = (int)(tid % NL);
+    const long long e = tid / NL;
+    int ce[3] = {0, 0, 0};
+    elem_coords<NDIM>(g, e, ce);
+    int qo[3] = {0, 0, 0};
+    {
+      int x = lo;
+#pragma unroll
+      for (int b = 0; b < NDIM; ++b) {
+        qo[b] = x % n;
+        x /= n;
+      }
+    }
+    double acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0;
+    for (int li = 0; li < NL; ++li) {
+      int x = li;
+      int qi[3] = {0, 0, 0};
+#pragma unroll
+      for (int b = 0; b < NDIM; ++b) {
+        qi[b] = x % n;
+        x /= n;
+      }
+      double w = 1.0;
+#pragma unroll
+      for (int b = NDIM - 1; b >= 0; --b)
+        w = (b == NDIM - 1) ? op.c2s[qo[b]][qi[b]] : w * op.c2s[qo[b]][qi[b]];
+      int cc[3];
+#pragma unroll
+      for (int b = 0; b < NDIM; ++b) cc[b] = ce[b] * n + qi[b];
+      const long long ci = cell_index<NDIM>(g, cc);
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] += w * Ucv[v * g.Nc + ci];
+    }
+    int cc[3];
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) cc[b] = ce[b] * n + qo[b];
+    const long long ci = cell_index<NDIM>(g, cc);
+    double u[V], W[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      Usp[((long long)v * g.NE + e) * NL + lo] = acc[v];
+      u[v] = Ucv[v * g.Nc + ci];
+    }
+    prims<NDIM>(u, W, 0, prm.gamma);
+    if (Ucv_sd) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        Ucv_sd[((long long)v * g.NE + e) * NL + lo] = u[v];
+        Wcv_sd[((long long)v * g.NE + e) * NL + lo] = W[v];
+      }
+    }
+    cw = sound(W[P_], W[0], prm.gamma, prm.min_c2) * NDIM;
+#pragma unroll
+    for (int k = 1; k <= NDIM; ++k) cw = cw + fabs(W[k]);
+  }
+  if (cmax) {
+    // wavefront maximum, then one atomic per wavefront
+    for (int off = warpSize / 2; off > 0; off >>= 1) cw = nmax(cw, __shfl_xor(cw, off));
+    if ((threadIdx.x % warpSize) == 0) atomic_max_pos(cmax, cw);
+  }
+}
+
+// Maximum wave speed over a primitive subcell-mean array in element layout.
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_wave_max(Geom g, Params prm, int NL, const double* __restrict__ W, unsigned long long* cmax) {
+  constexpr int P = Dims<NDIM>::P;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long npts = g.NE * NL;
+  double cw = 0.0;
+  if (tid < npts) {
+    cw = sound(W[P * npts + tid], W[tid], prm.gamma, prm.min_c2) * NDIM;
+#pragma unroll
+    for (int k = 1; k <= NDIM; ++k) cw = cw + fabs(W[k * npts + tid]);
+  }
+  for (int off = warpSize / 2; off > 0; off >>= 1) cw = nmax(cw, __shfl_xor(cw, off));
+  if ((threadIdx.x % warpSize) == 0) atomic_max_pos(cmax, cw);
+}
+
+__global__ void k_dt_final(const unsigned long long* cmax, double h, double cfl, double* dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double c = __longlong_as_double((long long)*cmax);
+    dt[0] = cfl * (h / c);
+  }
+}
+
+__global__ void k_zero_counters(int* any_flag, unsigned long long* cmax, int which) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (which & 1) *any_flag = 0;
+    if (which & 2) *cmax = 0ull;
+  }
+}
+
+template <int NDIM>
+__global__ void k_prims_points(Params prm, const double* __restrict__ U, double* __restrict__ W,
+                               long long count) {
+  constexpr int V = Dims<NDIM>::V;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  double u[V], w[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) u[v] = U[v * count + i];
+  prims<NDIM>(u, w, 0, prm.gamma);
+#pragma unroll
+  for (int v = 0; v < V; ++v) W[v * count + i] = w[v];
+}
+
+template <int NDIM>
+__global__ void k_cons_points(Params prm, const double* __restrict__ W, double* __restrict__ U,
+                              long long count) {
+  constexpr int V = Dims<NDIM>::V;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  double w[V], u[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) w[v] = W[v * count + i];
+  cons<NDIM>(w, u, 0, prm.gamma);
+#pragma unroll
+  for (int v = 0; v < V; ++v) U[v * count + i] = u[v];
+}
+
+// Debug tap: flags -> doubles, theta copy, face coefficients.
+template <int NDIM>
+__global__ void k_tap(Geom g, const uint8_t* __restrict__ flags, const double* __restrict__ theta,
+                      double* tr_out, double* th_out, double* aff0, double* aff1, double* aff2) {
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tr_out && tid < g.Nc) tr_out[tid] = flags[tid];
+  if (th_out && tid < g.Ngc) th_out[tid] = theta[tid];
+  double* aff[3] = {aff0, aff1, aff2};
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    if (!aff[a]) continue;
+    long long ext[3] = {1, 1, 1}, nface = 1;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      ext[b] = g.Ns[b] + (b == a ? 1 : 0);
+      nface *= ext[b];
+    }
+    if (tid >= nface) continue;
+    int c[3] = {0, 0, 0};
+    long long x = tid;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      c[b] = (int)(x % ext[b]);
+      x /= ext[b];
+    }
+    long long gl = 0, gr = 0;
+#pragma unroll
+    for (int b = NDIM - 1; b >= 0; --b) {
+      gl = gl * g.Ng[b] + c[b] + NGH - (b == a ? 1 : 0);
+      gr = gr * g.Ng[b] + c[b] + NGH;
+    }
+    aff[a][tid] = nmax(theta[gl], theta[gr]);
+  }
+}
+
+}  // namespace spd
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+using namespace spd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define SPD_HIP_CHECK(expr)                                                          \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess)                                                            \
+      return fail(SPD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+  } while (0)
+
+inline unsigned grid_for(long long n, int bs = 256) {
+  return (unsigned)((n + bs - 1) / bs);
+}
+
+}  // namespace
+
+struct spd_ctx {
+  spd_config cfg{};
+  int ndim = 0, p = 0, n = 0, nf = 0, T = 0, V = 0, NL = 0, NLP = 0;
+  Geom g{};
+  Params prm{};
+  std::unique_ptr<host::OpsHost> ops;
+  double h_dt = 0.0;  // h_min / (p+1)
+  int device = 0;
+  std::vector<void*> allocs;
+  size_t scratch_bytes = 0;
+  double* geom_buf = nullptr;
+  double* Uad = nullptr;
+  double* Fface[3] = {nullptr, nullptr, nullptr};
+  double* Fho[3] = {nullptr, nullptr, nullptr};
+  double* Ucv = nullptr;
+  double* Unew = nullptr;
+  uint8_t* flags = nullptr;
+  double* theta1 = nullptr;
+  double* theta = nullptr;
+  int* any_flag = nullptr;
+  unsigned long long* cmax = nullptr;
+  // debug tap
+  int dbg_sub = -1;
+  double* dbg_tr = nullptr;
+  double* dbg_th = nullptr;
+  double* dbg_aff[3] = {nullptr, nullptr, nullptr};
+
+  ~spd_ctx() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+  template <typename X> int alloc(X** ptr, size_t count) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(count * sizeof(X), 256);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+      return fail(SPD_E_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
+                                   "): " + hipGetErrorString(e));
+    allocs.push_back(p);
+    scratch_bytes += bytes;
+    *ptr = (X*)p;
+    return SPD_OK;
+  }
+};
+
+namespace {
+
+template <int P> Ops<P> make_ops(const host::OpsHost& h) {
+  Ops<P> o{};
+  constexpr int n = P + 1, nf = P + 2, T = P + 1;
+  for (int a = 0; a < nf; ++a)
+    for (int b = 0; b < n; ++b) o.s2f[a][b] = h.sp_to_fp[a][b];
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < nf; ++b) o.d2s[a][b] = h.dfp_to_sp[a][b];
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) {
+      o.s2c[a][b] = h.sp_to_cv[a][b];
+      o.c2s[a][b] = h.cv_to_sp[a][b];
+    }
+  for (int a = 0; a < T; ++a) {
+    for (int b = 0; b < T; ++b) o.inv[a][b] = h.invader[a][b];
+    o.w[a] = h.w_tp[a];
+  }
+  return o;
+}
+
+// Typed launcher of one (ndim, p) instantiation.
+template <int NDIM, int P> struct Stepper {
+  static constexpr int V = NDIM + 2, n = P + 1, nf = P + 2, T = P + 1;
+  static constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
+
+  static_assert(T * NLP <= 256, "one element's lines must fit one block");
+  static int epb() {
+    int e = std::max(1, 256 / (T * NLP));
+    const int lds_per = T * V * NL * (int)sizeof(double);
+    e = std::max(1, std::min(e, 49152 / lds_per));
+    return e;
+  }
+
+  template <int RS>
+  static void faces_rs(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, hipStream_t s) {
+    for (int a = 0; a < NDIM; ++a) {
+      const long long Na = c->g.N[a];
+      const long long tot = (long long)Tsrc * (Na + 1) * (c->g.NE / Na) * NLP;
+      hipLaunchKernelGGL((k_face_sd<NDIM, P, RS>), dim3(grid_for(tot)), dim3(256), 0, s, op,
+                         c->g, c->prm, a, src, Tsrc, c->Fface[a]);
+    }
+  }
+  static void faces(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, hipStream_t s) {
+    if (c->cfg.riemann_sd == SPD_RS_HLLC)
+      faces_rs<RS_HLLC>(c, op, src, Tsrc, s);
+    else
+      faces_rs<RS_LLF>(c, op, src, Tsrc, s);
+  }
+  static void pred(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, const double* Usp,
+                   const double* dt, double* out, int mode, hipStream_t s) {
+    const int E = epb();
+    const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
+    const size_t lds = (size_t)E * T * V * NL * sizeof(double);
+    hipLaunchKernelGGL((k_pred<NDIM, P>), dim3(blocks), dim3(E * T * NLP), lds, s, op, c->g,
+                       c->prm, src, Tsrc, c->Fface[0], c->Fface[1], c->Fface[2], Usp, dt, out,
+                       mode, E);
+  }
+  template <int RS>
+  static void fb_rs(spd_ctx* c, const double* dt, double w, hipStream_t s) {
+    for (int a = 0; a < NDIM; ++a) {
+      long long nface = 1;
+      for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == a ? 1 : 0);
+      hipLaunchKernelGGL((k_fbflux<NDIM, RS>), dim3(grid_for(nface)), dim3(256), 0, s, c->g,
+                         c->prm, a, c->Ucv, c->theta, dt, w, c->Fho[a]);
+    }
+  }
+
+  static int step(spd_ctx* c, double* Usp, double* Ucv_sd, double* Wcv_sd, const double* dt,
+                  int snapshot, hipStream_t s) {
+    const Ops<P> op = make_ops<P>(*c->ops);
+    const int m = c->cfg.m;
+    // --- ADER predictor (src/sdader_simulator.py:160-200)
+    faces(c, op, Usp, 1, s);
+    pred(c, op, Usp, 1, Usp, dt, c->Uad, 0, s);
+    for (int it = 1; it < m; ++it) {
+      faces(c, op, c->Uad, T, s);
+      pred(c, op, c->Uad, T, Usp, dt, c->Uad, 0, s);
+    }
+    faces(c, op, c->Uad, T, s);
+    if (c->cfg.update == SPD_UPDATE_SD) {
+      pred(c, op, c->Uad, T, Usp, dt, Usp, 1, s);
+      return hipGetLastError() == hipSuccess ? SPD_OK : fail(SPD_E_HIP, "kernel launch");
+    }
+    // --- FV corrector (src/sdader_simulator.py:288-349)
+    hipLaunchKernelGGL((k_tensor<NDIM, P>), dim3(grid_for(c->g.NE * V * NL)), dim3(256), 0, s, op,
+                       c->g, 1, Usp, c->Ucv, 1);
+    for (int i = 0; i < T; ++i) {
+      const double w = c->ops->w_tp[i];
+      for (int a = 0; a < NDIM; ++a)
+        hipLaunchKernelGGL((k_hoflux<NDIM, P>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s, op,
+                           c->g, c->prm, a, i, c->Uad, c->Fface[a], c->Fho[a]);
+      if (c->cfg.FB) {
+        hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
+                           c->Ucv, c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Unew);
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag, c->cmax, 1);
+        hipLaunchKernelGGL((k_detect<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
+                           c->prm, c->Ucv, c->Unew, c->flags, c->any_flag);
+        double* th_out = c->prm.blending ? c->theta1 : c->theta;
+        hipLaunchKernelGGL((k_theta1<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
+                           c->prm, c->flags, c->Ucv, c->Unew, c->any_flag, th_out);
+        if (c->prm.blending)
+          hipLaunchKernelGGL((k_ring<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
+                             c->theta1, c->theta);
+        if (i == c->dbg_sub || c->dbg_sub == SPD_TAP_ALL) {
+          // with SPD_TAP_ALL the buffers hold T consecutive blocks
+          const long long k = (c->dbg_sub == SPD_TAP_ALL) ? i : 0;
+          double* aff_o[3] = {nullptr, nullptr, nullptr};
+          for (int a = 0; a < NDIM; ++a) {
+            long long nface = 1;
+            for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == a ? 1 : 0);
+            if (c->dbg_aff[a]) aff_o[a] = c->dbg_aff[a] + k * nface;
+          }
+          long long mx = std::max(c->g.Nc, c->g.Ngc);
+          for (int a = 0; a < NDIM; ++a) {
+            long long nface = 1;
+            for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == a ? 1 : 0);
+            mx = std::max(mx, nface);
+          }
+          hipLaunchKernelGGL((k_tap<NDIM>), dim3(grid_for(mx)), dim3(256), 0, s, c->g, c->flags,
+                             c->theta, c->dbg_tr ? c->dbg_tr + k * c->g.Nc : nullptr,
+                             c->dbg_th ? c->dbg_th + k * c->g.Ngc : nullptr, aff_o[0], aff_o[1],
+                             aff_o[2]);
+        }
+        if (c->cfg.riemann_fv == SPD_RS_HLLC)
+          fb_rs<RS_HLLC>(c, dt, w, s);
+        else
+          fb_rs<RS_LLF>(c, dt, w, s);
+      }
+      hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g, c->Ucv,
+                         c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Ucv);
+    }
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag, c->cmax, 2);
+    hipLaunchKernelGGL((k_cv2sp<NDIM, P>), dim3(grid_for(c->g.NE * NL)), dim3(256), 0, s, op,
+                       c->g, c->prm, c->Ucv, Usp, snapshot ? Ucv_sd : nullptr,
+                       snapshot ? Wcv_sd : nullptr, c->cmax);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SPD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return SPD_OK;
+  }
+
+  static int tensor(spd_ctx* c, int to_cv, const double* in, double* out, hipStream_t s) {
+    const Ops<P> op = make_ops<P>(*c->ops);
+    hipLaunchKernelGGL((k_tensor<NDIM, P>), dim3(grid_for(c->g.NE * V * NL)), dim3(256), 0, s, op,
+                       c->g, to_cv, in, out, 0);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+  }
+};
+
+using StepFn = int (*)(spd_ctx*, double*, double*, double*, const double*, int, hipStream_t);
+using TensorFn = int (*)(spd_ctx*, int, const double*, double*, hipStream_t);
+
+struct Entry {
+  int ndim, p;
+  StepFn step;
+  TensorFn tensor;
+};
+
+#define SPD_ENTRY(D, P) {D, P, &Stepper<D, P>::step, &Stepper<D, P>::tensor}
+const Entry kEntries[] = {
+    SPD_ENTRY(1, 1), SPD_ENTRY(1, 2), SPD_ENTRY(1, 3), SPD_ENTRY(1, 4), SPD_ENTRY(1, 5),
+    SPD_ENTRY(2, 1), SPD_ENTRY(2, 2), SPD_ENTRY(2, 3), SPD_ENTRY(2, 4), SPD_ENTRY(2, 5),
+    SPD_ENTRY(3, 1), SPD_ENTRY(3, 2), SPD_ENTRY(3, 3),
+};
+#undef SPD_ENTRY
+
+const Entry* find_entry(int ndim, int p) {
+  for (const Entry& e : kEntries)
+    if (e.ndim == ndim && e.p == p) return &e;
+  return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spd_last_error(void) { return g_last_error.c_str(); }
+
+int spd_abi_version(void) { return SPD_ABI_VERSION; }
+
+size_t spd_config_size(void) { return sizeof(spd_config); }
+
+int spd_host_operators(int p, int m, double* out, int len) {
+  if (!out || p < 0 || m < 0) return fail(SPD_E_INVALID, "invalid argument");
+  try {
+    host::OpsHost o(p, m);
+    std::vector<double> v;
+    auto put = [&](const host::Mat& M) {
+      for (auto& r : M) v.insert(v.end(), r.begin(), r.end());
+    };
+    put(o.sp_to_fp);
+    put(o.dfp_to_sp);
+    put(o.sp_to_cv);
+    put(o.cv_to_sp);
+    put(o.invader);
+    v.insert(v.end(), o.w_tp.begin(), o.w_tp.end());
+    v.insert(v.end(), o.x_tp.begin(), o.x_tp.end());
+    if ((int)v.size() > len) return fail(SPD_E_INVALID, "buffer too small");
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return (int)v.size();
+  } catch (const std::exception& ex) {
+    return fail(SPD_E_INVALID, ex.what());
+  }
+}
+
+int spd_host_geometry(int N, double h, int p, double* out, int len) {
+  if (!out || N < 1 || p < 0) return fail(SPD_E_INVALID, "invalid argument");
+  host::GeomHost gh(N, h, host::flux_points(p), NGH);
+  std::vector<double> v;
+  v.insert(v.end(), gh.faces.begin(), gh.faces.end());
+  v.insert(v.end(), gh.centers.begin(), gh.centers.end());
+  v.insert(v.end(), gh.hfp.begin(), gh.hfp.end());
+  v.insert(v.end(), gh.hcv.begin(), gh.hcv.end());
+  if ((int)v.size() > len) return fail(SPD_E_INVALID, "buffer too small");
+  std::memcpy(out, v.data(), v.size() * sizeof(double));
+  return (int)v.size();
+}
+
+int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
+  if (!cfg || !out) return fail(SPD_E_INVALID, "null argument");
+  *out = nullptr;
+  const int nd = cfg->ndim;
+  if (nd < 1 || nd > 3) return fail(SPD_E_INVALID, "ndim must be 1, 2 or 3");
+  if (!find_entry(nd, cfg->p))
+    return fail(SPD_E_UNSUPPORTED, "no kernels built for ndim=" + std::to_string(nd) +
+                                       " p=" + std::to_string(cfg->p));
+  if (cfg->m != cfg->p) return fail(SPD_E_UNSUPPORTED, "only m == p is built");
+  if (cfg->update != SPD_UPDATE_SD && cfg->update != SPD_UPDATE_FV)
+    return fail(SPD_E_INVALID, "update must be SD or FV");
+  if ((cfg->riemann_sd != SPD_RS_LLF && cfg->riemann_sd != SPD_RS_HLLC) ||
+      (cfg->riemann_fv != SPD_RS_LLF && cfg->riemann_fv != SPD_RS_HLLC))
+    return fail(SPD_E_UNSUPPORTED, "riemann solver must be llf or hllc");
+  for (int a = 0; a < nd; ++a) {
+    if (cfg->N[a] < 1) return fail(SPD_E_INVALID, "N must be >= 1");
+    if (cfg->N[a] * (cfg->p + 1) < NGH)
+      return fail(SPD_E_INVALID, "fewer subcells than ghost layers");
+    for (int s = 0; s < 2; ++s)
+      if (cfg->bc[a][s] < SPD_BC_PERIODIC || cfg->bc[a][s] > SPD_BC_REFLECTIVE)
+        return fail(SPD_E_UNSUPPORTED, "boundary kind");
+    if (!(cfg->lim[a][1] > cfg->lim[a][0])) return fail(SPD_E_INVALID, "empty domain");
+  }
+  std::unique_ptr<spd_ctx> c(new spd_ctx());
+  c->cfg = *cfg;
+  SPD_HIP_CHECK(hipGetDevice(&c->device));
+  c->ndim = nd;
+  c->p = cfg->p;
+  c->n = cfg->p + 1;
+  c->nf = cfg->p + 2;
+  c->T = cfg->m + 1;
+  c->V = nd + 2;
+  c->NL = 1;
+  for (int a = 0; a < nd; ++a) c->NL *= c->n;
+  c->NLP = c->NL / c->n;
+  try {
+    c->ops.reset(new host::OpsHost(cfg->p, cfg->m));
+  } catch (const std::exception& ex) {
+    return fail(SPD_E_INVALID, ex.what());
+  }
+  Geom& g = c->g;
+  g.NE = 1;
+  g.Nc = 1;
+  g.Ngc = 1;
+  double hmin = 1e10;
+  for (int a = 0; a < 3; ++a) {
+    const bool on = a < nd;
+    g.N[a] = on ? cfg->N[a] : 1;
+    g.Ns[a] = on ? cfg->N[a] * c->n : 1;
+    g.Ng[a] = on ? g.Ns[a] + 2 * NGH : 1;
+    g.bc[a][0] = on ? cfg->bc[a][0] : BC_PERIODIC;
+    g.bc[a][1] = on ? cfg->bc[a][1] : BC_PERIODIC;
+    const double len = cfg->lim[a][1] - cfg->lim[a][0];
+    g.h[a] = on ? len / cfg->N[a] : 1.0;
+    if (on) {
+      hmin = std::min(hmin, g.h[a]);
+      g.NE *= g.N[a];
+      g.Nc *= g.Ns[a];
+      g.Ngc *= g.Ng[a];
+    }
+  }
+  c->h_dt = hmin / (cfg->p + 1);
+  Params& pr = c->prm;
+  pr.gamma = cfg->gamma;
+  pr.min_c2 = cfg->min_c2;
+  pr.tol = cfg->tolerance;
+  pr.min_rho = cfg->min_rho;
+  pr.max_rho = cfg->max_rho;
+  pr.min_P = cfg->min_P;
+  pr.p = cfg->p;
+  pr.nad_delta = cfg->NAD == SPD_NAD_DELTA;
+  pr.sed = cfg->SED;
+  pr.pad = cfg->PAD;
+  pr.blending = cfg->blending;
+  pr.godunov = cfg->godunov;
+
+  // geometry arrays
+  std::vector<double> gbuf;
+  std::vector<size_t> offs[3];
+  for (int a = 0; a < nd; ++a) {
+    host::GeomHost gh(g.N[a], g.h[a], c->ops->fp, NGH);
+    offs[a] = {gbuf.size()};
+    gbuf.insert(gbuf.end(), gh.faces.begin(), gh.faces.end());
+    offs[a].push_back(gbuf.size());
+    gbuf.insert(gbuf.end(), gh.centers.begin(), gh.centers.end());
+    offs[a].push_back(gbuf.size());
+    gbuf.insert(gbuf.end(), gh.hfp.begin(), gh.hfp.end());
+    offs[a].push_back(gbuf.size());
+    gbuf.insert(gbuf.end(), gh.hcv.begin(), gh.hcv.end());
+  }
+  int rc;
+  if ((rc = c->alloc(&c->geom_buf, gbuf.size())) != SPD_OK) return rc;
+  SPD_HIP_CHECK(hipMemcpy(c->geom_buf, gbuf.data(), gbuf.size() * sizeof(double),
+                          hipMemcpyHostToDevice));
+  for (int a = 0; a < 3; ++a) {
+    if (a < nd) {
+      g.faces[a] = c->geom_buf + offs[a][0];
+      g.centers[a] = c->geom_buf + offs[a][1];
+      g.hfp[a] = c->geom_buf + offs[a][2];
+      g.hcv[a] = c->geom_buf + offs[a][3];
+    } else {
+      g.faces[a] = g.centers[a] = g.hfp[a] = g.hcv[a] = c->geom_buf;
+    }
+  }
+  // scratch
+  const long long V = c->V, T = c->T, NL = c->NL, NLP = c->NLP;
+  if ((rc = c->alloc(&c->Uad, (size_t)(T * V * g.NE * NL))) != SPD_OK) return rc;
+  for (int a = 0; a < nd; ++a) {
+    const long long Na = g.N[a];
+    if ((rc = c->alloc(&c->Fface[a], (size_t)(T * V * (Na + 1) * (g.NE / Na) * NLP))) != SPD_OK)
+      return rc;
+  }
+  if (cfg->update == SPD_UPDATE_FV) {
+    for (int a = 0; a < nd; ++a) {
+      long long nface = 1;
+      for (int b = 0; b < nd; ++b) nface *= g.Ns[b] + (b == a ? 1 : 0);
+      if ((rc = c->alloc(&c->Fho[a], (size_t)(V * nface))) != SPD_OK) return rc;
+    }
+    if ((rc = c->alloc(&c->Ucv, (size_t)(V * g.Nc))) != SPD_OK) return rc;
+    if (cfg->FB) {
+      if ((rc = c->alloc(&c->Unew, (size_t)(V * g.Nc))) != SPD_OK) return rc;
+      if ((rc = c->alloc(&c->flags, (size_t)g.Nc)) != SPD_OK) return rc;
+      if ((rc = c->alloc(&c->theta1, (size_t)g.Ngc)) != SPD_OK) return rc;
+      if ((rc = c->alloc(&c->theta, (size_t)g.Ngc)) != SPD_OK) return rc;
+    }
+  }
+  if ((rc = c->alloc(&c->any_flag, 1)) != SPD_OK) return rc;
+  if ((rc = c->alloc(&c->cmax, 1)) != SPD_OK) return rc;
+  SPD_HIP_CHECK(hipMemset(c->any_flag, 0, sizeof(int)));
+  SPD_HIP_CHECK(hipMemset(c->cmax, 0, sizeof(unsigned long long)));
+  for (int a = 0; a < 3; ++a) {
+    if (!c->Fface[a]) c->Fface[a] = c->Uad;  // unused dims: any valid pointer
+    if (!c->Fho[a]) c->Fho[a] = c->Uad;
+  }
+  *out = c.release();
+  return SPD_OK;
+}
+
+int spd_ctx_destroy(spd_ctx* ctx) {
+  delete ctx;
+  return SPD_OK;
+}
+
+size_t spd_ctx_scratch_bytes(const spd_ctx* ctx) { return ctx ? ctx->scratch_bytes : 0; }
+
+int spd_get_operators(const spd_ctx* ctx, double* out, int len) {
+  if (!ctx || !out) return fail(SPD_E_INVALID, "null argument");
+  return spd_host_operators(ctx->cfg.p, ctx->cfg.m, out, len);
+}
+
+int spd_compute_dt(spd_ctx* ctx, const double* W_cv, double* dt_dev, void* stream) {
+  if (!ctx || !dt_dev) return fail(SPD_E_INVALID, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (W_cv) {
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, ctx->any_flag, ctx->cmax, 2);
+    const long long npts = ctx->g.NE * ctx->NL;
+    switch (ctx->ndim) {
+      case 1:
+        hipLaunchKernelGGL((k_wave_max<1>), dim3(grid_for(npts)), dim3(256), 0, s, ctx->g, ctx->prm,
+                           ctx->NL, W_cv, ctx->cmax);
+        break;
+      case 2:
+        hipLaunchKernelGGL((k_wave_max<2>), dim3(grid_for(npts)), dim3(256), 0, s, ctx->g, ctx->prm,
+                           ctx->NL, W_cv, ctx->cmax);
+        break;
+      default:
+        hipLaunchKernelGGL((k_wave_max<3>), dim3(grid_for(npts)), dim3(256), 0, s, ctx->g, ctx->prm,
+                           ctx->NL, W_cv, ctx->cmax);
+    }
+  }
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, s, ctx->cmax, ctx->h_dt, ctx->cfg.cfl_coeff,
+                     dt_dev);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+}
+
+int spd_step(spd_ctx* ctx, double* U_sp, double* U_cv, double* W_cv, const double* dt_dev,
+             int snapshot, void* stream) {
+  if (!ctx || !U_sp || !dt_dev) return fail(SPD_E_INVALID, "null argument");
+  if (snapshot && ctx->cfg.update == SPD_UPDATE_FV && (!U_cv || !W_cv))
+    return fail(SPD_E_INVALID, "snapshot needs U_cv and W_cv");
+  const Entry* e = find_entry(ctx->ndim, ctx->p);
+  return e->step(ctx, U_sp, U_cv, W_cv, dt_dev, snapshot, (hipStream_t)stream);
+}
+
+int spd_sp_to_cv(spd_ctx* ctx, const double* in, double* out, void* stream) {
+  if (!ctx || !in || !out) return fail(SPD_E_INVALID, "null argument");
+  return find_entry(ctx->ndim, ctx->p)->tensor(ctx, 1, in, out, (hipStream_t)stream);
+}
+
+int spd_cv_to_sp(spd_ctx* ctx, const double* in, double* out, void* stream) {
+  if (!ctx || !in || !out) return fail(SPD_E_INVALID, "null argument");
+  return find_entry(ctx->ndim, ctx->p)->tensor(ctx, 0, in, out, (hipStream_t)stream);
+}
+
+int spd_cons_to_prims(spd_ctx* ctx, const double* U, double* W, long long count, void* stream) {
+  if (!ctx || !U || !W || count < 0) return fail(SPD_E_INVALID, "invalid argument");
+  if (count == 0) return SPD_OK;
+  hipStream_t s = (hipStream_t)stream;
+  switch (ctx->ndim) {
+    case 1:
+      hipLaunchKernelGGL((k_prims_points<1>), dim3(grid_for(count)), dim3(256), 0, s, ctx->prm, U,
+                         W, count);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_prims_points<2>), dim3(grid_for(count)), dim3(256), 0, s, ctx->prm, U,
+                         W, count);
+      break;
+    default:
+      hipLaunchKernelGGL((k_prims_points<3>), dim3(grid_for(count)), dim3(256), 0, s, ctx->prm, U,
+                         W, count);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+}
+
+int spd_prims_to_cons(spd_ctx* ctx, const double* W, double* U, long long count, void* stream) {
+  if (!ctx || !U || !W || count < 0) return fail(SPD_E_INVALID, "invalid argument");
+  if (count == 0) return SPD_OK;
+  hipStream_t s = (hipStream_t)stream;
+  switch (ctx->ndim) {
+    case 1:
+      hipLaunchKernelGGL((k_cons_points<1>), dim3(grid_for(count)), dim3(256), 0, s, ctx->prm, W,
+                         U, count);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_cons_points<2>), dim3(grid_for(count)), dim3(256), 0, s, ctx->prm, W,
+                         U, count);
+      break;
+    default:
+      hipLaunchKernelGGL((k_cons_points<3>), dim3(grid_for(count)), dim3(256), 0, s, ctx->prm, W,
+                         U, count);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+}
+
+int spd_set_debug_tap(spd_ctx* ctx, int substep, double* troubles, double* theta, double* aff_x,
+                      double* aff_y, double* aff_z) {
+  if (!ctx) return fail(SPD_E_INVALID, "null argument");
+  if (substep != -1 && !(ctx->cfg.update == SPD_UPDATE_FV && ctx->cfg.FB))
+    return fail(SPD_E_INVALID, "the debug tap needs update=FV and FB");
+  if (substep >= ctx->T || substep < SPD_TAP_ALL) return fail(SPD_E_INVALID, "substep out of range");
+  ctx->dbg_sub = substep;
+  ctx->dbg_tr = troubles;
+  ctx->dbg_th = theta;
+  ctx->dbg_aff[0] = aff_x;
+  ctx->dbg_aff[1] = aff_y;
+  ctx->dbg_aff[2] = aff_z;
+  return SPD_OK;
+}
+
+}  // extern "C"
