@@ -4,8 +4,9 @@ reference's golden vectors and with the CPU oracle run in lock-step.
 Bar (FP64): every state array within 1e-12 norm-wise relative difference per
 field (momenta share one scale), dt within 1e-12 relative, troubled-cell flags
 identical to the oracle's except where the oracle's decision lies within 1e-10
-(relative) of its threshold; theta and the face coefficients identical when the
-flags are.
+(relative) of its threshold; theta and the face coefficients within 1e-12 when
+the flags are identical (exact 0/0.25/.../1 values inside the domain; ghost
+densities at outflow and reflective sides).
 """
 import numpy as np
 import pytest
@@ -67,10 +68,12 @@ def test_steps_vs_golden_and_oracle(name):
                 assert bad == 0, (k, i, bad)
                 if ndiff:
                     continue
-                assert relerr(rg["theta"], ro["theta"]) == 0.0, (k, i)
+                # theta holds densities in outflow/reflective ghosts (the
+                # reference's stale ghost fill), so it carries state round-off
+                assert relerr(rg["theta"], ro["theta"]) < TOL, (k, i)
                 for a in range(s.ndim):
                     d = "xyz"[a]
-                    assert relerr(rg[f"aff_{d}"], ro[f"aff_{d}"]) == 0.0, (k, i, d)
+                    assert relerr(rg[f"aff_{d}"], ro[f"aff_{d}"]) < TOL, (k, i, d)
 
 
 @pytest.mark.parametrize("name", [n for n in NAMES if "final_U_sp" in load(n)])
