@@ -1,0 +1,253 @@
+"""Benchmark of the SD-ADER step on one MI355X.
+
+Workload (the metric's configuration): 2-D Kelvin-Helmholtz on [0,1]^2,
+periodic, 1024^2 elements, p = 3 (16.8 M solution points), FP64, update="FV"
+with the a-posteriori fallback (NAD + SED + PAD, blending), HLLC on element and
+fallback faces, minmod, CFL 0.8; rho = 2, vx = 0.5 for |y-0.5| < 0.25 else
+rho = 1, vx = -0.5, P = 2.5, vy = 0.01 sin(4 pi x), plus seeded uniform noise
+(amplitude 1e-3, numpy default_rng(0)) on the vx, vy subcell means.
+
+A step is one pass of the hot path over the whole state: compute_dt on the
+device (its scalar read back to the host, as in the reference's loop) and
+one spd_step (predictor + FV/FB corrector).  The state is resident in HBM
+before the timed region.  Output: one JSON line with DOF-updates/s (DOF =
+solution points; one DOF-update = one point advanced one full step), the
+roofline of the dominant kernel (HIP-event durations measured over the timed
+region on the kernel's own stream) and the CPU baseline (the NumPy oracle on a
+bounded sample, one host core).
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBPS = 8000.0          # MI355X HBM3E, AMD published peak
+FP64_PEAK_TFLOPS = 78.6         # MI355X FP64 vector / matrix dense peak
+
+
+def kh_kwargs(N, p):
+    return dict(p=p, N=(N, N), BC=(("periodic", "periodic"), ("periodic", "periodic")),
+                update="FV", FB=True, riemann_solver_sd="hllc", riemann_solver_fv="hllc",
+                slope_limiter="minmod", cfl_coeff=0.8)
+
+
+def algorithmic_bytes(N, p, launches):
+    """Compulsory HBM bytes per launch of each kernel class for one step of the
+    2-D FV+FB path (each array element read or written once, FP64; flags 1 B).
+
+    Per step (V = 4, T = n = p+1, E = N^2 elements, C = E n^2 cells,
+    Fx = Fy = V (N n + 1) N n face values, G = (N n + 4)^2 ghosted cells):
+      k_face_sd  m+1 launches per dim: read the predictor state (T V C, first
+                 launch V C), write T V (N+1) N n element-face fluxes
+      k_pred     m launches: read state (T V C, first V C) + U_sp (V C) +
+                 both element-face arrays, write T V C
+      k_hoflux   per substep and dim: read V C of the state slice + V (N+1) N n,
+                 write V (N n + 1) N n
+      k_tensor   once: read + write V C
+      k_apply    per substep, twice: read V C + Fx + Fy, write V C
+      k_detect   per substep: read 2 V C, write C flags
+      k_theta1   per substep: read C flags, write G
+      k_ring     per substep: read + write G
+      k_fbflux   per substep and dim: read G theta + V C state, read+write the face array
+      k_cv2sp    once: read V C, write V C
+    """
+    V, n = 4, p + 1
+    T = n
+    m = p
+    E = N * N
+    C = E * n * n
+    Fd = V * (N * n + 1) * N * n
+    Fe = T * V * (N + 1) * N * n
+    G = (N * n + 4) ** 2
+    per_step = {
+        "k_face_sd": 2 * ((V * C + Fe // T) + m * (T * V * C + Fe)),
+        "k_pred": (V * C + V * C + 2 * Fe // T + T * V * C) + (m - 1) * (T * V * C + V * C + 2 * Fe + T * V * C),
+        "k_hoflux": T * 2 * (V * C + V * (N + 1) * N * n + Fd),
+        "k_tensor": 2 * V * C,
+        "k_apply": T * 2 * (2 * V * C + 2 * Fd),
+        "k_detect": T * (2 * V * C),
+        "k_theta1": T * G,
+        "k_ring": T * 2 * G,
+        "k_fbflux": T * 2 * (G + V * C + 2 * Fd),
+        "k_cv2sp": 2 * V * C,
+    }
+    extra_bytes = {"k_detect": T * C * 1.0, "k_theta1": T * C * 1.0}
+    out = {}
+    steps = launches["_steps"]
+    for k, val in launches.items():
+        if k not in per_step:
+            continue
+        cnt = val[1]
+        out[k] = (per_step[k] * 8.0 + extra_bytes.get(k, 0.0)) * steps / cnt
+    return out
+
+
+def copy_bandwidth(torch, nbytes=2 << 30, reps=10):
+    """Measured device-to-device copy bandwidth (read + write bytes / time)."""
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    del a, b
+    return 2.0 * nbytes * reps / dt / 1e9
+
+
+def cpu_baseline_child(N, steps, warmup):
+    """Time the NumPy oracle on one core (run in a child with 1 thread)."""
+    import numpy as np  # noqa: F401
+    from spd_amd import initial_conditions as ic
+    from oracle.spd_oracle import OracleSimulator
+    p = 3
+    noise = ic.subcell_noise((N, N), p, [1, 2], 1e-3, seed=0)
+    o = OracleSimulator(ic.kelvin_helmholtz, noise=noise, **kh_kwargs(N, p))
+    for _ in range(warmup):
+        o.compute_dt()
+        o.perform_update()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.compute_dt()
+        o.perform_update()
+    dt = time.perf_counter() - t0
+    dof = (N * (p + 1)) ** 2
+    print(json.dumps({"value": dof * steps / dt, "seconds": dt, "steps": steps}))
+
+
+def run_cpu_baseline(N=256, steps=1, warmup=1):
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
+               CUDA_VISIBLE_DEVICES="", HIP_VISIBLE_DEVICES="", ROCR_VISIBLE_DEVICES="")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
+           "--cpu-N", str(N), "--cpu-steps", str(steps), "--cpu-warmup", str(warmup)]
+
+    def pin():
+        try:
+            os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        except Exception:
+            pass
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, preexec_fn=pin, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + r.stderr[-2000:])
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    cpu = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": res["value"], "unit": "DOF-updates/s", "cores": 1, "kind": "port",
+            "sample": (f"NumPy oracle (FP64 restatement of the reference), KH {N}^2 p=3 FV+FB, "
+                       f"{warmup} warm-up + {steps} timed step(s) = {res['seconds']:.1f} s on one "
+                       f"core of {os.cpu_count()} ({cpu})")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--N", type=int, default=1024)
+    ap.add_argument("--p", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-child", action="store_true")
+    ap.add_argument("--cpu-N", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-warmup", type=int, default=1)
+    args = ap.parse_args()
+
+    if args.cpu_baseline_child:
+        cpu_baseline_child(args.cpu_N, args.cpu_steps, args.cpu_warmup)
+        return
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} "
+              f"(WORLD_SIZE={world}) is not supported (multi-GPU halo exchange is design-only).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np  # noqa: F401
+    import torch
+    from spd_amd import initial_conditions as ic
+    from spd_amd.simulator import SDADER_Simulator
+
+    N, p = args.N, args.p
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    s = SDADER_Simulator(init_fct=ic.kelvin_helmholtz, device=dev, verbose=False,
+                         **kh_kwargs(N, p))
+    s.perturb_initial_condition(ic.subcell_noise((N, N), p, [1, 2], 1e-3, seed=0))
+    dof = (N * (p + 1)) ** 2
+
+    s.advance(args.warmup)
+    torch.cuda.synchronize()
+    s.profile_begin(max_launches=200 * max(args.steps, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.advance(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = s.profile_end()
+    frac = s.troubled_fraction_next_step()
+
+    ms_step = elapsed / args.steps * 1e3
+    value = dof * args.steps / elapsed
+    launches = dict(prof)
+    launches["_steps"] = args.steps
+    bpl = algorithmic_bytes(N, p, launches)
+    dom = max(prof, key=lambda k: prof[k][0])
+    dom_ms, dom_cnt = prof[dom]
+    avg_ms = dom_ms / dom_cnt
+    achieved = bpl[dom] / (avg_ms * 1e-3) / 1e9
+    copy_gbps = copy_bandwidth(torch)
+    kernels = {k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps,
+                   "GBps_alg": (bpl[k] / (v[0] / v[1] * 1e-3) / 1e9) if k in bpl else None}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+
+    out = {
+        "metric": "DOF-updates/s, 2D p=3 1024^2 KH (FP64)",
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Kelvin-Helmholtz initial condition)",
+        "config": {"workload": f"Kelvin-Helmholtz {N}^2 p={p} FV+FB HLLC/HLLC minmod "
+                               f"CFL 0.8 periodic, FP64 (plain form)",
+                   "N": N, "p": p, "dof_per_step": dof, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "alg_bytes_per_launch": bpl[dom]},
+        "step_hbm_roofline_frac": value * 64.0 / (HBM_PEAK_GBPS * 1e9),
+        "copy_GBps": copy_gbps,
+        "troubled_fraction": frac,
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = run_cpu_baseline(args.cpu_N, args.cpu_steps, args.cpu_warmup)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

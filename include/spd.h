@@ -146,6 +146,19 @@ int spd_prims_to_cons(spd_ctx* ctx, const double* W, double* U, long long count,
 int spd_set_debug_tap(spd_ctx* ctx, int substep, double* troubles, double* theta,
                       double* aff_x, double* aff_y, double* aff_z);
 
+/* Optional HIP-event timing of the step's kernels (no counterpart in the
+ * reference, which only times whole runs with a wall clock,
+ * src/sdader_simulator.py:383,387).  spd_profile_begin makes the following
+ * calls record an event pair around each of at most `max_launches` kernel
+ * launches on their own stream; spd_profile_end waits for the last event and
+ * returns, per kernel class k < n_classes, the summed duration in ms and the
+ * launch count (return value: launches recorded).  Class names:
+ * spd_kernel_class_name(k) for k < spd_kernel_class_count(). */
+int spd_profile_begin(spd_ctx* ctx, int max_launches);
+int spd_profile_end(spd_ctx* ctx, double* ms_out, long long* count_out, int n_classes);
+int spd_kernel_class_count(void);
+const char* spd_kernel_class_name(int k);
+
 #ifdef __cplusplus
 }
 #endif

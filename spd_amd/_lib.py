@@ -24,6 +24,7 @@ EXPORTS = [
     "spd_ctx_scratch_bytes", "spd_get_operators", "spd_compute_dt", "spd_step",
     "spd_sp_to_cv", "spd_cv_to_sp", "spd_cons_to_prims", "spd_prims_to_cons",
     "spd_set_debug_tap", "spd_host_operators", "spd_host_geometry",
+    "spd_profile_begin", "spd_profile_end", "spd_kernel_class_count", "spd_kernel_class_name",
 ]
 
 
@@ -100,6 +101,15 @@ def load():
     L.spd_host_operators.restype = i
     L.spd_host_geometry.argtypes = [i, ctypes.c_double, i, ctypes.POINTER(ctypes.c_double), i]
     L.spd_host_geometry.restype = i
+    L.spd_profile_begin.argtypes = [vp, i]
+    L.spd_profile_begin.restype = i
+    L.spd_profile_end.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_longlong), i]
+    L.spd_profile_end.restype = i
+    L.spd_kernel_class_count.argtypes = []
+    L.spd_kernel_class_count.restype = i
+    L.spd_kernel_class_name.argtypes = [i]
+    L.spd_kernel_class_name.restype = ctypes.c_char_p
     _lib = L
     return L
 

@@ -1160,8 +1160,22 @@ inline unsigned grid_for(long long n, int bs = 256) {
 
 }  // namespace
 
+// Kernel classes timed by the optional HIP-event profiler.
+enum KernelClass : int {
+  KC_FACE_SD = 0, KC_PRED, KC_HOFLUX, KC_TENSOR, KC_APPLY, KC_DETECT, KC_THETA1, KC_RING,
+  KC_FBFLUX, KC_CV2SP, KC_OTHER, KC_COUNT
+};
+static const char* const kKernelClassNames[KC_COUNT] = {
+    "k_face_sd", "k_pred", "k_hoflux", "k_tensor", "k_apply", "k_detect", "k_theta1", "k_ring",
+    "k_fbflux", "k_cv2sp", "other"};
+
 struct spd_ctx {
   spd_config cfg{};
+  // HIP-event profiler: one event pair per recorded launch
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<int> prof_cls;
+  size_t prof_used = 0;
   int ndim = 0, p = 0, n = 0, nf = 0, T = 0, V = 0, NL = 0, NLP = 0;
   Geom g{};
   Params prm{};
@@ -1188,7 +1202,19 @@ struct spd_ctx {
   double* dbg_aff[3] = {nullptr, nullptr, nullptr};
 
   ~spd_ctx() {
+    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
+  }
+  // Launch `fn` on stream s, bracketed by an event pair when profiling.
+  template <typename F> void timed(int cls, hipStream_t s, F&& fn) {
+    const bool rec = prof_on && prof_used < prof_cls.size();
+    if (rec) (void)hipEventRecord(prof_ev[2 * prof_used], s);
+    fn();
+    if (rec) {
+      (void)hipEventRecord(prof_ev[2 * prof_used + 1], s);
+      prof_cls[prof_used] = cls;
+      ++prof_used;
+    }
   }
   template <typename X> int alloc(X** ptr, size_t count) {
     void* p = nullptr;
@@ -1243,8 +1269,10 @@ template <int NDIM, int P> struct Stepper {
     for (int a = 0; a < NDIM; ++a) {
       const long long Na = c->g.N[a];
       const long long tot = (long long)Tsrc * (Na + 1) * (c->g.NE / Na) * NLP;
-      hipLaunchKernelGGL((k_face_sd<NDIM, P, RS>), dim3(grid_for(tot)), dim3(256), 0, s, op,
-                         c->g, c->prm, a, src, Tsrc, c->Fface[a]);
+      c->timed(KC_FACE_SD, s, [&] {
+        hipLaunchKernelGGL((k_face_sd<NDIM, P, RS>), dim3(grid_for(tot)), dim3(256), 0, s, op,
+                           c->g, c->prm, a, src, Tsrc, c->Fface[a]);
+      });
     }
   }
   static void faces(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, hipStream_t s) {
@@ -1258,17 +1286,21 @@ template <int NDIM, int P> struct Stepper {
     const int E = epb();
     const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
     const size_t lds = (size_t)E * T * V * NL * sizeof(double);
-    hipLaunchKernelGGL((k_pred<NDIM, P>), dim3(blocks), dim3(E * T * NLP), lds, s, op, c->g,
-                       c->prm, src, Tsrc, c->Fface[0], c->Fface[1], c->Fface[2], Usp, dt, out,
-                       mode, E);
+    c->timed(KC_PRED, s, [&] {
+      hipLaunchKernelGGL((k_pred<NDIM, P>), dim3(blocks), dim3(E * T * NLP), lds, s, op, c->g,
+                         c->prm, src, Tsrc, c->Fface[0], c->Fface[1], c->Fface[2], Usp, dt, out,
+                         mode, E);
+    });
   }
   template <int RS>
   static void fb_rs(spd_ctx* c, const double* dt, double w, hipStream_t s) {
     for (int a = 0; a < NDIM; ++a) {
       long long nface = 1;
       for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == a ? 1 : 0);
-      hipLaunchKernelGGL((k_fbflux<NDIM, RS>), dim3(grid_for(nface)), dim3(256), 0, s, c->g,
-                         c->prm, a, c->Ucv, c->theta, dt, w, c->Fho[a]);
+      c->timed(KC_FBFLUX, s, [&] {
+        hipLaunchKernelGGL((k_fbflux<NDIM, RS>), dim3(grid_for(nface)), dim3(256), 0, s, c->g,
+                           c->prm, a, c->Ucv, c->theta, dt, w, c->Fho[a]);
+      });
     }
   }
 
@@ -1289,25 +1321,37 @@ template <int NDIM, int P> struct Stepper {
       return hipGetLastError() == hipSuccess ? SPD_OK : fail(SPD_E_HIP, "kernel launch");
     }
     // --- FV corrector (src/sdader_simulator.py:288-349)
-    hipLaunchKernelGGL((k_tensor<NDIM, P>), dim3(grid_for(c->g.NE * V * NL)), dim3(256), 0, s, op,
-                       c->g, 1, Usp, c->Ucv, 1);
+    c->timed(KC_TENSOR, s, [&] {
+      hipLaunchKernelGGL((k_tensor<NDIM, P>), dim3(grid_for(c->g.NE * V * NL)), dim3(256), 0, s,
+                         op, c->g, 1, Usp, c->Ucv, 1);
+    });
     for (int i = 0; i < T; ++i) {
       const double w = c->ops->w_tp[i];
       for (int a = 0; a < NDIM; ++a)
-        hipLaunchKernelGGL((k_hoflux<NDIM, P>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s, op,
-                           c->g, c->prm, a, i, c->Uad, c->Fface[a], c->Fho[a]);
+        c->timed(KC_HOFLUX, s, [&] {
+          hipLaunchKernelGGL((k_hoflux<NDIM, P>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s,
+                             op, c->g, c->prm, a, i, c->Uad, c->Fface[a], c->Fho[a]);
+        });
       if (c->cfg.FB) {
-        hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
-                           c->Ucv, c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Unew);
+        c->timed(KC_APPLY, s, [&] {
+          hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
+                             c->Ucv, c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Unew);
+        });
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag, c->cmax, 1);
-        hipLaunchKernelGGL((k_detect<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
-                           c->prm, c->Ucv, c->Unew, c->flags, c->any_flag);
+        c->timed(KC_DETECT, s, [&] {
+          hipLaunchKernelGGL((k_detect<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
+                             c->prm, c->Ucv, c->Unew, c->flags, c->any_flag);
+        });
         double* th_out = c->prm.blending ? c->theta1 : c->theta;
-        hipLaunchKernelGGL((k_theta1<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
-                           c->prm, c->flags, c->Ucv, c->Unew, c->any_flag, th_out);
+        c->timed(KC_THETA1, s, [&] {
+          hipLaunchKernelGGL((k_theta1<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
+                             c->prm, c->flags, c->Ucv, c->Unew, c->any_flag, th_out);
+        });
         if (c->prm.blending)
-          hipLaunchKernelGGL((k_ring<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
-                             c->theta1, c->theta);
+          c->timed(KC_RING, s, [&] {
+            hipLaunchKernelGGL((k_ring<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
+                               c->theta1, c->theta);
+          });
         if (i == c->dbg_sub || c->dbg_sub == SPD_TAP_ALL) {
           // with SPD_TAP_ALL the buffers hold T consecutive blocks
           const long long k = (c->dbg_sub == SPD_TAP_ALL) ? i : 0;
@@ -1333,13 +1377,17 @@ template <int NDIM, int P> struct Stepper {
         else
           fb_rs<RS_LLF>(c, dt, w, s);
       }
-      hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g, c->Ucv,
-                         c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Ucv);
+      c->timed(KC_APPLY, s, [&] {
+        hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
+                           c->Ucv, c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Ucv);
+      });
     }
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag, c->cmax, 2);
-    hipLaunchKernelGGL((k_cv2sp<NDIM, P>), dim3(grid_for(c->g.NE * NL)), dim3(256), 0, s, op,
-                       c->g, c->prm, c->Ucv, Usp, snapshot ? Ucv_sd : nullptr,
-                       snapshot ? Wcv_sd : nullptr, c->cmax);
+    c->timed(KC_CV2SP, s, [&] {
+      hipLaunchKernelGGL((k_cv2sp<NDIM, P>), dim3(grid_for(c->g.NE * NL)), dim3(256), 0, s, op,
+                         c->g, c->prm, c->Ucv, Usp, snapshot ? Ucv_sd : nullptr,
+                         snapshot ? Wcv_sd : nullptr, c->cmax);
+    });
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(SPD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return SPD_OK;
@@ -1658,6 +1706,48 @@ int spd_prims_to_cons(spd_ctx* ctx, const double* W, double* U, long long count,
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+}
+
+int spd_profile_begin(spd_ctx* ctx, int max_launches) {
+  if (!ctx || max_launches < 1) return fail(SPD_E_INVALID, "invalid argument");
+  for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+  ctx->prof_ev.assign(2 * (size_t)max_launches, nullptr);
+  for (auto& e : ctx->prof_ev) SPD_HIP_CHECK(hipEventCreate(&e));
+  ctx->prof_cls.assign((size_t)max_launches, KC_OTHER);
+  ctx->prof_used = 0;
+  ctx->prof_on = true;
+  return SPD_OK;
+}
+
+int spd_profile_end(spd_ctx* ctx, double* ms_out, long long* count_out, int n_classes) {
+  if (!ctx || !ms_out || !count_out) return fail(SPD_E_INVALID, "null argument");
+  ctx->prof_on = false;
+  for (int k = 0; k < n_classes; ++k) {
+    ms_out[k] = 0.0;
+    count_out[k] = 0;
+  }
+  if (ctx->prof_used) SPD_HIP_CHECK(hipEventSynchronize(ctx->prof_ev[2 * ctx->prof_used - 1]));
+  for (size_t i = 0; i < ctx->prof_used; ++i) {
+    float ms = 0.f;
+    SPD_HIP_CHECK(hipEventElapsedTime(&ms, ctx->prof_ev[2 * i], ctx->prof_ev[2 * i + 1]));
+    const int k = ctx->prof_cls[i];
+    if (k < n_classes) {
+      ms_out[k] += ms;
+      count_out[k] += 1;
+    }
+  }
+  const int used = (int)ctx->prof_used;
+  for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+  ctx->prof_ev.clear();
+  ctx->prof_cls.clear();
+  ctx->prof_used = 0;
+  return used;
+}
+
+int spd_kernel_class_count(void) { return KC_COUNT; }
+
+const char* spd_kernel_class_name(int k) {
+  return (k >= 0 && k < KC_COUNT) ? kKernelClassNames[k] : "";
 }
 
 int spd_set_debug_tap(spd_ctx* ctx, int substep, double* troubles, double* theta, double* aff_x,

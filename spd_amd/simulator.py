@@ -436,6 +436,15 @@ class SDADER_Simulator:
         self.n_step += 1
         self.time += self.dm.dt
 
+    def advance(self, nsteps=1):
+        """``nsteps`` of (compute_dt, step) without snapshot writes and without a
+        t_end clamp -- the inner loop of ``perform_iterations`` without
+        ``init_sim``/``end_sim``.  ``dm.U_cv``/``dm.W_cv`` are left stale until
+        ``end_sim`` (as in the reference, where they are recomputed there)."""
+        for _ in range(nsteps):
+            self.compute_dt()
+            self._step(snapshot=False)
+
     def perform_update(self) -> bool:
         """One step; ``dm.U_sp``, ``dm.U_cv``, ``dm.W_cv``, ``time`` and ``n_step``
         are updated as in ``src/sdader_simulator.py:361-378``."""
@@ -512,6 +521,39 @@ class SDADER_Simulator:
             for a in range(nd):
                 rec[f"aff_{'xyz'[a]}"] = affn[a][i]
             out.append(rec)
+        return out
+
+    def troubled_fraction_next_step(self):
+        """Advance one step (compute_dt + step) and return the fraction of
+        troubled subcells averaged over its FV substeps (computed on the GPU)."""
+        if not (self.update == "FV" and self.FB):
+            return 0.0
+        T = self.m + 1
+        Ns = [self.N[d] * (self.p + 1) for d in self.dims]
+        tr = torch.empty([T] + Ns[::-1], dtype=torch.float64, device=self.device)
+        _lib.check(self._lib.spd_set_debug_tap(self._ctx, _lib.TAP_ALL, _lib.ptr(tr), None,
+                                               None, None, None), "spd_set_debug_tap")
+        try:
+            self.compute_dt()
+            self._step(snapshot=False)
+        finally:
+            self._lib.spd_set_debug_tap(self._ctx, -1, None, None, None, None, None)
+        return float(tr.mean().item())
+
+    def profile_begin(self, max_launches=100000):
+        """Start HIP-event timing of every kernel launch of the following steps."""
+        _lib.check(self._lib.spd_profile_begin(self._ctx, int(max_launches)), "spd_profile_begin")
+
+    def profile_end(self):
+        """Stop timing; return {kernel: (total_ms, launches)}."""
+        k = self._lib.spd_kernel_class_count()
+        ms = (ctypes.c_double * k)()
+        cnt = (ctypes.c_longlong * k)()
+        _lib.check(self._lib.spd_profile_end(self._ctx, ms, cnt, k), "spd_profile_end")
+        out = {}
+        for i in range(k):
+            if cnt[i]:
+                out[self._lib.spd_kernel_class_name(i).decode()] = (float(ms[i]), int(cnt[i]))
         return out
 
     @property
