@@ -57,8 +57,12 @@ def algorithmic_bytes(N, p, launches):
       k_detect   per substep: read 2 V C, write C flags
       k_theta1   per substep: read C flags, write G
       k_ring     per substep: read + write G
-      k_fbflux   per substep and dim: read G theta + V C state, read+write the face array
+      k_hancock  per substep: read G theta + V C state, write (1+2) V C states/slopes
+      k_fbface   per substep and dim: read 2 G theta + 2 V C states/slopes, read+write
+                 the face array
       k_cv2sp    once: read V C, write V C
+    (k_hancock / k_fbface only touch cells and faces near troubled cells; the
+    figures are upper bounds for them.)
     """
     V, n = 4, p + 1
     T = n
@@ -77,7 +81,8 @@ def algorithmic_bytes(N, p, launches):
         "k_detect": T * (2 * V * C),
         "k_theta1": T * G,
         "k_ring": T * 2 * G,
-        "k_fbflux": T * 2 * (G + V * C + 2 * Fd),
+        "k_hancock": T * (G + V * C + 3 * V * C),
+        "k_fbface": T * 2 * (2 * G + 2 * V * C + 2 * Fd),
         "k_cv2sp": 2 * V * C,
     }
     extra_bytes = {"k_detect": T * C * 1.0, "k_theta1": T * C * 1.0}

@@ -250,11 +250,4 @@ __device__ __forceinline__ void load_W(const double* __restrict__ U, const Geom&
     if (flip[a]) W[a + 1] = -W[a + 1];
 }
 
-// Atomic max of a non-negative double held as its IEEE bit pattern (the
-// ordering of non-negative doubles, +inf and positive NaN matches the
-// unsigned ordering of their bits, so np.max semantics carry over).
-__device__ __forceinline__ void atomic_max_pos(unsigned long long* addr, double v) {
-  atomicMax(addr, (unsigned long long)__double_as_longlong(v));
-}
-
 }  // namespace spd

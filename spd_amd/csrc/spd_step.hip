@@ -14,7 +14,8 @@
 //              FV : sp->cv (k_sp2cv), then T substeps of
 //                   [HO face fluxes (k_hoflux), candidate (k_apply),
 //                    detection (k_detect), theta (k_theta1, k_ring),
-//                    fallback + blend (k_fbflux), apply (k_apply)],
+//                    fallback states (k_hancock) + flux and blend (k_fbface),
+//                    apply (k_apply)],
 //                   then cv->sp with the dt reduction (k_cv2sp).
 #include <hip/hip_runtime.h>
 
@@ -100,6 +101,21 @@ template <int n> __device__ __forceinline__ int stride_of(int a) {
   int s = 1;
   for (int b = 0; b < a; ++b) s *= n;
   return s;
+}
+
+// Maximum over the block (NaN-propagating); the result is valid in thread 0.
+__device__ __forceinline__ double block_max(double v) {
+  __shared__ double red[32];
+  for (int off = warpSize / 2; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off));
+  const int w = threadIdx.x / warpSize, lane = threadIdx.x % warpSize;
+  const int nw = (blockDim.x + warpSize - 1) / warpSize;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  if (w == 0) {
+    v = (lane < nw) ? red[lane] : 0.0;
+    for (int off = warpSize / 2; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off));
+  }
+  return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -659,7 +675,11 @@ k_detect(Geom g, Params prm, const double* __restrict__ Ucv, const double* __res
     }
     flags[cell] = tr ? 1 : 0;
   }
-  if (__any(cand) && (threadIdx.x % warpSize) == 0) atomicOr(any_flag, 1);
+  // the global gate only matters when some side is not periodic
+  if (any_flag) {
+    const int blk = __syncthreads_or(cand ? 1 : 0);
+    if (blk && threadIdx.x == 0 && *(volatile int*)any_flag == 0) atomicOr(any_flag, 1);
+  }
 }
 
 // Trouble value on the ghosted grid after the flag fill that refreshes only
@@ -850,61 +870,121 @@ k_ring(Geom g, const double* __restrict__ t1, double* __restrict__ theta) {
 // place (src/muscl.py:12-35,61-75,129-221, src/fv_simulator.py:88-175,
 // src/sdader_simulator.py:325-331).
 // ---------------------------------------------------------------------------
+// Hancock-predicted primitive state and half slopes of every cell of the
+// one-layer-ghosted grid that borders a face with a non-zero blending
+// coefficient (or every cell with godunov).  Output layout per cell:
+// [Mp (V), S_0 (V), ..., S_{NDIM-1} (V)] on the grid (Ns+2)^NDIM.
 template <int NDIM>
-__device__ __forceinline__ void hancock_state(const double* __restrict__ Ucv, const Geom& g,
-                                              const Params& prm, const int* cc, int a, double dti,
-                                              double* Mp, double* Sa) {
-  constexpr int V = Dims<NDIM>::V, P = Dims<NDIM>::P;
-  double W[V];
-  load_W<NDIM>(Ucv, g, cc, W, prm.gamma);
-  double dMh[NDIM][V];
+__global__ void __launch_bounds__(256)
+k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ theta,
+          const double* __restrict__ dt_dev, double w_i, double* __restrict__ HS) {
+  constexpr int V = Dims<NDIM>::V;
+  long long n1 = 1;
+  int e1[3] = {1, 1, 1};
 #pragma unroll
-  for (int b = 0; b < NDIM; ++b) {
-    int cp[3] = {cc[0], cc[1], cc[2]}, cm[3] = {cc[0], cc[1], cc[2]};
-    cp[b] += 1;
-    cm[b] -= 1;
-    double Wp[V], Wm[V];
-    load_W<NDIM>(Ucv, g, cp, Wp, prm.gamma);
-    load_W<NDIM>(Ucv, g, cm, Wm, prm.gamma);
-    const int G = cc[b] + NGH;
-    const double hl = g.hcv[b][G - 1], hr = g.hcv[b][G];
+  for (int a = 0; a < NDIM; ++a) {
+    e1[a] = g.Ns[a] + 2;
+    n1 *= e1[a];
+  }
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n1) return;
+  int c[3] = {0, 0, 0};
+  {
+    long long x = tid;
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const double L = (W[v] - Wm[v]) / hl;
-      const double R = (Wp[v] - W[v]) / hr;
-      double r = R / L;
-      r = (r < 1.0) ? r : 1.0;
-      dMh[b][v] = ((r > 0.0) ? r : 0.0) * L;
+    for (int a = 0; a < NDIM; ++a) {
+      c[a] = (int)(x % e1[a]) - 1;
+      x /= e1[a];
     }
   }
-  // Hancock half step with the reference's unrolled transverse loop (Q19) and
-  // zeroed pressure row (Q1)
-  double dtW[V];
+  // needed when the cell is L or R of a face with theta_face > 0 (or NaN)
+  bool need = prm.godunov != 0;
+  if (!need) {
+    long long gi = 0;
 #pragma unroll
-  for (int v = 0; v < V; ++v) dtW[v] = 0.0;
+    for (int b = NDIM - 1; b >= 0; --b) gi = gi * g.Ng[b] + (c[b] + NGH);
+    const double th0 = theta[gi];
+    need = !(th0 == 0.0);
+    long long st = 1;
 #pragma unroll
-  for (int b = 0; b < NDIM; ++b) {
-    const int vel = b + 1;
-    const double* dW = dMh[b];
-    dtW[0] = dtW[0] - (W[vel] * dW[0] + W[0] * dW[vel]);
-    dtW[P] = dtW[P] - (W[vel] * dW[P] + prm.gamma * W[P] * dW[vel]);
-    dtW[vel] = dtW[vel] - (W[vel] * dW[vel] + dW[P] / W[0]);
+    for (int a = 0; a < NDIM; ++a) {
+      if (need) break;
+      // faces of this cell along a that exist: the cell must be within the
+      // face range of the active transverse cells
+      bool trans_ok = true;
 #pragma unroll
-    for (int v2 = 2; v2 <= NDIM; ++v2) dtW[v2] = dtW[v2] - W[vel] * dW[v2];
+      for (int b = 0; b < NDIM; ++b)
+        if (b != a && (c[b] < 0 || c[b] >= g.Ns[b])) trans_ok = false;
+      if (trans_ok) {
+        if (c[a] + 1 < g.Ng[a] - NGH && !(theta[gi + st] == 0.0)) need = true;  // face to the right
+        if (c[a] - 1 >= -1 && !(theta[gi - st] == 0.0)) need = true;            // face to the left
+      }
+      st *= g.Ng[a];
+    }
   }
-  dtW[P] = dtW[P] * 0.0;
-  const double hf = g.hfp[a][cc[a] + NGH];
+  if (!need) return;
+  const double dti = (*dt_dev) * w_i;
+  double Mp[V], S[NDIM][V];
+  {
+    constexpr int P = Dims<NDIM>::P;
+    double W[V];
+    load_W<NDIM>(Ucv, g, c, W, prm.gamma);
+    double dMh[NDIM][V];
 #pragma unroll
-  for (int v = 0; v < V; ++v) {
-    Mp[v] = W[v] + 0.5 * dtW[v] * dti;
-    Sa[v] = 0.5 * dMh[a][v] * hf;
+    for (int b = 0; b < NDIM; ++b) {
+      int cp[3] = {c[0], c[1], c[2]}, cm[3] = {c[0], c[1], c[2]};
+      cp[b] += 1;
+      cm[b] -= 1;
+      double Wp[V], Wm[V];
+      load_W<NDIM>(Ucv, g, cp, Wp, prm.gamma);
+      load_W<NDIM>(Ucv, g, cm, Wm, prm.gamma);
+      const int G = c[b] + NGH;
+      const double hl = g.hcv[b][G - 1], hr = g.hcv[b][G];
+      const double hf = g.hfp[b][G];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const double L = (W[v] - Wm[v]) / hl;
+        const double R = (Wp[v] - W[v]) / hr;
+        double r = R / L;
+        r = (r < 1.0) ? r : 1.0;
+        dMh[b][v] = ((r > 0.0) ? r : 0.0) * L;
+        S[b][v] = 0.5 * dMh[b][v] * hf;
+      }
+    }
+    double dtW[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) dtW[v] = 0.0;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      const int vel = b + 1;
+      const double* dW = dMh[b];
+      dtW[0] = dtW[0] - (W[vel] * dW[0] + W[0] * dW[vel]);
+      dtW[P] = dtW[P] - (W[vel] * dW[P] + prm.gamma * W[P] * dW[vel]);
+      dtW[vel] = dtW[vel] - (W[vel] * dW[vel] + dW[P] / W[0]);
+#pragma unroll
+      for (int v2 = 2; v2 <= NDIM; ++v2) dtW[v2] = dtW[v2] - W[vel] * dW[v2];
+    }
+    dtW[P] = dtW[P] * 0.0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) Mp[v] = W[v] + 0.5 * dtW[v] * dti;
   }
+  constexpr int REC = (1 + NDIM) * V;
+  double* out = HS + tid;
+#pragma unroll
+  for (int v = 0; v < V; ++v) out[(long long)v * n1] = Mp[v];
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b)
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[(long long)((1 + b) * V + v) * n1] = S[b][v];
+  (void)REC;
 }
 
+// Fallback flux from the stored Hancock states at the faces of dimension a
+// with a non-zero coefficient, blended into the HO flux in place.
 template <int NDIM, int RS>
 __global__ void __launch_bounds__(256)
-k_fbflux(Geom g, Params prm, int a, const double* __restrict__ Ucv, const double* __restrict__ theta,
-         const double* __restrict__ dt_dev, double w_i, double* __restrict__ Fho) {
+k_fbface(Geom g, Params prm, int a, const double* __restrict__ HS,
+         const double* __restrict__ theta, double* __restrict__ Fho) {
   constexpr int V = Dims<NDIM>::V;
   long long ext[3] = {1, 1, 1};
   long long nface = 1;
@@ -924,29 +1004,24 @@ k_fbflux(Geom g, Params prm, int a, const double* __restrict__ Ucv, const double
       x /= ext[b];
     }
   }
-  // theta at the two cells (ghosted indices)
-  long long gl = 0, gr = 0;
+  long long gl = 0, gr = 0, hl = 0, hr = 0, n1 = 1;
 #pragma unroll
   for (int b = NDIM - 1; b >= 0; --b) {
-    const int xl = c[b] + NGH - (b == a ? 1 : 0);
-    const int xr = c[b] + NGH;
-    gl = gl * g.Ng[b] + xl;
-    gr = gr * g.Ng[b] + xr;
+    gl = gl * g.Ng[b] + c[b] + NGH - (b == a ? 1 : 0);
+    gr = gr * g.Ng[b] + c[b] + NGH;
+    hl = hl * (g.Ns[b] + 2) + c[b] + 1 - (b == a ? 1 : 0);
+    hr = hr * (g.Ns[b] + 2) + c[b] + 1;
   }
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) n1 *= g.Ns[b] + 2;
   double th = nmax(theta[gl], theta[gr]);
   if (prm.godunov) th = 1.0;
   if (th == 0.0) return;
-  const double dti = (*dt_dev) * w_i;
-  int cL[3] = {c[0], c[1], c[2]}, cR[3] = {c[0], c[1], c[2]};
-  cL[a] -= 1;
-  double MpL[V], SL[V], MpR[V], SR[V];
-  hancock_state<NDIM>(Ucv, g, prm, cL, a, dti, MpL, SL);
-  hancock_state<NDIM>(Ucv, g, prm, cR, a, dti, MpR, SR);
   double WL[V], WR[V], UL[V], UR[V], F[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) {
-    WL[v] = MpL[v] + SL[v];
-    WR[v] = MpR[v] - SR[v];
+    WL[v] = HS[(long long)v * n1 + hl] + HS[(long long)((1 + a) * V + v) * n1 + hl];
+    WR[v] = HS[(long long)v * n1 + hr] - HS[(long long)((1 + a) * V + v) * n1 + hr];
   }
   cons<NDIM>(WL, UL, a, prm.gamma);
   cons<NDIM>(WR, UR, a, prm.gamma);
@@ -967,7 +1042,7 @@ k_fbflux(Geom g, Params prm, int a, const double* __restrict__ Ucv, const double
 template <int NDIM, int P>
 __global__ void __launch_bounds__(256)
 k_cv2sp(Ops<P> op, Geom g, Params prm, const double* __restrict__ Ucv, double* __restrict__ Usp,
-        double* __restrict__ Ucv_sd, double* __restrict__ Wcv_sd, unsigned long long* cmax) {
+        double* __restrict__ Ucv_sd, double* __restrict__ Wcv_sd, double* __restrict__ partial) {
   constexpr int V = Dims<NDIM>::V, P_ = Dims<NDIM>::P, n = P + 1;
   constexpr int NL = IPow<n, NDIM>::v;
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1030,17 +1105,17 @@ k_cv2sp(Ops<P> op, Geom g, Params prm, const double* __restrict__ Ucv, double* _
 #pragma unroll
     for (int k = 1; k <= NDIM; ++k) cw = cw + fabs(W[k]);
   }
-  if (cmax) {
-    // wavefront maximum, then one atomic per wavefront
-    for (int off = warpSize / 2; off > 0; off >>= 1) cw = nmax(cw, __shfl_xor(cw, off));
-    if ((threadIdx.x % warpSize) == 0) atomic_max_pos(cmax, cw);
+  if (partial) {
+    cw = block_max(cw);
+    if (threadIdx.x == 0) partial[blockIdx.x] = cw;
   }
 }
 
-// Maximum wave speed over a primitive subcell-mean array in element layout.
+// Maximum wave speed over a primitive subcell-mean array in element layout
+// (per-block partial maxima).
 template <int NDIM>
 __global__ void __launch_bounds__(256)
-k_wave_max(Geom g, Params prm, int NL, const double* __restrict__ W, unsigned long long* cmax) {
+k_wave_max(Geom g, Params prm, int NL, const double* __restrict__ W, double* __restrict__ partial) {
   constexpr int P = Dims<NDIM>::P;
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long npts = g.NE * NL;
@@ -1050,22 +1125,25 @@ k_wave_max(Geom g, Params prm, int NL, const double* __restrict__ W, unsigned lo
 #pragma unroll
     for (int k = 1; k <= NDIM; ++k) cw = cw + fabs(W[k * npts + tid]);
   }
-  for (int off = warpSize / 2; off > 0; off >>= 1) cw = nmax(cw, __shfl_xor(cw, off));
-  if ((threadIdx.x % warpSize) == 0) atomic_max_pos(cmax, cw);
+  cw = block_max(cw);
+  if (threadIdx.x == 0) partial[blockIdx.x] = cw;
 }
 
-__global__ void k_dt_final(const unsigned long long* cmax, double h, double cfl, double* dt) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const double c = __longlong_as_double((long long)*cmax);
-    dt[0] = cfl * (h / c);
-  }
+// Reduce the block partial maxima into *cmax (one block of 1024 threads).
+__global__ void __launch_bounds__(1024)
+k_reduce_max(const double* __restrict__ partial, long long n, double* cmax) {
+  double v = 0.0;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) v = nmax(v, partial[i]);
+  v = block_max(v);
+  if (threadIdx.x == 0) *cmax = v;
 }
 
-__global__ void k_zero_counters(int* any_flag, unsigned long long* cmax, int which) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (which & 1) *any_flag = 0;
-    if (which & 2) *cmax = 0ull;
-  }
+__global__ void k_dt_final(const double* cmax, double h, double cfl, double* dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dt[0] = cfl * (h / *cmax);
+}
+
+__global__ void k_zero_counters(int* any_flag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *any_flag = 0;
 }
 
 template <int NDIM>
@@ -1163,11 +1241,11 @@ inline unsigned grid_for(long long n, int bs = 256) {
 // Kernel classes timed by the optional HIP-event profiler.
 enum KernelClass : int {
   KC_FACE_SD = 0, KC_PRED, KC_HOFLUX, KC_TENSOR, KC_APPLY, KC_DETECT, KC_THETA1, KC_RING,
-  KC_FBFLUX, KC_CV2SP, KC_OTHER, KC_COUNT
+  KC_FBFLUX, KC_CV2SP, KC_HANCOCK, KC_OTHER, KC_COUNT
 };
 static const char* const kKernelClassNames[KC_COUNT] = {
     "k_face_sd", "k_pred", "k_hoflux", "k_tensor", "k_apply", "k_detect", "k_theta1", "k_ring",
-    "k_fbflux", "k_cv2sp", "other"};
+    "k_fbface", "k_cv2sp", "k_hancock", "other"};
 
 struct spd_ctx {
   spd_config cfg{};
@@ -1193,8 +1271,12 @@ struct spd_ctx {
   uint8_t* flags = nullptr;
   double* theta1 = nullptr;
   double* theta = nullptr;
+  double* HS = nullptr;       // Hancock states and half slopes, (1+ndim)*V per cell
   int* any_flag = nullptr;
-  unsigned long long* cmax = nullptr;
+  double* cmax = nullptr;     // max wave speed of the last state (for dt)
+  double* partial = nullptr;  // per-block partial maxima
+  long long npartial = 0;
+  bool need_gate = false;     // some side is not periodic (global NAD gate needed)
   // debug tap
   int dbg_sub = -1;
   double* dbg_tr = nullptr;
@@ -1294,12 +1376,18 @@ template <int NDIM, int P> struct Stepper {
   }
   template <int RS>
   static void fb_rs(spd_ctx* c, const double* dt, double w, hipStream_t s) {
+    long long n1 = 1;
+    for (int b = 0; b < NDIM; ++b) n1 *= c->g.Ns[b] + 2;
+    c->timed(KC_HANCOCK, s, [&] {
+      hipLaunchKernelGGL((k_hancock<NDIM>), dim3(grid_for(n1)), dim3(256), 0, s, c->g, c->prm,
+                         c->Ucv, c->theta, dt, w, c->HS);
+    });
     for (int a = 0; a < NDIM; ++a) {
       long long nface = 1;
       for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == a ? 1 : 0);
       c->timed(KC_FBFLUX, s, [&] {
-        hipLaunchKernelGGL((k_fbflux<NDIM, RS>), dim3(grid_for(nface)), dim3(256), 0, s, c->g,
-                           c->prm, a, c->Ucv, c->theta, dt, w, c->Fho[a]);
+        hipLaunchKernelGGL((k_fbface<NDIM, RS>), dim3(grid_for(nface)), dim3(256), 0, s, c->g,
+                           c->prm, a, c->HS, c->theta, c->Fho[a]);
       });
     }
   }
@@ -1337,10 +1425,11 @@ template <int NDIM, int P> struct Stepper {
           hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
                              c->Ucv, c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Unew);
         });
-        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag, c->cmax, 1);
+        if (c->need_gate) hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
         c->timed(KC_DETECT, s, [&] {
           hipLaunchKernelGGL((k_detect<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
-                             c->prm, c->Ucv, c->Unew, c->flags, c->any_flag);
+                             c->prm, c->Ucv, c->Unew, c->flags,
+                             c->need_gate ? c->any_flag : nullptr);
         });
         double* th_out = c->prm.blending ? c->theta1 : c->theta;
         c->timed(KC_THETA1, s, [&] {
@@ -1382,12 +1471,13 @@ template <int NDIM, int P> struct Stepper {
                            c->Ucv, c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Ucv);
       });
     }
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag, c->cmax, 2);
+    const unsigned nb = grid_for(c->g.NE * NL);
     c->timed(KC_CV2SP, s, [&] {
-      hipLaunchKernelGGL((k_cv2sp<NDIM, P>), dim3(grid_for(c->g.NE * NL)), dim3(256), 0, s, op,
-                         c->g, c->prm, c->Ucv, Usp, snapshot ? Ucv_sd : nullptr,
-                         snapshot ? Wcv_sd : nullptr, c->cmax);
+      hipLaunchKernelGGL((k_cv2sp<NDIM, P>), dim3(nb), dim3(256), 0, s, op, c->g, c->prm, c->Ucv,
+                         Usp, snapshot ? Ucv_sd : nullptr, snapshot ? Wcv_sd : nullptr,
+                         c->partial);
     });
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, c->partial, (long long)nb, c->cmax);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(SPD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return SPD_OK;
@@ -1595,12 +1685,20 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
       if ((rc = c->alloc(&c->flags, (size_t)g.Nc)) != SPD_OK) return rc;
       if ((rc = c->alloc(&c->theta1, (size_t)g.Ngc)) != SPD_OK) return rc;
       if ((rc = c->alloc(&c->theta, (size_t)g.Ngc)) != SPD_OK) return rc;
+      long long n1 = 1;
+      for (int a = 0; a < nd; ++a) n1 *= g.Ns[a] + 2;
+      if ((rc = c->alloc(&c->HS, (size_t)((1 + nd) * V * n1))) != SPD_OK) return rc;
     }
   }
   if ((rc = c->alloc(&c->any_flag, 1)) != SPD_OK) return rc;
   if ((rc = c->alloc(&c->cmax, 1)) != SPD_OK) return rc;
+  c->npartial = (g.NE * c->NL + 255) / 256;
+  if ((rc = c->alloc(&c->partial, (size_t)c->npartial)) != SPD_OK) return rc;
   SPD_HIP_CHECK(hipMemset(c->any_flag, 0, sizeof(int)));
-  SPD_HIP_CHECK(hipMemset(c->cmax, 0, sizeof(unsigned long long)));
+  SPD_HIP_CHECK(hipMemset(c->cmax, 0, sizeof(double)));
+  for (int a = 0; a < nd; ++a)
+    for (int sd = 0; sd < 2; ++sd)
+      if (cfg->bc[a][sd] != SPD_BC_PERIODIC) c->need_gate = true;
   for (int a = 0; a < 3; ++a) {
     if (!c->Fface[a]) c->Fface[a] = c->Uad;  // unused dims: any valid pointer
     if (!c->Fho[a]) c->Fho[a] = c->Uad;
@@ -1625,21 +1723,23 @@ int spd_compute_dt(spd_ctx* ctx, const double* W_cv, double* dt_dev, void* strea
   if (!ctx || !dt_dev) return fail(SPD_E_INVALID, "null argument");
   hipStream_t s = (hipStream_t)stream;
   if (W_cv) {
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, ctx->any_flag, ctx->cmax, 2);
     const long long npts = ctx->g.NE * ctx->NL;
+    const unsigned nb = grid_for(npts);
     switch (ctx->ndim) {
       case 1:
-        hipLaunchKernelGGL((k_wave_max<1>), dim3(grid_for(npts)), dim3(256), 0, s, ctx->g, ctx->prm,
-                           ctx->NL, W_cv, ctx->cmax);
+        hipLaunchKernelGGL((k_wave_max<1>), dim3(nb), dim3(256), 0, s, ctx->g, ctx->prm, ctx->NL,
+                           W_cv, ctx->partial);
         break;
       case 2:
-        hipLaunchKernelGGL((k_wave_max<2>), dim3(grid_for(npts)), dim3(256), 0, s, ctx->g, ctx->prm,
-                           ctx->NL, W_cv, ctx->cmax);
+        hipLaunchKernelGGL((k_wave_max<2>), dim3(nb), dim3(256), 0, s, ctx->g, ctx->prm, ctx->NL,
+                           W_cv, ctx->partial);
         break;
       default:
-        hipLaunchKernelGGL((k_wave_max<3>), dim3(grid_for(npts)), dim3(256), 0, s, ctx->g, ctx->prm,
-                           ctx->NL, W_cv, ctx->cmax);
+        hipLaunchKernelGGL((k_wave_max<3>), dim3(nb), dim3(256), 0, s, ctx->g, ctx->prm, ctx->NL,
+                           W_cv, ctx->partial);
     }
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, ctx->partial, (long long)nb,
+                       ctx->cmax);
   }
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, s, ctx->cmax, ctx->h_dt, ctx->cfg.cfl_coeff,
                      dt_dev);
