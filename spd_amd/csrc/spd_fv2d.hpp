@@ -1,0 +1,515 @@
+// 2-D LDS-tiled kernels of the finite-volume corrector: high-order face
+// fluxes, trouble detection, blending coefficients and the MUSCL-Hancock
+// correction.  Each block owns a tile of cells (or a strip of elements),
+// stages what it needs (tile + halo, ghost layers resolved by the index maps
+// of spd_device.hpp) in LDS and computes its outputs from there.  The
+// arithmetic is that of the generic kernels in spd_step.hip, which this file
+// is included into (it uses trouble_at, ring1, kRules2 and Ops from there).
+#pragma once
+
+namespace spd {
+namespace fv2d {
+
+constexpr int TX = 32;  // tile width (cells along x)
+constexpr int TY = 8;   // tile height (cells along y)
+constexpr int NT = TX * TY;
+
+__device__ __forceinline__ double qnan() { return __longlong_as_double(0x7ff8000000000000ll); }
+
+__device__ __forceinline__ bool in_ghost_range(const Geom& g, int x, int y) {
+  return x >= -NGH && x < g.Ns[0] + NGH && y >= -NGH && y < g.Ns[1] + NGH;
+}
+
+// ---------------------------------------------------------------------------
+// Trouble detection (src/trouble_detection.py:5-71) on a TX x TY tile.
+//   wo: old primitives on the tile + 1 halo (all variables)
+//   wn: new (rho, P) on the tile + 3 halo (smooth-extrema stencil)
+// ---------------------------------------------------------------------------
+constexpr int DH = 3;
+constexpr int DNX = TX + 2 * DH, DNY = TY + 2 * DH;
+constexpr int ONX = TX + 2, ONY = TY + 2;
+
+// Smooth-extrema coefficient along one axis from LDS; w points at the cell,
+// w[k*stride] is its k-th neighbour along the axis (src/trouble_detection.py:
+// 104-137 with the compute_min edge rules).
+__device__ __forceinline__ double sed_alpha_lds(const double* w, int stride, int ca, int Ns,
+                                                const double* cen, const double* hf) {
+  double dU[5];
+#pragma unroll
+  for (int k = -2; k <= 2; ++k) {
+    const int x = ca + k;
+    dU[k + 2] = (x >= -1 && x <= Ns)
+                    ? (w[(k + 1) * stride] - w[(k - 1) * stride]) / (cen[x + 1] - cen[x - 1])
+                    : 0.0;
+  }
+  double A[3], aLc = 1.0;
+#pragma unroll
+  for (int k = -1; k <= 1; ++k) {
+    const int x = ca + k;
+    if (x < 0 || x >= Ns) {
+      A[k + 1] = 0.0;
+      continue;
+    }
+    const double d2U = (dU[k + 3] - dU[k + 1]) / (cen[x + 1] - cen[x - 1]);
+    const double dv = 0.5 * hf[x] * d2U;
+    const double vL = dU[k + 1] - dU[k + 2];
+    const double vR = dU[k + 3] - dU[k + 2];
+    double aL = -((dv < 0.0) ? ((vL > 0.0) ? vL : 0.0) : ((vL < 0.0) ? vL : 0.0)) / dv;
+    aL = (fabs(dv) <= 0.0) ? 1.0 : aL;
+    aL = (aL < 1.0) ? aL : 1.0;
+    double aR = ((dv > 0.0) ? ((vR > 0.0) ? vR : 0.0) : ((vR < 0.0) ? vR : 0.0)) / dv;
+    aR = (fabs(dv) <= 0.0) ? 1.0 : aR;
+    aR = (aR < 1.0) ? aR : 1.0;
+    A[k + 1] = (aR < aL) ? aR : aL;
+    if (k == 0) aLc = aL;
+  }
+  if (Ns == 1) return aLc;
+  if (ca == 0) return nmin(A[1], A[2]);
+  if (ca == Ns - 1) return nmin(A[0], aLc);
+  return nmin(A[0], nmin(A[1], A[2]));
+}
+
+__global__ void __launch_bounds__(NT)
+k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ Unew,
+           uint8_t* __restrict__ flags, int* __restrict__ any_flag) {
+  __shared__ double wn[2][DNY][DNX];  // new rho, P
+  __shared__ double wo[4][ONY][ONX];  // old primitives
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+  const int Nx = g.Ns[0], Ny = g.Ns[1];
+  for (int k = threadIdx.x; k < DNX * DNY; k += NT) {
+    const int lx = k % DNX, ly = k / DNX;
+    const int x = x0 + lx - DH, y = y0 + ly - DH;
+    double r = qnan(), p = qnan();
+    if (in_ghost_range(g, x, y)) {
+      int c[3] = {x, y, 0};
+      double W[4];
+      load_W<2>(Unew, g, c, W, prm.gamma);
+      r = W[0];
+      p = W[3];
+    }
+    wn[0][ly][lx] = r;
+    wn[1][ly][lx] = p;
+  }
+  for (int k = threadIdx.x; k < ONX * ONY; k += NT) {
+    const int lx = k % ONX, ly = k / ONX;
+    const int x = x0 + lx - 1, y = y0 + ly - 1;
+    double W[4] = {qnan(), qnan(), qnan(), qnan()};
+    if (in_ghost_range(g, x, y)) {
+      int c[3] = {x, y, 0};
+      load_W<2>(Ucv, g, c, W, prm.gamma);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) wo[v][ly][lx] = W[v];
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int x = x0 + tx, y = y0 + ty;
+  bool cand = false;
+  if (x < Nx && y < Ny) {
+    const int ox = tx + 1, oy = ty + 1;
+    double Wn[4];
+    Wn[0] = wn[0][ty + DH][tx + DH];
+    Wn[3] = wn[1][ty + DH][tx + DH];
+    Wn[1] = Wn[2] = 0.0;
+    const int nvar_nad = any_flag ? 4 : 2;  // velocities only feed the global gate
+    if (any_flag) {
+      int c[3] = {x, y, 0};
+      double W[4];
+      load_W<2>(Unew, g, c, W, prm.gamma);
+      Wn[1] = W[1];
+      Wn[2] = W[2];
+    }
+    bool pt[4] = {false, false, false, false};
+    for (int k = 0; k < nvar_nad; ++k) {
+      const int v = (k == 0) ? 0 : (k == 1 ? 3 : k - 1);
+      double mx = wo[v][oy][ox], mn = mx;
+      const double nb[4] = {wo[v][oy][ox - 1], wo[v][oy][ox + 1], wo[v][oy - 1][ox],
+                            wo[v][oy + 1][ox]};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        mx = nmax(mx, nb[q]);
+        mn = nmin(mn, nb[q]);
+      }
+      double lo = mn, hi = mx;
+      if (prm.p > 0) {
+        if (prm.nad_delta) {
+          const double eps = prm.tol * (hi - lo);
+          lo = lo - eps;
+          hi = hi + eps;
+        } else {
+          lo = lo - fabs(lo) * prm.tol;
+          hi = hi + fabs(hi) * prm.tol;
+        }
+      }
+      pt[v] = !(Wn[v] >= lo) || !(Wn[v] <= hi);
+      cand = cand || pt[v];
+    }
+    if (prm.p > 1 && prm.sed) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int var = (k == 0) ? 0 : 3;
+        if (!pt[var]) continue;
+        const double* w = &wn[k][ty + DH][tx + DH];
+        double alpha = 1.0;
+        const double ax = sed_alpha_lds(w, 1, x, Nx, g.centers[0] + NGH, g.hfp[0] + NGH);
+        alpha = (ax < alpha) ? ax : alpha;
+        const double ay = sed_alpha_lds(w, DNX, y, Ny, g.centers[1] + NGH, g.hfp[1] + NGH);
+        alpha = (ay < alpha) ? ay : alpha;
+        pt[var] = pt[var] && (alpha < 1.0);
+      }
+    }
+    bool tr = pt[0] || pt[3];
+    if (prm.pad) {
+      if (!(Wn[0] >= prm.min_rho) || !(Wn[0] <= prm.max_rho) || !(Wn[3] >= prm.min_P)) tr = true;
+    }
+    flags[(long long)y * Nx + x] = tr ? 1 : 0;
+  }
+  if (any_flag) {
+    const int blk = __syncthreads_or(cand ? 1 : 0);
+    if (blk && threadIdx.x == 0 && *(volatile int*)any_flag == 0) atomicOr(any_flag, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Blending coefficients theta on the ghosted grid (src/trouble_detection.py:
+// 73-87,139-169): trouble values on the tile + 2 halo, theta1 on tile + 1
+// halo, then the x and y 0.25 rings.  Tiles cover the ghosted grid; (gx, gy)
+// are ghosted indices.  Without blending theta = trouble.
+// ---------------------------------------------------------------------------
+constexpr int TH = 2;
+constexpr int TNX = TX + 2 * TH, TNY = TY + 2 * TH;
+
+__global__ void __launch_bounds__(NT)
+k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* __restrict__ Ucv,
+          const double* __restrict__ Unew, const int* __restrict__ any_flag,
+          double* __restrict__ theta) {
+  __shared__ double tr[TNY][TNX];
+  __shared__ double t1[TY + 2][TX + 2];
+  const int gx0 = blockIdx.x * TX, gy0 = blockIdx.y * TY;
+  const int NGx = g.Ng[0], NGy = g.Ng[1];
+  const bool gate = (*any_flag != 0) && prm.p > 1 && prm.sed;
+  const double* Ulast = gate ? Unew : Ucv;
+  for (int k = threadIdx.x; k < TNX * TNY; k += NT) {
+    const int lx = k % TNX, ly = k / TNX;
+    const int gx = gx0 + lx - TH, gy = gy0 + ly - TH;
+    double v = 0.0;
+    if (gx >= 0 && gx < NGx && gy >= 0 && gy < NGy) {
+      int c[3] = {gx - NGH, gy - NGH, 0};
+      v = trouble_at<2>(g, flags, Ulast, c);
+    }
+    tr[ly][lx] = v;
+  }
+  __syncthreads();
+  if (!prm.blending) {
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int gx = gx0 + tx, gy = gy0 + ty;
+    if (gx < NGx && gy < NGy) theta[(long long)gy * NGx + gx] = tr[ty + TH][tx + TH];
+    return;
+  }
+  for (int k = threadIdx.x; k < (TX + 2) * (TY + 2); k += NT) {
+    const int lx = k % (TX + 2), ly = k / (TX + 2);
+    const int gx = gx0 + lx - 1, gy = gy0 + ly - 1;
+    double th = 0.0;
+    if (gx >= 0 && gx < NGx && gy >= 0 && gy < NGy) {
+      const int sx = lx + 1, sy = ly + 1;  // position in tr
+      th = tr[sy][sx];
+      if (gx >= 1) th = nmax(0.75 * tr[sy][sx - 1], th);
+      if (gx <= NGx - 2) th = nmax(0.75 * tr[sy][sx + 1], th);
+      if (gy >= 1) th = nmax(0.75 * tr[sy - 1][sx], th);
+      if (gy <= NGy - 2) th = nmax(0.75 * tr[sy + 1][sx], th);
+      // diagonals (target slice of each update, src/trouble_detection.py:148-151)
+      if (gx <= NGx - 2 && gy <= NGy - 2) th = nmax(0.5 * tr[sy + 1][sx + 1], th);
+      if (gx >= 1 && gy <= NGy - 2) th = nmax(0.5 * tr[sy + 1][sx - 1], th);
+      if (gx <= NGx - 2 && gy >= 1) th = nmax(0.5 * tr[sy - 1][sx + 1], th);
+      if (gx >= 1 && gy >= 1) th = nmax(0.5 * tr[sy - 1][sx - 1], th);
+    }
+    t1[ly][lx] = th;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int gx = gx0 + tx, gy = gy0 + ty;
+  if (gx >= NGx || gy >= NGy) return;
+  const int lx = tx + 1, ly = ty + 1;
+  auto rx = [&](int dy) -> double {
+    const double lo = (gx >= 1) ? t1[ly + dy][lx - 1] : 0.0;
+    const double hi = (gx <= NGx - 2) ? t1[ly + dy][lx + 1] : 0.0;
+    return ring1(gx, NGx, lo, t1[ly + dy][lx], hi);
+  };
+  const double lo = (gy >= 1) ? rx(-1) : 0.0;
+  const double hi = (gy <= NGy - 2) ? rx(+1) : 0.0;
+  theta[(long long)gy * NGx + gx] = ring1(gy, NGy, lo, rx(0), hi);
+}
+
+// ---------------------------------------------------------------------------
+// MUSCL-Hancock correction on a TX x TY tile of cells (src/muscl.py:61-75,
+// 129-221, src/fv_simulator.py:88-175,189-205, src/sdader_simulator.py:325-331):
+//   1. theta on tile + 1 halo, old primitives on tile + 2 halo;
+//   2. Hancock state and half slopes of the cells that border a face with a
+//      non-zero coefficient (tile + 1 halo, plus-shaped);
+//   3. per face of the tile: fallback HLLC/LLF flux where the coefficient is
+//      non-zero, blended with the high-order flux;
+//   4. U_out = U_cv - dt_i * sum(dF/dx) for the tile's cells.
+// U_out must differ from U_cv (the halo of neighbouring tiles is read).
+// ---------------------------------------------------------------------------
+constexpr int CX = TX + 2, CY = TY + 2;   // tile + 1 halo
+constexpr int WX = TX + 4, WY = TY + 4;   // tile + 2 halo
+
+template <int RS>
+__global__ void __launch_bounds__(NT)
+k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
+              const double* __restrict__ theta, const double* __restrict__ Fx,
+              const double* __restrict__ Fy, const double* __restrict__ dt_dev, double w_i,
+              double* __restrict__ Uout) {
+  // LDS: W (4 x WY x WX) is reused for the blended face fluxes after step 2
+  constexpr int SW = (4 * WY * WX > 4 * (TY * (TX + 1) + (TY + 1) * TX))
+                         ? 4 * WY * WX
+                         : 4 * (TY * (TX + 1) + (TY + 1) * TX);
+  __shared__ double sW[SW];
+  __shared__ double sH[12][CY][CX];  // Mp(4), Sx(4), Sy(4)
+  __shared__ double sT[CY][CX];
+  __shared__ unsigned char sNeed[CY][CX];
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+  const int Nx = g.Ns[0], Ny = g.Ns[1];
+  const int NGx = g.Ng[0];
+  const double gamma = prm.gamma;
+  const double dti = (*dt_dev) * w_i;
+  double(*W)[WY][WX] = reinterpret_cast<double(*)[WY][WX]>(sW);
+
+  for (int k = threadIdx.x; k < CX * CY; k += NT) {
+    const int lx = k % CX, ly = k / CX;
+    const int x = x0 + lx - 1, y = y0 + ly - 1;
+    double th = 0.0;
+    if (x >= -1 && x <= Nx && y >= -1 && y <= Ny)
+      th = theta[(long long)(y + NGH) * NGx + (x + NGH)];
+    sT[ly][lx] = th;
+  }
+  for (int k = threadIdx.x; k < WX * WY; k += NT) {
+    const int lx = k % WX, ly = k / WX;
+    const int x = x0 + lx - 2, y = y0 + ly - 2;
+    double Wv[4] = {qnan(), qnan(), qnan(), qnan()};
+    if (in_ghost_range(g, x, y)) {
+      int c[3] = {x, y, 0};
+      load_W<2>(Ucv, g, c, Wv, gamma);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) W[v][ly][lx] = Wv[v];
+  }
+  __syncthreads();
+
+  // 2. which cells border an active face of this tile
+  for (int k = threadIdx.x; k < CX * CY; k += NT) {
+    const int lx = k % CX, ly = k / CX;
+    const int x = x0 + lx - 1, y = y0 + ly - 1;
+    bool need = false;
+    const bool godunov = prm.godunov != 0;
+    // x-faces of tile rows: cell is L of face x+1 (x in [x0-1, x0+TX-1]) or
+    // R of face x (x in [x0, x0+TX]); faces exist for x+1 <= Nx / x <= Nx
+    if (ly >= 1 && ly <= TY && y < Ny) {
+      if (lx <= TX && x + 1 <= Nx) need |= godunov || !(nmax(sT[ly][lx], sT[ly][lx + 1]) == 0.0);
+      if (lx >= 1 && x <= Nx) need |= godunov || !(nmax(sT[ly][lx - 1], sT[ly][lx]) == 0.0);
+    }
+    if (lx >= 1 && lx <= TX && x < Nx) {
+      if (ly <= TY && y + 1 <= Ny) need |= godunov || !(nmax(sT[ly][lx], sT[ly + 1][lx]) == 0.0);
+      if (ly >= 1 && y <= Ny) need |= godunov || !(nmax(sT[ly - 1][lx], sT[ly][lx]) == 0.0);
+    }
+    sNeed[ly][lx] = need ? 1 : 0;
+    if (!need) continue;
+    const int wx = lx + 1, wy = ly + 1;  // position in W
+    double Wc[4], dMh[2][4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Wc[v] = W[v][wy][wx];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int cb = (b == 0) ? x : y;
+      const int G = cb + NGH;
+      const double hl = g.hcv[b][G - 1], hr = g.hcv[b][G];
+      const double hf = g.hfp[b][G];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double Wm = (b == 0) ? W[v][wy][wx - 1] : W[v][wy - 1][wx];
+        const double Wp = (b == 0) ? W[v][wy][wx + 1] : W[v][wy + 1][wx];
+        const double L = (Wc[v] - Wm) / hl;
+        const double R = (Wp - Wc[v]) / hr;
+        double r = R / L;
+        r = (r < 1.0) ? r : 1.0;
+        dMh[b][v] = ((r > 0.0) ? r : 0.0) * L;
+        sH[4 + 4 * b + v][ly][lx] = 0.5 * dMh[b][v] * hf;
+      }
+    }
+    double dtW[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int vel = b + 1;
+      const double* dW = dMh[b];
+      dtW[0] = dtW[0] - (Wc[vel] * dW[0] + Wc[0] * dW[vel]);
+      dtW[3] = dtW[3] - (Wc[vel] * dW[3] + gamma * Wc[3] * dW[vel]);
+      dtW[vel] = dtW[vel] - (Wc[vel] * dW[vel] + dW[3] / Wc[0]);
+      dtW[2] = dtW[2] - Wc[vel] * dW[2];
+    }
+    dtW[3] = dtW[3] * 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sH[v][ly][lx] = Wc[v] + 0.5 * dtW[v] * dti;
+  }
+  __syncthreads();
+
+  // 3. blended fluxes of the tile's faces, in the (now free) W buffer
+  double* bFx = sW;                       // [TY][TX+1][4]
+  double* bFy = sW + TY * (TX + 1) * 4;   // [TY+1][TX][4]
+  const long long nfx = (long long)Ny * (Nx + 1), nfy = (long long)(Ny + 1) * Nx;
+  for (int k = threadIdx.x; k < TY * (TX + 1); k += NT) {
+    const int fx = k % (TX + 1), ly = k / (TX + 1);
+    const int x = x0 + fx, y = y0 + ly;
+    if (x > Nx || y >= Ny) continue;
+    const long long fi = (long long)y * (Nx + 1) + x;
+    double F[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) F[v] = Fx[v * nfx + fi];
+    double th = nmax(sT[ly + 1][fx], sT[ly + 1][fx + 1]);
+    if (prm.godunov) th = 1.0;
+    if (!(th == 0.0)) {
+      double WL[4], WR[4], UL[4], UR[4], Ff[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        WL[v] = sH[v][ly + 1][fx] + sH[4 + v][ly + 1][fx];
+        WR[v] = sH[v][ly + 1][fx + 1] - sH[4 + v][ly + 1][fx + 1];
+      }
+      cons<2>(WL, UL, 0, gamma);
+      cons<2>(WR, UR, 0, gamma);
+      riemann<2, RS>(WL, WR, UL, UR, Ff, 0, gamma, prm.min_c2);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) F[v] = th * Ff[v] + (1.0 - th) * F[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bFx[(ly * (TX + 1) + fx) * 4 + v] = F[v];
+  }
+  for (int k = threadIdx.x; k < (TY + 1) * TX; k += NT) {
+    const int lx = k % TX, fy = k / TX;
+    const int x = x0 + lx, y = y0 + fy;
+    if (x >= Nx || y > Ny) continue;
+    const long long fi = (long long)y * Nx + x;
+    double F[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) F[v] = Fy[v * nfy + fi];
+    double th = nmax(sT[fy][lx + 1], sT[fy + 1][lx + 1]);
+    if (prm.godunov) th = 1.0;
+    if (!(th == 0.0)) {
+      double WL[4], WR[4], UL[4], UR[4], Ff[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        WL[v] = sH[v][fy][lx + 1] + sH[8 + v][fy][lx + 1];
+        WR[v] = sH[v][fy + 1][lx + 1] - sH[8 + v][fy + 1][lx + 1];
+      }
+      cons<2>(WL, UL, 1, gamma);
+      cons<2>(WR, UR, 1, gamma);
+      riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, prm.min_c2);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) F[v] = th * Ff[v] + (1.0 - th) * F[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bFy[(fy * TX + lx) * 4 + v] = F[v];
+  }
+  __syncthreads();
+
+  // 4. apply
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= Nx || y >= Ny) return;
+  const double dx = g.hfp[0][x + NGH], dy = g.hfp[1][y + NGH];
+  const long long ci = (long long)y * Nx + x;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const double u = Ucv[v * g.Nc + ci];
+    double d = u * 0.0;
+    d = d + (bFx[(ty * (TX + 1) + tx + 1) * 4 + v] - bFx[(ty * (TX + 1) + tx) * 4 + v]) / dx;
+    d = d + (bFy[((ty + 1) * TX + tx) * 4 + v] - bFy[(ty * TX + tx) * 4 + v]) / dy;
+    Uout[v * g.Nc + ci] = u - d * dti;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// High-order fluxes of time node t on the FV faces of both dimensions
+// (src/sdader_simulator.py:288-323, src/sd_simulator.py:208-212).  A block
+// owns EPB consecutive elements: their predictor slab of node t is staged in
+// LDS with coalesced loads, then one work item per (dimension, element, flux
+// point) evaluates the flux on the element's lines, averages it over the
+// transverse subcells and writes the face values.
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ void __launch_bounds__(256)
+k_ho2d(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
+       const double* __restrict__ Ffx, const double* __restrict__ Ffy,
+       double* __restrict__ Fhx, double* __restrict__ Fhy, int EPB) {
+  constexpr int V = 4, n = P + 1, nf = P + 2, NL = n * n;
+  extern __shared__ double slab[];  // [V][EPB][NL]
+  const long long e0 = (long long)blockIdx.x * EPB;
+  const int ne = (int)((g.NE - e0 < EPB) ? (g.NE - e0) : EPB);
+  for (int k = threadIdx.x; k < V * ne * NL; k += blockDim.x) {
+    const int v = k / (ne * NL), r = k % (ne * NL);
+    slab[v * EPB * NL + r] = Uad[((long long)(t * V + v) * g.NE + e0) * NL + r];
+  }
+  __syncthreads();
+  const int Nx = g.N[0], Ny = g.N[1];
+  for (int item = threadIdx.x; item < 2 * ne * nf; item += blockDim.x) {
+    const int a = item / (ne * nf);
+    const int rem = item % (ne * nf);
+    const int el = rem / nf, f = rem % nf;
+    const long long e = e0 + el;
+    const int cx = (int)(e % Nx), cy = (int)(e / Nx);
+    const int ca = (a == 0) ? cx : cy;
+    const int Na = (a == 0) ? Nx : Ny;
+    if (f == nf - 1 && ca != Na - 1) continue;
+    double Fl[n][V];
+    if (f == 0 || f == nf - 1) {
+      // element-face Riemann flux; layout (T, V, Na+1, E_perp, n)
+      const double* Ff = (a == 0) ? Ffx : Ffy;
+      const long long ep = (a == 0) ? cy : cx;
+      const long long NEp = (a == 0) ? Ny : Nx;
+      const long long plane = NEp * n;
+      const int fa = ca + (f == 0 ? 0 : 1);
+#pragma unroll
+      for (int lp = 0; lp < n; ++lp)
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          Fl[lp][v] = Ff[((long long)(t * V + v) * (Na + 1) + fa) * plane + ep * n + lp];
+    } else {
+#pragma unroll
+      for (int lp = 0; lp < n; ++lp) {
+        double M[V], W[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const double* u = slab + (v * EPB + el) * NL;
+          double acc = 0.0;
+#pragma unroll
+          for (int s = 0; s < n; ++s) {
+            const int l = (a == 0) ? (lp * n + s) : (s * n + lp);
+            acc += op.s2f[f][s] * u[l];
+          }
+          M[v] = acc;
+        }
+        prims<2>(M, W, a, prm.gamma);
+        flux<2>(W, Fl[lp], a, prm.gamma);
+      }
+    }
+    double* Fh = (a == 0) ? Fhx : Fhy;
+    const long long nxf = (a == 0) ? (long long)Nx * n + 1 : (long long)Nx * n;
+    const long long nface = nxf * ((a == 0) ? (long long)Ny * n : (long long)Ny * n + 1);
+#pragma unroll
+    for (int q = 0; q < n; ++q) {
+      double Fa[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        double acc = 0.0;
+#pragma unroll
+        for (int s = 0; s < n; ++s) acc += op.s2c[q][s] * Fl[s][v];
+        Fa[v] = acc;
+      }
+      const long long fx = (a == 0) ? (long long)cx * n + f : (long long)cx * n + q;
+      const long long fy = (a == 0) ? (long long)cy * n + q : (long long)cy * n + f;
+#pragma unroll
+      for (int v = 0; v < V; ++v) Fh[v * nface + fy * nxf + fx] = Fa[v];
+    }
+  }
+}
+
+}  // namespace fv2d
+}  // namespace spd

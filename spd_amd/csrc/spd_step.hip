@@ -864,6 +864,12 @@ k_ring(Geom g, const double* __restrict__ t1, double* __restrict__ theta) {
   theta[tid] = out;
 }
 
+}  // namespace spd
+
+#include "spd_fv2d.hpp"
+
+namespace spd {
+
 // ---------------------------------------------------------------------------
 // MUSCL-Hancock fallback flux on the FV faces of dimension a whose blending
 // coefficient max(theta_L, theta_R) is non-zero, blended into the HO flux in
@@ -1392,6 +1398,70 @@ template <int NDIM, int P> struct Stepper {
     }
   }
 
+  static int ho_epb() { return std::max(1, 32768 / (V * NL * (int)sizeof(double))); }
+
+  // One FV substep with the 2-D tiled kernels; the corrected state goes to the
+  // other state buffer and the buffers are swapped.
+  static void substep2d(spd_ctx* c, const Ops<P>& op, int i, double w, const double* dt,
+                        hipStream_t s) {
+    const int E = ho_epb();
+    const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
+    c->timed(KC_HOFLUX, s, [&] {
+      hipLaunchKernelGGL((fv2d::k_ho2d<P>), dim3(blocks), dim3(256),
+                         (size_t)V * E * NL * sizeof(double), s, op, c->g, c->prm, i, c->Uad,
+                         c->Fface[0], c->Fface[1], c->Fho[0], c->Fho[1], E);
+    });
+    if (!c->cfg.FB) {
+      c->timed(KC_APPLY, s, [&] {
+        hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
+                           c->Ucv, c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Ucv);
+      });
+      return;
+    }
+    c->timed(KC_APPLY, s, [&] {
+      hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g, c->Ucv,
+                         c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Unew);
+    });
+    if (c->need_gate) hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
+    const dim3 tiles((c->g.Ns[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ns[1] + fv2d::TY - 1) / fv2d::TY);
+    const dim3 gtiles((c->g.Ng[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ng[1] + fv2d::TY - 1) / fv2d::TY);
+    c->timed(KC_DETECT, s, [&] {
+      hipLaunchKernelGGL(fv2d::k_detect2d, tiles, dim3(fv2d::NT), 0, s, c->g, c->prm, c->Ucv,
+                         c->Unew, c->flags, c->need_gate ? c->any_flag : nullptr);
+    });
+    c->timed(KC_THETA1, s, [&] {
+      hipLaunchKernelGGL(fv2d::k_theta2d, gtiles, dim3(fv2d::NT), 0, s, c->g, c->prm, c->flags,
+                         c->Ucv, c->Unew, c->any_flag, c->theta);
+    });
+    tap(c, i, s);
+    c->timed(KC_FBFLUX, s, [&] {
+      if (c->cfg.riemann_fv == SPD_RS_HLLC)
+        hipLaunchKernelGGL(fv2d::k_fbcorrect2d<RS_HLLC>, tiles, dim3(fv2d::NT), 0, s, c->g,
+                           c->prm, c->Ucv, c->theta, c->Fho[0], c->Fho[1], dt, w, c->Unew);
+      else
+        hipLaunchKernelGGL(fv2d::k_fbcorrect2d<RS_LLF>, tiles, dim3(fv2d::NT), 0, s, c->g,
+                           c->prm, c->Ucv, c->theta, c->Fho[0], c->Fho[1], dt, w, c->Unew);
+    });
+    std::swap(c->Ucv, c->Unew);
+  }
+
+  static void tap(spd_ctx* c, int i, hipStream_t s) {
+    if (!(i == c->dbg_sub || c->dbg_sub == SPD_TAP_ALL)) return;
+    const long long k = (c->dbg_sub == SPD_TAP_ALL) ? i : 0;
+    double* aff_o[3] = {nullptr, nullptr, nullptr};
+    long long mx = std::max(c->g.Nc, c->g.Ngc);
+    for (int a = 0; a < NDIM; ++a) {
+      long long nface = 1;
+      for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == a ? 1 : 0);
+      if (c->dbg_aff[a]) aff_o[a] = c->dbg_aff[a] + k * nface;
+      mx = std::max(mx, nface);
+    }
+    hipLaunchKernelGGL((k_tap<NDIM>), dim3(grid_for(mx)), dim3(256), 0, s, c->g, c->flags,
+                       c->theta, c->dbg_tr ? c->dbg_tr + k * c->g.Nc : nullptr,
+                       c->dbg_th ? c->dbg_th + k * c->g.Ngc : nullptr, aff_o[0], aff_o[1],
+                       aff_o[2]);
+  }
+
   static int step(spd_ctx* c, double* Usp, double* Ucv_sd, double* Wcv_sd, const double* dt,
                   int snapshot, hipStream_t s) {
     const Ops<P> op = make_ops<P>(*c->ops);
@@ -1415,6 +1485,10 @@ template <int NDIM, int P> struct Stepper {
     });
     for (int i = 0; i < T; ++i) {
       const double w = c->ops->w_tp[i];
+      if constexpr (NDIM == 2) {
+        substep2d(c, op, i, w, dt, s);
+        continue;
+      }
       for (int a = 0; a < NDIM; ++a)
         c->timed(KC_HOFLUX, s, [&] {
           hipLaunchKernelGGL((k_hoflux<NDIM, P>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s,
@@ -1441,26 +1515,7 @@ template <int NDIM, int P> struct Stepper {
             hipLaunchKernelGGL((k_ring<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
                                c->theta1, c->theta);
           });
-        if (i == c->dbg_sub || c->dbg_sub == SPD_TAP_ALL) {
-          // with SPD_TAP_ALL the buffers hold T consecutive blocks
-          const long long k = (c->dbg_sub == SPD_TAP_ALL) ? i : 0;
-          double* aff_o[3] = {nullptr, nullptr, nullptr};
-          for (int a = 0; a < NDIM; ++a) {
-            long long nface = 1;
-            for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == a ? 1 : 0);
-            if (c->dbg_aff[a]) aff_o[a] = c->dbg_aff[a] + k * nface;
-          }
-          long long mx = std::max(c->g.Nc, c->g.Ngc);
-          for (int a = 0; a < NDIM; ++a) {
-            long long nface = 1;
-            for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == a ? 1 : 0);
-            mx = std::max(mx, nface);
-          }
-          hipLaunchKernelGGL((k_tap<NDIM>), dim3(grid_for(mx)), dim3(256), 0, s, c->g, c->flags,
-                             c->theta, c->dbg_tr ? c->dbg_tr + k * c->g.Nc : nullptr,
-                             c->dbg_th ? c->dbg_th + k * c->g.Ngc : nullptr, aff_o[0], aff_o[1],
-                             aff_o[2]);
-        }
+        tap(c, i, s);
         if (c->cfg.riemann_fv == SPD_RS_HLLC)
           fb_rs<RS_HLLC>(c, dt, w, s);
         else
