@@ -460,17 +460,17 @@ k_ho2d(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
     if (f == nf - 1 && ca != Na - 1) continue;
     double Fl[n][V];
     if (f == 0 || f == nf - 1) {
-      // element-face Riemann flux; layout (T, V, Na+1, E_perp, n)
+      // element-face Riemann flux on the face grid of dimension a
       const double* Ff = (a == 0) ? Ffx : Ffy;
-      const long long ep = (a == 0) ? cy : cx;
-      const long long NEp = (a == 0) ? Ny : Nx;
-      const long long plane = NEp * n;
       const int fa = ca + (f == 0 ? 0 : 1);
+      const long long plane = (a == 0) ? (long long)(Nx + 1) * Ny * n : (long long)Nx * n * (Ny + 1);
 #pragma unroll
-      for (int lp = 0; lp < n; ++lp)
+      for (int lp = 0; lp < n; ++lp) {
+        const long long i0 = (a == 0) ? ((long long)cy * n + lp) * (Nx + 1) + fa
+                                      : (long long)fa * Nx * n + (long long)cx * n + lp;
 #pragma unroll
-        for (int v = 0; v < V; ++v)
-          Fl[lp][v] = Ff[((long long)(t * V + v) * (Na + 1) + fa) * plane + ep * n + lp];
+        for (int v = 0; v < V; ++v) Fl[lp][v] = Ff[(long long)(t * V + v) * plane + i0];
+      }
     } else {
 #pragma unroll
       for (int lp = 0; lp < n; ++lp) {

@@ -60,27 +60,6 @@ __device__ __forceinline__ long long elem_index(const Geom& g, const int* c) {
   for (int a = NDIM - 1; a >= 0; --a) e = e * g.N[a] + c[a];
   return e;
 }
-// element coordinates from (perpendicular element index, coordinate along a)
-template <int NDIM>
-__device__ __forceinline__ void elem_from_perp(const Geom& g, int a, long long ep, int ea, int* c) {
-#pragma unroll
-  for (int b = 0; b < NDIM; ++b) {
-    if (b == a) continue;
-    c[b] = (int)(ep % g.N[b]);
-    ep /= g.N[b];
-  }
-  c[a] = ea;
-}
-template <int NDIM>
-__device__ __forceinline__ long long perp_index(const Geom& g, int a, const int* c) {
-  long long r = 0;
-#pragma unroll
-  for (int b = NDIM - 1; b >= 0; --b) {
-    if (b == a) continue;
-    r = r * g.N[b] + c[b];
-  }
-  return r;
-}
 // local point index of the line `lp` (local coordinates other than a) at
 // position 0 along a; the stride along a is n^a.
 template <int NDIM, int n>
@@ -118,6 +97,36 @@ __device__ __forceinline__ double block_max(double v) {
   return v;
 }
 
+// Element-face arrays use a "face grid" per dimension a: coordinates q_b =
+// E_b * n + l_b for b != a (transverse solution point) and q_a = face index in
+// [0, N_a]; x fastest, then the variable, then the time node:
+//   idx = ((t * V + v) * prod_b ext_b) + sum_b q_b * stride_b
+template <int NDIM, int n>
+__device__ __forceinline__ long long face_grid_base(const Geom& g, int a, const int* ce,
+                                                    const int* cl, int f, long long* plane) {
+  long long idx = 0, tot = 1;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b) {
+    const long long ext = (b == a) ? (long long)g.N[a] + 1 : (long long)g.N[b] * n;
+    const long long q = (b == a) ? f : (long long)ce[b] * n + cl[b];
+    idx = idx * ext + q;
+    tot *= ext;
+  }
+  *plane = tot;
+  return idx;
+}
+
+template <int NDIM, int n>
+__device__ __forceinline__ long long stride_face(const Geom& g, int a) {
+  long long st = 1;
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    if (b == a) break;
+    st *= (long long)g.N[b] * n;
+  }
+  return st;
+}
+
 // ---------------------------------------------------------------------------
 // Predictor: Riemann problems at element faces for every time node
 // (src/sdader_simulator.py:213-240, src/sd_boundary.py:7-76).
@@ -128,25 +137,43 @@ __global__ void __launch_bounds__(256)
 k_face_sd(Ops<P> op, Geom g, Params prm, int a, const double* __restrict__ src, int Tsrc,
           double* __restrict__ Fface) {
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
-  constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
+  constexpr int NL = IPow<n, NDIM>::v;
   const int Na = g.N[a];
-  const long long NEp = g.NE / Na;
-  const long long per_t = (long long)(Na + 1) * NEp * NLP;
+  long long ext[3] = {1, 1, 1}, plane = 1;
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    ext[b] = (b == a) ? (long long)Na + 1 : (long long)g.N[b] * n;
+    plane *= ext[b];
+  }
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= per_t * Tsrc) return;
-  const int lp = (int)(tid % NLP);
-  long long r = tid / NLP;
-  const long long ep = r % NEp;
-  r /= NEp;
-  const int f = (int)(r % (Na + 1));
-  const int t = (int)(r / (Na + 1));
-  const int lb = line_base<NDIM, n>(a, lp);
+  if (tid >= plane * Tsrc) return;
+  const int t = (int)(tid / plane);
+  const long long pidx = tid % plane;
+  int ce[3] = {0, 0, 0}, cl[3] = {0, 0, 0};
+  int f = 0;
+  {
+    long long x = pidx;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      const int q = (int)(x % ext[b]);
+      x /= ext[b];
+      if (b == a) {
+        f = q;
+      } else {
+        ce[b] = q / n;
+        cl[b] = q % n;
+      }
+    }
+  }
+  int lb = 0;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b) lb = lb * n + ((b == a) ? 0 : cl[b]);
   const int sa = stride_of<n>(a);
   const double* base = src + (long long)t * V * g.NE * NL;
 
   auto trace = [&](int ea, int fp, double* q) {
-    int c[3];
-    elem_from_perp<NDIM>(g, a, ep, ea, c);
+    int c[3] = {ce[0], ce[1], ce[2]};
+    c[a] = ea;
     const long long e = elem_index<NDIM>(g, c);
 #pragma unroll
     for (int v = 0; v < V; ++v) {
@@ -184,10 +211,8 @@ k_face_sd(Ops<P> op, Geom g, Params prm, int a, const double* __restrict__ src, 
   prims<NDIM>(qL, WL, a, prm.gamma);
   prims<NDIM>(qR, WR, a, prm.gamma);
   riemann<NDIM, RS>(WL, WR, qL, qR, F, a, prm.gamma, prm.min_c2);
-  const long long plane = NEp * NLP;
 #pragma unroll
-  for (int v = 0; v < V; ++v)
-    Fface[((long long)(t * V + v) * (Na + 1) + f) * plane + ep * NLP + lp] = F[v];
+  for (int v = 0; v < V; ++v) Fface[(long long)(t * V + v) * plane + pidx] = F[v];
 }
 
 // ---------------------------------------------------------------------------
@@ -236,15 +261,24 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
         for (int s = 0; s < n; ++s) u[v][s] = s0[s * sa];
       }
       double F[nf][V];
-      const int Na = g.N[a];
-      const long long NEp = g.NE / Na;
-      const long long ep = perp_index<NDIM>(g, a, c);
-      const long long plane = NEp * NLP;
+      int cl[3] = {0, 0, 0};
+      {
+        int r = lp;
+#pragma unroll
+        for (int b = 0; b < NDIM; ++b) {
+          if (b == a) continue;
+          cl[b] = r % n;
+          r /= n;
+        }
+      }
+      long long plane;
+      const long long i0 = face_grid_base<NDIM, n>(g, a, c, cl, c[a], &plane);
+      const long long sfa = stride_face<NDIM, n>(g, a);
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const double* fb = Ff[a] + (long long)(t * V + v) * (Na + 1) * plane + ep * NLP + lp;
-        F[0][v] = fb[(long long)c[a] * plane];
-        F[nf - 1][v] = fb[(long long)(c[a] + 1) * plane];
+        const double* fb = Ff[a] + (long long)(t * V + v) * plane + i0;
+        F[0][v] = fb[0];
+        F[nf - 1][v] = fb[sfa];
       }
 #pragma unroll
       for (int fp = 1; fp < nf - 1; ++fp) {
@@ -331,18 +365,26 @@ k_hoflux(Ops<P> op, Geom g, Params prm, int a, int t, const double* __restrict__
   elem_coords<NDIM>(g, e, c);
   const int Na = g.N[a];
   if (f == nf - 1 && c[a] != Na - 1) return;
-  const long long NEp = g.NE / Na;
-  const long long ep = perp_index<NDIM>(g, a, c);
-  const long long plane = NEp * NLP;
   const int sa = stride_of<n>(a);
   double Fl[NLP][V];
 #pragma unroll
   for (int lp = 0; lp < NLP; ++lp) {
     if (f == 0 || f == nf - 1) {
       const int fa = c[a] + (f == 0 ? 0 : 1);
+      int cl[3] = {0, 0, 0};
+      {
+        int r = lp;
 #pragma unroll
-      for (int v = 0; v < V; ++v)
-        Fl[lp][v] = Fface[((long long)(t * V + v) * (Na + 1) + fa) * plane + ep * NLP + lp];
+        for (int b = 0; b < NDIM; ++b) {
+          if (b == a) continue;
+          cl[b] = r % n;
+          r /= n;
+        }
+      }
+      long long plane;
+      const long long i0 = face_grid_base<NDIM, n>(g, a, c, cl, fa, &plane);
+#pragma unroll
+      for (int v = 0; v < V; ++v) Fl[lp][v] = Fface[(long long)(t * V + v) * plane + i0];
     } else {
       const int lb = line_base<NDIM, n>(a, lp);
       double M[V], W[V];
