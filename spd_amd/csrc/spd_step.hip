@@ -132,6 +132,114 @@ __device__ __forceinline__ long long stride_face(const Geom& g, int a) {
 // (src/sdader_simulator.py:213-240, src/sd_boundary.py:7-76).
 // One thread per (t, face, perpendicular element, perpendicular line).
 // ---------------------------------------------------------------------------
+// x-faces (a = 0) with NDIM >= 2: a block takes FPB consecutive faces of all
+// NLP transverse lines of one element row, so that the lanes reading one
+// element's rows are adjacent; the fluxes are transposed through LDS and
+// written face-innermost.
+template <int NDIM, int P, int RS>
+__global__ void __launch_bounds__(256)
+k_face_sd_x(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
+            double* __restrict__ Fface) {
+  constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
+  constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
+  constexpr int FPB = 256 / NLP;
+  __shared__ double buf[V][NLP][FPB];
+  const int Na = g.N[0];
+  const int nchunk = (Na + 1 + FPB - 1) / FPB;
+  const long long NEp = g.NE / Na;
+  long long b = blockIdx.x;
+  const int fc = (int)(b % nchunk);
+  b /= nchunk;
+  const long long ep = b % NEp;
+  const int t = (int)(b / NEp);
+  const int lp = threadIdx.x % NLP, fi = threadIdx.x / NLP;
+  const int f = fc * FPB + fi;
+  int ce[3] = {0, 0, 0}, cl[3] = {0, 0, 0};
+  {
+    long long x = ep;
+#pragma unroll
+    for (int d = 1; d < NDIM; ++d) {
+      ce[d] = (int)(x % g.N[d]);
+      x /= g.N[d];
+    }
+    int r = lp;
+#pragma unroll
+    for (int d = 1; d < NDIM; ++d) {
+      cl[d] = r % n;
+      r /= n;
+    }
+  }
+  int lb = 0;
+#pragma unroll
+  for (int d = NDIM - 1; d >= 1; --d) lb = lb * n + cl[d];
+  const double* base = src + (long long)t * V * g.NE * NL;
+  auto trace = [&](int ea, int fp, double* q) {
+    int c[3] = {ea, ce[1], ce[2]};
+    const long long e = elem_index<NDIM>(g, c);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const double* u = base + ((long long)v * g.NE + e) * NL + lb;
+      double acc = 0.0;
+#pragma unroll
+      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[s];
+      q[v] = acc;
+    }
+  };
+  if (f <= Na) {
+    double qL[V], qR[V];
+    if (f > 0) trace(f - 1, nf - 1, qL);
+    if (f < Na) trace(f, 0, qR);
+    if (f == 0) {
+      const int k = g.bc[0][0];
+      if (k == BC_PERIODIC) {
+        trace(Na - 1, nf - 1, qL);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) qL[v] = qR[v];
+        if (k == BC_REFLECTIVE) qL[1] = -qL[1];
+      }
+    }
+    if (f == Na) {
+      const int k = g.bc[0][1];
+      if (k == BC_PERIODIC) {
+        trace(0, 0, qR);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) qR[v] = qL[v];
+        if (k == BC_REFLECTIVE) qR[1] = -qR[1];
+      }
+    }
+    double WL[V], WR[V], F[V];
+    prims<NDIM>(qL, WL, 0, prm.gamma);
+    prims<NDIM>(qR, WR, 0, prm.gamma);
+    riemann<NDIM, RS>(WL, WR, qL, qR, F, 0, prm.gamma, prm.min_c2);
+#pragma unroll
+    for (int v = 0; v < V; ++v) buf[v][lp][fi] = F[v];
+  }
+  __syncthreads();
+  // write: face grid of x, idx = (row) * (Na+1) + f with row = transverse coordinate
+  long long plane = 1;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) plane *= (d == 0) ? (long long)Na + 1 : (long long)g.N[d] * n;
+  for (int k = threadIdx.x; k < V * NLP * FPB; k += 256) {
+    const int v = k / (NLP * FPB), r = k % (NLP * FPB);
+    const int l = r / FPB, fo = r % FPB;
+    const int ff = fc * FPB + fo;
+    if (ff > Na) continue;
+    int cq[3] = {0, 0, 0};
+    int rr = l;
+#pragma unroll
+    for (int d = 1; d < NDIM; ++d) {
+      cq[d] = rr % n;
+      rr /= n;
+    }
+    long long row = 0;
+#pragma unroll
+    for (int d = NDIM - 1; d >= 1; --d) row = row * ((long long)g.N[d] * n) + (long long)ce[d] * n + cq[d];
+    Fface[(long long)(t * V + v) * plane + row * (Na + 1) + ff] = buf[v][l][fo];
+  }
+}
+
 template <int NDIM, int P, int RS>
 __global__ void __launch_bounds__(256)
 k_face_sd(Ops<P> op, Geom g, Params prm, int a, const double* __restrict__ src, int Tsrc,
@@ -237,7 +345,18 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
        double* out, int mode, int EPB) {
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2, T = P + 1;
   constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
-  extern __shared__ double lds[];  // [EPB][T][V][NL]
+  extern __shared__ double lds[];  // dudt [EPB][T][V][NL], then source [Tsrc*V][EPB][NL]
+  double* us = lds + (long long)EPB * T * V * NL;
+  {
+    const long long e0 = (long long)blockIdx.x * EPB;
+    const int ne = (int)((g.NE - e0 < EPB) ? (g.NE - e0) : EPB);
+    const int chunk = ne * NL;
+    for (int k = threadIdx.x; k < Tsrc * V * chunk; k += blockDim.x) {
+      const int tv = k / chunk, r = k % chunk;
+      us[(long long)tv * EPB * NL + r] = src[((long long)tv * g.NE + e0) * NL + r];
+    }
+    __syncthreads();
+  }
   const int el = threadIdx.x / (T * NLP);
   const int rem = threadIdx.x % (T * NLP);
   const int t = rem / NLP;
@@ -256,7 +375,7 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
       double u[V][n];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const double* s0 = src + ((long long)(t * V + v) * g.NE + e) * NL + lb;
+        const double* s0 = us + ((long long)(t * V + v) * EPB + el) * NL + lb;
 #pragma unroll
         for (int s = 0; s < n; ++s) u[v][s] = s0[s * sa];
       }
@@ -1389,14 +1508,23 @@ template <int NDIM, int P> struct Stepper {
   static_assert(T * NLP <= 256, "one element's lines must fit one block");
   static int epb() {
     int e = std::max(1, 256 / (T * NLP));
-    const int lds_per = T * V * NL * (int)sizeof(double);
+    const int lds_per = 2 * T * V * NL * (int)sizeof(double);  // dudt + staged source
     e = std::max(1, std::min(e, 49152 / lds_per));
     return e;
   }
 
   template <int RS>
   static void faces_rs(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, hipStream_t s) {
-    for (int a = 0; a < NDIM; ++a) {
+    if constexpr (NDIM >= 2) {
+      constexpr int FPB = 256 / NLP;
+      const long long Na = c->g.N[0];
+      const long long nblk = (long long)Tsrc * (c->g.NE / Na) * ((Na + 1 + FPB - 1) / FPB);
+      c->timed(KC_FACE_SD, s, [&] {
+        hipLaunchKernelGGL((k_face_sd_x<NDIM, P, RS>), dim3((unsigned)nblk), dim3(256), 0, s, op,
+                           c->g, c->prm, src, Tsrc, c->Fface[0]);
+      });
+    }
+    for (int a = (NDIM >= 2 ? 1 : 0); a < NDIM; ++a) {
       const long long Na = c->g.N[a];
       const long long tot = (long long)Tsrc * (Na + 1) * (c->g.NE / Na) * NLP;
       c->timed(KC_FACE_SD, s, [&] {
@@ -1415,7 +1543,7 @@ template <int NDIM, int P> struct Stepper {
                    const double* dt, double* out, int mode, hipStream_t s) {
     const int E = epb();
     const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
-    const size_t lds = (size_t)E * T * V * NL * sizeof(double);
+    const size_t lds = (size_t)2 * E * T * V * NL * sizeof(double);
     c->timed(KC_PRED, s, [&] {
       hipLaunchKernelGGL((k_pred<NDIM, P>), dim3(blocks), dim3(E * T * NLP), lds, s, op, c->g,
                          c->prm, src, Tsrc, c->Fface[0], c->Fface[1], c->Fface[2], Usp, dt, out,
