@@ -169,9 +169,10 @@ k_face_sd_x(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int T
       r /= n;
     }
   }
-  int lb = 0;
+  int lb = 0;  // local index of point 0 of the line (x coordinate 0)
 #pragma unroll
   for (int d = NDIM - 1; d >= 1; --d) lb = lb * n + cl[d];
+  lb *= n;
   const double* base = src + (long long)t * V * g.NE * NL;
   auto trace = [&](int ea, int fp, double* q) {
     int c[3] = {ea, ce[1], ce[2]};
@@ -185,7 +186,7 @@ k_face_sd_x(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int T
       q[v] = acc;
     }
   };
-  if (f <= Na) {
+  if (fi < FPB && f <= Na) {
     double qL[V], qR[V];
     if (f > 0) trace(f - 1, nf - 1, qL);
     if (f < Na) trace(f, 0, qR);
