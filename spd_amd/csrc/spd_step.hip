@@ -346,18 +346,10 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
        double* out, int mode, int EPB) {
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2, T = P + 1;
   constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
-  extern __shared__ double lds[];  // dudt [EPB][T][V][NL], then source [Tsrc*V][EPB][NL]
-  double* us = lds + (long long)EPB * T * V * NL;
-  {
-    const long long e0 = (long long)blockIdx.x * EPB;
-    const int ne = (int)((g.NE - e0 < EPB) ? (g.NE - e0) : EPB);
-    const int chunk = ne * NL;
-    for (int k = threadIdx.x; k < Tsrc * V * chunk; k += blockDim.x) {
-      const int tv = k / chunk, r = k % chunk;
-      us[(long long)tv * EPB * NL + r] = src[((long long)tv * g.NE + e0) * NL + r];
-    }
-    __syncthreads();
-  }
+  // dudt [EPB][T] slabs of V*NL values, padded by one double so that lanes of
+  // different time nodes / elements fall into different LDS banks
+  constexpr int SLAB = V * NL + 1;
+  extern __shared__ double lds[];
   const int el = threadIdx.x / (T * NLP);
   const int rem = threadIdx.x % (T * NLP);
   const int t = rem / NLP;
@@ -376,7 +368,7 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
       double u[V][n];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const double* s0 = us + ((long long)(t * V + v) * EPB + el) * NL + lb;
+        const double* s0 = src + ((long long)(t * V + v) * g.NE + e) * NL + lb;
 #pragma unroll
         for (int s = 0; s < n; ++s) u[v][s] = s0[s * sa];
       }
@@ -414,7 +406,7 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
         flux<NDIM>(W, F[fp], a, prm.gamma);
       }
       const double ha = g.h[a];
-      double* d0 = lds + ((long long)(el * T + t) * V) * NL + lb;
+      double* d0 = lds + (long long)(el * T + t) * SLAB + lb;
 #pragma unroll
       for (int s = 0; s < n; ++s) {
 #pragma unroll
@@ -444,7 +436,7 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
     if (ee >= g.NE) continue;
     const long long off = ((long long)v * g.NE + ee) * NL + l;
     const double u0 = Usp[off];
-    const double* dd = lds + ((long long)le * T * V + v) * NL + l;  // + p*V*NL
+    const double* dd = lds + (long long)le * T * SLAB + v * NL + l;  // + p*SLAB
     if (mode == 0) {
 #pragma unroll
       for (int tn = 0; tn < T; ++tn) {
@@ -452,14 +444,14 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
 #pragma unroll
         for (int p = 0; p < T; ++p) {
           const int ps = (p < Tsrc) ? p : (Tsrc - 1);
-          acc += op.inv[tn][p] * dd[(long long)ps * V * NL];
+          acc += op.inv[tn][p] * dd[(long long)ps * SLAB];
         }
         out[((long long)(tn * V + v) * g.NE + ee) * NL + l] = u0 - acc * dt;
       }
     } else {
       double acc = 0.0;
 #pragma unroll
-      for (int p = 0; p < T; ++p) acc += op.w[p] * dd[(long long)p * V * NL];
+      for (int p = 0; p < T; ++p) acc += op.w[p] * dd[(long long)p * SLAB];
       out[off] = u0 - acc * dt;
     }
   }
@@ -1509,7 +1501,7 @@ template <int NDIM, int P> struct Stepper {
   static_assert(T * NLP <= 256, "one element's lines must fit one block");
   static int epb() {
     int e = std::max(1, 256 / (T * NLP));
-    const int lds_per = 2 * T * V * NL * (int)sizeof(double);  // dudt + staged source
+    const int lds_per = T * (V * NL + 1) * (int)sizeof(double);
     e = std::max(1, std::min(e, 49152 / lds_per));
     return e;
   }
@@ -1544,7 +1536,7 @@ template <int NDIM, int P> struct Stepper {
                    const double* dt, double* out, int mode, hipStream_t s) {
     const int E = epb();
     const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
-    const size_t lds = (size_t)2 * E * T * V * NL * sizeof(double);
+    const size_t lds = (size_t)E * T * (V * NL + 1) * sizeof(double);
     c->timed(KC_PRED, s, [&] {
       hipLaunchKernelGGL((k_pred<NDIM, P>), dim3(blocks), dim3(E * T * NLP), lds, s, op, c->g,
                          c->prm, src, Tsrc, c->Fface[0], c->Fface[1], c->Fface[2], Usp, dt, out,
