@@ -434,6 +434,12 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
 // point) evaluates the flux on the element's lines, averages it over the
 // transverse subcells and writes the face values.
 // ---------------------------------------------------------------------------
+template <int P, int A>
+__device__ __forceinline__ void ho2d_items(const Ops<P>& op, const Geom& g, const Params& prm,
+                                           int t, const double* __restrict__ Ff,
+                                           double* __restrict__ Fh, const double* slab,
+                                           long long e0, int ne, int EPB);
+
 template <int P>
 __global__ void __launch_bounds__(256)
 k_ho2d(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
@@ -448,10 +454,20 @@ k_ho2d(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
     slab[v * EPB * NL + r] = Uad[((long long)(t * V + v) * g.NE + e0) * NL + r];
   }
   __syncthreads();
+  ho2d_items<P, 0>(op, g, prm, t, Ffx, Fhx, slab, e0, ne, EPB);
+  ho2d_items<P, 1>(op, g, prm, t, Ffy, Fhy, slab, e0, ne, EPB);
+}
+
+template <int P, int A>
+__device__ __forceinline__ void ho2d_items(const Ops<P>& op, const Geom& g, const Params& prm,
+                                           int t, const double* __restrict__ Ff,
+                                           double* __restrict__ Fh, const double* slab,
+                                           long long e0, int ne, int EPB) {
+  constexpr int V = 4, n = P + 1, nf = P + 2, NL = n * n;
+  constexpr int a = A;
   const int Nx = g.N[0], Ny = g.N[1];
-  for (int item = threadIdx.x; item < 2 * ne * nf; item += blockDim.x) {
-    const int a = item / (ne * nf);
-    const int rem = item % (ne * nf);
+  for (int item = threadIdx.x; item < ne * nf; item += blockDim.x) {
+    const int rem = item;
     const int el = rem / nf, f = rem % nf;
     const long long e = e0 + el;
     const int cx = (int)(e % Nx), cy = (int)(e / Nx);
@@ -461,7 +477,6 @@ k_ho2d(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
     double Fl[n][V];
     if (f == 0 || f == nf - 1) {
       // element-face Riemann flux on the face grid of dimension a
-      const double* Ff = (a == 0) ? Ffx : Ffy;
       const int fa = ca + (f == 0 ? 0 : 1);
       const long long plane = (a == 0) ? (long long)(Nx + 1) * Ny * n : (long long)Nx * n * (Ny + 1);
 #pragma unroll
@@ -490,7 +505,6 @@ k_ho2d(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
         flux<2>(W, Fl[lp], a, prm.gamma);
       }
     }
-    double* Fh = (a == 0) ? Fhx : Fhy;
     const long long nxf = (a == 0) ? (long long)Nx * n + 1 : (long long)Nx * n;
     const long long nface = nxf * ((a == 0) ? (long long)Ny * n : (long long)Ny * n + 1);
 #pragma unroll

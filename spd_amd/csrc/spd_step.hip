@@ -241,10 +241,11 @@ k_face_sd_x(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int T
   }
 }
 
-template <int NDIM, int P, int RS>
+template <int NDIM, int P, int RS, int A>
 __global__ void __launch_bounds__(256)
-k_face_sd(Ops<P> op, Geom g, Params prm, int a, const double* __restrict__ src, int Tsrc,
+k_face_sd(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
           double* __restrict__ Fface) {
+  constexpr int a = A;
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
   constexpr int NL = IPow<n, NDIM>::v;
   const int Na = g.N[a];
@@ -360,7 +361,7 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
   int c[3] = {0, 0, 0};
   if (e < g.NE) elem_coords<NDIM>(g, e, c);
 
-#pragma unroll 1
+#pragma unroll
   for (int a = 0; a < NDIM; ++a) {
     if (active) {
       const int lb = line_base<NDIM, n>(a, lp);
@@ -463,10 +464,11 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
 // subcell extent in the transverse dimensions (src/sdader_simulator.py:288-323,
 // src/sd_simulator.py:208-212).  One thread per (element, flux point).
 // ---------------------------------------------------------------------------
-template <int NDIM, int P>
+template <int NDIM, int P, int A>
 __global__ void __launch_bounds__(256)
-k_hoflux(Ops<P> op, Geom g, Params prm, int a, int t, const double* __restrict__ Uad,
+k_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
          const double* __restrict__ Fface, double* __restrict__ Fho) {
+  constexpr int a = A;
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
   constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1141,10 +1143,11 @@ k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __re
 
 // Fallback flux from the stored Hancock states at the faces of dimension a
 // with a non-zero coefficient, blended into the HO flux in place.
-template <int NDIM, int RS>
+template <int NDIM, int RS, int A>
 __global__ void __launch_bounds__(256)
-k_fbface(Geom g, Params prm, int a, const double* __restrict__ HS,
-         const double* __restrict__ theta, double* __restrict__ Fho) {
+k_fbface(Geom g, Params prm, const double* __restrict__ HS, const double* __restrict__ theta,
+         double* __restrict__ Fho) {
+  constexpr int a = A;
   constexpr int V = Dims<NDIM>::V;
   long long ext[3] = {1, 1, 1};
   long long nface = 1;
@@ -1506,6 +1509,17 @@ template <int NDIM, int P> struct Stepper {
     return e;
   }
 
+  template <int RS, int A>
+  static void face_generic(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc,
+                           hipStream_t s) {
+    const long long Na = c->g.N[A];
+    const long long tot = (long long)Tsrc * (Na + 1) * (c->g.NE / Na) * NLP;
+    c->timed(KC_FACE_SD, s, [&] {
+      hipLaunchKernelGGL((k_face_sd<NDIM, P, RS, A>), dim3(grid_for(tot)), dim3(256), 0, s, op,
+                         c->g, c->prm, src, Tsrc, c->Fface[A]);
+    });
+  }
+
   template <int RS>
   static void faces_rs(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, hipStream_t s) {
     if constexpr (NDIM >= 2) {
@@ -1517,14 +1531,9 @@ template <int NDIM, int P> struct Stepper {
                            c->g, c->prm, src, Tsrc, c->Fface[0]);
       });
     }
-    for (int a = (NDIM >= 2 ? 1 : 0); a < NDIM; ++a) {
-      const long long Na = c->g.N[a];
-      const long long tot = (long long)Tsrc * (Na + 1) * (c->g.NE / Na) * NLP;
-      c->timed(KC_FACE_SD, s, [&] {
-        hipLaunchKernelGGL((k_face_sd<NDIM, P, RS>), dim3(grid_for(tot)), dim3(256), 0, s, op,
-                           c->g, c->prm, a, src, Tsrc, c->Fface[a]);
-      });
-    }
+    if constexpr (NDIM == 1) face_generic<RS, 0>(c, op, src, Tsrc, s);
+    if constexpr (NDIM >= 2) face_generic<RS, 1>(c, op, src, Tsrc, s);
+    if constexpr (NDIM >= 3) face_generic<RS, 2>(c, op, src, Tsrc, s);
   }
   static void faces(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, hipStream_t s) {
     if (c->cfg.riemann_sd == SPD_RS_HLLC)
@@ -1551,14 +1560,25 @@ template <int NDIM, int P> struct Stepper {
       hipLaunchKernelGGL((k_hancock<NDIM>), dim3(grid_for(n1)), dim3(256), 0, s, c->g, c->prm,
                          c->Ucv, c->theta, dt, w, c->HS);
     });
-    for (int a = 0; a < NDIM; ++a) {
-      long long nface = 1;
-      for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == a ? 1 : 0);
-      c->timed(KC_FBFLUX, s, [&] {
-        hipLaunchKernelGGL((k_fbface<NDIM, RS>), dim3(grid_for(nface)), dim3(256), 0, s, c->g,
-                           c->prm, a, c->HS, c->theta, c->Fho[a]);
-      });
-    }
+    fbface_one<RS, 0>(c, s);
+    if constexpr (NDIM >= 2) fbface_one<RS, 1>(c, s);
+    if constexpr (NDIM >= 3) fbface_one<RS, 2>(c, s);
+  }
+  template <int RS, int A> static void fbface_one(spd_ctx* c, hipStream_t s) {
+    long long nface = 1;
+    for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == A ? 1 : 0);
+    c->timed(KC_FBFLUX, s, [&] {
+      hipLaunchKernelGGL((k_fbface<NDIM, RS, A>), dim3(grid_for(nface)), dim3(256), 0, s, c->g,
+                         c->prm, c->HS, c->theta, c->Fho[A]);
+    });
+  }
+
+  template <int A>
+  static void hoflux_generic(spd_ctx* c, const Ops<P>& op, int i, hipStream_t s) {
+    c->timed(KC_HOFLUX, s, [&] {
+      hipLaunchKernelGGL((k_hoflux<NDIM, P, A>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s, op,
+                         c->g, c->prm, i, c->Uad, c->Fface[A], c->Fho[A]);
+    });
   }
 
   static int ho_epb() { return std::max(1, 32768 / (V * NL * (int)sizeof(double))); }
@@ -1652,11 +1672,9 @@ template <int NDIM, int P> struct Stepper {
         substep2d(c, op, i, w, dt, s);
         continue;
       }
-      for (int a = 0; a < NDIM; ++a)
-        c->timed(KC_HOFLUX, s, [&] {
-          hipLaunchKernelGGL((k_hoflux<NDIM, P>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s,
-                             op, c->g, c->prm, a, i, c->Uad, c->Fface[a], c->Fho[a]);
-        });
+      hoflux_generic<0>(c, op, i, s);
+      if constexpr (NDIM >= 2) hoflux_generic<1>(c, op, i, s);
+      if constexpr (NDIM >= 3) hoflux_generic<2>(c, op, i, s);
       if (c->cfg.FB) {
         c->timed(KC_APPLY, s, [&] {
           hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
