@@ -137,18 +137,23 @@ __device__ __forceinline__ void hllc(const double* WL, const double* WR, const d
   const double sR = ((vL > vR) ? vL : vR) + cmax;
   const double rcL = WL[0] * (vL - sL);
   const double rcR = WR[0] * (sR - vR);
-  const double vs = (rcR * vR + rcL * vL + (WL[P] - WR[P])) / (rcR + rcL);
-  const double Ps = (rcR * WL[P] + rcL * WR[P] + rcL * rcR * (vL - vR)) / (rcR + rcL);
+  // one reciprocal per shared denominator (differs from two divisions by at
+  // most an ulp per value)
+  const double rden = 1.0 / (rcR + rcL);
+  const double vs = (rcR * vR + rcL * vL + (WL[P] - WR[P])) * rden;
+  const double Ps = (rcR * WL[P] + rcL * WR[P] + rcL * rcR * (vL - vR)) * rden;
   double r, v, p, e;
   if (sL > 0.0) {
     r = WL[0]; v = vL; p = WL[P]; e = UL[P];
   } else if (vs > 0.0) {
-    r = WL[0] * (sL - vL) / (sL - vs);
-    e = ((sL - vL) * UL[P] - WL[P] * vL + Ps * vs) / (sL - vs);
+    const double rs = 1.0 / (sL - vs);
+    r = WL[0] * (sL - vL) * rs;
+    e = ((sL - vL) * UL[P] - WL[P] * vL + Ps * vs) * rs;
     v = vs; p = Ps;
   } else if (sR > 0.0) {
-    r = WR[0] * (sR - vR) / (sR - vs);
-    e = ((sR - vR) * UR[P] - WR[P] * vR + Ps * vs) / (sR - vs);
+    const double rs = 1.0 / (sR - vs);
+    r = WR[0] * (sR - vR) * rs;
+    e = ((sR - vR) * UR[P] - WR[P] * vR + Ps * vs) * rs;
     v = vs; p = Ps;
   } else {
     r = WR[0]; v = vR; p = WR[P]; e = UR[P];
@@ -187,6 +192,10 @@ struct Geom {
   const double* centers[3];  // Ns + 4 cell centres
   const double* hfp[3];      // Ns + 4 cell widths
   const double* hcv[3];      // Ns + 3 centre spacings
+  const double* rhfp[3];     // 1 / hfp
+  const double* rhcv[3];     // 1 / hcv
+  const double* rdc[3];      // 1 / (centers[k+1] - centers[k-1]), k in [1, Ns+2]
+  double rh[3];              // 1 / h
   long long NE;   // number of elements
   long long Nc;   // number of active subcells
   long long Ngc;  // number of ghosted subcells

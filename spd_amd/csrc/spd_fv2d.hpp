@@ -33,14 +33,12 @@ constexpr int ONX = TX + 2, ONY = TY + 2;
 // w[k*stride] is its k-th neighbour along the axis (src/trouble_detection.py:
 // 104-137 with the compute_min edge rules).
 __device__ __forceinline__ double sed_alpha_lds(const double* w, int stride, int ca, int Ns,
-                                                const double* cen, const double* hf) {
+                                                const double* rdc, const double* hf) {
   double dU[5];
 #pragma unroll
   for (int k = -2; k <= 2; ++k) {
     const int x = ca + k;
-    dU[k + 2] = (x >= -1 && x <= Ns)
-                    ? (w[(k + 1) * stride] - w[(k - 1) * stride]) / (cen[x + 1] - cen[x - 1])
-                    : 0.0;
+    dU[k + 2] = (x >= -1 && x <= Ns) ? (w[(k + 1) * stride] - w[(k - 1) * stride]) * rdc[x] : 0.0;
   }
   double A[3], aLc = 1.0;
 #pragma unroll
@@ -50,7 +48,7 @@ __device__ __forceinline__ double sed_alpha_lds(const double* w, int stride, int
       A[k + 1] = 0.0;
       continue;
     }
-    const double d2U = (dU[k + 3] - dU[k + 1]) / (cen[x + 1] - cen[x - 1]);
+    const double d2U = (dU[k + 3] - dU[k + 1]) * rdc[x];
     const double dv = 0.5 * hf[x] * d2U;
     const double vL = dU[k + 1] - dU[k + 2];
     const double vR = dU[k + 3] - dU[k + 2];
@@ -151,9 +149,9 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
         if (!pt[var]) continue;
         const double* w = &wn[k][ty + DH][tx + DH];
         double alpha = 1.0;
-        const double ax = sed_alpha_lds(w, 1, x, Nx, g.centers[0] + NGH, g.hfp[0] + NGH);
+        const double ax = sed_alpha_lds(w, 1, x, Nx, g.rdc[0] + NGH, g.hfp[0] + NGH);
         alpha = (ax < alpha) ? ax : alpha;
-        const double ay = sed_alpha_lds(w, DNX, y, Ny, g.centers[1] + NGH, g.hfp[1] + NGH);
+        const double ay = sed_alpha_lds(w, DNX, y, Ny, g.rdc[1] + NGH, g.hfp[1] + NGH);
         alpha = (ay < alpha) ? ay : alpha;
         pt[var] = pt[var] && (alpha < 1.0);
       }
@@ -322,14 +320,14 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
     for (int b = 0; b < 2; ++b) {
       const int cb = (b == 0) ? x : y;
       const int G = cb + NGH;
-      const double hl = g.hcv[b][G - 1], hr = g.hcv[b][G];
+      const double rhl = g.rhcv[b][G - 1], rhr = g.rhcv[b][G];
       const double hf = g.hfp[b][G];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const double Wm = (b == 0) ? W[v][wy][wx - 1] : W[v][wy - 1][wx];
         const double Wp = (b == 0) ? W[v][wy][wx + 1] : W[v][wy + 1][wx];
-        const double L = (Wc[v] - Wm) / hl;
-        const double R = (Wp - Wc[v]) / hr;
+        const double L = (Wc[v] - Wm) * rhl;
+        const double R = (Wp - Wc[v]) * rhr;
         double r = R / L;
         r = (r < 1.0) ? r : 1.0;
         dMh[b][v] = ((r > 0.0) ? r : 0.0) * L;
@@ -414,14 +412,14 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int x = x0 + tx, y = y0 + ty;
   if (x >= Nx || y >= Ny) return;
-  const double dx = g.hfp[0][x + NGH], dy = g.hfp[1][y + NGH];
+  const double rdx = g.rhfp[0][x + NGH], rdy = g.rhfp[1][y + NGH];
   const long long ci = (long long)y * Nx + x;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const double u = Ucv[v * g.Nc + ci];
     double d = u * 0.0;
-    d = d + (bFx[(ty * (TX + 1) + tx + 1) * 4 + v] - bFx[(ty * (TX + 1) + tx) * 4 + v]) / dx;
-    d = d + (bFy[((ty + 1) * TX + tx) * 4 + v] - bFy[(ty * TX + tx) * 4 + v]) / dy;
+    d = d + (bFx[(ty * (TX + 1) + tx + 1) * 4 + v] - bFx[(ty * (TX + 1) + tx) * 4 + v]) * rdx;
+    d = d + (bFy[((ty + 1) * TX + tx) * 4 + v] - bFy[(ty * TX + tx) * 4 + v]) * rdy;
     Uout[v * g.Nc + ci] = u - d * dti;
   }
 }

@@ -197,6 +197,8 @@ struct OpsHost {
 // face positions (Ns+5), centres (Ns+4), widths (Ns+4), centre spacings (Ns+3).
 struct GeomHost {
   std::vector<double> faces, centers, hfp, hcv;
+  // reciprocals used by the kernels: 1/hfp, 1/hcv and 1/(c[k+1] - c[k-1])
+  std::vector<double> rhfp, rhcv, rdc;
   GeomHost(int N, double h, const std::vector<double>& fp_local, int ngh) {
     const int n = (int)fp_local.size() - 1;
     const int Ns = N * n;
@@ -216,6 +218,12 @@ struct GeomHost {
     }
     hcv.resize(L - 2);
     for (int k = 0; k < L - 2; ++k) hcv[k] = centers[k + 1] - centers[k];
+    rhfp.resize(hfp.size());
+    for (size_t k = 0; k < hfp.size(); ++k) rhfp[k] = 1.0 / hfp[k];
+    rhcv.resize(hcv.size());
+    for (size_t k = 0; k < hcv.size(); ++k) rhcv[k] = 1.0 / hcv[k];
+    rdc.assign(centers.size(), 0.0);
+    for (size_t k = 1; k + 1 < centers.size(); ++k) rdc[k] = 1.0 / (centers[k + 1] - centers[k - 1]);
   }
 };
 

@@ -406,7 +406,7 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
         prims<NDIM>(M, W, a, prm.gamma);
         flux<NDIM>(W, F[fp], a, prm.gamma);
       }
-      const double ha = g.h[a];
+      const double rha = g.rh[a];
       double* d0 = lds + (long long)(el * T + t) * SLAB + lb;
 #pragma unroll
       for (int s = 0; s < n; ++s) {
@@ -415,7 +415,7 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
           double acc = 0.0;
 #pragma unroll
           for (int fp = 0; fp < nf; ++fp) acc += op.d2s[s][fp] * F[fp][v];
-          const double d = acc / ha;
+          const double d = acc * rha;
           double* dst = d0 + v * NL + s * sa;
           if (a == 0)
             *dst = d;
@@ -676,7 +676,7 @@ k_apply(Geom g, const double* Uin, const double* __restrict__ F0, const double* 
     fi[a] = idx;
     fs[a] = sa;
     nf[a] = tot;
-    dx[a] = g.hfp[a][c[a] + NGH];
+    dx[a] = g.rhfp[a][c[a] + NGH];  // reciprocal width
   }
   const double dti = (*dt_dev) * w_i;
 #pragma unroll
@@ -686,7 +686,7 @@ k_apply(Geom g, const double* Uin, const double* __restrict__ F0, const double* 
 #pragma unroll
     for (int a = 0; a < NDIM; ++a) {
       const double* Fa = F[a] + v * nf[a] + fi[a];
-      d = d + (Fa[fs[a]] - Fa[0]) / dx[a];
+      d = d + (Fa[fs[a]] - Fa[0]) * dx[a];
     }
     Uout[v * g.Nc + cell] = u - d * dti;
   }
@@ -704,7 +704,7 @@ __device__ __forceinline__ double sed_alpha(const double* __restrict__ Unew, con
   constexpr int V = Dims<NDIM>::V;
   const int Ns = g.Ns[a];
   const int ca = c[a];
-  const double* cen = g.centers[a] + NGH;  // cen[k] = centre of active-relative cell k
+  const double* rdc = g.rdc[a] + NGH;  // 1 / (centre spacing over two cells), active-relative
   const double* hf = g.hfp[a] + NGH;
   double w[7];
 #pragma unroll
@@ -725,7 +725,7 @@ __device__ __forceinline__ double sed_alpha(const double* __restrict__ Unew, con
   for (int k = -2; k <= 2; ++k) {
     const int x = ca + k;
     if (x >= -1 && x <= Ns)
-      dU[k + 2] = (w[k + 4] - w[k + 2]) / (cen[x + 1] - cen[x - 1]);
+      dU[k + 2] = (w[k + 4] - w[k + 2]) * rdc[x];
     else
       dU[k + 2] = 0.0;
   }
@@ -737,7 +737,7 @@ __device__ __forceinline__ double sed_alpha(const double* __restrict__ Unew, con
       A[k + 1] = 0.0;
       continue;
     }
-    const double d2U = (dU[k + 3] - dU[k + 1]) / (cen[x + 1] - cen[x - 1]);
+    const double d2U = (dU[k + 3] - dU[k + 1]) * rdc[x];
     const double dv = 0.5 * hf[x] * d2U;
     const double vL = dU[k + 1] - dU[k + 2];
     const double vR = dU[k + 3] - dU[k + 2];
@@ -1101,12 +1101,12 @@ k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __re
       load_W<NDIM>(Ucv, g, cp, Wp, prm.gamma);
       load_W<NDIM>(Ucv, g, cm, Wm, prm.gamma);
       const int G = c[b] + NGH;
-      const double hl = g.hcv[b][G - 1], hr = g.hcv[b][G];
+      const double rhl = g.rhcv[b][G - 1], rhr = g.rhcv[b][G];
       const double hf = g.hfp[b][G];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const double L = (W[v] - Wm[v]) / hl;
-        const double R = (Wp[v] - W[v]) / hr;
+        const double L = (W[v] - Wm[v]) * rhl;
+        const double R = (Wp[v] - W[v]) * rhr;
         double r = R / L;
         r = (r < 1.0) ? r : 1.0;
         dMh[b][v] = ((r > 0.0) ? r : 0.0) * L;
@@ -1886,6 +1886,12 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
     gbuf.insert(gbuf.end(), gh.hfp.begin(), gh.hfp.end());
     offs[a].push_back(gbuf.size());
     gbuf.insert(gbuf.end(), gh.hcv.begin(), gh.hcv.end());
+    offs[a].push_back(gbuf.size());
+    gbuf.insert(gbuf.end(), gh.rhfp.begin(), gh.rhfp.end());
+    offs[a].push_back(gbuf.size());
+    gbuf.insert(gbuf.end(), gh.rhcv.begin(), gh.rhcv.end());
+    offs[a].push_back(gbuf.size());
+    gbuf.insert(gbuf.end(), gh.rdc.begin(), gh.rdc.end());
   }
   int rc;
   if ((rc = c->alloc(&c->geom_buf, gbuf.size())) != SPD_OK) return rc;
@@ -1897,9 +1903,14 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
       g.centers[a] = c->geom_buf + offs[a][1];
       g.hfp[a] = c->geom_buf + offs[a][2];
       g.hcv[a] = c->geom_buf + offs[a][3];
+      g.rhfp[a] = c->geom_buf + offs[a][4];
+      g.rhcv[a] = c->geom_buf + offs[a][5];
+      g.rdc[a] = c->geom_buf + offs[a][6];
     } else {
       g.faces[a] = g.centers[a] = g.hfp[a] = g.hcv[a] = c->geom_buf;
+      g.rhfp[a] = g.rhcv[a] = g.rdc[a] = c->geom_buf;
     }
+    g.rh[a] = 1.0 / g.h[a];
   }
   // scratch
   const long long V = c->V, T = c->T, NL = c->NL, NLP = c->NLP;
