@@ -121,6 +121,23 @@ int spd_compute_dt(spd_ctx* ctx, const double* W_cv, double* dt_dev, void* strea
 int spd_step(spd_ctx* ctx, double* U_sp, double* U_cv, double* W_cv, const double* dt_dev,
              int snapshot, void* stream);
 
+/* Device-side time loop (SDADER_Simulator.perform_time_evolution /
+ * perform_iterations, src/sdader_simulator.py:394-413).  spd_run_begin sets
+ * the device run state: current time and step count, t_end, a step budget
+ * (max_steps < 0: unlimited) and whether dt is clamped to reach t_end (with
+ * the reference's stop when the clamped dt < 1e-14).  spd_run_steps enqueues
+ * `nsteps` iterations of [dt from the last recorded maximum wave speed, clamp,
+ * spd_step, time += dt]; once the run is done the remaining iterations' kernels
+ * return at once, so a fixed chunk of steps can be captured in a HIP graph
+ * and replayed until spd_run_status reports done.  The first dt of a run uses
+ * the maximum recorded by spd_compute_dt (call it with the initial W_cv) or by
+ * the last step.  spd_run_status synchronises the stream and reads the state. */
+int spd_run_begin(spd_ctx* ctx, double time, double t_end, long long n_step, long long max_steps,
+                  int clamp, void* stream);
+int spd_run_steps(spd_ctx* ctx, double* U_sp, int nsteps, void* stream);
+int spd_run_status(spd_ctx* ctx, double* time, long long* n_step, int* done, double* dt,
+                   void* stream);
+
 /* Element-layout tensor-product transforms (src/transforms.py:27-45 via
  * SD_Simulator.compute_cv_from_sp / compute_sp_from_cv, src/sd_simulator.py:184-188). */
 int spd_sp_to_cv(spd_ctx* ctx, const double* in, double* out, void* stream);

@@ -25,6 +25,7 @@ EXPORTS = [
     "spd_sp_to_cv", "spd_cv_to_sp", "spd_cons_to_prims", "spd_prims_to_cons",
     "spd_set_debug_tap", "spd_host_operators", "spd_host_geometry",
     "spd_profile_begin", "spd_profile_end", "spd_kernel_class_count", "spd_kernel_class_name",
+    "spd_run_begin", "spd_run_steps", "spd_run_status",
 ]
 
 
@@ -110,6 +111,13 @@ def load():
     L.spd_kernel_class_count.restype = i
     L.spd_kernel_class_name.argtypes = [i]
     L.spd_kernel_class_name.restype = ctypes.c_char_p
+    L.spd_run_begin.argtypes = [vp, ctypes.c_double, ctypes.c_double, ll, ll, i, vp]
+    L.spd_run_begin.restype = i
+    L.spd_run_steps.argtypes = [vp, dp, i, vp]
+    L.spd_run_steps.restype = i
+    L.spd_run_status.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ll),
+                                 ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double), vp]
+    L.spd_run_status.restype = i
     _lib = L
     return L
 

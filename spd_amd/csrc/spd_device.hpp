@@ -196,10 +196,14 @@ struct Geom {
   const double* rhcv[3];     // 1 / hcv
   const double* rdc[3];      // 1 / (centers[k+1] - centers[k-1]), k in [1, Ns+2]
   double rh[3];              // 1 / h
+  const int* skip;  // device flag: when non-zero every step kernel returns at once
   long long NE;   // number of elements
   long long Nc;   // number of active subcells
   long long Ngc;  // number of ghosted subcells
 };
+
+#define SPD_SKIP(g) \
+  if (*(g).skip) return
 
 struct Params {
   double gamma, min_c2;

@@ -70,6 +70,7 @@ __device__ __forceinline__ double sed_alpha_lds(const double* w, int stride, int
 __global__ void __launch_bounds__(NT)
 k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ Unew,
            uint8_t* __restrict__ flags, int* __restrict__ any_flag) {
+  SPD_SKIP(g);
   __shared__ double wn[2][DNY][DNX];  // new rho, P
   __shared__ double wo[4][ONY][ONX];  // old primitives
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
@@ -181,6 +182,7 @@ __global__ void __launch_bounds__(NT)
 k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* __restrict__ Ucv,
           const double* __restrict__ Unew, const int* __restrict__ any_flag,
           double* __restrict__ theta) {
+  SPD_SKIP(g);
   __shared__ double tr[TNY][TNX];
   __shared__ double t1[TY + 2][TX + 2];
   const int gx0 = blockIdx.x * TX, gy0 = blockIdx.y * TY;
@@ -258,6 +260,7 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
               const double* __restrict__ theta, const double* __restrict__ Fx,
               const double* __restrict__ Fy, const double* __restrict__ dt_dev, double w_i,
               double* __restrict__ Uout) {
+  SPD_SKIP(g);
   // LDS: W (4 x WY x WX) is reused for the blended face fluxes after step 2
   constexpr int SW = (4 * WY * WX > 4 * (TY * (TX + 1) + (TY + 1) * TX))
                          ? 4 * WY * WX
@@ -443,6 +446,7 @@ __global__ void __launch_bounds__(256)
 k_ho2d(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
        const double* __restrict__ Ffx, const double* __restrict__ Ffy,
        double* __restrict__ Fhx, double* __restrict__ Fhy, int EPB) {
+  SPD_SKIP(g);
   constexpr int V = 4, n = P + 1, nf = P + 2, NL = n * n;
   extern __shared__ double slab[];  // [V][EPB][NL]
   const long long e0 = (long long)blockIdx.x * EPB;

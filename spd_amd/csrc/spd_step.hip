@@ -140,6 +140,7 @@ template <int NDIM, int P, int RS>
 __global__ void __launch_bounds__(256)
 k_face_sd_x(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
             double* __restrict__ Fface) {
+  SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
   constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
   constexpr int FPB = 256 / NLP;
@@ -245,6 +246,7 @@ template <int NDIM, int P, int RS, int A>
 __global__ void __launch_bounds__(256)
 k_face_sd(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
           double* __restrict__ Fface) {
+  SPD_SKIP(g);
   constexpr int a = A;
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
   constexpr int NL = IPow<n, NDIM>::v;
@@ -345,6 +347,7 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
        const double* __restrict__ Ff0, const double* __restrict__ Ff1,
        const double* __restrict__ Ff2, const double* Usp, const double* __restrict__ dt_dev,
        double* out, int mode, int EPB) {
+  SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2, T = P + 1;
   constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
   // dudt [EPB][T] slabs of V*NL values, padded by one double so that lanes of
@@ -468,6 +471,7 @@ template <int NDIM, int P, int A>
 __global__ void __launch_bounds__(256)
 k_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
          const double* __restrict__ Fface, double* __restrict__ Fho) {
+  SPD_SKIP(g);
   constexpr int a = A;
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
   constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
@@ -592,6 +596,7 @@ template <int NDIM, int P>
 __global__ void __launch_bounds__(256)
 k_tensor(Ops<P> op, Geom g, int to_cv, const double* __restrict__ in, double* __restrict__ out,
          int out_fv) {
+  SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V, n = P + 1;
   constexpr int NL = IPow<n, NDIM>::v;
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -647,6 +652,7 @@ __global__ void __launch_bounds__(256)
 k_apply(Geom g, const double* Uin, const double* __restrict__ F0, const double* __restrict__ F1,
         const double* __restrict__ F2, const double* __restrict__ dt_dev, double w_i,
         double* Uout) {
+  SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V;
   const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= g.Nc) return;
@@ -761,6 +767,7 @@ template <int NDIM>
 __global__ void __launch_bounds__(256)
 k_detect(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ Unew,
          uint8_t* __restrict__ flags, int* __restrict__ any_flag) {
+  SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V, P = Dims<NDIM>::P;
   const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   bool cand = false;
@@ -916,6 +923,7 @@ __global__ void __launch_bounds__(256)
 k_theta1(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* __restrict__ Ucv,
          const double* __restrict__ Unew, const int* __restrict__ any_flag,
          double* __restrict__ theta1) {
+  SPD_SKIP(g);
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= g.Ngc) return;
   int gx[3] = {0, 0, 0};
@@ -976,6 +984,7 @@ __device__ __forceinline__ double ring1(int i, int n, double lo, double mid, dou
 template <int NDIM>
 __global__ void __launch_bounds__(256)
 k_ring(Geom g, const double* __restrict__ t1, double* __restrict__ theta) {
+  SPD_SKIP(g);
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= g.Ngc) return;
   int gx[3] = {0, 0, 0};
@@ -1040,6 +1049,7 @@ template <int NDIM>
 __global__ void __launch_bounds__(256)
 k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ theta,
           const double* __restrict__ dt_dev, double w_i, double* __restrict__ HS) {
+  SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V;
   long long n1 = 1;
   int e1[3] = {1, 1, 1};
@@ -1147,6 +1157,7 @@ template <int NDIM, int RS, int A>
 __global__ void __launch_bounds__(256)
 k_fbface(Geom g, Params prm, const double* __restrict__ HS, const double* __restrict__ theta,
          double* __restrict__ Fho) {
+  SPD_SKIP(g);
   constexpr int a = A;
   constexpr int V = Dims<NDIM>::V;
   long long ext[3] = {1, 1, 1};
@@ -1206,6 +1217,7 @@ template <int NDIM, int P>
 __global__ void __launch_bounds__(256)
 k_cv2sp(Ops<P> op, Geom g, Params prm, const double* __restrict__ Ucv, double* __restrict__ Usp,
         double* __restrict__ Ucv_sd, double* __restrict__ Wcv_sd, double* __restrict__ partial) {
+  SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V, P_ = Dims<NDIM>::P, n = P + 1;
   constexpr int NL = IPow<n, NDIM>::v;
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1279,6 +1291,7 @@ k_cv2sp(Ops<P> op, Geom g, Params prm, const double* __restrict__ Ucv, double* _
 template <int NDIM>
 __global__ void __launch_bounds__(256)
 k_wave_max(Geom g, Params prm, int NL, const double* __restrict__ W, double* __restrict__ partial) {
+  SPD_SKIP(g);
   constexpr int P = Dims<NDIM>::P;
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long npts = g.NE * NL;
@@ -1294,7 +1307,8 @@ k_wave_max(Geom g, Params prm, int NL, const double* __restrict__ W, double* __r
 
 // Reduce the block partial maxima into *cmax (one block of 1024 threads).
 __global__ void __launch_bounds__(1024)
-k_reduce_max(const double* __restrict__ partial, long long n, double* cmax) {
+k_reduce_max(const double* __restrict__ partial, long long n, double* cmax, const int* skip) {
+  if (*skip) return;
   double v = 0.0;
   for (long long i = threadIdx.x; i < n; i += blockDim.x) v = nmax(v, partial[i]);
   v = block_max(v);
@@ -1303,6 +1317,70 @@ k_reduce_max(const double* __restrict__ partial, long long n, double* cmax) {
 
 __global__ void k_dt_final(const double* cmax, double h, double cfl, double* dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) dt[0] = cfl * (h / *cmax);
+}
+
+// Device-side time loop (src/sdader_simulator.py:394-413): time, dt, step
+// count and the stop condition live on the device so that chunks of steps can
+// be captured in a graph and replayed without host synchronisation.
+struct RunState {
+  double time, t_end, dt;
+  long long n_step;
+  long long n_left;  // steps still allowed (< 0: unlimited)
+  int done;
+  int clamp;         // clamp dt to reach t_end and stop when dt < 1e-14
+};
+
+__global__ void k_run_init(RunState* st, int* skip, double time, double t_end, long long n_step,
+                           long long n_left, int clamp) {
+  if (threadIdx.x || blockIdx.x) return;
+  st->time = time;
+  st->t_end = t_end;
+  st->dt = 0.0;
+  st->n_step = n_step;
+  st->n_left = n_left;
+  st->done = (n_left == 0) ? 1 : 0;
+  st->clamp = clamp;
+  *skip = st->done;
+}
+
+// dt of the next step from the recorded maximum wave speed, with the
+// reference's clamp to t_end and its dt < 1e-14 stop.
+__global__ void k_run_dt(const double* cmax, double h, double cfl, RunState* st, double* dt_out,
+                         int* skip) {
+  if (threadIdx.x || blockIdx.x) return;
+  if (st->done) {
+    *skip = 1;
+    return;
+  }
+  if (st->clamp && !(st->time < st->t_end)) {
+    st->done = 1;
+    *skip = 1;
+    return;
+  }
+  double dt = cfl * (h / *cmax);
+  if (st->clamp) {
+    if (st->time + dt >= st->t_end) dt = st->t_end - st->time;
+    if (dt < 1e-14) {
+      st->dt = dt;
+      st->done = 1;
+      *skip = 1;
+      return;
+    }
+  }
+  st->dt = dt;
+  *dt_out = dt;
+  *skip = 0;
+}
+
+__global__ void k_run_advance(RunState* st, const int* skip) {
+  if (threadIdx.x || blockIdx.x) return;
+  if (*skip) return;
+  st->time = st->time + st->dt;
+  st->n_step += 1;
+  if (st->n_left > 0) {
+    st->n_left -= 1;
+    if (st->n_left == 0) st->done = 1;
+  }
 }
 
 __global__ void k_zero_counters(int* any_flag) {
@@ -1341,6 +1419,7 @@ __global__ void k_cons_points(Params prm, const double* __restrict__ W, double* 
 template <int NDIM>
 __global__ void k_tap(Geom g, const uint8_t* __restrict__ flags, const double* __restrict__ theta,
                       double* tr_out, double* th_out, double* aff0, double* aff1, double* aff2) {
+  SPD_SKIP(g);
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tr_out && tid < g.Nc) tr_out[tid] = flags[tid];
   if (th_out && tid < g.Ngc) th_out[tid] = theta[tid];
@@ -1436,6 +1515,10 @@ struct spd_ctx {
   double* theta = nullptr;
   double* HS = nullptr;       // Hancock states and half slopes, (1+ndim)*V per cell
   int* any_flag = nullptr;
+  int* zero_flag = nullptr;   // always 0: the skip flag of ordinary calls
+  int* run_skip = nullptr;    // skip flag of device-loop steps
+  RunState* run = nullptr;
+  double* run_dt = nullptr;
   double* cmax = nullptr;     // max wave speed of the last state (for dt)
   double* partial = nullptr;  // per-block partial maxima
   long long npartial = 0;
@@ -1648,6 +1731,7 @@ template <int NDIM, int P> struct Stepper {
   static int step(spd_ctx* c, double* Usp, double* Ucv_sd, double* Wcv_sd, const double* dt,
                   int snapshot, hipStream_t s) {
     const Ops<P> op = make_ops<P>(*c->ops);
+    if (!c->g.skip) c->g.skip = c->zero_flag;
     const int m = c->cfg.m;
     // --- ADER predictor (src/sdader_simulator.py:160-200)
     faces(c, op, Usp, 1, s);
@@ -1713,7 +1797,8 @@ template <int NDIM, int P> struct Stepper {
                          Usp, snapshot ? Ucv_sd : nullptr, snapshot ? Wcv_sd : nullptr,
                          c->partial);
     });
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, c->partial, (long long)nb, c->cmax);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, c->partial, (long long)nb, c->cmax,
+                       c->g.skip);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(SPD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return SPD_OK;
@@ -1938,6 +2023,15 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
     }
   }
   if ((rc = c->alloc(&c->any_flag, 1)) != SPD_OK) return rc;
+  if ((rc = c->alloc(&c->zero_flag, 1)) != SPD_OK) return rc;
+  if ((rc = c->alloc(&c->run_skip, 1)) != SPD_OK) return rc;
+  if ((rc = c->alloc(&c->run, 1)) != SPD_OK) return rc;
+  if ((rc = c->alloc(&c->run_dt, 1)) != SPD_OK) return rc;
+  SPD_HIP_CHECK(hipMemset(c->zero_flag, 0, sizeof(int)));
+  SPD_HIP_CHECK(hipMemset(c->run_skip, 0, sizeof(int)));
+  SPD_HIP_CHECK(hipMemset(c->run, 0, sizeof(RunState)));
+  SPD_HIP_CHECK(hipMemset(c->run_dt, 0, sizeof(double)));
+  c->g.skip = c->zero_flag;
   if ((rc = c->alloc(&c->cmax, 1)) != SPD_OK) return rc;
   c->npartial = (g.NE * c->NL + 255) / 256;
   if ((rc = c->alloc(&c->partial, (size_t)c->npartial)) != SPD_OK) return rc;
@@ -1986,7 +2080,7 @@ int spd_compute_dt(spd_ctx* ctx, const double* W_cv, double* dt_dev, void* strea
                            W_cv, ctx->partial);
     }
     hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, ctx->partial, (long long)nb,
-                       ctx->cmax);
+                       ctx->cmax, ctx->zero_flag);
   }
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, s, ctx->cmax, ctx->h_dt, ctx->cfg.cfl_coeff,
                      dt_dev);
@@ -2001,6 +2095,45 @@ int spd_step(spd_ctx* ctx, double* U_sp, double* U_cv, double* W_cv, const doubl
     return fail(SPD_E_INVALID, "snapshot needs U_cv and W_cv");
   const Entry* e = find_entry(ctx->ndim, ctx->p);
   return e->step(ctx, U_sp, U_cv, W_cv, dt_dev, snapshot, (hipStream_t)stream);
+}
+
+int spd_run_begin(spd_ctx* ctx, double time, double t_end, long long n_step, long long max_steps,
+                  int clamp, void* stream) {
+  if (!ctx) return fail(SPD_E_INVALID, "null argument");
+  hipLaunchKernelGGL(k_run_init, dim3(1), dim3(64), 0, (hipStream_t)stream, ctx->run,
+                     ctx->run_skip, time, t_end, n_step, max_steps, clamp);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+}
+
+int spd_run_steps(spd_ctx* ctx, double* U_sp, int nsteps, void* stream) {
+  if (!ctx || !U_sp || nsteps < 0) return fail(SPD_E_INVALID, "invalid argument");
+  hipStream_t s = (hipStream_t)stream;
+  const Entry* en = find_entry(ctx->ndim, ctx->p);
+  for (int k = 0; k < nsteps; ++k) {
+    hipLaunchKernelGGL(k_run_dt, dim3(1), dim3(64), 0, s, ctx->cmax, ctx->h_dt, ctx->cfg.cfl_coeff,
+                       ctx->run, ctx->run_dt, ctx->run_skip);
+    ctx->g.skip = ctx->run_skip;
+    const int rc = en->step(ctx, U_sp, nullptr, nullptr, ctx->run_dt, 0, s);
+    ctx->g.skip = ctx->zero_flag;
+    if (rc != SPD_OK) return rc;
+    hipLaunchKernelGGL(k_run_advance, dim3(1), dim3(64), 0, s, ctx->run, ctx->run_skip);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+}
+
+int spd_run_status(spd_ctx* ctx, double* time, long long* n_step, int* done, double* dt,
+                   void* stream) {
+  if (!ctx) return fail(SPD_E_INVALID, "null argument");
+  RunState st;
+  SPD_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  SPD_HIP_CHECK(hipMemcpy(&st, ctx->run, sizeof(RunState), hipMemcpyDeviceToHost));
+  if (time) *time = st.time;
+  if (n_step) *n_step = st.n_step;
+  if (done) *done = st.done;
+  if (dt) *dt = st.dt;
+  return SPD_OK;
 }
 
 int spd_sp_to_cv(spd_ctx* ctx, const double* in, double* out, void* stream) {

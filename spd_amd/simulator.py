@@ -116,6 +116,13 @@ class SDADER_Simulator:
     ``isothermal``, ``"ic"``/``"eq"``/``"pressure"`` sides, ``moncen``,
     ``lhllc``, ``predictor=False``, ``m != p``, ``Nghc != 2``) raise
     ``NotImplementedError``.  ``device`` selects the GPU (default: current).
+
+    ``perform_time_evolution`` and ``perform_iterations`` run the time loop on
+    the device when ``device_loop`` is true: dt, the t_end clamp, the dt < 1e-14
+    stop, ``time`` and ``n_step`` are computed by kernels, and chunks of
+    ``graph_steps`` steps are captured once in a HIP graph and replayed (0: no
+    graph, the chunk's kernels are enqueued directly), with one host
+    synchronisation per chunk.  The numbers are those of the host loop.
     """
 
     def __init__(self, init_fct=_ic.sine_wave, eq_fct=_ic.sine_wave, p=1, m=-1, N=(32, 32),
@@ -126,7 +133,7 @@ class SDADER_Simulator:
                  riemann_solver_sd="llf", update="SD", riemann_solver_fv="llf",
                  slope_limiter="minmod", predictor=True, FB=False, tolerance=1e-5,
                  SED=True, NAD="", PAD=True, blending=True, min_rho=1e-10, max_rho=1e10,
-                 min_P=1e-10, godunov=False, device=None):
+                 min_P=1e-10, godunov=False, device=None, device_loop=True, graph_steps=16):
         if m == -1:
             m = p
         for flag, name in ((isothermal, "isothermal"), (viscosity, "viscosity"),
@@ -228,6 +235,9 @@ class SDADER_Simulator:
         self._dt_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
         self._dt_from_step = False   # the library holds the max wave speed of the state
         self._tap = None
+        self.device_loop, self.graph_steps = bool(device_loop), int(graph_steps)
+        self._graph = None           # (U_sp pointer, steps, torch.cuda.CUDAGraph)
+        self._profiling = False
 
         # operator matrices (host copies, as built by the library)
         n, nf, T = p + 1, p + 2, m + 1
@@ -259,6 +269,7 @@ class SDADER_Simulator:
 
     # -- lifetime ---------------------------------------------------------------
     def __del__(self):
+        self._graph = None
         ctx = getattr(self, "_ctx", None)
         if ctx is not None and ctx.value:
             try:
@@ -466,18 +477,77 @@ class SDADER_Simulator:
         if self.rank == 0:
             print(f"t={self.time}, steps taken {self.n_step}, time taken {self.execution_time}")
 
+    def _run_chunk(self, U_sp, k):
+        """Enqueue ``k`` device-loop steps: replay of a captured graph of
+        ``graph_steps`` steps when ``k`` equals it, else direct launches."""
+        s = self._stream
+        if k == self.graph_steps and k > 0 and not self._profiling:
+            key = (U_sp.data_ptr(), k)
+            if self._graph is None or self._graph[0] != key:
+                self._graph = None
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.device(self.device), torch.cuda.graph(g):
+                    _lib.check(self._lib.spd_run_steps(self._ctx, _lib.ptr(U_sp), k,
+                                                       self._stream), "spd_run_steps")
+                self._graph = (key, g)
+            self._graph[1].replay()
+        else:
+            _lib.check(self._lib.spd_run_steps(self._ctx, _lib.ptr(U_sp), k, s), "spd_run_steps")
+
+    def _run_on_device(self, t_end, max_steps, clamp):
+        """Device-side time loop (``src/sdader_simulator.py:394-413``); returns the
+        final dt.  ``max_steps`` < 0 means no step budget."""
+        self.compute_dt()                       # records the max wave speed of the state
+        U_sp = self.dm.device_value("U_sp")
+        _lib.check(self._lib.spd_run_begin(self._ctx, float(self.time), float(t_end),
+                                           int(self.n_step), int(max_steps), int(clamp),
+                                           self._stream), "spd_run_begin")
+        chunk = self.graph_steps if self.graph_steps > 0 else 16
+        t, n, done, dt = (ctypes.c_double(), ctypes.c_longlong(), ctypes.c_int(),
+                          ctypes.c_double())
+        left = max_steps
+        while True:
+            k = chunk if left < 0 else min(chunk, left)
+            if k <= 0:
+                break
+            before = self.n_step
+            self._run_chunk(U_sp, k)
+            _lib.check(self._lib.spd_run_status(self._ctx, ctypes.byref(t), ctypes.byref(n),
+                                                ctypes.byref(done), ctypes.byref(dt),
+                                                self._stream), "spd_run_status")
+            self.time, self.n_step, self.dm.dt = t.value, int(n.value), dt.value
+            if left >= 0:
+                left -= self.n_step - before
+            if self.verbose and self.rank == 0 and self.n_step // 100 > before // 100:
+                print(f"Time step #{self.n_step} (t = {self.time})", end="\r")
+            if done.value:
+                break
+        if self.update == "FV":
+            self._dt_from_step = True
+        return self.dm.dt
+
     def perform_iterations(self, n_step: int) -> None:
         """``src/sdader_simulator.py:394-399``."""
         self.init_sim()
-        for _ in range(n_step):
-            self.compute_dt()
-            self._step(snapshot=False)
+        if self.device_loop and n_step > 0:
+            self._run_on_device(0.0, n_step, 0)
+        else:
+            for _ in range(n_step):
+                self.compute_dt()
+                self._step(snapshot=False)
         self.end_sim()
 
     def perform_time_evolution(self, t_end: float, nsteps=0) -> None:
         """``src/sdader_simulator.py:401-413``: dt clamped to reach ``t_end``; the
         loop stops if dt < 1e-14."""
         self.init_sim()
+        if self.device_loop:
+            if self.time < t_end:
+                dt = self._run_on_device(t_end, -1, 1)
+                if dt < 1e-14:
+                    print(f"dt={dt}")
+            self.end_sim()
+            return
         while self.time < t_end:
             if not self.n_step % 100 and self.rank == 0 and self.verbose:
                 print(f"Time step #{self.n_step} (t = {self.time})", end="\r")
@@ -543,6 +613,7 @@ class SDADER_Simulator:
     def profile_begin(self, max_launches=100000):
         """Start HIP-event timing of every kernel launch of the following steps."""
         _lib.check(self._lib.spd_profile_begin(self._ctx, int(max_launches)), "spd_profile_begin")
+        self._profiling = True
 
     def profile_end(self):
         """Stop timing; return {kernel: (total_ms, launches)}."""
@@ -550,6 +621,7 @@ class SDADER_Simulator:
         ms = (ctypes.c_double * k)()
         cnt = (ctypes.c_longlong * k)()
         _lib.check(self._lib.spd_profile_end(self._ctx, ms, cnt, k), "spd_profile_end")
+        self._profiling = False
         out = {}
         for i in range(k):
             if cnt[i]:
