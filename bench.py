@@ -194,21 +194,37 @@ def main():
     N, p = args.N, args.p
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    s = SDADER_Simulator(init_fct=ic.kelvin_helmholtz, device=dev, verbose=False,
-                         **kh_kwargs(N, p))
-    s.perturb_initial_condition(ic.subcell_noise((N, N), p, [1, 2], 1e-3, seed=0))
     dof = (N * (p + 1)) ** 2
+    noise = ic.subcell_noise((N, N), p, [1, 2], 1e-3, seed=0)
 
-    s.advance(args.warmup)
+    def run_form(precision):
+        """Warm-up, then the timed steps (HIP events around every kernel), then
+        one more step that measures the troubled fraction."""
+        s = SDADER_Simulator(init_fct=ic.kelvin_helmholtz, device=dev, verbose=False,
+                             precision=precision, **kh_kwargs(N, p))
+        s.perturb_initial_condition(noise)
+        s.advance(args.warmup)
+        torch.cuda.synchronize()
+        s.profile_begin(max_launches=200 * max(args.steps, 1))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.advance(args.steps)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        prof = s.profile_end()
+        frac = s.troubled_fraction_next_step()
+        return s, elapsed, prof, frac
+
+    s, elapsed, prof, frac = run_form("fp64")
+    s32, elapsed32, prof32, frac32 = run_form("fp32")
+    # the two forms after the same number of steps from the same state
     torch.cuda.synchronize()
-    s.profile_begin(max_launches=200 * max(args.steps, 1))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s.advance(args.steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = s.profile_end()
-    frac = s.troubled_fraction_next_step()
+    r64 = s.dm.device_value("U_sp")
+    r32 = s32.dm.device_value("U_sp")
+    l1 = {name: float((r32[v] - r64[v]).abs().sum() / r64[v].abs().sum())
+          for name, v in (("rho", 0), ("E", 3))}
+    l1["momenta"] = float((r32[1:3] - r64[1:3]).abs().sum() / r64[1:3].abs().sum())
+    del s32, r32
 
     ms_step = elapsed / args.steps * 1e3
     value = dof * args.steps / elapsed
@@ -224,8 +240,9 @@ def main():
                    "GBps_alg": (bpl[k] / (v[0] / v[1] * 1e-3) / 1e9) if k in bpl else None}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
 
+    value32 = dof * args.steps / elapsed32
     out = {
-        "metric": "DOF-updates/s, 2D p=3 1024^2 KH (FP64)",
+        "metric": "DOF-updates/s, 2D p=3 1024\u00b2 KH: FP64 vs FP32-bulk/batched; %HBM roofline",
         "value": value,
         "unit": "DOF-updates/s",
         "n_gpus": 1,
@@ -248,6 +265,15 @@ def main():
         "copy_GBps": copy_gbps,
         "troubled_fraction": frac,
         "kernels": kernels,
+        "fp32_bulk": {
+            "value": value32, "unit": "DOF-updates/s", "ms_per_step": elapsed32 / args.steps * 1e3,
+            "speedup_vs_fp64": value32 / value,
+            "step_hbm_roofline_frac": value32 * 64.0 / (HBM_PEAK_GBPS * 1e9),
+            "l1_rel_vs_fp64": l1, "steps_compared": args.warmup + args.steps + 1,
+            "troubled_fraction": frac32,
+            "kernels_ms_per_step": {k: v[0] / args.steps
+                                    for k, v in sorted(prof32.items(), key=lambda kv: -kv[1][0])},
+        },
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline(args.cpu_N, args.cpu_steps, args.cpu_warmup)

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define SPD_ABI_VERSION 1
+#define SPD_ABI_VERSION 2
 
 enum spd_status {
   SPD_OK = 0,
@@ -41,6 +41,12 @@ enum spd_riemann { SPD_RS_LLF = 0, SPD_RS_HLLC = 1 };
 enum spd_bc { SPD_BC_PERIODIC = 0, SPD_BC_GRADFREE = 1, SPD_BC_REFLECTIVE = 2 };
 enum spd_update { SPD_UPDATE_SD = 0, SPD_UPDATE_FV = 1 };
 enum spd_nad { SPD_NAD_RELATIVE = 0, SPD_NAD_DELTA = 1 };
+/* Arithmetic of the step.  FP64: the reference's arithmetic (parity form).
+ * FP32: predictor, interpolation, element-face and fallback Riemann solvers and
+ * MUSCL-Hancock in FP32, with FP64 kept for the state, the conservative update
+ * accumulation, the pressure recovery from total energy, trouble detection
+ * and the dt reduction (an extension; no reference counterpart). */
+enum spd_precision { SPD_PRECISION_FP64 = 0, SPD_PRECISION_FP32 = 1 };
 
 /* Constructor keywords of SDADER_Simulator (src/sdader_simulator.py:15-27,
  * src/sd_simulator.py:20-26, src/fv_simulator.py:11-17, src/simulator.py:10-36)
@@ -68,6 +74,7 @@ typedef struct spd_config {
   int blending;
   double min_rho, max_rho, min_P;
   int godunov;
+  int precision;          /* spd_precision */
 } spd_config;
 
 typedef struct spd_ctx spd_ctx;

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libspd_hip.so")
 RS = {"llf": 0, "hllc": 1}
 BC = {"periodic": 0, "gradfree": 1, "reflective": 2}
 UPDATE = {"SD": 0, "FV": 1}
+PRECISION = {"fp64": 0, "fp32": 1}
 TAP_ALL = -2
 
 EXPORTS = [
@@ -54,6 +55,7 @@ class SpdConfig(ctypes.Structure):
         ("max_rho", ctypes.c_double),
         ("min_P", ctypes.c_double),
         ("godunov", ctypes.c_int),
+        ("precision", ctypes.c_int),
     ]
 
 

@@ -1,6 +1,7 @@
 // Device-side building blocks of the SD-ADER step: pointwise Euler algebra,
 // the two Riemann solvers, NaN-propagating extrema and the index arithmetic
-// of the element and finite-volume layouts.  Everything here is FP64.
+// of the element and finite-volume layouts.  The pointwise functions take the
+// arithmetic type of their caller: FP64, or FP32 in the FP32-bulk form.
 //
 // Variable order (both sets): 0 = density, 1..NDIM = momentum / velocity
 // components (x first), NDIM+1 = total energy / pressure.  A "rolled" velocity
@@ -26,13 +27,38 @@ template <int NDIM> struct Dims {
 template <int B, int E> struct IPow { static constexpr int v = B * IPow<B, E - 1>::v; };
 template <int B> struct IPow<B, 0> { static constexpr int v = 1; };
 
+// Bulk arithmetic type of the FP32 form.  The FP64 form keeps true divisions
+// and the correctly rounded square root; the FP32 form uses the hardware
+// reciprocal and square root (1 ulp).
+template <typename R> struct IsF32 { static constexpr bool v = false; };
+template <> struct IsF32<float> { static constexpr bool v = true; };
+
+template <typename R> __device__ __forceinline__ R qdiv(R a, R b) {
+  if constexpr (IsF32<R>::v)
+    return a * __builtin_amdgcn_rcpf(b);
+  else
+    return a / b;
+}
+template <typename R> __device__ __forceinline__ R qrcp(R b) {
+  if constexpr (IsF32<R>::v)
+    return __builtin_amdgcn_rcpf(b);
+  else
+    return R(1) / b;
+}
+template <typename R> __device__ __forceinline__ R qsqrt(R x) {
+  if constexpr (IsF32<R>::v)
+    return __builtin_amdgcn_sqrtf(x);
+  else
+    return sqrt(x);
+}
+
 // numpy.maximum / numpy.minimum: NaN in either operand propagates.
-__device__ __forceinline__ double nmax(double a, double b) {
+template <typename R> __device__ __forceinline__ R nmax(R a, R b) {
   if (a != a) return a;
   if (b != b) return b;
   return a > b ? a : b;
 }
-__device__ __forceinline__ double nmin(double a, double b) {
+template <typename R> __device__ __forceinline__ R nmin(R a, R b) {
   if (a != a) return a;
   if (b != b) return b;
   return a < b ? a : b;
@@ -43,46 +69,59 @@ template <int NDIM> __device__ __forceinline__ constexpr int rolled(int a, int k
   return 1 + (a + k) % NDIM;
 }
 
-// conservative -> primitive (src/hydro.py:48-86), rolled list of dim a.
-template <int NDIM>
-__device__ __forceinline__ void prims(const double* U, double* W, int a, double gamma) {
+// conservative -> primitive (src/hydro.py:48-86), rolled list of dim a.  The
+// FP32 form recovers the pressure from the total energy in FP64.
+template <int NDIM, typename R>
+__device__ __forceinline__ void prims(const R* U, R* W, int a, R gamma) {
   constexpr int P = Dims<NDIM>::P;
   W[0] = U[0];
-  double K = U[0] * 0.0;
+  if constexpr (IsF32<R>::v) {
+    const float r = __builtin_amdgcn_rcpf(U[0]);
+    double K = 0.0;
 #pragma unroll
-  for (int k = 0; k < NDIM; ++k) {
-    const int v = rolled<NDIM>(a, k);
-    W[v] = U[v] / U[0];
-    K = K + W[v] * W[v];
+    for (int k = 0; k < NDIM; ++k) {
+      const int v = rolled<NDIM>(a, k);
+      W[v] = U[v] * r;
+      K = K + (double)U[v] * (double)W[v];
+    }
+    W[P] = (float)(((double)gamma - 1.0) * ((double)U[P] - 0.5 * K));
+  } else {
+    double K = U[0] * 0.0;
+#pragma unroll
+    for (int k = 0; k < NDIM; ++k) {
+      const int v = rolled<NDIM>(a, k);
+      W[v] = U[v] / U[0];
+      K = K + W[v] * W[v];
+    }
+    K = K * (0.5 * U[0]);
+    W[P] = (gamma - 1.0) * (U[P] - K);
   }
-  K = K * (0.5 * U[0]);
-  W[P] = (gamma - 1.0) * (U[P] - K);
 }
 
 // primitive -> conservative (src/hydro.py:88-125), rolled list of dim a.
-template <int NDIM>
-__device__ __forceinline__ void cons(const double* W, double* U, int a, double gamma) {
+template <int NDIM, typename R>
+__device__ __forceinline__ void cons(const R* W, R* U, int a, R gamma) {
   constexpr int P = Dims<NDIM>::P;
   U[0] = W[0];
-  double K = W[0] * 0.0;
+  R K = W[0] * R(0);
 #pragma unroll
   for (int k = 0; k < NDIM; ++k) {
     const int v = rolled<NDIM>(a, k);
     U[v] = W[v] * U[0];
     K = K + W[v] * W[v];
   }
-  K = K * (0.5 * U[0]);
-  U[P] = W[P] / (gamma - 1.0) + K;
+  K = K * (R(0.5) * U[0]);
+  U[P] = qdiv<R>(W[P], gamma - R(1)) + K;
 }
 
 // Euler flux along dim a from primitives (src/hydro.py:127-170): the kinetic
 // sum runs over the rolled list in reverse, so the normal component is last.
-template <int NDIM>
-__device__ __forceinline__ void flux(const double* W, double* F, int a, double gamma) {
+template <int NDIM, typename R>
+__device__ __forceinline__ void flux(const R* W, R* F, int a, R gamma) {
   constexpr int P = Dims<NDIM>::P;
   const int v1 = a + 1;
-  double K = W[0] * 0.0;
-  double m = 0.0;
+  R K = W[0] * R(0);
+  R m = R(0);
 #pragma unroll
   for (int k = NDIM - 1; k >= 0; --k) {
     const int v = rolled<NDIM>(a, k);
@@ -90,68 +129,67 @@ __device__ __forceinline__ void flux(const double* W, double* F, int a, double g
     K = K + m * W[v];
     F[v] = m * W[v1];
   }
-  const double E = W[P] / (gamma - 1.0) + 0.5 * K;
+  const R E = qdiv<R>(W[P], gamma - R(1)) + R(0.5) * K;
   F[0] = m;
   F[v1] = m * W[v1] + W[P];
   F[P] = W[v1] * (E + W[P]);
 }
 
 // Sound speed with the where-clamp of src/hydro.py:3-46 (NaN -> min_c2).
-__device__ __forceinline__ double sound(double P, double rho, double gamma, double min_c2) {
-  double c2 = gamma * P / rho;
+template <typename R> __device__ __forceinline__ R sound(R P, R rho, R gamma, R min_c2) {
+  R c2 = qdiv<R>(gamma * P, rho);
   c2 = (c2 > min_c2) ? c2 : min_c2;
-  return sqrt(c2);
+  return qsqrt<R>(c2);
 }
 
 // Local Lax-Friedrichs flux (src/riemann_solver.py:31-65).
-template <int NDIM>
-__device__ __forceinline__ void llf(const double* WL, const double* WR, const double* UL,
-                                    const double* UR, double* F, int a, double gamma,
-                                    double min_c2) {
+template <int NDIM, typename R>
+__device__ __forceinline__ void llf(const R* WL, const R* WR, const R* UL, const R* UR, R* F,
+                                    int a, R gamma, R min_c2) {
   constexpr int V = Dims<NDIM>::V;
   constexpr int P = Dims<NDIM>::P;
   const int v1 = a + 1;
-  double FL[V], FR[V];
-  flux<NDIM>(WL, FL, a, gamma);
-  flux<NDIM>(WR, FR, a, gamma);
-  const double cL = sound(WL[P], WL[0], gamma, min_c2) + fabs(WL[v1]);
-  const double cR = sound(WR[P], WR[0], gamma, min_c2) + fabs(WR[v1]);
-  const double cmax = (cL > cR) ? cL : cR;
+  R FL[V], FR[V];
+  flux<NDIM, R>(WL, FL, a, gamma);
+  flux<NDIM, R>(WR, FR, a, gamma);
+  const R cL = sound<R>(WL[P], WL[0], gamma, min_c2) + fabs(WL[v1]);
+  const R cR = sound<R>(WR[P], WR[0], gamma, min_c2) + fabs(WR[v1]);
+  const R cmax = (cL > cR) ? cL : cR;
 #pragma unroll
-  for (int v = 0; v < V; ++v) F[v] = 0.5 * (FR[v] + FL[v]) - 0.5 * cmax * (UR[v] - UL[v]);
+  for (int v = 0; v < V; ++v)
+    F[v] = R(0.5) * (FR[v] + FL[v]) - R(0.5) * cmax * (UR[v] - UL[v]);
 }
 
 // HLLC flux in the Godunov-state form of src/riemann_solver.py:67-145.  Only
 // the selected star state is evaluated; its formula is the reference's.
-template <int NDIM>
-__device__ __forceinline__ void hllc(const double* WL, const double* WR, const double* UL,
-                                     const double* UR, double* F, int a, double gamma,
-                                     double min_c2) {
+template <int NDIM, typename R>
+__device__ __forceinline__ void hllc(const R* WL, const R* WR, const R* UL, const R* UR, R* F,
+                                     int a, R gamma, R min_c2) {
   constexpr int P = Dims<NDIM>::P;
   const int v1 = a + 1;
-  const double cL = sound(WL[P], WL[0], gamma, min_c2) + fabs(WL[v1]);
-  const double cR = sound(WR[P], WR[0], gamma, min_c2) + fabs(WR[v1]);
-  const double cmax = (cL > cR) ? cL : cR;
-  const double vL = WL[v1], vR = WR[v1];
-  const double sL = ((vL < vR) ? vL : vR) - cmax;
-  const double sR = ((vL > vR) ? vL : vR) + cmax;
-  const double rcL = WL[0] * (vL - sL);
-  const double rcR = WR[0] * (sR - vR);
+  const R cL = sound<R>(WL[P], WL[0], gamma, min_c2) + fabs(WL[v1]);
+  const R cR = sound<R>(WR[P], WR[0], gamma, min_c2) + fabs(WR[v1]);
+  const R cmax = (cL > cR) ? cL : cR;
+  const R vL = WL[v1], vR = WR[v1];
+  const R sL = ((vL < vR) ? vL : vR) - cmax;
+  const R sR = ((vL > vR) ? vL : vR) + cmax;
+  const R rcL = WL[0] * (vL - sL);
+  const R rcR = WR[0] * (sR - vR);
   // one reciprocal per shared denominator (differs from two divisions by at
   // most an ulp per value)
-  const double rden = 1.0 / (rcR + rcL);
-  const double vs = (rcR * vR + rcL * vL + (WL[P] - WR[P])) * rden;
-  const double Ps = (rcR * WL[P] + rcL * WR[P] + rcL * rcR * (vL - vR)) * rden;
-  double r, v, p, e;
-  if (sL > 0.0) {
+  const R rden = qrcp<R>(rcR + rcL);
+  const R vs = (rcR * vR + rcL * vL + (WL[P] - WR[P])) * rden;
+  const R Ps = (rcR * WL[P] + rcL * WR[P] + rcL * rcR * (vL - vR)) * rden;
+  R r, v, p, e;
+  if (sL > R(0)) {
     r = WL[0]; v = vL; p = WL[P]; e = UL[P];
-  } else if (vs > 0.0) {
-    const double rs = 1.0 / (sL - vs);
+  } else if (vs > R(0)) {
+    const R rs = qrcp<R>(sL - vs);
     r = WL[0] * (sL - vL) * rs;
     e = ((sL - vL) * UL[P] - WL[P] * vL + Ps * vs) * rs;
     v = vs; p = Ps;
-  } else if (sR > 0.0) {
-    const double rs = 1.0 / (sR - vs);
+  } else if (sR > R(0)) {
+    const R rs = qrcp<R>(sR - vs);
     r = WR[0] * (sR - vR) * rs;
     e = ((sR - vR) * UR[P] - WR[P] * vR + Ps * vs) * rs;
     v = vs; p = Ps;
@@ -164,18 +202,17 @@ __device__ __forceinline__ void hllc(const double* WL, const double* WR, const d
 #pragma unroll
   for (int k = 1; k < NDIM; ++k) {
     const int vt = rolled<NDIM>(a, k);
-    F[vt] = F[0] * ((vs > 0.0) ? WL[vt] : WR[vt]);
+    F[vt] = F[0] * ((vs > R(0)) ? WL[vt] : WR[vt]);
   }
 }
 
-template <int NDIM, int RS>
-__device__ __forceinline__ void riemann(const double* WL, const double* WR, const double* UL,
-                                        const double* UR, double* F, int a, double gamma,
-                                        double min_c2) {
+template <int NDIM, int RS, typename R>
+__device__ __forceinline__ void riemann(const R* WL, const R* WR, const R* UL, const R* UR, R* F,
+                                        int a, R gamma, R min_c2) {
   if constexpr (RS == RS_HLLC)
-    hllc<NDIM>(WL, WR, UL, UR, F, a, gamma, min_c2);
+    hllc<NDIM, R>(WL, WR, UL, UR, F, a, gamma, min_c2);
   else
-    llf<NDIM>(WL, WR, UL, UR, F, a, gamma, min_c2);
+    llf<NDIM, R>(WL, WR, UL, UR, F, a, gamma, min_c2);
 }
 
 // ---------------------------------------------------------------------------

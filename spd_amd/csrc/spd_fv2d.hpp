@@ -254,27 +254,28 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
 constexpr int CX = TX + 2, CY = TY + 2;   // tile + 1 halo
 constexpr int WX = TX + 4, WY = TY + 4;   // tile + 2 halo
 
-template <int RS>
+template <int RS, typename B>
 __global__ void __launch_bounds__(NT)
 k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
-              const double* __restrict__ theta, const double* __restrict__ Fx,
-              const double* __restrict__ Fy, const double* __restrict__ dt_dev, double w_i,
+              const double* __restrict__ theta, const B* __restrict__ Fx,
+              const B* __restrict__ Fy, const double* __restrict__ dt_dev, double w_i,
               double* __restrict__ Uout) {
   SPD_SKIP(g);
   // LDS: W (4 x WY x WX) is reused for the blended face fluxes after step 2
   constexpr int SW = (4 * WY * WX > 4 * (TY * (TX + 1) + (TY + 1) * TX))
                          ? 4 * WY * WX
                          : 4 * (TY * (TX + 1) + (TY + 1) * TX);
-  __shared__ double sW[SW];
-  __shared__ double sH[12][CY][CX];  // Mp(4), Sx(4), Sy(4)
+  __shared__ B sW[SW];
+  __shared__ B sH[12][CY][CX];  // Mp(4), Sx(4), Sy(4)
   __shared__ double sT[CY][CX];
   __shared__ unsigned char sNeed[CY][CX];
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
   const int Nx = g.Ns[0], Ny = g.Ns[1];
   const int NGx = g.Ng[0];
-  const double gamma = prm.gamma;
+  const B gamma = (B)prm.gamma;
   const double dti = (*dt_dev) * w_i;
-  double(*W)[WY][WX] = reinterpret_cast<double(*)[WY][WX]>(sW);
+  const B dtib = (B)dti;
+  B(*W)[WY][WX] = reinterpret_cast<B(*)[WY][WX]>(sW);
 
   for (int k = threadIdx.x; k < CX * CY; k += NT) {
     const int lx = k % CX, ly = k / CX;
@@ -290,10 +291,10 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
     double Wv[4] = {qnan(), qnan(), qnan(), qnan()};
     if (in_ghost_range(g, x, y)) {
       int c[3] = {x, y, 0};
-      load_W<2>(Ucv, g, c, Wv, gamma);
+      load_W<2>(Ucv, g, c, Wv, prm.gamma);
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) W[v][ly][lx] = Wv[v];
+    for (int v = 0; v < 4; ++v) W[v][ly][lx] = (B)Wv[v];
   }
   __syncthreads();
 
@@ -316,59 +317,59 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
     sNeed[ly][lx] = need ? 1 : 0;
     if (!need) continue;
     const int wx = lx + 1, wy = ly + 1;  // position in W
-    double Wc[4], dMh[2][4];
+    B Wc[4], dMh[2][4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) Wc[v] = W[v][wy][wx];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int cb = (b == 0) ? x : y;
       const int G = cb + NGH;
-      const double rhl = g.rhcv[b][G - 1], rhr = g.rhcv[b][G];
-      const double hf = g.hfp[b][G];
+      const B rhl = (B)g.rhcv[b][G - 1], rhr = (B)g.rhcv[b][G];
+      const B hf = (B)g.hfp[b][G];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const double Wm = (b == 0) ? W[v][wy][wx - 1] : W[v][wy - 1][wx];
-        const double Wp = (b == 0) ? W[v][wy][wx + 1] : W[v][wy + 1][wx];
-        const double L = (Wc[v] - Wm) * rhl;
-        const double R = (Wp - Wc[v]) * rhr;
-        double r = R / L;
-        r = (r < 1.0) ? r : 1.0;
-        dMh[b][v] = ((r > 0.0) ? r : 0.0) * L;
-        sH[4 + 4 * b + v][ly][lx] = 0.5 * dMh[b][v] * hf;
+        const B Wm = (b == 0) ? W[v][wy][wx - 1] : W[v][wy - 1][wx];
+        const B Wp = (b == 0) ? W[v][wy][wx + 1] : W[v][wy + 1][wx];
+        const B L = (Wc[v] - Wm) * rhl;
+        const B R = (Wp - Wc[v]) * rhr;
+        B r = qdiv<B>(R, L);
+        r = (r < B(1)) ? r : B(1);
+        dMh[b][v] = ((r > B(0)) ? r : B(0)) * L;
+        sH[4 + 4 * b + v][ly][lx] = B(0.5) * dMh[b][v] * hf;
       }
     }
-    double dtW[4] = {0.0, 0.0, 0.0, 0.0};
+    B dtW[4] = {B(0), B(0), B(0), B(0)};
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int vel = b + 1;
-      const double* dW = dMh[b];
+      const B* dW = dMh[b];
       dtW[0] = dtW[0] - (Wc[vel] * dW[0] + Wc[0] * dW[vel]);
       dtW[3] = dtW[3] - (Wc[vel] * dW[3] + gamma * Wc[3] * dW[vel]);
-      dtW[vel] = dtW[vel] - (Wc[vel] * dW[vel] + dW[3] / Wc[0]);
+      dtW[vel] = dtW[vel] - (Wc[vel] * dW[vel] + qdiv<B>(dW[3], Wc[0]));
       dtW[2] = dtW[2] - Wc[vel] * dW[2];
     }
-    dtW[3] = dtW[3] * 0.0;
+    dtW[3] = dtW[3] * B(0);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) sH[v][ly][lx] = Wc[v] + 0.5 * dtW[v] * dti;
+    for (int v = 0; v < 4; ++v) sH[v][ly][lx] = Wc[v] + B(0.5) * dtW[v] * dtib;
   }
   __syncthreads();
 
   // 3. blended fluxes of the tile's faces, in the (now free) W buffer
-  double* bFx = sW;                       // [TY][TX+1][4]
-  double* bFy = sW + TY * (TX + 1) * 4;   // [TY+1][TX][4]
+  B* bFx = sW;                       // [TY][TX+1][4]
+  B* bFy = sW + TY * (TX + 1) * 4;   // [TY+1][TX][4]
   const long long nfx = (long long)Ny * (Nx + 1), nfy = (long long)(Ny + 1) * Nx;
   for (int k = threadIdx.x; k < TY * (TX + 1); k += NT) {
     const int fx = k % (TX + 1), ly = k / (TX + 1);
     const int x = x0 + fx, y = y0 + ly;
     if (x > Nx || y >= Ny) continue;
     const long long fi = (long long)y * (Nx + 1) + x;
-    double F[4];
+    B F[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) F[v] = Fx[v * nfx + fi];
     double th = nmax(sT[ly + 1][fx], sT[ly + 1][fx + 1]);
     if (prm.godunov) th = 1.0;
     if (!(th == 0.0)) {
-      double WL[4], WR[4], UL[4], UR[4], Ff[4];
+      B WL[4], WR[4], UL[4], UR[4], Ff[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         WL[v] = sH[v][ly + 1][fx] + sH[4 + v][ly + 1][fx];
@@ -376,9 +377,10 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
       }
       cons<2>(WL, UL, 0, gamma);
       cons<2>(WR, UR, 0, gamma);
-      riemann<2, RS>(WL, WR, UL, UR, Ff, 0, gamma, prm.min_c2);
+      riemann<2, RS>(WL, WR, UL, UR, Ff, 0, gamma, (B)prm.min_c2);
+      const B tb = (B)th;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) F[v] = th * Ff[v] + (1.0 - th) * F[v];
+      for (int v = 0; v < 4; ++v) F[v] = tb * Ff[v] + (B(1) - tb) * F[v];
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) bFx[(ly * (TX + 1) + fx) * 4 + v] = F[v];
@@ -388,13 +390,13 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
     const int x = x0 + lx, y = y0 + fy;
     if (x >= Nx || y > Ny) continue;
     const long long fi = (long long)y * Nx + x;
-    double F[4];
+    B F[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) F[v] = Fy[v * nfy + fi];
     double th = nmax(sT[fy][lx + 1], sT[fy + 1][lx + 1]);
     if (prm.godunov) th = 1.0;
     if (!(th == 0.0)) {
-      double WL[4], WR[4], UL[4], UR[4], Ff[4];
+      B WL[4], WR[4], UL[4], UR[4], Ff[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         WL[v] = sH[v][fy][lx + 1] + sH[8 + v][fy][lx + 1];
@@ -402,9 +404,10 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
       }
       cons<2>(WL, UL, 1, gamma);
       cons<2>(WR, UR, 1, gamma);
-      riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, prm.min_c2);
+      riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, (B)prm.min_c2);
+      const B tb = (B)th;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) F[v] = th * Ff[v] + (1.0 - th) * F[v];
+      for (int v = 0; v < 4; ++v) F[v] = tb * Ff[v] + (B(1) - tb) * F[v];
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) bFy[(fy * TX + lx) * 4 + v] = F[v];
@@ -421,8 +424,9 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
   for (int v = 0; v < 4; ++v) {
     const double u = Ucv[v * g.Nc + ci];
     double d = u * 0.0;
-    d = d + (bFx[(ty * (TX + 1) + tx + 1) * 4 + v] - bFx[(ty * (TX + 1) + tx) * 4 + v]) * rdx;
-    d = d + (bFy[((ty + 1) * TX + tx) * 4 + v] - bFy[(ty * TX + tx) * 4 + v]) * rdy;
+    d = d + ((double)bFx[(ty * (TX + 1) + tx + 1) * 4 + v] -
+             (double)bFx[(ty * (TX + 1) + tx) * 4 + v]) * rdx;
+    d = d + ((double)bFy[((ty + 1) * TX + tx) * 4 + v] - (double)bFy[(ty * TX + tx) * 4 + v]) * rdy;
     Uout[v * g.Nc + ci] = u - d * dti;
   }
 }
@@ -435,20 +439,19 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
 // point) evaluates the flux on the element's lines, averages it over the
 // transverse subcells and writes the face values.
 // ---------------------------------------------------------------------------
-template <int P, int A>
-__device__ __forceinline__ void ho2d_items(const Ops<P>& op, const Geom& g, const Params& prm,
-                                           int t, const double* __restrict__ Ff,
-                                           double* __restrict__ Fh, const double* slab,
-                                           long long e0, int ne, int EPB);
+template <int P, int A, typename B>
+__device__ __forceinline__ void ho2d_items(const Ops<P, B>& op, const Geom& g, const Params& prm,
+                                           int t, const B* __restrict__ Ff, B* __restrict__ Fh,
+                                           const B* slab, long long e0, int ne, int EPB);
 
-template <int P>
+template <int P, typename B>
 __global__ void __launch_bounds__(256)
-k_ho2d(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
-       const double* __restrict__ Ffx, const double* __restrict__ Ffy,
-       double* __restrict__ Fhx, double* __restrict__ Fhy, int EPB) {
+k_ho2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
+       const B* __restrict__ Ffx, const B* __restrict__ Ffy, B* __restrict__ Fhx,
+       B* __restrict__ Fhy, int EPB) {
   SPD_SKIP(g);
   constexpr int V = 4, n = P + 1, nf = P + 2, NL = n * n;
-  extern __shared__ double slab[];  // [V][EPB][NL]
+  B* slab = reinterpret_cast<B*>(spd_dyn_lds);  // [V][EPB][NL]
   const long long e0 = (long long)blockIdx.x * EPB;
   const int ne = (int)((g.NE - e0 < EPB) ? (g.NE - e0) : EPB);
   for (int k = threadIdx.x; k < V * ne * NL; k += blockDim.x) {
@@ -456,15 +459,14 @@ k_ho2d(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
     slab[v * EPB * NL + r] = Uad[((long long)(t * V + v) * g.NE + e0) * NL + r];
   }
   __syncthreads();
-  ho2d_items<P, 0>(op, g, prm, t, Ffx, Fhx, slab, e0, ne, EPB);
-  ho2d_items<P, 1>(op, g, prm, t, Ffy, Fhy, slab, e0, ne, EPB);
+  ho2d_items<P, 0, B>(op, g, prm, t, Ffx, Fhx, slab, e0, ne, EPB);
+  ho2d_items<P, 1, B>(op, g, prm, t, Ffy, Fhy, slab, e0, ne, EPB);
 }
 
-template <int P, int A>
-__device__ __forceinline__ void ho2d_items(const Ops<P>& op, const Geom& g, const Params& prm,
-                                           int t, const double* __restrict__ Ff,
-                                           double* __restrict__ Fh, const double* slab,
-                                           long long e0, int ne, int EPB) {
+template <int P, int A, typename B>
+__device__ __forceinline__ void ho2d_items(const Ops<P, B>& op, const Geom& g, const Params& prm,
+                                           int t, const B* __restrict__ Ff, B* __restrict__ Fh,
+                                           const B* slab, long long e0, int ne, int EPB) {
   constexpr int V = 4, n = P + 1, nf = P + 2, NL = n * n;
   constexpr int a = A;
   const int Nx = g.N[0], Ny = g.N[1];
@@ -476,7 +478,7 @@ __device__ __forceinline__ void ho2d_items(const Ops<P>& op, const Geom& g, cons
     const int ca = (a == 0) ? cx : cy;
     const int Na = (a == 0) ? Nx : Ny;
     if (f == nf - 1 && ca != Na - 1) continue;
-    double Fl[n][V];
+    B Fl[n][V];
     if (f == 0 || f == nf - 1) {
       // element-face Riemann flux on the face grid of dimension a
       const int fa = ca + (f == 0 ? 0 : 1);
@@ -491,11 +493,11 @@ __device__ __forceinline__ void ho2d_items(const Ops<P>& op, const Geom& g, cons
     } else {
 #pragma unroll
       for (int lp = 0; lp < n; ++lp) {
-        double M[V], W[V];
+        B M[V], W[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          const double* u = slab + (v * EPB + el) * NL;
-          double acc = 0.0;
+          const B* u = slab + (v * EPB + el) * NL;
+          B acc = B(0);
 #pragma unroll
           for (int s = 0; s < n; ++s) {
             const int l = (a == 0) ? (lp * n + s) : (s * n + lp);
@@ -503,18 +505,18 @@ __device__ __forceinline__ void ho2d_items(const Ops<P>& op, const Geom& g, cons
           }
           M[v] = acc;
         }
-        prims<2>(M, W, a, prm.gamma);
-        flux<2>(W, Fl[lp], a, prm.gamma);
+        prims<2>(M, W, a, (B)prm.gamma);
+        flux<2>(W, Fl[lp], a, (B)prm.gamma);
       }
     }
     const long long nxf = (a == 0) ? (long long)Nx * n + 1 : (long long)Nx * n;
     const long long nface = nxf * ((a == 0) ? (long long)Ny * n : (long long)Ny * n + 1);
 #pragma unroll
     for (int q = 0; q < n; ++q) {
-      double Fa[V];
+      B Fa[V];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        double acc = 0.0;
+        B acc = B(0);
 #pragma unroll
         for (int s = 0; s < n; ++s) acc += op.s2c[q][s] * Fl[s][v];
         Fa[v] = acc;

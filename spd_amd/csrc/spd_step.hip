@@ -1,6 +1,7 @@
 // MI355X (gfx950) kernels of the SD-ADER step and the C ABI of libspd_hip.so.
 //
-// Data layouts (all FP64):
+// Data layouts (FP64; in the FP32-bulk form the predictor state, the element-
+// face fluxes, the FV-face fluxes and the Hancock states are FP32):
 //   element layout   (V, E, L)        E = elements (x fastest), L = (p+1)^d points
 //   predictor state  (T, V, E, L)     T = m+1 time nodes
 //   element faces    (T, V, N_a+1, E_perp, L_perp) per dimension a
@@ -32,15 +33,20 @@
 
 namespace spd {
 
-template <int P> struct Ops {
+// Operator matrices in the arithmetic type R of the kernels that use them;
+// the quadrature weights stay FP64 (they scale the FP64 update).
+template <int P, typename R = double> struct Ops {
   static constexpr int n = P + 1, nf = P + 2, T = P + 1;
-  double s2f[nf][n];  // sp_to_fp
-  double d2s[n][nf];  // dfp_to_sp
-  double s2c[n][n];   // sp_to_cv
-  double c2s[n][n];   // cv_to_sp
-  double inv[T][T];   // invader
-  double w[T];        // w_tp
+  R s2f[nf][n];  // sp_to_fp
+  R d2s[n][nf];  // dfp_to_sp
+  R s2c[n][n];   // sp_to_cv
+  R c2s[n][n];   // cv_to_sp
+  R inv[T][T];   // invader
+  double w[T];   // w_tp
 };
+
+// Dynamic LDS of the templated kernels (typed at each use).
+extern __shared__ __attribute__((aligned(16))) unsigned char spd_dyn_lds[];
 
 // ---------------------------------------------------------------------------
 // index helpers
@@ -136,15 +142,15 @@ __device__ __forceinline__ long long stride_face(const Geom& g, int a) {
 // NLP transverse lines of one element row, so that the lanes reading one
 // element's rows are adjacent; the fluxes are transposed through LDS and
 // written face-innermost.
-template <int NDIM, int P, int RS>
+template <int NDIM, int P, int RS, typename S, typename B>
 __global__ void __launch_bounds__(256)
-k_face_sd_x(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
-            double* __restrict__ Fface) {
+k_face_sd_x(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
+            B* __restrict__ Fface) {
   SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
   constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
   constexpr int FPB = 256 / NLP;
-  __shared__ double buf[V][NLP][FPB];
+  __shared__ B buf[V][NLP][FPB];
   const int Na = g.N[0];
   const int nchunk = (Na + 1 + FPB - 1) / FPB;
   const long long NEp = g.NE / Na;
@@ -174,21 +180,21 @@ k_face_sd_x(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int T
 #pragma unroll
   for (int d = NDIM - 1; d >= 1; --d) lb = lb * n + cl[d];
   lb *= n;
-  const double* base = src + (long long)t * V * g.NE * NL;
-  auto trace = [&](int ea, int fp, double* q) {
+  const S* base = src + (long long)t * V * g.NE * NL;
+  auto trace = [&](int ea, int fp, B* q) {
     int c[3] = {ea, ce[1], ce[2]};
     const long long e = elem_index<NDIM>(g, c);
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-      const double* u = base + ((long long)v * g.NE + e) * NL + lb;
-      double acc = 0.0;
+      const S* u = base + ((long long)v * g.NE + e) * NL + lb;
+      B acc = B(0);
 #pragma unroll
-      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[s];
+      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * (B)u[s];
       q[v] = acc;
     }
   };
   if (fi < FPB && f <= Na) {
-    double qL[V], qR[V];
+    B qL[V], qR[V];
     if (f > 0) trace(f - 1, nf - 1, qL);
     if (f < Na) trace(f, 0, qR);
     if (f == 0) {
@@ -211,10 +217,10 @@ k_face_sd_x(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int T
         if (k == BC_REFLECTIVE) qR[1] = -qR[1];
       }
     }
-    double WL[V], WR[V], F[V];
-    prims<NDIM>(qL, WL, 0, prm.gamma);
-    prims<NDIM>(qR, WR, 0, prm.gamma);
-    riemann<NDIM, RS>(WL, WR, qL, qR, F, 0, prm.gamma, prm.min_c2);
+    B WL[V], WR[V], F[V];
+    prims<NDIM>(qL, WL, 0, (B)prm.gamma);
+    prims<NDIM>(qR, WR, 0, (B)prm.gamma);
+    riemann<NDIM, RS>(WL, WR, qL, qR, F, 0, (B)prm.gamma, (B)prm.min_c2);
 #pragma unroll
     for (int v = 0; v < V; ++v) buf[v][lp][fi] = F[v];
   }
@@ -242,10 +248,10 @@ k_face_sd_x(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int T
   }
 }
 
-template <int NDIM, int P, int RS, int A>
+template <int NDIM, int P, int RS, int A, typename S, typename B>
 __global__ void __launch_bounds__(256)
-k_face_sd(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
-          double* __restrict__ Fface) {
+k_face_sd(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
+          B* __restrict__ Fface) {
   SPD_SKIP(g);
   constexpr int a = A;
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
@@ -281,22 +287,22 @@ k_face_sd(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsr
 #pragma unroll
   for (int b = NDIM - 1; b >= 0; --b) lb = lb * n + ((b == a) ? 0 : cl[b]);
   const int sa = stride_of<n>(a);
-  const double* base = src + (long long)t * V * g.NE * NL;
+  const S* base = src + (long long)t * V * g.NE * NL;
 
-  auto trace = [&](int ea, int fp, double* q) {
+  auto trace = [&](int ea, int fp, B* q) {
     int c[3] = {ce[0], ce[1], ce[2]};
     c[a] = ea;
     const long long e = elem_index<NDIM>(g, c);
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-      const double* u = base + ((long long)v * g.NE + e) * NL + lb;
-      double acc = 0.0;
+      const S* u = base + ((long long)v * g.NE + e) * NL + lb;
+      B acc = B(0);
 #pragma unroll
-      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[s * sa];
+      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * (B)u[s * sa];
       q[v] = acc;
     }
   };
-  double qL[V], qR[V];
+  B qL[V], qR[V];
   if (f > 0) trace(f - 1, nf - 1, qL);
   if (f < Na) trace(f, 0, qR);
   if (f == 0) {
@@ -319,10 +325,10 @@ k_face_sd(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsr
       if (k == BC_REFLECTIVE) qR[a + 1] = -qR[a + 1];
     }
   }
-  double WL[V], WR[V], F[V];
-  prims<NDIM>(qL, WL, a, prm.gamma);
-  prims<NDIM>(qR, WR, a, prm.gamma);
-  riemann<NDIM, RS>(WL, WR, qL, qR, F, a, prm.gamma, prm.min_c2);
+  B WL[V], WR[V], F[V];
+  prims<NDIM>(qL, WL, a, (B)prm.gamma);
+  prims<NDIM>(qR, WR, a, (B)prm.gamma);
+  riemann<NDIM, RS>(WL, WR, qL, qR, F, a, (B)prm.gamma, (B)prm.min_c2);
 #pragma unroll
   for (int v = 0; v < V; ++v) Fface[(long long)(t * V + v) * plane + pidx] = F[v];
 }
@@ -339,28 +345,29 @@ k_face_sd(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsr
 //   mode 1 (SD corrector, src/sdader_simulator.py:202-208):
 //      U_sp  -= dt * sum_t w_t dudt[t]
 // With Tsrc == 1 the source is U_sp for every time node (first iteration)
-// and only t = 0 is evaluated.
+// and only t = 0 is evaluated.  Mode 0 writes out_ad (bulk type), mode 1
+// out_sp (FP64, accumulated in FP64).
 // ---------------------------------------------------------------------------
-template <int NDIM, int P>
+template <int NDIM, int P, typename S, typename B>
 __global__ void __launch_bounds__(256)
-k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
-       const double* __restrict__ Ff0, const double* __restrict__ Ff1,
-       const double* __restrict__ Ff2, const double* Usp, const double* __restrict__ dt_dev,
-       double* out, int mode, int EPB) {
+k_pred(Ops<P, B> op, Geom g, Params prm, const S* src, int Tsrc, const B* __restrict__ Ff0,
+       const B* __restrict__ Ff1, const B* __restrict__ Ff2, const double* Usp,
+       const double* __restrict__ dt_dev, B* out_ad, double* out_sp, int mode, int EPB) {
   SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2, T = P + 1;
   constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
-  // dudt [EPB][T] slabs of V*NL values, padded by one double so that lanes of
+  // dudt [EPB][T] slabs of V*NL values, padded by one value so that lanes of
   // different time nodes / elements fall into different LDS banks
   constexpr int SLAB = V * NL + 1;
-  extern __shared__ double lds[];
+  B* lds = reinterpret_cast<B*>(spd_dyn_lds);
   const int el = threadIdx.x / (T * NLP);
   const int rem = threadIdx.x % (T * NLP);
   const int t = rem / NLP;
   const int lp = rem % NLP;
   const long long e = (long long)blockIdx.x * EPB + el;
   const bool active = (el < EPB) && (e < g.NE) && (t < Tsrc);
-  const double* Ff[3] = {Ff0, Ff1, Ff2};
+  const B* Ff[3] = {Ff0, Ff1, Ff2};
+  const B gam = (B)prm.gamma;
   int c[3] = {0, 0, 0};
   if (e < g.NE) elem_coords<NDIM>(g, e, c);
 
@@ -369,14 +376,14 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
     if (active) {
       const int lb = line_base<NDIM, n>(a, lp);
       const int sa = stride_of<n>(a);
-      double u[V][n];
+      B u[V][n];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const double* s0 = src + ((long long)(t * V + v) * g.NE + e) * NL + lb;
+        const S* s0 = src + ((long long)(t * V + v) * g.NE + e) * NL + lb;
 #pragma unroll
-        for (int s = 0; s < n; ++s) u[v][s] = s0[s * sa];
+        for (int s = 0; s < n; ++s) u[v][s] = (B)s0[s * sa];
       }
-      double F[nf][V];
+      B F[nf][V];
       int cl[3] = {0, 0, 0};
       {
         int r = lp;
@@ -392,34 +399,34 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
       const long long sfa = stride_face<NDIM, n>(g, a);
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const double* fb = Ff[a] + (long long)(t * V + v) * plane + i0;
+        const B* fb = Ff[a] + (long long)(t * V + v) * plane + i0;
         F[0][v] = fb[0];
         F[nf - 1][v] = fb[sfa];
       }
 #pragma unroll
       for (int fp = 1; fp < nf - 1; ++fp) {
-        double M[V], W[V];
+        B M[V], W[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          double acc = 0.0;
+          B acc = B(0);
 #pragma unroll
           for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[v][s];
           M[v] = acc;
         }
-        prims<NDIM>(M, W, a, prm.gamma);
-        flux<NDIM>(W, F[fp], a, prm.gamma);
+        prims<NDIM>(M, W, a, gam);
+        flux<NDIM>(W, F[fp], a, gam);
       }
-      const double rha = g.rh[a];
-      double* d0 = lds + (long long)(el * T + t) * SLAB + lb;
+      const B rha = (B)g.rh[a];
+      B* d0 = lds + (long long)(el * T + t) * SLAB + lb;
 #pragma unroll
       for (int s = 0; s < n; ++s) {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          double acc = 0.0;
+          B acc = B(0);
 #pragma unroll
           for (int fp = 0; fp < nf; ++fp) acc += op.d2s[s][fp] * F[fp][v];
-          const double d = acc * rha;
-          double* dst = d0 + v * NL + s * sa;
+          const B d = acc * rha;
+          B* dst = d0 + v * NL + s * sa;
           if (a == 0)
             *dst = d;
           else
@@ -440,23 +447,23 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
     if (ee >= g.NE) continue;
     const long long off = ((long long)v * g.NE + ee) * NL + l;
     const double u0 = Usp[off];
-    const double* dd = lds + (long long)le * T * SLAB + v * NL + l;  // + p*SLAB
+    const B* dd = lds + (long long)le * T * SLAB + v * NL + l;  // + p*SLAB
     if (mode == 0) {
 #pragma unroll
       for (int tn = 0; tn < T; ++tn) {
-        double acc = 0.0;
+        B acc = B(0);
 #pragma unroll
         for (int p = 0; p < T; ++p) {
           const int ps = (p < Tsrc) ? p : (Tsrc - 1);
           acc += op.inv[tn][p] * dd[(long long)ps * SLAB];
         }
-        out[((long long)(tn * V + v) * g.NE + ee) * NL + l] = u0 - acc * dt;
+        out_ad[((long long)(tn * V + v) * g.NE + ee) * NL + l] = (B)(u0 - (double)acc * dt);
       }
     } else {
       double acc = 0.0;
 #pragma unroll
-      for (int p = 0; p < T; ++p) acc += op.w[p] * dd[(long long)p * SLAB];
-      out[off] = u0 - acc * dt;
+      for (int p = 0; p < T; ++p) acc += op.w[p] * (double)dd[(long long)p * SLAB];
+      out_sp[off] = u0 - acc * dt;
     }
   }
 }
@@ -467,10 +474,10 @@ k_pred(Ops<P> op, Geom g, Params prm, const double* src, int Tsrc,
 // subcell extent in the transverse dimensions (src/sdader_simulator.py:288-323,
 // src/sd_simulator.py:208-212).  One thread per (element, flux point).
 // ---------------------------------------------------------------------------
-template <int NDIM, int P, int A>
+template <int NDIM, int P, int A, typename B>
 __global__ void __launch_bounds__(256)
-k_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
-         const double* __restrict__ Fface, double* __restrict__ Fho) {
+k_hoflux(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
+         const B* __restrict__ Fface, B* __restrict__ Fho) {
   SPD_SKIP(g);
   constexpr int a = A;
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
@@ -484,7 +491,7 @@ k_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
   const int Na = g.N[a];
   if (f == nf - 1 && c[a] != Na - 1) return;
   const int sa = stride_of<n>(a);
-  double Fl[NLP][V];
+  B Fl[NLP][V];
 #pragma unroll
   for (int lp = 0; lp < NLP; ++lp) {
     if (f == 0 || f == nf - 1) {
@@ -505,21 +512,21 @@ k_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
       for (int v = 0; v < V; ++v) Fl[lp][v] = Fface[(long long)(t * V + v) * plane + i0];
     } else {
       const int lb = line_base<NDIM, n>(a, lp);
-      double M[V], W[V];
+      B M[V], W[V];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const double* u = Uad + ((long long)(t * V + v) * g.NE + e) * NL + lb;
-        double acc = 0.0;
+        const B* u = Uad + ((long long)(t * V + v) * g.NE + e) * NL + lb;
+        B acc = B(0);
 #pragma unroll
         for (int s = 0; s < n; ++s) acc += op.s2f[f][s] * u[s * sa];
         M[v] = acc;
       }
-      prims<NDIM>(M, W, a, prm.gamma);
-      flux<NDIM>(W, Fl[lp], a, prm.gamma);
+      prims<NDIM>(M, W, a, (B)prm.gamma);
+      flux<NDIM>(W, Fl[lp], a, (B)prm.gamma);
     }
   }
   // transverse subcell averages, lower dimension first
-  double Fa[NLP][V];
+  B Fa[NLP][V];
   if constexpr (NDIM == 1) {
 #pragma unroll
     for (int v = 0; v < V; ++v) Fa[0][v] = Fl[0][v];
@@ -528,20 +535,20 @@ k_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
     for (int q = 0; q < n; ++q)
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        double acc = 0.0;
+        B acc = B(0);
 #pragma unroll
         for (int s = 0; s < n; ++s) acc += op.s2c[q][s] * Fl[s][v];
         Fa[q][v] = acc;
       }
   } else {
-    double G[NLP][V];
+    B G[NLP][V];
 #pragma unroll
     for (int c2 = 0; c2 < n; ++c2)
 #pragma unroll
       for (int q1 = 0; q1 < n; ++q1)
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          double acc = 0.0;
+          B acc = B(0);
 #pragma unroll
           for (int s = 0; s < n; ++s) acc += op.s2c[q1][s] * Fl[s + n * c2][v];
           G[q1 + n * c2][v] = acc;
@@ -552,7 +559,7 @@ k_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
       for (int q1 = 0; q1 < n; ++q1)
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          double acc = 0.0;
+          B acc = B(0);
 #pragma unroll
           for (int s = 0; s < n; ++s) acc += op.s2c[q2][s] * G[q1 + n * s][v];
           Fa[q1 + n * q2][v] = acc;
@@ -647,11 +654,10 @@ k_tensor(Ops<P> op, Geom g, int to_cv, const double* __restrict__ in, double* __
 // FV update U_out = U_in - dt_i * sum_a (F_a[f+1] - F_a[f]) / dx_a
 // (src/fv_simulator.py:189-205); in place when U_out == U_in.
 // ---------------------------------------------------------------------------
-template <int NDIM>
+template <int NDIM, typename B>
 __global__ void __launch_bounds__(256)
-k_apply(Geom g, const double* Uin, const double* __restrict__ F0, const double* __restrict__ F1,
-        const double* __restrict__ F2, const double* __restrict__ dt_dev, double w_i,
-        double* Uout) {
+k_apply(Geom g, const double* Uin, const B* __restrict__ F0, const B* __restrict__ F1,
+        const B* __restrict__ F2, const double* __restrict__ dt_dev, double w_i, double* Uout) {
   SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V;
   const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -665,7 +671,7 @@ k_apply(Geom g, const double* Uin, const double* __restrict__ F0, const double* 
       x /= g.Ns[a];
     }
   }
-  const double* F[3] = {F0, F1, F2};
+  const B* F[3] = {F0, F1, F2};
   long long fi[3], fs[3], nf[3];
   double dx[3];
 #pragma unroll
@@ -691,8 +697,8 @@ k_apply(Geom g, const double* Uin, const double* __restrict__ F0, const double* 
     double d = u * 0.0;
 #pragma unroll
     for (int a = 0; a < NDIM; ++a) {
-      const double* Fa = F[a] + v * nf[a] + fi[a];
-      d = d + (Fa[fs[a]] - Fa[0]) * dx[a];
+      const B* Fa = F[a] + v * nf[a] + fi[a];
+      d = d + ((double)Fa[fs[a]] - (double)Fa[0]) * dx[a];
     }
     Uout[v * g.Nc + cell] = u - d * dti;
   }
@@ -1045,10 +1051,10 @@ namespace spd {
 // one-layer-ghosted grid that borders a face with a non-zero blending
 // coefficient (or every cell with godunov).  Output layout per cell:
 // [Mp (V), S_0 (V), ..., S_{NDIM-1} (V)] on the grid (Ns+2)^NDIM.
-template <int NDIM>
+template <int NDIM, typename B>
 __global__ void __launch_bounds__(256)
 k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ theta,
-          const double* __restrict__ dt_dev, double w_i, double* __restrict__ HS) {
+          const double* __restrict__ dt_dev, double w_i, B* __restrict__ HS) {
   SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V;
   long long n1 = 1;
@@ -1095,53 +1101,61 @@ k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __re
     }
   }
   if (!need) return;
-  const double dti = (*dt_dev) * w_i;
-  double Mp[V], S[NDIM][V];
+  const B dti = (B)((*dt_dev) * w_i);
+  const B gam = (B)prm.gamma;
+  B Mp[V], S[NDIM][V];
   {
     constexpr int P = Dims<NDIM>::P;
-    double W[V];
-    load_W<NDIM>(Ucv, g, c, W, prm.gamma);
-    double dMh[NDIM][V];
+    // primitives from the FP64 state (FP64 pressure recovery)
+    auto load = [&](const int* cc, B* Wb) {
+      double Wd[V];
+      load_W<NDIM>(Ucv, g, cc, Wd, prm.gamma);
+#pragma unroll
+      for (int v = 0; v < V; ++v) Wb[v] = (B)Wd[v];
+    };
+    B W[V];
+    load(c, W);
+    B dMh[NDIM][V];
 #pragma unroll
     for (int b = 0; b < NDIM; ++b) {
       int cp[3] = {c[0], c[1], c[2]}, cm[3] = {c[0], c[1], c[2]};
       cp[b] += 1;
       cm[b] -= 1;
-      double Wp[V], Wm[V];
-      load_W<NDIM>(Ucv, g, cp, Wp, prm.gamma);
-      load_W<NDIM>(Ucv, g, cm, Wm, prm.gamma);
+      B Wp[V], Wm[V];
+      load(cp, Wp);
+      load(cm, Wm);
       const int G = c[b] + NGH;
-      const double rhl = g.rhcv[b][G - 1], rhr = g.rhcv[b][G];
-      const double hf = g.hfp[b][G];
+      const B rhl = (B)g.rhcv[b][G - 1], rhr = (B)g.rhcv[b][G];
+      const B hf = (B)g.hfp[b][G];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const double L = (W[v] - Wm[v]) * rhl;
-        const double R = (Wp[v] - W[v]) * rhr;
-        double r = R / L;
-        r = (r < 1.0) ? r : 1.0;
-        dMh[b][v] = ((r > 0.0) ? r : 0.0) * L;
-        S[b][v] = 0.5 * dMh[b][v] * hf;
+        const B L = (W[v] - Wm[v]) * rhl;
+        const B R = (Wp[v] - W[v]) * rhr;
+        B r = qdiv<B>(R, L);
+        r = (r < B(1)) ? r : B(1);
+        dMh[b][v] = ((r > B(0)) ? r : B(0)) * L;
+        S[b][v] = B(0.5) * dMh[b][v] * hf;
       }
     }
-    double dtW[V];
+    B dtW[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) dtW[v] = 0.0;
+    for (int v = 0; v < V; ++v) dtW[v] = B(0);
 #pragma unroll
     for (int b = 0; b < NDIM; ++b) {
       const int vel = b + 1;
-      const double* dW = dMh[b];
+      const B* dW = dMh[b];
       dtW[0] = dtW[0] - (W[vel] * dW[0] + W[0] * dW[vel]);
-      dtW[P] = dtW[P] - (W[vel] * dW[P] + prm.gamma * W[P] * dW[vel]);
-      dtW[vel] = dtW[vel] - (W[vel] * dW[vel] + dW[P] / W[0]);
+      dtW[P] = dtW[P] - (W[vel] * dW[P] + gam * W[P] * dW[vel]);
+      dtW[vel] = dtW[vel] - (W[vel] * dW[vel] + qdiv<B>(dW[P], W[0]));
 #pragma unroll
       for (int v2 = 2; v2 <= NDIM; ++v2) dtW[v2] = dtW[v2] - W[vel] * dW[v2];
     }
-    dtW[P] = dtW[P] * 0.0;
+    dtW[P] = dtW[P] * B(0);
 #pragma unroll
-    for (int v = 0; v < V; ++v) Mp[v] = W[v] + 0.5 * dtW[v] * dti;
+    for (int v = 0; v < V; ++v) Mp[v] = W[v] + B(0.5) * dtW[v] * dti;
   }
   constexpr int REC = (1 + NDIM) * V;
-  double* out = HS + tid;
+  B* out = HS + tid;
 #pragma unroll
   for (int v = 0; v < V; ++v) out[(long long)v * n1] = Mp[v];
 #pragma unroll
@@ -1153,10 +1167,10 @@ k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __re
 
 // Fallback flux from the stored Hancock states at the faces of dimension a
 // with a non-zero coefficient, blended into the HO flux in place.
-template <int NDIM, int RS, int A>
+template <int NDIM, int RS, int A, typename B>
 __global__ void __launch_bounds__(256)
-k_fbface(Geom g, Params prm, const double* __restrict__ HS, const double* __restrict__ theta,
-         double* __restrict__ Fho) {
+k_fbface(Geom g, Params prm, const B* __restrict__ HS, const double* __restrict__ theta,
+         B* __restrict__ Fho) {
   SPD_SKIP(g);
   constexpr int a = A;
   constexpr int V = Dims<NDIM>::V;
@@ -1191,19 +1205,21 @@ k_fbface(Geom g, Params prm, const double* __restrict__ HS, const double* __rest
   double th = nmax(theta[gl], theta[gr]);
   if (prm.godunov) th = 1.0;
   if (th == 0.0) return;
-  double WL[V], WR[V], UL[V], UR[V], F[V];
+  B WL[V], WR[V], UL[V], UR[V], F[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) {
     WL[v] = HS[(long long)v * n1 + hl] + HS[(long long)((1 + a) * V + v) * n1 + hl];
     WR[v] = HS[(long long)v * n1 + hr] - HS[(long long)((1 + a) * V + v) * n1 + hr];
   }
-  cons<NDIM>(WL, UL, a, prm.gamma);
-  cons<NDIM>(WR, UR, a, prm.gamma);
-  riemann<NDIM, RS>(WL, WR, UL, UR, F, a, prm.gamma, prm.min_c2);
+  const B gam = (B)prm.gamma;
+  cons<NDIM>(WL, UL, a, gam);
+  cons<NDIM>(WR, UR, a, gam);
+  riemann<NDIM, RS>(WL, WR, UL, UR, F, a, gam, (B)prm.min_c2);
+  const B tb = (B)th;
 #pragma unroll
   for (int v = 0; v < V; ++v) {
-    double* dst = Fho + v * nface + tid;
-    *dst = th * F[v] + (1.0 - th) * (*dst);
+    B* dst = Fho + v * nface + tid;
+    *dst = tb * F[v] + (B(1) - tb) * (*dst);
   }
 }
 
@@ -1505,15 +1521,16 @@ struct spd_ctx {
   std::vector<void*> allocs;
   size_t scratch_bytes = 0;
   double* geom_buf = nullptr;
-  double* Uad = nullptr;
-  double* Fface[3] = {nullptr, nullptr, nullptr};
-  double* Fho[3] = {nullptr, nullptr, nullptr};
+  // bulk-type buffers (FP64, or FP32 in the FP32-bulk form)
+  void* Uad = nullptr;
+  void* Fface[3] = {nullptr, nullptr, nullptr};
+  void* Fho[3] = {nullptr, nullptr, nullptr};
   double* Ucv = nullptr;
   double* Unew = nullptr;
   uint8_t* flags = nullptr;
   double* theta1 = nullptr;
   double* theta = nullptr;
-  double* HS = nullptr;       // Hancock states and half slopes, (1+ndim)*V per cell
+  void* HS = nullptr;         // Hancock states and half slopes, (1+ndim)*V per cell (bulk)
   int* any_flag = nullptr;
   int* zero_flag = nullptr;   // always 0: the skip flag of ordinary calls
   int* run_skip = nullptr;    // skip flag of device-loop steps
@@ -1560,79 +1577,84 @@ struct spd_ctx {
 
 namespace {
 
-template <int P> Ops<P> make_ops(const host::OpsHost& h) {
-  Ops<P> o{};
+template <int P, typename R = double> Ops<P, R> make_ops(const host::OpsHost& h) {
+  Ops<P, R> o{};
   constexpr int n = P + 1, nf = P + 2, T = P + 1;
   for (int a = 0; a < nf; ++a)
-    for (int b = 0; b < n; ++b) o.s2f[a][b] = h.sp_to_fp[a][b];
+    for (int b = 0; b < n; ++b) o.s2f[a][b] = (R)h.sp_to_fp[a][b];
   for (int a = 0; a < n; ++a)
-    for (int b = 0; b < nf; ++b) o.d2s[a][b] = h.dfp_to_sp[a][b];
+    for (int b = 0; b < nf; ++b) o.d2s[a][b] = (R)h.dfp_to_sp[a][b];
   for (int a = 0; a < n; ++a)
     for (int b = 0; b < n; ++b) {
-      o.s2c[a][b] = h.sp_to_cv[a][b];
-      o.c2s[a][b] = h.cv_to_sp[a][b];
+      o.s2c[a][b] = (R)h.sp_to_cv[a][b];
+      o.c2s[a][b] = (R)h.cv_to_sp[a][b];
     }
   for (int a = 0; a < T; ++a) {
-    for (int b = 0; b < T; ++b) o.inv[a][b] = h.invader[a][b];
+    for (int b = 0; b < T; ++b) o.inv[a][b] = (R)h.invader[a][b];
     o.w[a] = h.w_tp[a];
   }
   return o;
 }
 
-// Typed launcher of one (ndim, p) instantiation.
-template <int NDIM, int P> struct Stepper {
+// Typed launcher of one (ndim, p, bulk type) instantiation.
+template <int NDIM, int P, typename B> struct Stepper {
+  using OpsB = Ops<P, B>;
+  static B* Uad(spd_ctx* c) { return (B*)c->Uad; }
+  static B* Ff(spd_ctx* c, int a) { return (B*)c->Fface[a]; }
+  static B* Fh(spd_ctx* c, int a) { return (B*)c->Fho[a]; }
   static constexpr int V = NDIM + 2, n = P + 1, nf = P + 2, T = P + 1;
   static constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
 
   static_assert(T * NLP <= 256, "one element's lines must fit one block");
   static int epb() {
     int e = std::max(1, 256 / (T * NLP));
-    const int lds_per = T * (V * NL + 1) * (int)sizeof(double);
+    const int lds_per = T * (V * NL + 1) * (int)sizeof(B);
     e = std::max(1, std::min(e, 49152 / lds_per));
     return e;
   }
 
-  template <int RS, int A>
-  static void face_generic(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc,
-                           hipStream_t s) {
+  template <int RS, int A, typename S>
+  static void face_generic(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, hipStream_t s) {
     const long long Na = c->g.N[A];
     const long long tot = (long long)Tsrc * (Na + 1) * (c->g.NE / Na) * NLP;
     c->timed(KC_FACE_SD, s, [&] {
-      hipLaunchKernelGGL((k_face_sd<NDIM, P, RS, A>), dim3(grid_for(tot)), dim3(256), 0, s, op,
-                         c->g, c->prm, src, Tsrc, c->Fface[A]);
+      hipLaunchKernelGGL((k_face_sd<NDIM, P, RS, A, S, B>), dim3(grid_for(tot)), dim3(256), 0, s,
+                         op, c->g, c->prm, src, Tsrc, Ff(c, A));
     });
   }
 
-  template <int RS>
-  static void faces_rs(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, hipStream_t s) {
+  template <int RS, typename S>
+  static void faces_rs(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, hipStream_t s) {
     if constexpr (NDIM >= 2) {
       constexpr int FPB = 256 / NLP;
       const long long Na = c->g.N[0];
       const long long nblk = (long long)Tsrc * (c->g.NE / Na) * ((Na + 1 + FPB - 1) / FPB);
       c->timed(KC_FACE_SD, s, [&] {
-        hipLaunchKernelGGL((k_face_sd_x<NDIM, P, RS>), dim3((unsigned)nblk), dim3(256), 0, s, op,
-                           c->g, c->prm, src, Tsrc, c->Fface[0]);
+        hipLaunchKernelGGL((k_face_sd_x<NDIM, P, RS, S, B>), dim3((unsigned)nblk), dim3(256), 0,
+                           s, op, c->g, c->prm, src, Tsrc, Ff(c, 0));
       });
     }
     if constexpr (NDIM == 1) face_generic<RS, 0>(c, op, src, Tsrc, s);
     if constexpr (NDIM >= 2) face_generic<RS, 1>(c, op, src, Tsrc, s);
     if constexpr (NDIM >= 3) face_generic<RS, 2>(c, op, src, Tsrc, s);
   }
-  static void faces(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, hipStream_t s) {
+  template <typename S>
+  static void faces(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, hipStream_t s) {
     if (c->cfg.riemann_sd == SPD_RS_HLLC)
       faces_rs<RS_HLLC>(c, op, src, Tsrc, s);
     else
       faces_rs<RS_LLF>(c, op, src, Tsrc, s);
   }
-  static void pred(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, const double* Usp,
-                   const double* dt, double* out, int mode, hipStream_t s) {
+  template <typename S>
+  static void pred(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
+                   const double* dt, B* out_ad, double* out_sp, int mode, hipStream_t s) {
     const int E = epb();
     const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
-    const size_t lds = (size_t)E * T * (V * NL + 1) * sizeof(double);
+    const size_t lds = (size_t)E * T * (V * NL + 1) * sizeof(B);
     c->timed(KC_PRED, s, [&] {
-      hipLaunchKernelGGL((k_pred<NDIM, P>), dim3(blocks), dim3(E * T * NLP), lds, s, op, c->g,
-                         c->prm, src, Tsrc, c->Fface[0], c->Fface[1], c->Fface[2], Usp, dt, out,
-                         mode, E);
+      hipLaunchKernelGGL((k_pred<NDIM, P, S, B>), dim3(blocks), dim3(E * T * NLP), lds, s, op,
+                         c->g, c->prm, src, Tsrc, Ff(c, 0), Ff(c, 1), Ff(c, 2), Usp, dt, out_ad,
+                         out_sp, mode, E);
     });
   }
   template <int RS>
@@ -1640,8 +1662,8 @@ template <int NDIM, int P> struct Stepper {
     long long n1 = 1;
     for (int b = 0; b < NDIM; ++b) n1 *= c->g.Ns[b] + 2;
     c->timed(KC_HANCOCK, s, [&] {
-      hipLaunchKernelGGL((k_hancock<NDIM>), dim3(grid_for(n1)), dim3(256), 0, s, c->g, c->prm,
-                         c->Ucv, c->theta, dt, w, c->HS);
+      hipLaunchKernelGGL((k_hancock<NDIM, B>), dim3(grid_for(n1)), dim3(256), 0, s, c->g, c->prm,
+                         c->Ucv, c->theta, dt, w, (B*)c->HS);
     });
     fbface_one<RS, 0>(c, s);
     if constexpr (NDIM >= 2) fbface_one<RS, 1>(c, s);
@@ -1651,43 +1673,43 @@ template <int NDIM, int P> struct Stepper {
     long long nface = 1;
     for (int b = 0; b < NDIM; ++b) nface *= c->g.Ns[b] + (b == A ? 1 : 0);
     c->timed(KC_FBFLUX, s, [&] {
-      hipLaunchKernelGGL((k_fbface<NDIM, RS, A>), dim3(grid_for(nface)), dim3(256), 0, s, c->g,
-                         c->prm, c->HS, c->theta, c->Fho[A]);
+      hipLaunchKernelGGL((k_fbface<NDIM, RS, A, B>), dim3(grid_for(nface)), dim3(256), 0, s, c->g,
+                         c->prm, (const B*)c->HS, c->theta, Fh(c, A));
     });
   }
 
   template <int A>
-  static void hoflux_generic(spd_ctx* c, const Ops<P>& op, int i, hipStream_t s) {
+  static void hoflux_generic(spd_ctx* c, const OpsB& op, int i, hipStream_t s) {
     c->timed(KC_HOFLUX, s, [&] {
-      hipLaunchKernelGGL((k_hoflux<NDIM, P, A>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s, op,
-                         c->g, c->prm, i, c->Uad, c->Fface[A], c->Fho[A]);
+      hipLaunchKernelGGL((k_hoflux<NDIM, P, A, B>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s,
+                         op, c->g, c->prm, i, Uad(c), Ff(c, A), Fh(c, A));
+    });
+  }
+  static void apply(spd_ctx* c, const double* dt, double w, double* out, hipStream_t s) {
+    c->timed(KC_APPLY, s, [&] {
+      hipLaunchKernelGGL((k_apply<NDIM, B>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
+                         c->Ucv, Fh(c, 0), Fh(c, 1), Fh(c, 2), dt, w, out);
     });
   }
 
-  static int ho_epb() { return std::max(1, 32768 / (V * NL * (int)sizeof(double))); }
+  static int ho_epb() { return std::max(1, 32768 / (V * NL * (int)sizeof(B))); }
 
   // One FV substep with the 2-D tiled kernels; the corrected state goes to the
   // other state buffer and the buffers are swapped.
-  static void substep2d(spd_ctx* c, const Ops<P>& op, int i, double w, const double* dt,
+  static void substep2d(spd_ctx* c, const OpsB& op, int i, double w, const double* dt,
                         hipStream_t s) {
     const int E = ho_epb();
     const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
     c->timed(KC_HOFLUX, s, [&] {
-      hipLaunchKernelGGL((fv2d::k_ho2d<P>), dim3(blocks), dim3(256),
-                         (size_t)V * E * NL * sizeof(double), s, op, c->g, c->prm, i, c->Uad,
-                         c->Fface[0], c->Fface[1], c->Fho[0], c->Fho[1], E);
+      hipLaunchKernelGGL((fv2d::k_ho2d<P, B>), dim3(blocks), dim3(256),
+                         (size_t)V * E * NL * sizeof(B), s, op, c->g, c->prm, i, Uad(c), Ff(c, 0),
+                         Ff(c, 1), Fh(c, 0), Fh(c, 1), E);
     });
     if (!c->cfg.FB) {
-      c->timed(KC_APPLY, s, [&] {
-        hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
-                           c->Ucv, c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Ucv);
-      });
+      apply(c, dt, w, c->Ucv, s);
       return;
     }
-    c->timed(KC_APPLY, s, [&] {
-      hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g, c->Ucv,
-                         c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Unew);
-    });
+    apply(c, dt, w, c->Unew, s);
     if (c->need_gate) hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
     const dim3 tiles((c->g.Ns[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ns[1] + fv2d::TY - 1) / fv2d::TY);
     const dim3 gtiles((c->g.Ng[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ng[1] + fv2d::TY - 1) / fv2d::TY);
@@ -1702,11 +1724,11 @@ template <int NDIM, int P> struct Stepper {
     tap(c, i, s);
     c->timed(KC_FBFLUX, s, [&] {
       if (c->cfg.riemann_fv == SPD_RS_HLLC)
-        hipLaunchKernelGGL(fv2d::k_fbcorrect2d<RS_HLLC>, tiles, dim3(fv2d::NT), 0, s, c->g,
-                           c->prm, c->Ucv, c->theta, c->Fho[0], c->Fho[1], dt, w, c->Unew);
+        hipLaunchKernelGGL((fv2d::k_fbcorrect2d<RS_HLLC, B>), tiles, dim3(fv2d::NT), 0, s, c->g,
+                           c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, c->Unew);
       else
-        hipLaunchKernelGGL(fv2d::k_fbcorrect2d<RS_LLF>, tiles, dim3(fv2d::NT), 0, s, c->g,
-                           c->prm, c->Ucv, c->theta, c->Fho[0], c->Fho[1], dt, w, c->Unew);
+        hipLaunchKernelGGL((fv2d::k_fbcorrect2d<RS_LLF, B>), tiles, dim3(fv2d::NT), 0, s, c->g,
+                           c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, c->Unew);
     });
     std::swap(c->Ucv, c->Unew);
   }
@@ -1731,18 +1753,19 @@ template <int NDIM, int P> struct Stepper {
   static int step(spd_ctx* c, double* Usp, double* Ucv_sd, double* Wcv_sd, const double* dt,
                   int snapshot, hipStream_t s) {
     const Ops<P> op = make_ops<P>(*c->ops);
+    const OpsB opb = make_ops<P, B>(*c->ops);
     if (!c->g.skip) c->g.skip = c->zero_flag;
     const int m = c->cfg.m;
     // --- ADER predictor (src/sdader_simulator.py:160-200)
-    faces(c, op, Usp, 1, s);
-    pred(c, op, Usp, 1, Usp, dt, c->Uad, 0, s);
+    faces(c, opb, (const double*)Usp, 1, s);
+    pred(c, opb, (const double*)Usp, 1, Usp, dt, Uad(c), nullptr, 0, s);
     for (int it = 1; it < m; ++it) {
-      faces(c, op, c->Uad, T, s);
-      pred(c, op, c->Uad, T, Usp, dt, c->Uad, 0, s);
+      faces(c, opb, (const B*)Uad(c), T, s);
+      pred(c, opb, (const B*)Uad(c), T, Usp, dt, Uad(c), nullptr, 0, s);
     }
-    faces(c, op, c->Uad, T, s);
+    faces(c, opb, (const B*)Uad(c), T, s);
     if (c->cfg.update == SPD_UPDATE_SD) {
-      pred(c, op, c->Uad, T, Usp, dt, Usp, 1, s);
+      pred(c, opb, (const B*)Uad(c), T, Usp, dt, nullptr, Usp, 1, s);
       return hipGetLastError() == hipSuccess ? SPD_OK : fail(SPD_E_HIP, "kernel launch");
     }
     // --- FV corrector (src/sdader_simulator.py:288-349)
@@ -1753,17 +1776,14 @@ template <int NDIM, int P> struct Stepper {
     for (int i = 0; i < T; ++i) {
       const double w = c->ops->w_tp[i];
       if constexpr (NDIM == 2) {
-        substep2d(c, op, i, w, dt, s);
+        substep2d(c, opb, i, w, dt, s);
         continue;
       }
-      hoflux_generic<0>(c, op, i, s);
-      if constexpr (NDIM >= 2) hoflux_generic<1>(c, op, i, s);
-      if constexpr (NDIM >= 3) hoflux_generic<2>(c, op, i, s);
+      hoflux_generic<0>(c, opb, i, s);
+      if constexpr (NDIM >= 2) hoflux_generic<1>(c, opb, i, s);
+      if constexpr (NDIM >= 3) hoflux_generic<2>(c, opb, i, s);
       if (c->cfg.FB) {
-        c->timed(KC_APPLY, s, [&] {
-          hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
-                             c->Ucv, c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Unew);
-        });
+        apply(c, dt, w, c->Unew, s);
         if (c->need_gate) hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
         c->timed(KC_DETECT, s, [&] {
           hipLaunchKernelGGL((k_detect<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
@@ -1786,10 +1806,7 @@ template <int NDIM, int P> struct Stepper {
         else
           fb_rs<RS_LLF>(c, dt, w, s);
       }
-      c->timed(KC_APPLY, s, [&] {
-        hipLaunchKernelGGL((k_apply<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
-                           c->Ucv, c->Fho[0], c->Fho[1], c->Fho[2], dt, w, c->Ucv);
-      });
+      apply(c, dt, w, c->Ucv, s);
     }
     const unsigned nb = grid_for(c->g.NE * NL);
     c->timed(KC_CV2SP, s, [&] {
@@ -1817,24 +1834,28 @@ using StepFn = int (*)(spd_ctx*, double*, double*, double*, const double*, int, 
 using TensorFn = int (*)(spd_ctx*, int, const double*, double*, hipStream_t);
 
 struct Entry {
-  int ndim, p;
+  int ndim, p, precision;
   StepFn step;
   TensorFn tensor;
 };
 
-#define SPD_ENTRY(D, P) {D, P, &Stepper<D, P>::step, &Stepper<D, P>::tensor}
+#define SPD_ENTRY(D, P, PR, B) {D, P, PR, &Stepper<D, P, B>::step, &Stepper<D, P, B>::tensor}
+#define SPD_ENTRIES(D, P) \
+  SPD_ENTRY(D, P, SPD_PRECISION_FP64, double), SPD_ENTRY(D, P, SPD_PRECISION_FP32, float)
 const Entry kEntries[] = {
-    SPD_ENTRY(1, 1), SPD_ENTRY(1, 2), SPD_ENTRY(1, 3), SPD_ENTRY(1, 4), SPD_ENTRY(1, 5),
-    SPD_ENTRY(2, 1), SPD_ENTRY(2, 2), SPD_ENTRY(2, 3), SPD_ENTRY(2, 4), SPD_ENTRY(2, 5),
-    SPD_ENTRY(3, 1), SPD_ENTRY(3, 2), SPD_ENTRY(3, 3),
+    SPD_ENTRIES(1, 1), SPD_ENTRIES(1, 2), SPD_ENTRIES(1, 3), SPD_ENTRIES(1, 4), SPD_ENTRIES(1, 5),
+    SPD_ENTRIES(2, 1), SPD_ENTRIES(2, 2), SPD_ENTRIES(2, 3), SPD_ENTRIES(2, 4), SPD_ENTRIES(2, 5),
+    SPD_ENTRIES(3, 1), SPD_ENTRIES(3, 2), SPD_ENTRIES(3, 3),
 };
+#undef SPD_ENTRIES
 #undef SPD_ENTRY
 
-const Entry* find_entry(int ndim, int p) {
+const Entry* find_entry(int ndim, int p, int precision) {
   for (const Entry& e : kEntries)
-    if (e.ndim == ndim && e.p == p) return &e;
+    if (e.ndim == ndim && e.p == p && e.precision == precision) return &e;
   return nullptr;
 }
+size_t bulk_size(int precision) { return precision == SPD_PRECISION_FP32 ? 4 : 8; }
 
 }  // namespace
 
@@ -1887,7 +1908,9 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
   *out = nullptr;
   const int nd = cfg->ndim;
   if (nd < 1 || nd > 3) return fail(SPD_E_INVALID, "ndim must be 1, 2 or 3");
-  if (!find_entry(nd, cfg->p))
+  if (cfg->precision != SPD_PRECISION_FP64 && cfg->precision != SPD_PRECISION_FP32)
+    return fail(SPD_E_INVALID, "precision must be SPD_PRECISION_FP64 or SPD_PRECISION_FP32");
+  if (!find_entry(nd, cfg->p, cfg->precision))
     return fail(SPD_E_UNSUPPORTED, "no kernels built for ndim=" + std::to_string(nd) +
                                        " p=" + std::to_string(cfg->p));
   if (cfg->m != cfg->p) return fail(SPD_E_UNSUPPORTED, "only m == p is built");
@@ -1999,17 +2022,24 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
   }
   // scratch
   const long long V = c->V, T = c->T, NL = c->NL, NLP = c->NLP;
-  if ((rc = c->alloc(&c->Uad, (size_t)(T * V * g.NE * NL))) != SPD_OK) return rc;
+  const size_t bsz = bulk_size(cfg->precision);
+  auto alloc_bulk = [&](void** ptr, long long count) {
+    unsigned char* q = nullptr;
+    const int r = c->alloc(&q, (size_t)count * bsz);
+    *ptr = q;
+    return r;
+  };
+  if ((rc = alloc_bulk(&c->Uad, T * V * g.NE * NL)) != SPD_OK) return rc;
   for (int a = 0; a < nd; ++a) {
     const long long Na = g.N[a];
-    if ((rc = c->alloc(&c->Fface[a], (size_t)(T * V * (Na + 1) * (g.NE / Na) * NLP))) != SPD_OK)
+    if ((rc = alloc_bulk(&c->Fface[a], T * V * (Na + 1) * (g.NE / Na) * NLP)) != SPD_OK)
       return rc;
   }
   if (cfg->update == SPD_UPDATE_FV) {
     for (int a = 0; a < nd; ++a) {
       long long nface = 1;
       for (int b = 0; b < nd; ++b) nface *= g.Ns[b] + (b == a ? 1 : 0);
-      if ((rc = c->alloc(&c->Fho[a], (size_t)(V * nface))) != SPD_OK) return rc;
+      if ((rc = alloc_bulk(&c->Fho[a], V * nface)) != SPD_OK) return rc;
     }
     if ((rc = c->alloc(&c->Ucv, (size_t)(V * g.Nc))) != SPD_OK) return rc;
     if (cfg->FB) {
@@ -2019,7 +2049,7 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
       if ((rc = c->alloc(&c->theta, (size_t)g.Ngc)) != SPD_OK) return rc;
       long long n1 = 1;
       for (int a = 0; a < nd; ++a) n1 *= g.Ns[a] + 2;
-      if ((rc = c->alloc(&c->HS, (size_t)((1 + nd) * V * n1))) != SPD_OK) return rc;
+      if ((rc = alloc_bulk(&c->HS, (1 + nd) * V * n1)) != SPD_OK) return rc;
     }
   }
   if ((rc = c->alloc(&c->any_flag, 1)) != SPD_OK) return rc;
@@ -2093,7 +2123,7 @@ int spd_step(spd_ctx* ctx, double* U_sp, double* U_cv, double* W_cv, const doubl
   if (!ctx || !U_sp || !dt_dev) return fail(SPD_E_INVALID, "null argument");
   if (snapshot && ctx->cfg.update == SPD_UPDATE_FV && (!U_cv || !W_cv))
     return fail(SPD_E_INVALID, "snapshot needs U_cv and W_cv");
-  const Entry* e = find_entry(ctx->ndim, ctx->p);
+  const Entry* e = find_entry(ctx->ndim, ctx->p, ctx->cfg.precision);
   return e->step(ctx, U_sp, U_cv, W_cv, dt_dev, snapshot, (hipStream_t)stream);
 }
 
@@ -2109,7 +2139,7 @@ int spd_run_begin(spd_ctx* ctx, double time, double t_end, long long n_step, lon
 int spd_run_steps(spd_ctx* ctx, double* U_sp, int nsteps, void* stream) {
   if (!ctx || !U_sp || nsteps < 0) return fail(SPD_E_INVALID, "invalid argument");
   hipStream_t s = (hipStream_t)stream;
-  const Entry* en = find_entry(ctx->ndim, ctx->p);
+  const Entry* en = find_entry(ctx->ndim, ctx->p, ctx->cfg.precision);
   for (int k = 0; k < nsteps; ++k) {
     hipLaunchKernelGGL(k_run_dt, dim3(1), dim3(64), 0, s, ctx->cmax, ctx->h_dt, ctx->cfg.cfl_coeff,
                        ctx->run, ctx->run_dt, ctx->run_skip);
@@ -2138,12 +2168,12 @@ int spd_run_status(spd_ctx* ctx, double* time, long long* n_step, int* done, dou
 
 int spd_sp_to_cv(spd_ctx* ctx, const double* in, double* out, void* stream) {
   if (!ctx || !in || !out) return fail(SPD_E_INVALID, "null argument");
-  return find_entry(ctx->ndim, ctx->p)->tensor(ctx, 1, in, out, (hipStream_t)stream);
+  return find_entry(ctx->ndim, ctx->p, ctx->cfg.precision)->tensor(ctx, 1, in, out, (hipStream_t)stream);
 }
 
 int spd_cv_to_sp(spd_ctx* ctx, const double* in, double* out, void* stream) {
   if (!ctx || !in || !out) return fail(SPD_E_INVALID, "null argument");
-  return find_entry(ctx->ndim, ctx->p)->tensor(ctx, 0, in, out, (hipStream_t)stream);
+  return find_entry(ctx->ndim, ctx->p, ctx->cfg.precision)->tensor(ctx, 0, in, out, (hipStream_t)stream);
 }
 
 int spd_cons_to_prims(spd_ctx* ctx, const double* U, double* W, long long count, void* stream) {

@@ -123,6 +123,12 @@ class SDADER_Simulator:
     ``graph_steps`` steps are captured once in a HIP graph and replayed (0: no
     graph, the chunk's kernels are enqueued directly), with one host
     synchronisation per chunk.  The numbers are those of the host loop.
+
+    ``precision="fp32"`` selects the FP32-bulk form: predictor, interpolation,
+    Riemann solvers and MUSCL-Hancock in FP32; the state, the conservative
+    update accumulation, the pressure recovery from total energy, trouble
+    detection and dt stay FP64.  ``"fp64"`` (default) is the reference's
+    arithmetic.
     """
 
     def __init__(self, init_fct=_ic.sine_wave, eq_fct=_ic.sine_wave, p=1, m=-1, N=(32, 32),
@@ -133,7 +139,8 @@ class SDADER_Simulator:
                  riemann_solver_sd="llf", update="SD", riemann_solver_fv="llf",
                  slope_limiter="minmod", predictor=True, FB=False, tolerance=1e-5,
                  SED=True, NAD="", PAD=True, blending=True, min_rho=1e-10, max_rho=1e10,
-                 min_P=1e-10, godunov=False, device=None, device_loop=True, graph_steps=16):
+                 min_P=1e-10, godunov=False, device=None, device_loop=True, graph_steps=16,
+                 precision="fp64"):
         if m == -1:
             m = p
         for flag, name in ((isothermal, "isothermal"), (viscosity, "viscosity"),
@@ -155,6 +162,9 @@ class SDADER_Simulator:
             raise NotImplementedError("m != p is not implemented")
         if Nghc != 2:
             raise NotImplementedError("Nghc != 2 is not implemented")
+        if precision not in ("fp64", "fp32"):
+            raise ValueError(f"precision must be 'fp64' or 'fp32', got {precision!r}")
+        self.precision = precision
         if not torch.cuda.is_available():
             raise RuntimeError("SDADER_Simulator needs a ROCm GPU (torch.cuda is unavailable)")
 
@@ -226,6 +236,7 @@ class SDADER_Simulator:
         cfg.SED, cfg.PAD, cfg.blending = int(bool(SED)), int(bool(PAD)), int(bool(blending))
         cfg.min_rho, cfg.max_rho, cfg.min_P = min_rho, max_rho, min_P
         cfg.godunov = int(bool(godunov))
+        cfg.precision = _lib.PRECISION[precision]
         self._cfg = cfg
         ctx = ctypes.c_void_p()
         with torch.cuda.device(self.device):
