@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define SPD_ABI_VERSION 2
+#define SPD_ABI_VERSION 3
 
 enum spd_status {
   SPD_OK = 0,
@@ -37,7 +37,8 @@ enum spd_status {
   SPD_E_NOMEM = -4
 };
 
-enum spd_riemann { SPD_RS_LLF = 0, SPD_RS_HLLC = 1 };
+enum spd_riemann { SPD_RS_LLF = 0, SPD_RS_HLLC = 1, SPD_RS_LHLLC = 2 };
+enum spd_limiter { SPD_LIMITER_MINMOD = 0, SPD_LIMITER_MONCEN = 1 };
 enum spd_bc { SPD_BC_PERIODIC = 0, SPD_BC_GRADFREE = 1, SPD_BC_REFLECTIVE = 2 };
 enum spd_update { SPD_UPDATE_SD = 0, SPD_UPDATE_FV = 1 };
 enum spd_nad { SPD_NAD_RELATIVE = 0, SPD_NAD_DELTA = 1 };
@@ -75,6 +76,7 @@ typedef struct spd_config {
   double min_rho, max_rho, min_P;
   int godunov;
   int precision;          /* spd_precision */
+  int slope_limiter;      /* spd_limiter of the MUSCL-Hancock fallback */
 } spd_config;
 
 typedef struct spd_ctx spd_ctx;

@@ -208,8 +208,11 @@ def riemann_llf(WL, WR, UL, UR, vels, P, gamma, min_c2):
     return 0.5 * (FR + FL) - 0.5 * cmax * (UR - UL)
 
 
-def riemann_hllc(WL, WR, UL, UR, vels, P, gamma, min_c2):
-    """``src/riemann_solver.py:67-145`` (wave speeds Q8)."""
+def riemann_hllc(WL, WR, UL, UR, vels, P, gamma, min_c2, low_mach=False):
+    """``src/riemann_solver.py:67-145`` (wave speeds Q8).  With ``low_mach`` the
+    low-Mach variant ``lhllc`` (``src/riemann_solver.py:147-212``): the
+    velocity-jump term of the star pressure is scaled by phi = M (2 - M),
+    M = min(1, max(|v_L|, |v_R|) / c_max)."""
     v1 = vels[0]
     cL = np.sqrt(cs2(WL[P], WL[0], gamma, min_c2)) + np.abs(WL[v1])
     cR = np.sqrt(cs2(WR[P], WR[0], gamma, min_c2)) + np.abs(WR[v1])
@@ -221,7 +224,12 @@ def riemann_hllc(WL, WR, UL, UR, vels, P, gamma, min_c2):
     rcL = WL[0] * (vL - sL)
     rcR = WR[0] * (sR - vR)
     vs = (rcR * vR + rcL * vL + (WL[P] - WR[P])) / (rcR + rcL)
-    Ps = (rcR * WL[P] + rcL * WR[P] + rcL * rcR * (vL - vR)) / (rcR + rcL)
+    if low_mach:
+        mach = np.minimum(1, np.maximum(np.abs(vL), np.abs(vR)) / cmax)
+        phi = mach * (2 - mach)
+        Ps = (rcR * WL[P] + rcL * WR[P] + phi * rcL * rcR * (vL - vR)) / (rcR + rcL)
+    else:
+        Ps = (rcR * WL[P] + rcL * WR[P] + rcL * rcR * (vL - vR)) / (rcR + rcL)
     rsL = WL[0] * (sL - vL) / (sL - vs)
     esL = ((sL - vL) * UL[P] - WL[P] * vL + Ps * vs) / (sL - vs)
     rsR = WR[0] * (sR - vR) / (sR - vs)
@@ -247,8 +255,26 @@ def riemann(name, ML, MR, vels, P, gamma, min_c2, prims_in):
     else:
         UL, UR = ML, MR
         WL, WR = prims(UL, vels, P, gamma), prims(UR, vels, P, gamma)
+    if name == "lhllc":
+        return riemann_hllc(WL, WR, UL, UR, vels, P, gamma, min_c2, low_mach=True)
     f = {"llf": riemann_llf, "hllc": riemann_hllc}[name]
     return f(WL, WR, UL, UR, vels, P, gamma, min_c2)
+
+
+def minmod(dL, dR):
+    """``src/muscl.py:12-35`` (Q7): (R/L clipped to [0, 1]) * L."""
+    r = dR / dL
+    r = np.where(r < 1, r, 1)
+    return np.where(r > 0, r, 0) * dL
+
+
+def moncen(dL, dR, xL, xR, xM):
+    """``src/muscl.py:37-59``: monotonized central slope from the one-sided
+    slopes dL, dR, the centre spacings xL, xR and the cell width xM."""
+    dC = (xL * dL + xR * dR) / (xL + xR)
+    slope = np.minimum(np.abs(2 * dL * xL / xM), np.abs(2 * dR * xR / xM))
+    slope = np.sign(dC) * np.minimum(slope, np.abs(dC))
+    return np.where(dL * dR >= 0, slope, 0)
 
 
 # ---------------------------------------------------------------------------
@@ -296,8 +322,9 @@ class OracleSimulator:
                  **unused):
         if m == -1:
             m = p
-        if slope_limiter != "minmod":
+        if slope_limiter not in ("minmod", "moncen"):
             raise NotImplementedError(slope_limiter)
+        self.slope_limiter = slope_limiter
         self.p, self.m = p, m
         self.ndim = nd = len(N)
         self.N = list(N)
@@ -787,9 +814,12 @@ class OracleSimulator:
                 dM = (Mfv[_sl(nt, ax, slice(1, None))] - Mfv[_sl(nt, ax, slice(None, -1))]) / hcv
                 L = dM[_sl(nt, ax, slice(None, -1))]
                 R = dM[_sl(nt, ax, slice(1, None))]
-                r = R / L
-                r = np.where(r < 1, r, 1)
-                dMh = np.where(r > 0, r, 0) * L
+                if self.slope_limiter == "moncen":
+                    # src/muscl.py:61-75: dx_L = h_cv[:-1], dx_R = h_cv[1:], dx_M = h_fp[1:-1]
+                    dMh = moncen(L, R, hcv[_sl(nt, ax, slice(None, -1))],
+                                 hcv[_sl(nt, ax, slice(1, None))], hfp[_sl(nt, ax, slice(1, -1))])
+                else:
+                    dMh = minmod(L, R)
                 S[a] = 0.5 * dMh * hfp[_sl(nt, ax, slice(1, -1))]
                 crop = [slice(None)] * nt
                 for b in range(nd):

@@ -14,7 +14,8 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libspd_hip.so")
 
-RS = {"llf": 0, "hllc": 1}
+RS = {"llf": 0, "hllc": 1, "lhllc": 2}
+LIMITER = {"minmod": 0, "moncen": 1}
 BC = {"periodic": 0, "gradfree": 1, "reflective": 2}
 UPDATE = {"SD": 0, "FV": 1}
 PRECISION = {"fp64": 0, "fp32": 1}
@@ -56,6 +57,7 @@ class SpdConfig(ctypes.Structure):
         ("min_P", ctypes.c_double),
         ("godunov", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("slope_limiter", ctypes.c_int),
     ]
 
 

@@ -161,10 +161,13 @@ __device__ __forceinline__ void llf(const R* WL, const R* WR, const R* UL, const
 }
 
 // HLLC flux in the Godunov-state form of src/riemann_solver.py:67-145.  Only
-// the selected star state is evaluated; its formula is the reference's.
+// the selected star state is evaluated; its formula is the reference's.  With
+// low_mach the lhllc variant (src/riemann_solver.py:147-212): the velocity
+// jump term of the star pressure is scaled by phi = M (2 - M),
+// M = min(1, max(|v_L|, |v_R|) / c_max).
 template <int NDIM, typename R>
 __device__ __forceinline__ void hllc(const R* WL, const R* WR, const R* UL, const R* UR, R* F,
-                                     int a, R gamma, R min_c2) {
+                                     int a, R gamma, R min_c2, int low_mach) {
   constexpr int P = Dims<NDIM>::P;
   const int v1 = a + 1;
   const R cL = sound<R>(WL[P], WL[0], gamma, min_c2) + fabs(WL[v1]);
@@ -179,7 +182,13 @@ __device__ __forceinline__ void hllc(const R* WL, const R* WR, const R* UL, cons
   // most an ulp per value)
   const R rden = qrcp<R>(rcR + rcL);
   const R vs = (rcR * vR + rcL * vL + (WL[P] - WR[P])) * rden;
-  const R Ps = (rcR * WL[P] + rcL * WR[P] + rcL * rcR * (vL - vR)) * rden;
+  R jump = rcL * rcR * (vL - vR);
+  if (low_mach) {
+    const R aL = fabs(vL), aR = fabs(vR);
+    const R mach = nmin(R(1), qdiv<R>(nmax(aL, aR), cmax));
+    jump = mach * (R(2) - mach) * rcL * rcR * (vL - vR);
+  }
+  const R Ps = (rcR * WL[P] + rcL * WR[P] + jump) * rden;
   R r, v, p, e;
   if (sL > R(0)) {
     r = WL[0]; v = vL; p = WL[P]; e = UL[P];
@@ -206,13 +215,29 @@ __device__ __forceinline__ void hllc(const R* WL, const R* WR, const R* UL, cons
   }
 }
 
+// RS_HLLC covers hllc and lhllc (low_mach != 0).
 template <int NDIM, int RS, typename R>
 __device__ __forceinline__ void riemann(const R* WL, const R* WR, const R* UL, const R* UR, R* F,
-                                        int a, R gamma, R min_c2) {
+                                        int a, R gamma, R min_c2, int low_mach) {
   if constexpr (RS == RS_HLLC)
-    hllc<NDIM, R>(WL, WR, UL, UR, F, a, gamma, min_c2);
+    hllc<NDIM, R>(WL, WR, UL, UR, F, a, gamma, min_c2, low_mach);
   else
     llf<NDIM, R>(WL, WR, UL, UR, F, a, gamma, min_c2);
+}
+
+// Slope limiters of src/muscl.py:12-59 applied to the one-sided slopes dL, dR
+// of a cell; xL, xR = centre spacings, xM = cell width (moncen only).
+template <typename R> __device__ __forceinline__ R minmod(R dL, R dR) {
+  R r = qdiv<R>(dR, dL);
+  r = (r < R(1)) ? r : R(1);
+  return ((r > R(0)) ? r : R(0)) * dL;
+}
+template <typename R> __device__ __forceinline__ R moncen(R dL, R dR, R xL, R xR, R xM) {
+  const R dC = qdiv<R>(xL * dL + xR * dR, xL + xR);
+  R slope = nmin(fabs(qdiv<R>(R(2) * dL * xL, xM)), fabs(qdiv<R>(R(2) * dR * xR, xM)));
+  const R sg = (dC > R(0)) ? R(1) : ((dC < R(0)) ? R(-1) : dC);  // numpy.sign
+  slope = sg * nmin(slope, R(fabs(dC)));
+  return (dL * dR >= R(0)) ? slope : R(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -247,6 +272,8 @@ struct Params {
   double tol, min_rho, max_rho, min_P;
   int p;
   int nad_delta, sed, pad, blending, godunov;
+  int lowmach_sd, lowmach_fv;  // lhllc on the element / fallback faces
+  int moncen;                  // slope limiter: 0 minmod, 1 moncen
 };
 
 // Map a ghosted-grid coordinate (active range [0, Ns)) to the active cell whose

@@ -29,36 +29,46 @@ constexpr int DH = 3;
 constexpr int DNX = TX + 2 * DH, DNY = TY + 2 * DH;
 constexpr int ONX = TX + 2, ONY = TY + 2;
 
-// Smooth-extrema coefficient along one axis from LDS; w points at the cell,
-// w[k*stride] is its k-th neighbour along the axis (src/trouble_detection.py:
-// 104-137 with the compute_min edge rules).
+// Smooth-extrema quantities along one axis from LDS (src/trouble_detection.py:
+// 104-137).  w points at the cell of coordinate x; w[k*stride] is its k-th
+// neighbour along the axis.  dU is zero outside [-1, Ns].
+__device__ __forceinline__ double sed_dU(const double* w, int stride, int x, int Ns,
+                                         const double* rdc) {
+  return (x >= -1 && x <= Ns) ? (w[stride] - w[-stride]) * rdc[x] : 0.0;
+}
+
+// A = min(alpha_R, alpha_L) of the cell at x (0 outside [0, Ns)); *aL = alpha_L.
+__device__ __forceinline__ double sed_A(const double* w, int stride, int x, int Ns,
+                                        const double* rdc, const double* hf, double* aLout) {
+  if (x < 0 || x >= Ns) {
+    *aLout = 1.0;
+    return 0.0;
+  }
+  const double dUm = sed_dU(w - stride, stride, x - 1, Ns, rdc);
+  const double dU0 = sed_dU(w, stride, x, Ns, rdc);
+  const double dUp = sed_dU(w + stride, stride, x + 1, Ns, rdc);
+  const double d2U = (dUp - dUm) * rdc[x];
+  const double dv = 0.5 * hf[x] * d2U;
+  const double vL = dUm - dU0;
+  const double vR = dUp - dU0;
+  double aL = -((dv < 0.0) ? ((vL > 0.0) ? vL : 0.0) : ((vL < 0.0) ? vL : 0.0)) / dv;
+  aL = (fabs(dv) <= 0.0) ? 1.0 : aL;
+  aL = (aL < 1.0) ? aL : 1.0;
+  double aR = ((dv > 0.0) ? ((vR > 0.0) ? vR : 0.0) : ((vR < 0.0) ? vR : 0.0)) / dv;
+  aR = (fabs(dv) <= 0.0) ? 1.0 : aR;
+  aR = (aR < 1.0) ? aR : 1.0;
+  *aLout = aL;
+  return (aR < aL) ? aR : aL;
+}
+
+// Smooth-extrema coefficient of the cell at ca with the compute_min edge rules
+// (first cell: min(A[0], A[1]); last cell: min(A[-2], alpha_L[-1])).
 __device__ __forceinline__ double sed_alpha_lds(const double* w, int stride, int ca, int Ns,
                                                 const double* rdc, const double* hf) {
-  double dU[5];
-#pragma unroll
-  for (int k = -2; k <= 2; ++k) {
-    const int x = ca + k;
-    dU[k + 2] = (x >= -1 && x <= Ns) ? (w[(k + 1) * stride] - w[(k - 1) * stride]) * rdc[x] : 0.0;
-  }
-  double A[3], aLc = 1.0;
+  double A[3], aLc = 1.0, aL;
 #pragma unroll
   for (int k = -1; k <= 1; ++k) {
-    const int x = ca + k;
-    if (x < 0 || x >= Ns) {
-      A[k + 1] = 0.0;
-      continue;
-    }
-    const double d2U = (dU[k + 3] - dU[k + 1]) * rdc[x];
-    const double dv = 0.5 * hf[x] * d2U;
-    const double vL = dU[k + 1] - dU[k + 2];
-    const double vR = dU[k + 3] - dU[k + 2];
-    double aL = -((dv < 0.0) ? ((vL > 0.0) ? vL : 0.0) : ((vL < 0.0) ? vL : 0.0)) / dv;
-    aL = (fabs(dv) <= 0.0) ? 1.0 : aL;
-    aL = (aL < 1.0) ? aL : 1.0;
-    double aR = ((dv > 0.0) ? ((vR > 0.0) ? vR : 0.0) : ((vR < 0.0) ? vR : 0.0)) / dv;
-    aR = (fabs(dv) <= 0.0) ? 1.0 : aR;
-    aR = (aR < 1.0) ? aR : 1.0;
-    A[k + 1] = (aR < aL) ? aR : aL;
+    A[k + 1] = sed_A(w + k * stride, stride, ca + k, Ns, rdc, hf, &aL);
     if (k == 0) aLc = aL;
   }
   if (Ns == 1) return aLc;
@@ -67,12 +77,20 @@ __device__ __forceinline__ double sed_alpha_lds(const double* w, int stride, int
   return nmin(A[0], nmin(A[1], A[2]));
 }
 
+// LDS of k_detect2d: the old primitives (tile + 1) are needed by the NAD phase
+// only; their space then holds the smooth-extrema A values of the tile (+1
+// along the axis) for rho and P.
+constexpr int DWO = 4 * ONY * ONX;
+constexpr int DAX = 2 * TY * (TX + 2), DAY = 2 * (TY + 2) * TX;
+static_assert(DAX + DAY <= DWO, "smooth-extrema table must fit the old-state tile");
+
 __global__ void __launch_bounds__(NT)
 k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ Unew,
            uint8_t* __restrict__ flags, int* __restrict__ any_flag) {
   SPD_SKIP(g);
   __shared__ double wn[2][DNY][DNX];  // new rho, P
-  __shared__ double wo[4][ONY][ONX];  // old primitives
+  __shared__ double wbuf[DWO];        // old primitives, then the A table
+  double(*wo)[ONY][ONX] = reinterpret_cast<double(*)[ONY][ONX]>(wbuf);
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
   const int Nx = g.Ns[0], Ny = g.Ns[1];
   for (int k = threadIdx.x; k < DNX * DNY; k += NT) {
@@ -103,10 +121,12 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
   __syncthreads();
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int x = x0 + tx, y = y0 + ty;
+  const bool inside = x < Nx && y < Ny;
   bool cand = false;
-  if (x < Nx && y < Ny) {
+  bool pt[4] = {false, false, false, false};
+  double Wn[4];
+  if (inside) {
     const int ox = tx + 1, oy = ty + 1;
-    double Wn[4];
     Wn[0] = wn[0][ty + DH][tx + DH];
     Wn[3] = wn[1][ty + DH][tx + DH];
     Wn[1] = Wn[2] = 0.0;
@@ -118,7 +138,6 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
       Wn[1] = W[1];
       Wn[2] = W[2];
     }
-    bool pt[4] = {false, false, false, false};
     for (int k = 0; k < nvar_nad; ++k) {
       const int v = (k == 0) ? 0 : (k == 1 ? 3 : k - 1);
       double mx = wo[v][oy][ox], mn = mx;
@@ -143,20 +162,65 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
       pt[v] = !(Wn[v] >= lo) || !(Wn[v] <= hi);
       cand = cand || pt[v];
     }
-    if (prm.p > 1 && prm.sed) {
+  }
+  if (prm.p > 1 && prm.sed) {
+    // smooth-extrema relaxation of the rho / P candidates; the A table is
+    // built only for tiles that hold one
+    const int need = __syncthreads_or((pt[0] || pt[3]) ? 1 : 0);
+    if (need) {
+      double* Ax = wbuf;        // [2][TY][TX+2], x in [x0-1, x0+TX]
+      double* Ay = wbuf + DAX;  // [2][TY+2][TX], y in [y0-1, y0+TY]
+      const double* rdcx = g.rdc[0] + NGH;
+      const double* rdcy = g.rdc[1] + NGH;
+      const double* hfx = g.hfp[0] + NGH;
+      const double* hfy = g.hfp[1] + NGH;
+      for (int k = threadIdx.x; k < DAX + DAY; k += NT) {
+        double aL;
+        if (k < DAX) {
+          const int var = k / (TY * (TX + 2)), r = k % (TY * (TX + 2));
+          const int ly = r / (TX + 2), lx = r % (TX + 2);
+          const double* w = &wn[var][ly + DH][lx - 1 + DH];
+          Ax[k] = sed_A(w, 1, x0 + lx - 1, Nx, rdcx, hfx, &aL);
+        } else {
+          const int kk = k - DAX;
+          const int var = kk / ((TY + 2) * TX), r = kk % ((TY + 2) * TX);
+          const int ly = r / TX, lx = r % TX;
+          const double* w = &wn[var][ly - 1 + DH][lx + DH];
+          Ay[kk] = sed_A(w, DNX, y0 + ly - 1, Ny, rdcy, hfy, &aL);
+        }
+      }
+      __syncthreads();
+      if (inside) {
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const int var = (k == 0) ? 0 : 3;
-        if (!pt[var]) continue;
-        const double* w = &wn[k][ty + DH][tx + DH];
-        double alpha = 1.0;
-        const double ax = sed_alpha_lds(w, 1, x, Nx, g.rdc[0] + NGH, g.hfp[0] + NGH);
-        alpha = (ax < alpha) ? ax : alpha;
-        const double ay = sed_alpha_lds(w, DNX, y, Ny, g.rdc[1] + NGH, g.hfp[1] + NGH);
-        alpha = (ay < alpha) ? ay : alpha;
-        pt[var] = pt[var] && (alpha < 1.0);
+        for (int k = 0; k < 2; ++k) {
+          const int var = (k == 0) ? 0 : 3;
+          if (!pt[var]) continue;
+          const double* w = &wn[k][ty + DH][tx + DH];
+          const double* ax = Ax + (k * TY + ty) * (TX + 2) + tx + 1;   // A at x
+          const double* ay = Ay + (k * (TY + 2) + ty + 1) * TX + tx;   // A at y
+          double alpha = 1.0;
+          double a1;
+          if (Nx == 1 || x == Nx - 1)
+            a1 = sed_alpha_lds(w, 1, x, Nx, rdcx, hfx);
+          else if (x == 0)
+            a1 = nmin(ax[0], ax[1]);
+          else
+            a1 = nmin(ax[-1], nmin(ax[0], ax[1]));
+          alpha = (a1 < alpha) ? a1 : alpha;
+          double a2;
+          if (Ny == 1 || y == Ny - 1)
+            a2 = sed_alpha_lds(w, DNX, y, Ny, rdcy, hfy);
+          else if (y == 0)
+            a2 = nmin(ay[0], ay[TX]);
+          else
+            a2 = nmin(ay[-TX], nmin(ay[0], ay[TX]));
+          alpha = (a2 < alpha) ? a2 : alpha;
+          pt[var] = pt[var] && (alpha < 1.0);
+        }
       }
     }
+  }
+  if (inside) {
     bool tr = pt[0] || pt[3];
     if (prm.pad) {
       if (!(Wn[0] >= prm.min_rho) || !(Wn[0] <= prm.max_rho) || !(Wn[3] >= prm.min_P)) tr = true;
@@ -243,13 +307,17 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
 // ---------------------------------------------------------------------------
 // MUSCL-Hancock correction on a TX x TY tile of cells (src/muscl.py:61-75,
 // 129-221, src/fv_simulator.py:88-175,189-205, src/sdader_simulator.py:325-331):
-//   1. theta on tile + 1 halo, old primitives on tile + 2 halo;
+//   0. theta on tile + 1 halo; a tile none of whose faces has a non-zero
+//      coefficient (and no godunov) returns at once: its blended fluxes are
+//      the high-order ones, so the candidate state already in U_out is the
+//      result (k_apply evaluates the same expression);
+//   1. old primitives on tile + 2 halo;
 //   2. Hancock state and half slopes of the cells that border a face with a
 //      non-zero coefficient (tile + 1 halo, plus-shaped);
 //   3. per face of the tile: fallback HLLC/LLF flux where the coefficient is
 //      non-zero, blended with the high-order flux;
 //   4. U_out = U_cv - dt_i * sum(dF/dx) for the tile's cells.
-// U_out must differ from U_cv (the halo of neighbouring tiles is read).
+// U_out holds the candidate state on entry and must differ from U_cv.
 // ---------------------------------------------------------------------------
 constexpr int CX = TX + 2, CY = TY + 2;   // tile + 1 halo
 constexpr int WX = TX + 4, WY = TY + 4;   // tile + 2 halo
@@ -277,6 +345,7 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
   const B dtib = (B)dti;
   B(*W)[WY][WX] = reinterpret_cast<B(*)[WY][WX]>(sW);
 
+  int live = prm.godunov != 0;
   for (int k = threadIdx.x; k < CX * CY; k += NT) {
     const int lx = k % CX, ly = k / CX;
     const int x = x0 + lx - 1, y = y0 + ly - 1;
@@ -284,7 +353,10 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
     if (x >= -1 && x <= Nx && y >= -1 && y <= Ny)
       th = theta[(long long)(y + NGH) * NGx + (x + NGH)];
     sT[ly][lx] = th;
+    live |= !(th == 0.0);
   }
+  if (!__syncthreads_or(live)) return;
+
   for (int k = threadIdx.x; k < WX * WY; k += NT) {
     const int lx = k % WX, ly = k / WX;
     const int x = x0 + lx - 2, y = y0 + ly - 2;
@@ -332,9 +404,8 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
         const B Wp = (b == 0) ? W[v][wy][wx + 1] : W[v][wy + 1][wx];
         const B L = (Wc[v] - Wm) * rhl;
         const B R = (Wp - Wc[v]) * rhr;
-        B r = qdiv<B>(R, L);
-        r = (r < B(1)) ? r : B(1);
-        dMh[b][v] = ((r > B(0)) ? r : B(0)) * L;
+        dMh[b][v] = prm.moncen ? moncen<B>(L, R, (B)g.hcv[b][G - 1], (B)g.hcv[b][G], hf)
+                               : minmod<B>(L, R);
         sH[4 + 4 * b + v][ly][lx] = B(0.5) * dMh[b][v] * hf;
       }
     }
@@ -377,7 +448,7 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
       }
       cons<2>(WL, UL, 0, gamma);
       cons<2>(WR, UR, 0, gamma);
-      riemann<2, RS>(WL, WR, UL, UR, Ff, 0, gamma, (B)prm.min_c2);
+      riemann<2, RS>(WL, WR, UL, UR, Ff, 0, gamma, (B)prm.min_c2, prm.lowmach_fv);
       const B tb = (B)th;
 #pragma unroll
       for (int v = 0; v < 4; ++v) F[v] = tb * Ff[v] + (B(1) - tb) * F[v];
@@ -404,7 +475,7 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
       }
       cons<2>(WL, UL, 1, gamma);
       cons<2>(WR, UR, 1, gamma);
-      riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, (B)prm.min_c2);
+      riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, (B)prm.min_c2, prm.lowmach_fv);
       const B tb = (B)th;
 #pragma unroll
       for (int v = 0; v < 4; ++v) F[v] = tb * Ff[v] + (B(1) - tb) * F[v];

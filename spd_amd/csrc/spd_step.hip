@@ -220,7 +220,7 @@ k_face_sd_x(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsr
     B WL[V], WR[V], F[V];
     prims<NDIM>(qL, WL, 0, (B)prm.gamma);
     prims<NDIM>(qR, WR, 0, (B)prm.gamma);
-    riemann<NDIM, RS>(WL, WR, qL, qR, F, 0, (B)prm.gamma, (B)prm.min_c2);
+    riemann<NDIM, RS>(WL, WR, qL, qR, F, 0, (B)prm.gamma, (B)prm.min_c2, prm.lowmach_sd);
 #pragma unroll
     for (int v = 0; v < V; ++v) buf[v][lp][fi] = F[v];
   }
@@ -328,7 +328,7 @@ k_face_sd(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
   B WL[V], WR[V], F[V];
   prims<NDIM>(qL, WL, a, (B)prm.gamma);
   prims<NDIM>(qR, WR, a, (B)prm.gamma);
-  riemann<NDIM, RS>(WL, WR, qL, qR, F, a, (B)prm.gamma, (B)prm.min_c2);
+  riemann<NDIM, RS>(WL, WR, qL, qR, F, a, (B)prm.gamma, (B)prm.min_c2, prm.lowmach_sd);
 #pragma unroll
   for (int v = 0; v < V; ++v) Fface[(long long)(t * V + v) * plane + pidx] = F[v];
 }
@@ -1131,9 +1131,9 @@ k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __re
       for (int v = 0; v < V; ++v) {
         const B L = (W[v] - Wm[v]) * rhl;
         const B R = (Wp[v] - W[v]) * rhr;
-        B r = qdiv<B>(R, L);
-        r = (r < B(1)) ? r : B(1);
-        dMh[b][v] = ((r > B(0)) ? r : B(0)) * L;
+        dMh[b][v] = prm.moncen
+                        ? moncen<B>(L, R, (B)g.hcv[b][G - 1], (B)g.hcv[b][G], hf)
+                        : minmod<B>(L, R);
         S[b][v] = B(0.5) * dMh[b][v] * hf;
       }
     }
@@ -1214,7 +1214,7 @@ k_fbface(Geom g, Params prm, const B* __restrict__ HS, const double* __restrict_
   const B gam = (B)prm.gamma;
   cons<NDIM>(WL, UL, a, gam);
   cons<NDIM>(WR, UR, a, gam);
-  riemann<NDIM, RS>(WL, WR, UL, UR, F, a, gam, (B)prm.min_c2);
+  riemann<NDIM, RS>(WL, WR, UL, UR, F, a, gam, (B)prm.min_c2, prm.lowmach_fv);
   const B tb = (B)th;
 #pragma unroll
   for (int v = 0; v < V; ++v) {
@@ -1640,7 +1640,7 @@ template <int NDIM, int P, typename B> struct Stepper {
   }
   template <typename S>
   static void faces(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, hipStream_t s) {
-    if (c->cfg.riemann_sd == SPD_RS_HLLC)
+    if (c->cfg.riemann_sd != SPD_RS_LLF)
       faces_rs<RS_HLLC>(c, op, src, Tsrc, s);
     else
       faces_rs<RS_LLF>(c, op, src, Tsrc, s);
@@ -1712,18 +1712,19 @@ template <int NDIM, int P, typename B> struct Stepper {
     apply(c, dt, w, c->Unew, s);
     if (c->need_gate) hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
     const dim3 tiles((c->g.Ns[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ns[1] + fv2d::TY - 1) / fv2d::TY);
-    const dim3 gtiles((c->g.Ng[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ng[1] + fv2d::TY - 1) / fv2d::TY);
     c->timed(KC_DETECT, s, [&] {
       hipLaunchKernelGGL(fv2d::k_detect2d, tiles, dim3(fv2d::NT), 0, s, c->g, c->prm, c->Ucv,
                          c->Unew, c->flags, c->need_gate ? c->any_flag : nullptr);
     });
+    const dim3 gtiles((c->g.Ng[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ng[1] + fv2d::TY - 1) / fv2d::TY);
     c->timed(KC_THETA1, s, [&] {
       hipLaunchKernelGGL(fv2d::k_theta2d, gtiles, dim3(fv2d::NT), 0, s, c->g, c->prm, c->flags,
                          c->Ucv, c->Unew, c->any_flag, c->theta);
     });
     tap(c, i, s);
+    // the corrected state replaces the candidate in place, tile by tile
     c->timed(KC_FBFLUX, s, [&] {
-      if (c->cfg.riemann_fv == SPD_RS_HLLC)
+      if (c->cfg.riemann_fv != SPD_RS_LLF)
         hipLaunchKernelGGL((fv2d::k_fbcorrect2d<RS_HLLC, B>), tiles, dim3(fv2d::NT), 0, s, c->g,
                            c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, c->Unew);
       else
@@ -1733,8 +1734,11 @@ template <int NDIM, int P, typename B> struct Stepper {
     std::swap(c->Ucv, c->Unew);
   }
 
+  static bool tapping(const spd_ctx* c, int i) {
+    return i == c->dbg_sub || c->dbg_sub == SPD_TAP_ALL;
+  }
   static void tap(spd_ctx* c, int i, hipStream_t s) {
-    if (!(i == c->dbg_sub || c->dbg_sub == SPD_TAP_ALL)) return;
+    if (!tapping(c, i)) return;
     const long long k = (c->dbg_sub == SPD_TAP_ALL) ? i : 0;
     double* aff_o[3] = {nullptr, nullptr, nullptr};
     long long mx = std::max(c->g.Nc, c->g.Ngc);
@@ -1801,7 +1805,7 @@ template <int NDIM, int P, typename B> struct Stepper {
                                c->theta1, c->theta);
           });
         tap(c, i, s);
-        if (c->cfg.riemann_fv == SPD_RS_HLLC)
+        if (c->cfg.riemann_fv != SPD_RS_LLF)
           fb_rs<RS_HLLC>(c, dt, w, s);
         else
           fb_rs<RS_LLF>(c, dt, w, s);
@@ -1916,9 +1920,11 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
   if (cfg->m != cfg->p) return fail(SPD_E_UNSUPPORTED, "only m == p is built");
   if (cfg->update != SPD_UPDATE_SD && cfg->update != SPD_UPDATE_FV)
     return fail(SPD_E_INVALID, "update must be SD or FV");
-  if ((cfg->riemann_sd != SPD_RS_LLF && cfg->riemann_sd != SPD_RS_HLLC) ||
-      (cfg->riemann_fv != SPD_RS_LLF && cfg->riemann_fv != SPD_RS_HLLC))
-    return fail(SPD_E_UNSUPPORTED, "riemann solver must be llf or hllc");
+  auto rs_ok = [](int r) { return r == SPD_RS_LLF || r == SPD_RS_HLLC || r == SPD_RS_LHLLC; };
+  if (!rs_ok(cfg->riemann_sd) || !rs_ok(cfg->riemann_fv))
+    return fail(SPD_E_UNSUPPORTED, "riemann solver must be llf, hllc or lhllc");
+  if (cfg->slope_limiter != SPD_LIMITER_MINMOD && cfg->slope_limiter != SPD_LIMITER_MONCEN)
+    return fail(SPD_E_UNSUPPORTED, "slope limiter must be minmod or moncen");
   for (int a = 0; a < nd; ++a) {
     if (cfg->N[a] < 1) return fail(SPD_E_INVALID, "N must be >= 1");
     if (cfg->N[a] * (cfg->p + 1) < NGH)
@@ -1980,6 +1986,9 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
   pr.pad = cfg->PAD;
   pr.blending = cfg->blending;
   pr.godunov = cfg->godunov;
+  pr.lowmach_sd = cfg->riemann_sd == SPD_RS_LHLLC;
+  pr.lowmach_fv = cfg->riemann_fv == SPD_RS_LHLLC;
+  pr.moncen = cfg->slope_limiter == SPD_LIMITER_MONCEN;
 
   // geometry arrays
   std::vector<double> gbuf;

@@ -113,8 +113,8 @@ class SDADER_Simulator:
     (``src/sdader_simulator.py:15-27``, ``src/sd_simulator.py:20-26``,
     ``src/fv_simulator.py:11-17``, ``src/simulator.py:10-36``).  Options the
     reference offers beyond this path (``viscosity``, ``potential``, ``WB``,
-    ``isothermal``, ``"ic"``/``"eq"``/``"pressure"`` sides, ``moncen``,
-    ``lhllc``, ``predictor=False``, ``m != p``, ``Nghc != 2``) raise
+    ``isothermal``, ``"ic"``/``"eq"``/``"pressure"`` sides,
+    ``predictor=False``, ``m != p``, ``Nghc != 2``) raise
     ``NotImplementedError``.  ``device`` selects the GPU (default: current).
 
     ``perform_time_evolution`` and ``perform_iterations`` run the time loop on
@@ -149,7 +149,7 @@ class SDADER_Simulator:
                 raise NotImplementedError(f"{name}=True is not implemented")
         if not predictor:
             raise NotImplementedError("predictor=False is not implemented")
-        if slope_limiter != "minmod":
+        if slope_limiter not in _lib.LIMITER:
             raise NotImplementedError(f"slope limiter {slope_limiter!r} is not implemented")
         for rs in (riemann_solver_sd, riemann_solver_fv):
             if rs not in _lib.RS:
@@ -237,6 +237,7 @@ class SDADER_Simulator:
         cfg.min_rho, cfg.max_rho, cfg.min_P = min_rho, max_rho, min_P
         cfg.godunov = int(bool(godunov))
         cfg.precision = _lib.PRECISION[precision]
+        cfg.slope_limiter = _lib.LIMITER[slope_limiter]
         self._cfg = cfg
         ctx = ctypes.c_void_p()
         with torch.cuda.device(self.device):

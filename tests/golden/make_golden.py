@@ -237,6 +237,27 @@ def main():
                 run_case(name, kw, {"kind": "sine"}, steps=3,
                          capture_steps=(1,) if mode in ("FB", "FBG") else (),
                          t_end=1.0)
+    # Low-Mach HLLC (lhllc) and the moncen limiter: 1-D Sod to t_end, 2-D
+    # four-quadrant outflow, 2-D Kelvin-Helmholtz (periodic), 3-D step.
+    LM = dict(riemann_solver_sd="lhllc", riemann_solver_fv="lhllc")
+    MC = dict(FVFB, slope_limiter="moncen")
+    if want("sod_p3_lhllc_moncen"):
+        run_case("sod_p3_lhllc_moncen", dict(p=3, N=(60,), BC=GF1, **LM, **MC),
+                 {"kind": "sod"}, steps=20, capture_steps=(1, 2),
+                 record_steps=(1, 2, 3, 10, 20), t_end=0.2)
+    if want("quad12_p3_lhllc_moncen"):
+        states = ic.random_four_quadrant_states(np.random.default_rng(4321))
+        run_case("quad12_p3_lhllc_moncen", dict(p=3, N=(12, 12), BC=GF2, **LM, **MC),
+                 {"kind": "quad", "states": states.tolist()}, steps=2, capture_steps=(1,))
+    if want("kh16_p3_moncen"):
+        run_case("kh16_p3_moncen", dict(p=3, N=(16, 16), BC=PER2, riemann_solver_sd="hllc",
+                                        riemann_solver_fv="lhllc", **MC),
+                 {"kind": "kh"}, steps=3, capture_steps=(1,), noise=kh_noise)
+    if want("step3d_p2_moncen"):
+        run_case("step3d_p2_moncen", dict(p=2, N=(5, 4, 4),
+                                          BC=GF2 + (("gradfree", "gradfree"),), **LM, **MC),
+                 {"kind": "step", "kw": {"vx": 0.5, "vy": -0.3, "vz": 0.2}},
+                 steps=1, capture_steps=(1,))
     # 3-D Taylor-Green, 4^3, p=3, periodic on [0, 2 pi]^3.
     if want("tg4_p3"):
         twopi = 2 * np.pi

@@ -32,7 +32,7 @@ def test_abi_version_and_struct_size():
     import ctypes
     from spd_amd import _lib
     lib = _lib.load()
-    assert lib.spd_abi_version() == 2
+    assert lib.spd_abi_version() == 3
     assert lib.spd_config_size() == ctypes.sizeof(_lib.SpdConfig)
 
 
@@ -98,8 +98,8 @@ def test_facade_without_gpu_fails_loudly():
 
 def test_unsupported_options_raise():
     from spd_amd.simulator import SDADER_Simulator
-    for kw in (dict(viscosity=True), dict(WB=True), dict(slope_limiter="moncen"),
-               dict(riemann_solver_sd="lhllc"), dict(predictor=False)):
+    for kw in (dict(viscosity=True), dict(WB=True), dict(slope_limiter="vanleer"),
+               dict(riemann_solver_sd="roe"), dict(predictor=False)):
         with pytest.raises(NotImplementedError):
             SDADER_Simulator(p=1, N=(4,), **kw)
 
