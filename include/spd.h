@@ -147,6 +147,31 @@ int spd_run_steps(spd_ctx* ctx, double* U_sp, int nsteps, void* stream);
 int spd_run_status(spd_ctx* ctx, double* time, long long* n_step, int* done, double* dt,
                    void* stream);
 
+/* Launch shapes that the kernels take at run time (an extension with no
+ * reference counterpart; the autotuner of spd_amd.ensemble sets them):
+ * elements per block of the predictor update and of the 2-D high-order
+ * face-flux kernel.  value 0 restores the default; values above the maximum
+ * (256 threads / 48 KiB of LDS per block) are rejected.  spd_get_launch reports
+ * the value in effect and the maximum. */
+enum spd_launch_key { SPD_LAUNCH_PRED_EPB = 0, SPD_LAUNCH_HO_EPB = 1, SPD_LAUNCH_COUNT = 2 };
+int spd_set_launch(spd_ctx* ctx, int key, int value);
+int spd_get_launch(const spd_ctx* ctx, int key, int* value, int* max_value);
+
+/* Initial condition on the device (SD_Simulator.post_init / quadrature_mean,
+ * src/sd_simulator.py:59-90, src/polynomials.py:124-148).  spd_ic_points
+ * writes the 2-point Gauss-Legendre nodes of every subcell of the active
+ * elements, layout (ndim, E_{d-1}..E_0, l_{d-1}..l_0, q_{d-1}..q_0), from the
+ * subcell lower bounds and widths of the reference's element mesh (N_a*(p+1)
+ * values per dimension, concatenated x first; device arrays).  The caller
+ * evaluates the IC at those points (one variable at a time) and spd_ic_mean
+ * reduces the values (E.., l.., q..) to subcell means (E.., l..).
+ * spd_sp_to_regular interpolates element-layout values to p+1 equispaced
+ * subcell centres (SD_Simulator.regular_mesh, src/sd_simulator.py:92-102). */
+int spd_ic_points(spd_ctx* ctx, const double* lows, const double* widths, double* pts,
+                  void* stream);
+int spd_ic_mean(spd_ctx* ctx, const double* vals, double* W_cv_var, void* stream);
+int spd_sp_to_regular(spd_ctx* ctx, const double* in, double* out, void* stream);
+
 /* Element-layout tensor-product transforms (src/transforms.py:27-45 via
  * SD_Simulator.compute_cv_from_sp / compute_sp_from_cv, src/sd_simulator.py:184-188). */
 int spd_sp_to_cv(spd_ctx* ctx, const double* in, double* out, void* stream);

@@ -20,6 +20,7 @@ BC = {"periodic": 0, "gradfree": 1, "reflective": 2}
 UPDATE = {"SD": 0, "FV": 1}
 PRECISION = {"fp64": 0, "fp32": 1}
 TAP_ALL = -2
+LAUNCH = {"pred_epb": 0, "ho_epb": 1}
 
 EXPORTS = [
     "spd_last_error", "spd_abi_version", "spd_config_size", "spd_ctx_create", "spd_ctx_destroy",
@@ -27,7 +28,8 @@ EXPORTS = [
     "spd_sp_to_cv", "spd_cv_to_sp", "spd_cons_to_prims", "spd_prims_to_cons",
     "spd_set_debug_tap", "spd_host_operators", "spd_host_geometry",
     "spd_profile_begin", "spd_profile_end", "spd_kernel_class_count", "spd_kernel_class_name",
-    "spd_run_begin", "spd_run_steps", "spd_run_status",
+    "spd_run_begin", "spd_run_steps", "spd_run_status", "spd_set_launch", "spd_get_launch",
+    "spd_ic_points", "spd_ic_mean", "spd_sp_to_regular",
 ]
 
 
@@ -122,6 +124,16 @@ def load():
     L.spd_run_status.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ll),
                                  ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double), vp]
     L.spd_run_status.restype = i
+    L.spd_ic_points.argtypes = [vp, dp, dp, dp, vp]
+    L.spd_ic_points.restype = i
+    L.spd_ic_mean.argtypes = [vp, dp, dp, vp]
+    L.spd_ic_mean.restype = i
+    L.spd_sp_to_regular.argtypes = [vp, dp, dp, vp]
+    L.spd_sp_to_regular.restype = i
+    L.spd_set_launch.argtypes = [vp, i, i]
+    L.spd_set_launch.restype = i
+    L.spd_get_launch.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.spd_get_launch.restype = i
     _lib = L
     return L
 

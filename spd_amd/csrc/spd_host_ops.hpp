@@ -171,6 +171,7 @@ struct OpsHost {
   int p, m;
   std::vector<double> sp, fp, x_tp, w_tp;
   Mat sp_to_fp, dfp_to_sp, sp_to_cv, cv_to_sp, invader;
+  Mat sp_to_reg;  // solution points -> p+1 equispaced subcell centres (regular_mesh)
 
   OpsHost(int p_, int m_) : p(p_), m(m_) {
     sp = solution_points(p);
@@ -178,6 +179,12 @@ struct OpsHost {
     sp_to_fp = lagrange(fp, sp);
     dfp_to_sp = lagrange_prime(sp, fp);
     sp_to_cv = subcell_mean(sp, fp);
+    {
+      // centres of p+1 equal subcells: x = arange(p+2)/(p+1), 0.5 (x[1:] + x[:-1])
+      std::vector<double> xr(p + 1);
+      for (int i = 0; i <= p; ++i) xr[i] = 0.5 * (double(i + 1) / (p + 1) + double(i) / (p + 1));
+      sp_to_reg = lagrange(xr, sp);
+    }
     cv_to_sp = inverse(sp_to_cv);
     gauss_legendre(m + 1, x_tp, w_tp);
     const int T = m + 1;

@@ -41,6 +41,7 @@ template <int P, typename R = double> struct Ops {
   R d2s[n][nf];  // dfp_to_sp
   R s2c[n][n];   // sp_to_cv
   R c2s[n][n];   // cv_to_sp
+  R s2r[n][n];   // solution points -> regular subcell centres
   R inv[T][T];   // invader
   double w[T];   // w_tp
 };
@@ -612,7 +613,8 @@ k_tensor(Ops<P> op, Geom g, int to_cv, const double* __restrict__ in, double* __
   const long long r = tid / NL;
   const long long e = r % g.NE;
   const int v = (int)(r / g.NE);
-  const double(*M)[n] = to_cv ? op.s2c : op.c2s;
+  // to_cv: 1 sp -> cv, 0 cv -> sp, 2 sp -> regular subcell centres
+  const double(*M)[n] = (to_cv == 1) ? op.s2c : ((to_cv == 2) ? op.s2r : op.c2s);
   int qo[3] = {0, 0, 0};
   {
     int x = lo;
@@ -1399,6 +1401,66 @@ __global__ void k_run_advance(RunState* st, const int* skip) {
   }
 }
 
+// Quadrature points of the initial-condition subcell means
+// (src/polynomials.py:124-148, src/sd_simulator.py:59-90): for every element,
+// local subcell and 2-point Gauss-Legendre node per dimension,
+//   pts[a] = low_a + width_a * xq   (separately rounded, as in the reference)
+// Layout (ndim, E_{d-1}..E_0, l_{d-1}..l_0, q_{d-1}..q_0); lows / widths hold
+// N_a * n values per dimension, concatenated.
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_ic_points(Geom g, int n, const double* __restrict__ lows, const double* __restrict__ wids,
+            double xq0, double xq1, double* __restrict__ pts) {
+  constexpr int Q = 1 << NDIM;
+  const long long npts = g.Nc * Q;  // elements * n^d * 2^d
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= npts) return;
+  long long x = tid;
+  int q[3] = {0, 0, 0}, l[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    q[a] = (int)(x % 2);
+    x /= 2;
+  }
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    l[a] = (int)(x % n);
+    x /= n;
+  }
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    e[a] = (int)(x % g.N[a]);
+    x /= g.N[a];
+  }
+  long long off = 0;
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    const long long k = off + (long long)e[a] * n + l[a];
+    const double xq = q[a] ? xq1 : xq0;
+    pts[a * npts + tid] = __dadd_rn(lows[k], __dmul_rn(wids[k], xq));
+    off += (long long)g.N[a] * n;
+  }
+}
+
+// W[e, l] = sum over the 2^d nodes of vals[e, l, q] * prod_a w[q_a].
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_ic_mean(Geom g, const double* __restrict__ vals, double w0, double w1,
+          double* __restrict__ out) {
+  constexpr int Q = 1 << NDIM;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= g.Nc) return;
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    double w = 1.0;
+#pragma unroll
+    for (int a = NDIM - 1; a >= 0; --a) w = w * (((k >> a) & 1) ? w1 : w0);
+    acc += vals[tid * Q + k] * w;
+  }
+  out[tid] = acc;
+}
+
 __global__ void k_zero_counters(int* any_flag) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *any_flag = 0;
 }
@@ -1540,6 +1602,7 @@ struct spd_ctx {
   double* partial = nullptr;  // per-block partial maxima
   long long npartial = 0;
   bool need_gate = false;     // some side is not periodic (global NAD gate needed)
+  int launch[SPD_LAUNCH_COUNT] = {0, 0};  // tuned launch shapes (0 = default)
   // debug tap
   int dbg_sub = -1;
   double* dbg_tr = nullptr;
@@ -1588,6 +1651,7 @@ template <int P, typename R = double> Ops<P, R> make_ops(const host::OpsHost& h)
     for (int b = 0; b < n; ++b) {
       o.s2c[a][b] = (R)h.sp_to_cv[a][b];
       o.c2s[a][b] = (R)h.cv_to_sp[a][b];
+      o.s2r[a][b] = (R)h.sp_to_reg[a][b];
     }
   for (int a = 0; a < T; ++a) {
     for (int b = 0; b < T; ++b) o.inv[a][b] = (R)h.invader[a][b];
@@ -1606,11 +1670,15 @@ template <int NDIM, int P, typename B> struct Stepper {
   static constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
 
   static_assert(T * NLP <= 256, "one element's lines must fit one block");
-  static int epb() {
-    int e = std::max(1, 256 / (T * NLP));
+  // Elements per block of the predictor update: the largest that fits 256
+  // threads and 48 KiB of LDS, or the tuned value (clamped to that maximum).
+  static int epb_max() {
     const int lds_per = T * (V * NL + 1) * (int)sizeof(B);
-    e = std::max(1, std::min(e, 49152 / lds_per));
-    return e;
+    return std::max(1, std::min(256 / (T * NLP), 49152 / lds_per));
+  }
+  static int epb(const spd_ctx* c) {
+    const int mx = epb_max();
+    return c->launch[SPD_LAUNCH_PRED_EPB] > 0 ? std::min(mx, c->launch[SPD_LAUNCH_PRED_EPB]) : mx;
   }
 
   template <int RS, int A, typename S>
@@ -1648,7 +1716,7 @@ template <int NDIM, int P, typename B> struct Stepper {
   template <typename S>
   static void pred(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
                    const double* dt, B* out_ad, double* out_sp, int mode, hipStream_t s) {
-    const int E = epb();
+    const int E = epb(c);
     const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
     const size_t lds = (size_t)E * T * (V * NL + 1) * sizeof(B);
     c->timed(KC_PRED, s, [&] {
@@ -1692,13 +1760,24 @@ template <int NDIM, int P, typename B> struct Stepper {
     });
   }
 
-  static int ho_epb() { return std::max(1, 32768 / (V * NL * (int)sizeof(B))); }
+  // Elements per block of the 2-D high-order flux kernel (LDS slab of the
+  // block's elements, at most 48 KiB).
+  static int ho_epb_max() { return std::max(1, 49152 / (V * NL * (int)sizeof(B))); }
+  static int ho_epb(const spd_ctx* c) {
+    const int mx = ho_epb_max();
+    return c->launch[SPD_LAUNCH_HO_EPB] > 0 ? std::min(mx, c->launch[SPD_LAUNCH_HO_EPB])
+                                            : std::max(1, 32768 / (V * NL * (int)sizeof(B)));
+  }
+  static int launch_value(const spd_ctx* c, int key) {
+    return key == SPD_LAUNCH_PRED_EPB ? epb(c) : ho_epb(c);
+  }
+  static int launch_max(int key) { return key == SPD_LAUNCH_PRED_EPB ? epb_max() : ho_epb_max(); }
 
   // One FV substep with the 2-D tiled kernels; the corrected state goes to the
   // other state buffer and the buffers are swapped.
   static void substep2d(spd_ctx* c, const OpsB& op, int i, double w, const double* dt,
                         hipStream_t s) {
-    const int E = ho_epb();
+    const int E = ho_epb(c);
     const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
     c->timed(KC_HOFLUX, s, [&] {
       hipLaunchKernelGGL((fv2d::k_ho2d<P, B>), dim3(blocks), dim3(256),
@@ -1836,14 +1915,20 @@ template <int NDIM, int P, typename B> struct Stepper {
 
 using StepFn = int (*)(spd_ctx*, double*, double*, double*, const double*, int, hipStream_t);
 using TensorFn = int (*)(spd_ctx*, int, const double*, double*, hipStream_t);
+using LaunchFn = int (*)(const spd_ctx*, int);
+using LaunchMaxFn = int (*)(int);
 
 struct Entry {
   int ndim, p, precision;
   StepFn step;
   TensorFn tensor;
+  LaunchFn launch_value;
+  LaunchMaxFn launch_max;
 };
 
-#define SPD_ENTRY(D, P, PR, B) {D, P, PR, &Stepper<D, P, B>::step, &Stepper<D, P, B>::tensor}
+#define SPD_ENTRY(D, P, PR, B)                                              \
+  {D, P, PR, &Stepper<D, P, B>::step, &Stepper<D, P, B>::tensor,            \
+   &Stepper<D, P, B>::launch_value, &Stepper<D, P, B>::launch_max}
 #define SPD_ENTRIES(D, P) \
   SPD_ENTRY(D, P, SPD_PRECISION_FP64, double), SPD_ENTRY(D, P, SPD_PRECISION_FP32, float)
 const Entry kEntries[] = {
@@ -2058,7 +2143,7 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
       if ((rc = c->alloc(&c->theta, (size_t)g.Ngc)) != SPD_OK) return rc;
       long long n1 = 1;
       for (int a = 0; a < nd; ++a) n1 *= g.Ns[a] + 2;
-      if ((rc = alloc_bulk(&c->HS, (1 + nd) * V * n1)) != SPD_OK) return rc;
+      if (nd != 2 && (rc = alloc_bulk(&c->HS, (1 + nd) * V * n1)) != SPD_OK) return rc;
     }
   }
   if ((rc = c->alloc(&c->any_flag, 1)) != SPD_OK) return rc;
@@ -2136,6 +2221,25 @@ int spd_step(spd_ctx* ctx, double* U_sp, double* U_cv, double* W_cv, const doubl
   return e->step(ctx, U_sp, U_cv, W_cv, dt_dev, snapshot, (hipStream_t)stream);
 }
 
+int spd_set_launch(spd_ctx* ctx, int key, int value) {
+  if (!ctx || key < 0 || key >= SPD_LAUNCH_COUNT || value < 0)
+    return fail(SPD_E_INVALID, "invalid launch key or value");
+  const Entry* en = find_entry(ctx->ndim, ctx->p, ctx->cfg.precision);
+  if (value > en->launch_max(key))
+    return fail(SPD_E_INVALID, "launch value above its maximum " +
+                                   std::to_string(en->launch_max(key)));
+  ctx->launch[key] = value;
+  return SPD_OK;
+}
+
+int spd_get_launch(const spd_ctx* ctx, int key, int* value, int* max_value) {
+  if (!ctx || key < 0 || key >= SPD_LAUNCH_COUNT) return fail(SPD_E_INVALID, "invalid launch key");
+  const Entry* en = find_entry(ctx->ndim, ctx->p, ctx->cfg.precision);
+  if (value) *value = en->launch_value(ctx, key);
+  if (max_value) *max_value = en->launch_max(key);
+  return SPD_OK;
+}
+
 int spd_run_begin(spd_ctx* ctx, double time, double t_end, long long n_step, long long max_steps,
                   int clamp, void* stream) {
   if (!ctx) return fail(SPD_E_INVALID, "null argument");
@@ -2173,6 +2277,56 @@ int spd_run_status(spd_ctx* ctx, double* time, long long* n_step, int* done, dou
   if (done) *done = st.done;
   if (dt) *dt = st.dt;
   return SPD_OK;
+}
+
+int spd_sp_to_regular(spd_ctx* ctx, const double* in, double* out, void* stream) {
+  if (!ctx || !in || !out) return fail(SPD_E_INVALID, "null argument");
+  return find_entry(ctx->ndim, ctx->p, ctx->cfg.precision)
+      ->tensor(ctx, 2, in, out, (hipStream_t)stream);
+}
+
+int spd_ic_points(spd_ctx* ctx, const double* lows, const double* wids, double* pts,
+                  void* stream) {
+  if (!ctx || !lows || !wids || !pts) return fail(SPD_E_INVALID, "null argument");
+  std::vector<double> xq, wq;
+  host::gauss_legendre(2, xq, wq);
+  const long long npts = ctx->g.Nc << ctx->ndim;
+  hipStream_t s = (hipStream_t)stream;
+  switch (ctx->ndim) {
+    case 1:
+      hipLaunchKernelGGL(k_ic_points<1>, dim3(grid_for(npts)), dim3(256), 0, s, ctx->g, ctx->n,
+                         lows, wids, xq[0], xq[1], pts);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_ic_points<2>, dim3(grid_for(npts)), dim3(256), 0, s, ctx->g, ctx->n,
+                         lows, wids, xq[0], xq[1], pts);
+      break;
+    default:
+      hipLaunchKernelGGL(k_ic_points<3>, dim3(grid_for(npts)), dim3(256), 0, s, ctx->g, ctx->n,
+                         lows, wids, xq[0], xq[1], pts);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+}
+
+int spd_ic_mean(spd_ctx* ctx, const double* vals, double* out, void* stream) {
+  if (!ctx || !vals || !out) return fail(SPD_E_INVALID, "null argument");
+  std::vector<double> xq, wq;
+  host::gauss_legendre(2, xq, wq);
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned nb = grid_for(ctx->g.Nc);
+  switch (ctx->ndim) {
+    case 1:
+      hipLaunchKernelGGL(k_ic_mean<1>, dim3(nb), dim3(256), 0, s, ctx->g, vals, wq[0], wq[1], out);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_ic_mean<2>, dim3(nb), dim3(256), 0, s, ctx->g, vals, wq[0], wq[1], out);
+      break;
+    default:
+      hipLaunchKernelGGL(k_ic_mean<3>, dim3(nb), dim3(256), 0, s, ctx->g, vals, wq[0], wq[1], out);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
 }
 
 int spd_sp_to_cv(spd_ctx* ctx, const double* in, double* out, void* stream) {

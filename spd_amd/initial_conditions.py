@@ -9,8 +9,9 @@ This is the convention of ``src/initial_conditions.py:6-13`` and
 averages the callable over every subcell with a 2-point Gauss-Legendre rule
 per dimension (``src/polynomials.py:124-148``).
 
-The functions are evaluated once on the host at construction time; they are
-plain NumPy and do not touch the GPU.
+The callables here accept NumPy arrays or torch tensors (the simulator
+evaluates them on the GPU at the quadrature points it computes there);
+user callables written for NumPy are evaluated on the host instead.
 """
 from __future__ import annotations
 
@@ -28,6 +29,58 @@ def _nd(xyz):
     return xyz.shape[0]
 
 
+class _TorchNS:
+    """The few NumPy functions the callables use, on torch tensors."""
+
+    def __init__(self):
+        import torch
+        self.t = torch
+        self.pi = np.pi
+
+    def where(self, c, a, b):
+        # Python numbers become FP64 tensors (torch.where would otherwise
+        # produce the default FP32 dtype for two scalar operands)
+        def f64(v):
+            if isinstance(v, self.t.Tensor):
+                return v
+            return self.t.tensor(v, dtype=self.t.float64, device=c.device)
+        return self.t.where(c, f64(a), f64(b))
+
+    def sin(self, x):
+        return self.t.sin(x)
+
+    def cos(self, x):
+        return self.t.cos(x)
+
+    def exp(self, x):
+        return self.t.exp(x)
+
+    def abs(self, x):
+        return self.t.abs(x)
+
+    def round(self, x):
+        return self.t.round(x)
+
+    def ones_like(self, x):
+        return self.t.ones_like(x)
+
+    def zeros_like(self, x):
+        return self.t.zeros_like(x)
+
+
+_TORCH_NS = None
+
+
+def _xp(x):
+    """Array namespace of ``x``: NumPy, or the torch adapter for tensors."""
+    global _TORCH_NS
+    if type(x).__module__.startswith("torch"):
+        if _TORCH_NS is None:
+            _TORCH_NS = _TorchNS()
+        return _TORCH_NS
+    return np
+
+
 def sine_wave(xyz: np.ndarray, case: int, A=0.125, vx=1.0, vy=1.0, vz=1.0, P=1.0):
     """Density sine wave advected at constant velocity and pressure.
 
@@ -35,6 +88,7 @@ def sine_wave(xyz: np.ndarray, case: int, A=0.125, vx=1.0, vy=1.0, vz=1.0, P=1.0
     dispatches to): rho = 1 + A sin(2 pi (x [+ y [+ z]])), v = const, P = const.
     In 1-D only ``vx`` is used, in 2-D ``vx, vy``.
     """
+    xp = _xp(xyz)
     nd = _nd(xyz)
     x = xyz[0]
     vel = (vx, vy, vz)[:nd]
@@ -42,12 +96,12 @@ def sine_wave(xyz: np.ndarray, case: int, A=0.125, vx=1.0, vy=1.0, vz=1.0, P=1.0
         s = xyz[0]
         for d in range(1, nd):
             s = s + xyz[d]
-        return 1.0 + A * np.sin(2 * np.pi * s)
+        return 1.0 + A * xp.sin(2 * np.pi * s)
     if 1 <= case <= nd:
-        return vel[case - 1] * np.ones(x.shape)
+        return vel[case - 1] * xp.ones_like(x)
     if case == nd + 1:
-        return P * np.ones(x.shape)
-    return np.ones(x.shape)
+        return P * xp.ones_like(x)
+    return xp.ones_like(x)
 
 
 def step_function(xyz: np.ndarray, case: int, vx=1.0, vy=1.0, vz=1.0, P=1.0):
@@ -55,19 +109,20 @@ def step_function(xyz: np.ndarray, case: int, vx=1.0, vy=1.0, vz=1.0, P=1.0):
 
     Follows ``src/initial_conditions_3d.py:5-23`` and its 1-D/2-D variants.
     """
+    xp = _xp(xyz)
     nd = _nd(xyz)
     x = xyz[0]
     vel = (vx, vy, vz)[:nd]
     if case == 0:
-        inside = np.ones(x.shape, dtype=bool)
-        for d in range(nd):
-            inside &= np.fabs(xyz[d] - 0.5) < 0.25
-        return np.where(inside, 2.0, 1.0)
+        inside = xp.abs(xyz[0] - 0.5) < 0.25
+        for d in range(1, nd):
+            inside = inside & (xp.abs(xyz[d] - 0.5) < 0.25)
+        return xp.where(inside, 2.0, 1.0)
     if 1 <= case <= nd:
-        return vel[case - 1] * np.ones(x.shape)
+        return vel[case - 1] * xp.ones_like(x)
     if case == nd + 1:
-        return P * np.ones(x.shape)
-    return np.ones(x.shape)
+        return P * xp.ones_like(x)
+    return xp.ones_like(x)
 
 
 def sod_shock_tube(xyz: np.ndarray, case: int):
@@ -75,12 +130,13 @@ def sod_shock_tube(xyz: np.ndarray, case: int):
 
     Same states as ``src/initial_conditions_1d.py:31-40``.
     """
+    xp = _xp(xyz)
     x = xyz[0]
     if case == 0:
-        return np.where(x < 0.5, 1.0, 0.125)
+        return xp.where(x < 0.5, 1.0, 0.125)
     if case == _nd(xyz) + 1:
-        return np.where(x < 0.5, 1.0, 0.1)
-    return np.zeros(x.shape)
+        return xp.where(x < 0.5, 1.0, 0.1)
+    return xp.zeros_like(x)
 
 
 sod = sod_shock_tube
@@ -94,36 +150,38 @@ def kelvin_helmholtz(xyz: np.ndarray, case: int, rho_in=2.0, rho_out=1.0,
     vy = A sin(4 pi x); P constant.  (The seeded velocity noise of the
     benchmark is added to the subcell means afterwards, see ``subcell_noise``.)
     """
+    xp = _xp(xyz)
     x, y = xyz[0], xyz[1]
-    inner = np.fabs(y - 0.5) < 0.25
+    inner = xp.abs(y - 0.5) < 0.25
     if case == 0:
-        return np.where(inner, rho_in, rho_out)
+        return xp.where(inner, rho_in, rho_out)
     if case == 1:
-        return np.where(inner, v_in, v_out)
+        return xp.where(inner, v_in, v_out)
     if case == 2:
-        return A * np.sin(4 * np.pi * x)
+        return A * xp.sin(4 * np.pi * x)
     if case == 3:
-        return P * np.ones(x.shape)
-    return np.zeros(x.shape)
+        return P * xp.ones_like(x)
+    return xp.zeros_like(x)
 
 
 def KH_instability(xyz: np.ndarray, case: int):
     """The reference's own Kelvin-Helmholtz set-up (Gaussian-localised vy),
     ``src/initial_conditions_2d.py:40-52``."""
+    xp = _xp(xyz)
     y = xyz[1]
     w0 = 0.1
     sigma = 0.05 / np.sqrt(2)
     if case == 0:
-        return np.where(y < 0.25, 1.0, np.where(y < 0.75, 2.0, 1.0))
+        return xp.where(y < 0.25, 1.0, xp.where(y < 0.75, 2.0, 1.0))
     if case == 1:
-        return np.where(y < 0.25, -0.5, np.where(y < 0.75, 0.5, -0.5))
+        return xp.where(y < 0.25, -0.5, xp.where(y < 0.75, 0.5, -0.5))
     if case == 2:
-        return w0 * np.sin(4 * np.pi * xyz[0]) * (
-            np.exp(-(y - 0.25) ** 2 / (2 * sigma ** 2))
-            + np.exp(-(y - 0.75) ** 2 / (2 * sigma ** 2)))
+        return w0 * xp.sin(4 * np.pi * xyz[0]) * (
+            xp.exp(-(y - 0.25) ** 2 / (2 * sigma ** 2))
+            + xp.exp(-(y - 0.75) ** 2 / (2 * sigma ** 2)))
     if case == 3:
-        return 2.5 * np.ones(y.shape)
-    return np.zeros(xyz[0].shape)
+        return 2.5 * xp.ones_like(y)
+    return xp.zeros_like(xyz[0])
 
 
 def isentropic_vortex(xyz: np.ndarray, case: int, beta=5.0, gamma=1.4,
@@ -137,16 +195,17 @@ def isentropic_vortex(xyz: np.ndarray, case: int, beta=5.0, gamma=1.4,
     rho = T^{1/(gamma-1)}, P = rho T.  With ``t > 0`` the vortex centre is
     advected to (x0 + u0 t, y0 + v0 t), wrapped into the periodic box.
     """
+    xp = _xp(xyz)
     x, y = xyz[0], xyz[1]
     Lx, Ly = period
     xc = x0 + u0 * t
     yc = y0 + v0 * t
     dx = x - xc
     dy = y - yc
-    dx = dx - Lx * np.round(dx / Lx)
-    dy = dy - Ly * np.round(dy / Ly)
+    dx = dx - Lx * xp.round(dx / Lx)
+    dy = dy - Ly * xp.round(dy / Ly)
     r2 = dx * dx + dy * dy
-    f = np.exp(0.5 * (1.0 - r2))
+    f = xp.exp(0.5 * (1.0 - r2))
     if case == 1:
         return u0 - beta / (2 * np.pi) * f * dy
     if case == 2:
@@ -157,7 +216,7 @@ def isentropic_vortex(xyz: np.ndarray, case: int, beta=5.0, gamma=1.4,
         return rho
     if case == 3:
         return rho * T
-    return np.zeros(x.shape)
+    return xp.zeros_like(x)
 
 
 def random_four_quadrant_states(rng: np.random.Generator):
@@ -182,6 +241,10 @@ def four_quadrant(states, xc=0.5, yc=0.5):
 
     def fct(xyz: np.ndarray, case: int):
         x, y = xyz[0], xyz[1]
+        if _xp(xyz) is not np:
+            import torch
+            q = (x >= xc).long() + 2 * (y >= yc).long()
+            return torch.as_tensor(states[:, case], dtype=torch.float64, device=x.device)[q]
         q = (x >= xc).astype(np.int64) + 2 * (y >= yc).astype(np.int64)
         return states[:, case][q]
 
@@ -194,18 +257,19 @@ def taylor_green(xyz: np.ndarray, case: int, P0=100.0):
     rho = 1; u = sin x cos y cos z, v = -cos x sin y cos z, w = 0;
     P = P0 + (cos 2x + cos 2y)(cos 2z + 2)/16.
     """
+    xp = _xp(xyz)
     x, y, z = xyz[0], xyz[1], xyz[2]
     if case == 0:
-        return np.ones(x.shape)
+        return xp.ones_like(x)
     if case == 1:
-        return np.sin(x) * np.cos(y) * np.cos(z)
+        return xp.sin(x) * xp.cos(y) * xp.cos(z)
     if case == 2:
-        return -np.cos(x) * np.sin(y) * np.cos(z)
+        return -xp.cos(x) * xp.sin(y) * xp.cos(z)
     if case == 3:
-        return np.zeros(x.shape)
+        return xp.zeros_like(x)
     if case == 4:
-        return P0 + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
-    return np.zeros(x.shape)
+        return P0 + (xp.cos(2 * x) + xp.cos(2 * y)) * (xp.cos(2 * z) + 2.0) / 16.0
+    return xp.zeros_like(x)
 
 
 def subcell_noise(N, p, variables, amplitude, seed=0, nvar=None):

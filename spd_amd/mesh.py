@@ -1,9 +1,8 @@
-"""Host-side mesh and initial-condition helpers of the simulator facade.
-
-These run once, on the host, at construction: the user's initial-condition
-callable is NumPy code evaluated at quadrature points, exactly as in the
-reference (``src/sd_simulator.py:59-90``, ``src/polynomials.py:124-148``).
-Nothing here is on the time-step path.
+"""Host-side geometry helpers of the simulator facade: the point sets of the
+reference (``src/polynomials.py:5-66``) and the initial-condition mesh of
+``src/sd_simulator.py:59-72``.  They build small 1-D arrays once at
+construction; the quadrature points, the subcell means and everything on the
+time-step path are computed by the HIP kernels.
 """
 from __future__ import annotations
 
@@ -28,56 +27,18 @@ def flux_points(p):
     return np.concatenate(([0.0], x, [1.0]))
 
 
-def lagrange_matrix(x_to, x_from):
-    """L[a, b] = l_b(x_to[a]) (used for ``regular_mesh``)."""
-    x_to = np.asarray(x_to, dtype=np.float64)
-    x_from = np.asarray(x_from, dtype=np.float64)
-    diff_to = x_to[:, None, None] - x_from[None, None, :]
-    diff_to = np.broadcast_to(diff_to, (x_to.size, x_from.size, x_from.size)).copy()
-    idx = np.arange(x_from.size)
-    diff_to[:, idx, idx] = 1.0
-    den = x_from[:, None] - x_from[None, :]
-    np.fill_diagonal(den, 1.0)
-    return diff_to.prod(axis=-1) / den.prod(axis=-1)[None, :]
-
-
-def subcell_means(init_fct, nvar, N, p, lim, h, Nghe=1):
-    """Primitive subcell means of ``init_fct`` on the active elements, element
-    layout ``(V, [Nz,] [Ny,] Nx, [nz,] [ny,] nx)``.
-
-    Subcell bounds follow the reference's mesh (elements with ``Nghe`` ghost
-    layers, ``lim0 + (k + fp) * L_ghosted / N_ghosted - h``); each subcell mean
-    is a 2-point Gauss-Legendre rule per dimension.
-    """
-    nd = len(N)
-    n = p + 1
+def ic_mesh(N, p, lim, h, Nghe=1):
+    """Subcell lower bounds and widths of the active elements per dimension,
+    ``(N_a, p+1)`` arrays, from the reference's element mesh
+    (``src/sd_simulator.py:59-72``): elements with ``Nghe`` ghost layers,
+    faces ``lim0 + (k + fp) * L_ghosted / N_ghosted - h``."""
     fp = flux_points(p)
-    xq, wq = gl01(2)
     lows, wids = [], []
-    for a in range(nd):
+    for a in range(len(N)):
         Ng = N[a] + 2 * Nghe
         length = (lim[a][1] - lim[a][0]) + 2 * Nghe * h[a]
         mesh = lim[a][0] + (np.arange(Ng)[:, None] + fp[None, :]) * length / Ng - h[a]
         mesh = mesh[Nghe:Ng - Nghe]
-        lows.append(mesh[:, :-1])
-        wids.append(np.diff(mesh, axis=1))
-    shapeE = [N[a] for a in range(nd)][::-1]
-    full = shapeE + [n] * nd + [2] * nd
-    out = np.empty([nvar] + shapeE + [n] * nd)
-    pts = np.empty([nd] + full)
-    for a in range(nd):
-        k = nd - 1 - a
-        shp = [1] * (3 * nd)
-        shp[k] = N[a]
-        shp[nd + k] = n
-        qs = [1] * (3 * nd)
-        qs[2 * nd + k] = 2
-        pts[a] = lows[a].reshape(shp) + wids[a].reshape(shp) * xq.reshape(qs)
-    for var in range(nvar):
-        f = np.asarray(init_fct(pts, var), dtype=np.float64) * np.ones(full)
-        args = [f, list(range(3 * nd))]
-        for k in range(nd):
-            args += [wq, [2 * nd + k]]
-        args.append(list(range(2 * nd)))
-        out[var] = np.einsum(*args)
-    return out
+        lows.append(np.ascontiguousarray(mesh[:, :-1]))
+        wids.append(np.ascontiguousarray(np.diff(mesh, axis=1)))
+    return lows, wids

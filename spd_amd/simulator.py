@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from . import initial_conditions as _ic
-from .mesh import lagrange_matrix, solution_points, subcell_means
+from .mesh import ic_mesh, solution_points
 
 HOST, DEVICE = "host", "device"
 
@@ -305,14 +305,46 @@ class SDADER_Simulator:
 
     # -- initial condition (src/sd_simulator.py:59-90) -------------------------------
     def post_init(self):
-        W = subcell_means(self.init_fct, self.nvar, [self.N[d] for d in self.dims], self.p,
-                          [self.lim[d] for d in self.dims], [self.h[d] for d in self.dims],
-                          self.Nghe)
+        """Primitive subcell means of ``init_fct`` (2-point Gauss-Legendre rule
+        per dimension) computed on the device: the quadrature points come from
+        ``spd_ic_points``, the callable is evaluated on them as torch tensors
+        (the callables of :mod:`initial_conditions` accept tensors; a callable
+        written for NumPy is evaluated on a host copy of the points instead) and
+        ``spd_ic_mean`` reduces the values per subcell."""
+        dims = list(self.dims)
+        lows, wids = ic_mesh([self.N[d] for d in dims], self.p, [self.lim[d] for d in dims],
+                             [self.h[d] for d in dims], self.Nghe)
+        dev = self.device
+        lw = torch.from_numpy(np.concatenate([a.ravel() for a in lows])).to(dev)
+        ww = torch.from_numpy(np.concatenate([a.ravel() for a in wids])).to(dev)
+        nd = self.ndim
+        full = [self.N[d] for d in dims][::-1] + [self.p + 1] * nd + [2] * nd
+        pts = torch.empty([nd] + full, dtype=torch.float64, device=dev)
+        _lib.check(self._lib.spd_ic_points(self._ctx, _lib.ptr(lw), _lib.ptr(ww), _lib.ptr(pts),
+                                           self._stream), "spd_ic_points")
+        W = self._empty()
+        host_pts = None
+        for var in range(self.nvar):
+            vals = None
+            if host_pts is None:
+                try:
+                    vals = self.init_fct(pts, var)
+                except (TypeError, RuntimeError, AttributeError):
+                    host_pts = pts.cpu().numpy()
+            if host_pts is not None:
+                vals = np.asarray(self.init_fct(host_pts, var), dtype=np.float64)
+            vals = torch.as_tensor(vals, dtype=torch.float64, device=dev).expand(full).contiguous()
+            _lib.check(self._lib.spd_ic_mean(self._ctx, _lib.ptr(vals), _lib.ptr(W[var]),
+                                             self._stream), "spd_ic_mean")
         self._set_initial_primitives(W)
 
     def _set_initial_primitives(self, W_cv):
-        self.W_init_cv = np.array(W_cv, dtype=np.float64)
-        Wd = torch.from_numpy(self.W_init_cv).to(self.device)
+        if isinstance(W_cv, torch.Tensor):
+            Wd = W_cv.to(self.device, dtype=torch.float64).contiguous()
+            self.W_init_cv = Wd.cpu().numpy()
+        else:
+            self.W_init_cv = np.array(W_cv, dtype=np.float64)
+            Wd = torch.from_numpy(self.W_init_cv).to(self.device)
         self.dm.set_device_value("W_cv", Wd)
         W_sp = self._tensor_op("spd_cv_to_sp", Wd)
         self.dm.set_device_value("W_sp", W_sp)
@@ -331,14 +363,34 @@ class SDADER_Simulator:
 
     # -- library wrappers -----------------------------------------------------------
     def _tensor_op(self, name, src):
+        """Element-layout transform of an array (..., E.., L..) with any number
+        of leading rows; the kernels take V rows per call, so other counts are
+        processed in zero-padded chunks of V rows."""
         src = self._as_device(src)
-        out = torch.empty_like(src)
-        _lib.check(getattr(self._lib, name)(self._ctx, _lib.ptr(src), _lib.ptr(out), self._stream),
-                   name)
-        return out
+        per = 1
+        for d in self.dims:
+            per *= self.N[d] * (self.p + 1)
+        if src.numel() % per != 0 or list(src.shape[-2 * self.ndim:]) != self._shape_sd()[1:]:
+            raise ValueError(f"array of shape {tuple(src.shape)} is not in the element layout "
+                             f"{tuple(self._shape_sd()[1:])}")
+        rows = src.numel() // per
+        flat = src.reshape(rows, per)
+        V = self.nvar
+        nchunk = (rows + V - 1) // V
+        buf = torch.zeros(nchunk * V, per, dtype=torch.float64, device=self.device)
+        buf[:rows] = flat
+        out = torch.empty_like(buf)
+        for k in range(nchunk):
+            _lib.check(getattr(self._lib, name)(self._ctx, _lib.ptr(buf[k * V:(k + 1) * V]),
+                                                _lib.ptr(out[k * V:(k + 1) * V]), self._stream),
+                       name)
+        return out[:rows].reshape(src.shape)
 
     def _pointwise(self, name, src):
         src = self._as_device(src)
+        if src.dim() < 1 or src.shape[0] != self.nvar:
+            raise ValueError(f"expected {self.nvar} variables along the first axis, "
+                             f"got shape {tuple(src.shape)}")
         out = torch.empty_like(src)
         count = src.numel() // src.shape[0]
         _lib.check(getattr(self._lib, name)(self._ctx, _lib.ptr(src), _lib.ptr(out), count,
@@ -402,16 +454,7 @@ class SDADER_Simulator:
         p = self.p
         if p <= 1:
             return W
-        x = np.arange(p + 2) / (p + 1)
-        x = 0.5 * (x[1:] + x[:-1])
-        mat = lagrange_matrix(x, self.x_sp)
-        Wn = self.dm.asnumpy(W)
-        nd = self.ndim
-        if nd == 1:
-            return np.einsum("il,uxl->uxi", mat, Wn)
-        if nd == 2:
-            return np.einsum("jm,il,uyxml->uyxji", mat, mat, Wn)
-        return np.einsum("kn,jm,il,uzyxnml->uzyxkji", mat, mat, mat, Wn)
+        return self._like_input(W, self._tensor_op("spd_sp_to_regular", W))
 
     def crop(self, M, ngh=1):
         return M[(slice(None),) + (slice(ngh, -ngh),) * self.ndim + (Ellipsis,)]

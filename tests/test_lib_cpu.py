@@ -104,10 +104,38 @@ def test_unsupported_options_raise():
             SDADER_Simulator(p=1, N=(4,), **kw)
 
 
-def test_subcell_means_match_oracle_ic():
-    """The facade's host IC pipeline equals the oracle's (same quadrature)."""
+def _host_means(init_fct, nvar, N, p, lims, h):
+    """Subcell means from the facade's IC mesh with the reference's quadrature
+    (points low + width * x_q, 2-point Gauss-Legendre per dimension)."""
+    from spd_amd.mesh import gl01, ic_mesh
+    nd = len(N)
+    lows, wids = ic_mesh(N, p, lims, h)
+    xq, wq = gl01(2)
+    full = list(N)[::-1] + [p + 1] * nd + [2] * nd
+    pts = np.empty([nd] + full)
+    for a in range(nd):
+        k = nd - 1 - a
+        shp = [1] * (3 * nd)
+        shp[k] = N[a]
+        shp[nd + k] = p + 1
+        qs = [1] * (3 * nd)
+        qs[2 * nd + k] = 2
+        pts[a] = lows[a].reshape(shp) + wids[a].reshape(shp) * xq.reshape(qs)
+    out = np.empty([nvar] + full[:2 * nd])
+    for var in range(nvar):
+        f = np.asarray(init_fct(pts, var), dtype=np.float64) * np.ones(full)
+        args = [f, list(range(3 * nd))]
+        for k in range(nd):
+            args += [wq, [2 * nd + k]]
+        args.append(list(range(2 * nd)))
+        out[var] = np.einsum(*args)
+    return out, pts
+
+
+def test_ic_mesh_matches_oracle_ic():
+    """The IC mesh the device quadrature starts from gives the oracle's
+    subcell means (same points, same rule)."""
     from fixtures import load, make_ic, sim_kwargs
-    from spd_amd.mesh import subcell_means
     from oracle.spd_oracle import OracleSimulator
     for name in ("kh16_p3", "sod_p3", "vortex12_p5", "tg4_p3"):
         fx = load(name)
@@ -116,5 +144,25 @@ def test_subcell_means_match_oracle_ic():
         nd = len(kw["N"])
         lims = [kw.get(k, (0, 1)) for k in ("xlim", "ylim", "zlim")][:nd]
         h = [(lims[a][1] - lims[a][0]) / kw["N"][a] for a in range(nd)]
-        W = subcell_means(make_ic(fx["cfg"]["ic"]), nd + 2, list(kw["N"]), kw["p"], lims, h)
+        W, _ = _host_means(make_ic(fx["cfg"]["ic"]), nd + 2, list(kw["N"]), kw["p"], lims, h)
         assert np.array_equal(W, o.W_init_cv), name
+
+
+def test_ic_callables_accept_torch_tensors():
+    """Every IC callable gives the same values on torch tensors (the device
+    path) as on NumPy arrays."""
+    import torch
+    from spd_amd import initial_conditions as ic
+    rng = np.random.default_rng(3)
+    cases = [(ic.sine_wave, 1), (ic.sine_wave, 2), (ic.sine_wave, 3), (ic.step_function, 2),
+             (ic.step_function, 3), (ic.sod_shock_tube, 1), (ic.kelvin_helmholtz, 2),
+             (ic.KH_instability, 2), (ic.isentropic_vortex, 2), (ic.taylor_green, 3),
+             (ic.four_quadrant(ic.random_four_quadrant_states(rng)), 2)]
+    for fct, nd in cases:
+        pts = rng.uniform(-1.0, 7.0, size=(nd, 5, 7))
+        for var in range(nd + 2):
+            a = np.asarray(fct(pts, var), dtype=np.float64) * np.ones(pts.shape[1:])
+            b = fct(torch.from_numpy(pts), var)
+            b = torch.as_tensor(b, dtype=torch.float64).expand(pts.shape[1:]).numpy()
+            np.testing.assert_allclose(b, a, rtol=1e-14, atol=1e-15,
+                                       err_msg=f"{getattr(fct, '__name__', fct)} var {var}")
