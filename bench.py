@@ -1,8 +1,8 @@
 """Benchmark of the SD-ADER step on one MI355X.
 
 Workload (the metric's configuration): 2-D Kelvin-Helmholtz on [0,1]^2,
-periodic, 1024^2 elements, p = 3 (16.8 M solution points), FP64, update="FV"
-with the a-posteriori fallback (NAD + SED + PAD, blending), HLLC on element and
+periodic, 1024^2 elements, p = 3 (16.8 M solution points), update="FV" with
+the a-posteriori fallback (NAD + SED + PAD, blending), HLLC on element and
 fallback faces, minmod, CFL 0.8; rho = 2, vx = 0.5 for |y-0.5| < 0.25 else
 rho = 1, vx = -0.5, P = 2.5, vy = 0.01 sin(4 pi x), plus seeded uniform noise
 (amplitude 1e-3, numpy default_rng(0)) on the vx, vy subcell means.
@@ -10,11 +10,19 @@ rho = 1, vx = -0.5, P = 2.5, vy = 0.01 sin(4 pi x), plus seeded uniform noise
 A step is one pass of the hot path over the whole state: compute_dt on the
 device (its scalar read back to the host, as in the reference's loop) and
 one spd_step (predictor + FV/FB corrector).  The state is resident in HBM
-before the timed region.  Output: one JSON line with DOF-updates/s (DOF =
-solution points; one DOF-update = one point advanced one full step), the
-roofline of the dominant kernel (HIP-event durations measured over the timed
-region on the kernel's own stream) and the CPU baseline (the NumPy oracle on a
-bounded sample, one host core).
+before the timed region.  The JSON line carries:
+  * value: DOF-updates/s of the FP64 form (DOF = solution points; one
+    DOF-update = one point advanced one full step), with the roofline of the
+    dominant kernel (HIP-event durations over the timed region on the
+    kernel's own stream; HBM traffic per launch from the committed counter
+    summary) and per-kernel times;
+  * fp32_bulk: the same workload in the FP32-bulk form, its speedup and its
+    relative L1 difference from the FP64 form after the same steps;
+  * ensemble / ensemble_small_members: 64 four-quadrant Riemann problems
+    (N in {64,128,256,384} resp. {16,32,64}, p in {2..5}, outflow), a fixed
+    number of steps per member, batched (multi-branch HIP graph, autotuned
+    launch shapes) against the members run one after the other;
+  * cpu_baseline: the NumPy oracle on a bounded sample, one host core.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -42,27 +50,25 @@ def kh_kwargs(N, p):
 
 def algorithmic_bytes(N, p, launches):
     """Compulsory HBM bytes per launch of each kernel class for one step of the
-    2-D FV+FB path (each array element read or written once, FP64; flags 1 B).
+    2-D tiled FV+FB path in FP64 (each array element read or written once;
+    flags 1 B).
 
-    Per step (V = 4, T = n = p+1, E = N^2 elements, C = E n^2 cells,
-    Fx = Fy = V (N n + 1) N n face values, G = (N n + 4)^2 ghosted cells):
-      k_face_sd  m+1 launches per dim: read the predictor state (T V C, first
-                 launch V C), write T V (N+1) N n element-face fluxes
-      k_pred     m launches: read state (T V C, first V C) + U_sp (V C) +
+    Per step (V = 4, T = n = p+1, m = p, E = N^2 elements, C = E n^2 cells,
+    Fe = T V (N+1) N n element-face values per dimension, Fd = V (N n + 1) N n
+    FV-face values per dimension, G = (N n + 4)^2 ghosted cells):
+      k_face_sd  2 (m+1) launches: read the predictor state (T V C; the first
+                 iteration V C of U_sp), write the element-face fluxes (Fe; Fe/T)
+      k_pred     m launches: read the state (T V C; first V C), U_sp (V C) and
                  both element-face arrays, write T V C
-      k_hoflux   per substep and dim: read V C of the state slice + V (N+1) N n,
-                 write V (N n + 1) N n
+      k_hoflux   T launches (k_ho2d): read the node's state slice (V C) and its
+                 element-face fluxes (2 V (N+1) N n), write both FV-face arrays
       k_tensor   once: read + write V C
-      k_apply    per substep, twice: read V C + Fx + Fy, write V C
-      k_detect   per substep: read 2 V C, write C flags
-      k_theta1   per substep: read C flags, write G
-      k_ring     per substep: read + write G
-      k_hancock  per substep: read G theta + V C state, write (1+2) V C states/slopes
-      k_fbface   per substep and dim: read 2 G theta + 2 V C states/slopes, read+write
-                 the face array
+      k_apply    T launches: read U_cv + both face arrays, write the candidate
+      k_detect   T launches: read U_cv and the candidate (2 V C), write C flags
+      k_theta1   T launches (k_theta2d): read C flags, write G
+      k_fbface   T launches (k_fbcorrect2d): read U_cv, theta, both face arrays,
+                 write V C
       k_cv2sp    once: read V C, write V C
-    (k_hancock / k_fbface only touch cells and faces near troubled cells; the
-    figures are upper bounds for them.)
     """
     V, n = 4, p + 1
     T = n
@@ -74,15 +80,14 @@ def algorithmic_bytes(N, p, launches):
     G = (N * n + 4) ** 2
     per_step = {
         "k_face_sd": 2 * ((V * C + Fe // T) + m * (T * V * C + Fe)),
-        "k_pred": (V * C + V * C + 2 * Fe // T + T * V * C) + (m - 1) * (T * V * C + V * C + 2 * Fe + T * V * C),
-        "k_hoflux": T * 2 * (V * C + V * (N + 1) * N * n + Fd),
+        "k_pred": (V * C + V * C + 2 * Fe // T + T * V * C)
+                  + (m - 1) * (T * V * C + V * C + 2 * Fe + T * V * C),
+        "k_hoflux": T * (V * C + 2 * V * (N + 1) * N * n + 2 * Fd),
         "k_tensor": 2 * V * C,
-        "k_apply": T * 2 * (2 * V * C + 2 * Fd),
+        "k_apply": T * (2 * V * C + 2 * Fd),
         "k_detect": T * (2 * V * C),
         "k_theta1": T * G,
-        "k_ring": T * 2 * G,
-        "k_hancock": T * (G + V * C + 3 * V * C),
-        "k_fbface": T * 2 * (2 * G + 2 * V * C + 2 * Fd),
+        "k_fbface": T * (2 * V * C + G + 2 * Fd),
         "k_cv2sp": 2 * V * C,
     }
     extra_bytes = {"k_detect": T * C * 1.0, "k_theta1": T * C * 1.0}
@@ -94,6 +99,42 @@ def algorithmic_bytes(N, p, launches):
         cnt = val[1]
         out[k] = (per_step[k] * 8.0 + extra_bytes.get(k, 0.0)) * steps / cnt
     return out
+
+
+# rocprofv3 names of the kernels of each class in the FP64 2-D HLLC workload
+CLASS_KERNELS = {
+    "k_fbface": ["k_fbcorrect2d<1, double>"],
+    "k_face_sd": ["k_face_sd_x<2, 3, 1, double, double>", "k_face_sd<2, 3, 1, 1, double, double>"],
+    "k_pred": ["k_pred<2, 3, double, double>"],
+    "k_detect": ["k_detect2d"],
+    "k_hoflux": ["k_ho2d<3, double>"],
+    "k_apply": ["k_apply<2, double>"],
+    "k_theta1": ["k_theta2d"],
+    "k_cv2sp": ["k_cv2sp<2, 3>"],
+    "k_tensor": ["k_tensor<2, 3>"],
+}
+TRAFFIC_SUMMARY = os.path.join(HERE, "profiles", "kh1024_p3_hbm_traffic.md")
+
+
+def measured_traffic(cls):
+    """HBM bytes per launch of a kernel class from the committed counter
+    summary (FETCH_SIZE / WRITE_SIZE passes scaled by the copy calibration),
+    averaged over the class's kernels; None when not recorded."""
+    names = CLASS_KERNELS.get(cls)
+    if not names or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    vals = {}
+    with open(TRAFFIC_SUMMARY) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 6 and cells[0].strip("`") in names:
+                try:
+                    vals[cells[0].strip("`")] = (float(cells[4]) + float(cells[5])) * 1024.0 * 1024.0
+                except ValueError:
+                    pass
+    if len(vals) != len(names):
+        return None
+    return sum(vals.values()) / len(vals)
 
 
 def copy_bandwidth(torch, nbytes=2 << 30, reps=10):
@@ -162,6 +203,39 @@ def run_cpu_baseline(N=256, steps=1, warmup=1):
                        f"core of {os.cpu_count()} ({cpu})")}
 
 
+def run_ensemble(torch, n=64, steps=10, sizes=(64, 128, 256, 384), seed=0):
+    """The batched ensemble of 2-D four-quadrant Riemann problems (outflow,
+    FV+FB, HLLC, p and N drawn from a seed) on a bounded sample: ``steps``
+    steps of every member, batched (one multi-branch HIP graph launch per
+    chunk of steps, autotuned launch shapes) and then the same steps with the
+    members run one after the other by the host-driven loop."""
+    import contextlib
+    from spd_amd.ensemble import Ensemble, riemann_member_kwargs, riemann_members, timed
+    specs = riemann_members(n, seed=seed, sizes=sizes)
+    with contextlib.redirect_stdout(sys.stderr):
+        t0 = time.perf_counter()
+        ens = Ensemble([riemann_member_kwargs(m) for m in specs], graph_steps=4, branches=8,
+                       tune=True)
+        setup = time.perf_counter() - t0
+        dof = sum(ens.dof)
+        ens.run(steps=1, finish=False)                      # builds the graph once
+        tb = timed(lambda: ens.run(steps=steps, finish=False))
+        ts = timed(lambda: ens.run_sequential(steps=steps))
+    import numpy as np
+    bad = sum(1 for s_ in ens.sims
+              if not bool(torch.isfinite(s_.dm.device_value("U_sp")).all()))
+    out = {"members": n, "sizes": list(sizes), "orders": [2, 3, 4, 5], "seed": seed,
+           "steps_per_member": steps, "dof_total": dof,
+           "batched_DOF_updates_per_s": dof * steps / tb,
+           "sequential_DOF_updates_per_s": dof * steps / ts,
+           "speedup_vs_sequential": ts / tb, "graph_builds": ens.graph_builds,
+           "graph_build_s": ens.build_seconds, "setup_and_tune_s": setup,
+           "members_nonfinite": bad}
+    del ens
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +248,8 @@ def main():
     ap.add_argument("--cpu-N", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-warmup", type=int, default=1)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     args = ap.parse_args()
 
     if args.cpu_baseline_child:
@@ -259,8 +335,9 @@ def main():
                    "N": N, "p": p, "dof_per_step": dof, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": None, "avg_launch_ms": avg_ms,
-                     "alg_bytes_per_launch": bpl[dom]},
+                     "traffic": measured_traffic(dom), "avg_launch_ms": avg_ms,
+                     "alg_bytes_per_launch": bpl[dom],
+                     "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, HERE)},
         "step_hbm_roofline_frac": value * 64.0 / (HBM_PEAK_GBPS * 1e9),
         "copy_GBps": copy_gbps,
         "troubled_fraction": frac,
@@ -275,6 +352,12 @@ def main():
                                     for k, v in sorted(prof32.items(), key=lambda kv: -kv[1][0])},
         },
     }
+    if not args.no_ensemble:
+        del s
+        torch.cuda.empty_cache()
+        out["ensemble"] = run_ensemble(torch, steps=args.ensemble_steps)
+        out["ensemble_small_members"] = run_ensemble(torch, steps=2 * args.ensemble_steps,
+                                                     sizes=(16, 32, 64))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline(args.cpu_N, args.cpu_steps, args.cpu_warmup)
     print(json.dumps(out))
