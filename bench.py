@@ -56,10 +56,11 @@ def algorithmic_bytes(N, p, launches):
     Per step (V = 4, T = n = p+1, m = p, E = N^2 elements, C = E n^2 cells,
     Fe = T V (N+1) N n element-face values per dimension, Fd = V (N n + 1) N n
     FV-face values per dimension, G = (N n + 4)^2 ghosted cells):
-      k_face_sd  2 (m+1) launches: read the predictor state (T V C; the first
-                 iteration V C of U_sp), write the element-face fluxes (Fe; Fe/T)
-      k_pred     m launches: read the state (T V C; first V C), U_sp (V C) and
-                 both element-face arrays, write T V C
+      k_face_sd  2 launches (the final predictor state, for the corrector):
+                 read T V C, write the element-face fluxes Fe
+      k_pred     m launches (k_pred2d, element faces solved inside): read the
+                 state (T V C; the first iteration V C of U_sp) and U_sp (V C),
+                 write T V C (the one-element halo reads are not counted)
       k_hoflux   T launches (k_ho2d): read the node's state slice (V C) and its
                  element-face fluxes (2 V (N+1) N n), write both FV-face arrays
       k_tensor   once: read + write V C
@@ -79,9 +80,8 @@ def algorithmic_bytes(N, p, launches):
     Fe = T * V * (N + 1) * N * n
     G = (N * n + 4) ** 2
     per_step = {
-        "k_face_sd": 2 * ((V * C + Fe // T) + m * (T * V * C + Fe)),
-        "k_pred": (V * C + V * C + 2 * Fe // T + T * V * C)
-                  + (m - 1) * (T * V * C + V * C + 2 * Fe + T * V * C),
+        "k_face_sd": 2 * (T * V * C + Fe),
+        "k_pred": (V * C + V * C + T * V * C) + (m - 1) * (T * V * C + V * C + T * V * C),
         "k_hoflux": T * (V * C + 2 * V * (N + 1) * N * n + 2 * Fd),
         "k_tensor": 2 * V * C,
         "k_apply": T * (2 * V * C + 2 * Fd),
@@ -105,7 +105,7 @@ def algorithmic_bytes(N, p, launches):
 CLASS_KERNELS = {
     "k_fbface": ["k_fbcorrect2d<1, double>"],
     "k_face_sd": ["k_face_sd_x<2, 3, 1, double, double>", "k_face_sd<2, 3, 1, 1, double, double>"],
-    "k_pred": ["k_pred<2, 3, double, double>"],
+    "k_pred": ["k_pred2d<3, 1, double, double>", "k_pred2d<3, 1, double, double>"],
     "k_detect": ["k_detect2d"],
     "k_hoflux": ["k_ho2d<3, double>"],
     "k_apply": ["k_apply<2, double>"],

@@ -1,0 +1,266 @@
+// 2-D ADER predictor iteration with the element-face Riemann problems solved
+// in the same kernel (src/sdader_simulator.py:160-200,213-240,148-158,
+// src/sd_boundary.py:7-76).  A block owns a TX x TY tile of elements and all
+// time nodes; one thread per (element, time node, line) in each dimension
+// phase:
+//   1. load the line, evaluate its two end traces (sp -> fp 0 and fp nf-1);
+//      the upper trace goes to LDS, and a few threads evaluate the traces of
+//      the one-element halo across the tile's two sides along the axis;
+//   2. every element's lower face flux is solved by the element itself (the
+//      upper-side neighbour inside the tile reads it from LDS); the elements
+//      whose upper neighbour is outside the tile (or the domain) also solve
+//      their upper face;
+//   3. flux at the interior flux points, derivative back to the solution
+//      points, accumulated in the element's LDS slab (as k_pred);
+// then the Picard update writes the next predictor state to another buffer
+// (neighbouring blocks still read the current one).  The arithmetic of every
+// trace, state conversion, Riemann solve and derivative is that of k_face_sd
+// and k_pred.  This file is included into spd_step.hip.
+#pragma once
+
+namespace spd {
+namespace pred2d {
+
+template <int P> struct Tile;  // elements per block along x and y
+template <> struct Tile<1> { static constexpr int X = 8, Y = 8; };
+template <> struct Tile<2> { static constexpr int X = 5, Y = 5; };
+template <> struct Tile<3> { static constexpr int X = 4, Y = 4; };
+template <> struct Tile<4> { static constexpr int X = 3, Y = 2; };
+template <> struct Tile<5> { static constexpr int X = 2, Y = 2; };
+
+template <int P> struct Layout {
+  static constexpr int V = 4, n = P + 1, nf = P + 2, T = P + 1, NL = n * n;
+  static constexpr int TX = Tile<P>::X, TY = Tile<P>::Y, TE = TX * TY;
+  static constexpr int NTH = TE * T * n;             // threads per block
+  static constexpr int TM = TX > TY ? TX : TY;       // rows of a halo side
+  static constexpr int SLAB = V * NL + 1;            // dudt slab (padded)
+  static constexpr int LINE = T * n * V;             // traces of one element / halo row
+  // LDS offsets, in values of the bulk type
+  static constexpr int O_DUDT = 0;                   // [TE][T][SLAB]
+  static constexpr int O_TR = O_DUDT + TE * T * SLAB;  // [TE][T][n][V] upper traces, then lower-face fluxes
+  static constexpr int O_HL = O_TR + TE * LINE;      // [TM][T][n][V] upper traces of the lower halo
+  static constexpr int O_HU = O_HL + TM * LINE;      // [TM][T][n][V] lower traces of the upper halo
+  static constexpr int O_FB = O_HU + TM * LINE;      // [TM][T][n][V] upper faces at the tile side
+  static constexpr int TOTAL = O_FB + TM * LINE;
+  static_assert(NTH <= 256, "one block per tile");
+  static_assert(2 * TM * T * n <= NTH, "one halo line per thread");
+};
+
+template <int P, int RS, typename S, typename B>
+__global__ void __launch_bounds__(256)
+k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
+         const double* __restrict__ Usp, const double* __restrict__ dt_dev, B* __restrict__ out) {
+  SPD_SKIP(g);
+  using L = Layout<P>;
+  constexpr int V = 4, n = L::n, nf = L::nf, T = L::T, NL = L::NL;
+  constexpr int TX = L::TX, TY = L::TY, SLAB = L::SLAB;
+  B* lds = reinterpret_cast<B*>(spd_dyn_lds);
+  B* dudt = lds + L::O_DUDT;
+  B* sTR = lds + L::O_TR;
+  B* sHL = lds + L::O_HL;
+  B* sHU = lds + L::O_HU;
+  B* sFB = lds + L::O_FB;
+  const int Nx = g.N[0], Ny = g.N[1];
+  const long long NE = g.NE;
+  const int ntx = (Nx + TX - 1) / TX;
+  const int bx = (int)(blockIdx.x % ntx), by = (int)(blockIdx.x / ntx);
+  const int tid = threadIdx.x;
+  const int el = tid / (T * n), rem = tid % (T * n);
+  const int t = rem / n, lp = rem % n;
+  const int ex = el % TX, ey = el / TX;
+  const int gx = bx * TX + ex, gy = by * TY + ey;
+  const bool inside = tid < L::NTH && gx < Nx && gy < Ny;
+  const bool active = inside && t < Tsrc;
+  const B gam = (B)prm.gamma, mc2 = (B)prm.min_c2;
+
+  // line `l` of element (xe, ye) along dimension a at time node tt
+  auto load_line = [&](int xe, int ye, int tt, int a, int l, B (*u)[n]) {
+    const long long e = (long long)ye * Nx + xe;
+    const int lb = (a == 0) ? l * n : l;
+    const int sa = (a == 0) ? 1 : n;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const S* s0 = src + ((long long)(tt * V + v) * NE + e) * NL + lb;
+#pragma unroll
+      for (int s = 0; s < n; ++s) u[v][s] = (B)s0[s * sa];
+    }
+  };
+  auto trace = [&](B (*u)[n], int fp, B* q) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      B acc = B(0);
+#pragma unroll
+      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[v][s];
+      q[v] = acc;
+    }
+  };
+  auto solve = [&](const B* qa, const B* qb, int a, B* F) {
+    B WL[V], WR[V], UL[V], UR[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      UL[v] = qa[v];
+      UR[v] = qb[v];
+    }
+    prims<2>(UL, WL, a, gam);
+    prims<2>(UR, WR, a, gam);
+    riemann<2, RS>(WL, WR, UL, UR, F, a, gam, mc2, prm.lowmach_sd);
+  };
+
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int ca = (a == 0) ? gx : gy;        // element coordinate along a
+    const int la = (a == 0) ? ex : ey;        // tile coordinate along a
+    const int lt = (a == 0) ? ey : ex;        // tile coordinate across a (halo row)
+    const int TA = (a == 0) ? TX : TY;
+    const int Na = (a == 0) ? Nx : Ny;
+    const int nbr = (a == 0) ? 1 : TX;        // element offset of the upper neighbour in the tile
+    B u[V][n], qL[V], qR[V];
+    if (active) {
+      load_line(gx, gy, t, a, lp, u);
+      trace(u, 0, qL);
+      trace(u, nf - 1, qR);
+      B* d = sTR + ((el * T + t) * n + lp) * V;
+#pragma unroll
+      for (int v = 0; v < V; ++v) d[v] = qR[v];
+    }
+    // halo traces: side 0 = the element below the tile's first one along a
+    // (its upper trace), side 1 = the element above the tile's last element
+    // in the domain (its lower trace); wrapped on periodic sides
+    {
+      const int rows = (a == 0) ? TY : TX;
+      const int per = Tsrc * n;
+      if (tid < 2 * rows * per) {
+        const int side = tid / (rows * per);
+        const int r2 = tid % (rows * per);
+        const int k = r2 / per, th = (r2 % per) / n, lh = r2 % n;
+        const int across = ((a == 0) ? by * TY : bx * TX) + k;   // coordinate across a
+        const int Nacross = (a == 0) ? Ny : Nx;
+        int along = (side == 0) ? ((a == 0) ? bx * TX : by * TY) - 1
+                                : min(((a == 0) ? bx * TX : by * TY) + TA, Na);
+        bool ok = across < Nacross;
+        if (along < 0 || along >= Na) {
+          if (g.bc[a][side] == BC_PERIODIC)
+            along = (along < 0) ? Na - 1 : 0;
+          else
+            ok = false;
+        }
+        if (ok) {
+          B uh[V][n], q[V];
+          if (a == 0)
+            load_line(along, across, th, a, lh, uh);
+          else
+            load_line(across, along, th, a, lh, uh);
+          trace(uh, side == 0 ? nf - 1 : 0, q);
+          B* d = ((side == 0) ? sHL : sHU) + ((k * T + th) * n + lh) * V;
+#pragma unroll
+          for (int v = 0; v < V; ++v) d[v] = q[v];
+        }
+      }
+    }
+    __syncthreads();
+    // lower face (and, at the tile's upper side, the upper face)
+    B F[nf][V];
+    if (active) {
+      B qa[V];
+      if (ca == 0 && g.bc[a][0] != BC_PERIODIC) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) qa[v] = qL[v];
+        if (g.bc[a][0] == BC_REFLECTIVE) qa[a + 1] = -qa[a + 1];
+      } else {
+        const B* s = (la > 0) ? sTR + (((el - nbr) * T + t) * n + lp) * V
+                              : sHL + ((lt * T + t) * n + lp) * V;
+#pragma unroll
+        for (int v = 0; v < V; ++v) qa[v] = s[v];
+      }
+      solve(qa, qL, a, F[0]);
+      if (ca == Na - 1 || la == TA - 1) {
+        B qb[V];
+        if (ca == Na - 1 && g.bc[a][1] != BC_PERIODIC) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) qb[v] = qR[v];
+          if (g.bc[a][1] == BC_REFLECTIVE) qb[a + 1] = -qb[a + 1];
+        } else {
+          const B* s = sHU + ((lt * T + t) * n + lp) * V;
+#pragma unroll
+          for (int v = 0; v < V; ++v) qb[v] = s[v];
+        }
+        solve(qR, qb, a, F[nf - 1]);
+      }
+    }
+    __syncthreads();  // every upper trace has been read
+    if (active) {
+      B* d = sTR + ((el * T + t) * n + lp) * V;
+#pragma unroll
+      for (int v = 0; v < V; ++v) d[v] = F[0][v];
+    }
+    __syncthreads();
+    if (active) {
+      if (!(ca == Na - 1 || la == TA - 1)) {
+        const B* s = sTR + (((el + nbr) * T + t) * n + lp) * V;
+#pragma unroll
+        for (int v = 0; v < V; ++v) F[nf - 1][v] = s[v];
+      }
+      // interior flux points, derivative, accumulation (k_pred)
+#pragma unroll
+      for (int fp = 1; fp < nf - 1; ++fp) {
+        B M[V], W[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          B acc = B(0);
+#pragma unroll
+          for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[v][s];
+          M[v] = acc;
+        }
+        prims<2>(M, W, a, gam);
+        flux<2>(W, F[fp], a, gam);
+      }
+      const B rha = (B)g.rh[a];
+      const int lb = (a == 0) ? lp * n : lp;
+      const int sa = (a == 0) ? 1 : n;
+      B* d0 = dudt + (long long)(el * T + t) * SLAB + lb;
+#pragma unroll
+      for (int s = 0; s < n; ++s) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          B acc = B(0);
+#pragma unroll
+          for (int fp = 0; fp < nf; ++fp) acc += op.d2s[s][fp] * F[fp][v];
+          const B d = acc * rha;
+          B* dst = d0 + v * NL + s * sa;
+          if (a == 0)
+            *dst = d;
+          else
+            *dst = *dst + d;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // Picard update: out[tn] = U_sp - dt * sum_p invader[tn, p] dudt[p]
+  const double dt = *dt_dev;
+  for (int idx = tid; idx < L::TE * V * NL; idx += blockDim.x) {
+    const int le = idx / (V * NL);
+    const int v = (idx / NL) % V;
+    const int l = idx % NL;
+    const int xe = bx * TX + le % TX, ye = by * TY + le / TX;
+    if (xe >= Nx || ye >= Ny) continue;
+    const long long e = (long long)ye * Nx + xe;
+    const long long off = ((long long)v * NE + e) * NL + l;
+    const double u0 = Usp[off];
+    const B* dd = dudt + (long long)le * T * SLAB + v * NL + l;
+#pragma unroll
+    for (int tn = 0; tn < T; ++tn) {
+      B acc = B(0);
+#pragma unroll
+      for (int p = 0; p < T; ++p) {
+        const int ps = (p < Tsrc) ? p : (Tsrc - 1);
+        acc += op.inv[tn][p] * dd[(long long)ps * SLAB];
+      }
+      out[((long long)(tn * V + v) * NE + e) * NL + l] = (B)(u0 - (double)acc * dt);
+    }
+  }
+}
+
+}  // namespace pred2d
+}  // namespace spd

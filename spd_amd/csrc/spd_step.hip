@@ -1040,6 +1040,7 @@ k_ring(Geom g, const double* __restrict__ t1, double* __restrict__ theta) {
 }  // namespace spd
 
 #include "spd_fv2d.hpp"
+#include "spd_pred2d.hpp"
 
 namespace spd {
 
@@ -1585,6 +1586,8 @@ struct spd_ctx {
   double* geom_buf = nullptr;
   // bulk-type buffers (FP64, or FP32 in the FP32-bulk form)
   void* Uad = nullptr;
+  void* Uad2 = nullptr;       // second predictor buffer of the fused 2-D iteration
+  void* Uad_cur = nullptr;    // predictor state of the current step's last iteration
   void* Fface[3] = {nullptr, nullptr, nullptr};
   void* Fho[3] = {nullptr, nullptr, nullptr};
   double* Ucv = nullptr;
@@ -1663,7 +1666,7 @@ template <int P, typename R = double> Ops<P, R> make_ops(const host::OpsHost& h)
 // Typed launcher of one (ndim, p, bulk type) instantiation.
 template <int NDIM, int P, typename B> struct Stepper {
   using OpsB = Ops<P, B>;
-  static B* Uad(spd_ctx* c) { return (B*)c->Uad; }
+  static B* Uad(spd_ctx* c) { return (B*)c->Uad_cur; }
   static B* Ff(spd_ctx* c, int a) { return (B*)c->Fface[a]; }
   static B* Fh(spd_ctx* c, int a) { return (B*)c->Fho[a]; }
   static constexpr int V = NDIM + 2, n = P + 1, nf = P + 2, T = P + 1;
@@ -1713,6 +1716,27 @@ template <int NDIM, int P, typename B> struct Stepper {
     else
       faces_rs<RS_LLF>(c, op, src, Tsrc, s);
   }
+  // fused 2-D predictor iteration (element faces solved in the kernel)
+  template <int RS, typename S>
+  static void pred2d_rs(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
+                        const double* dt, B* out, hipStream_t s) {
+    using L = pred2d::Layout<P>;
+    const int ntx = (c->g.N[0] + L::TX - 1) / L::TX, nty = (c->g.N[1] + L::TY - 1) / L::TY;
+    c->timed(KC_PRED, s, [&] {
+      hipLaunchKernelGGL((pred2d::k_pred2d<P, RS, S, B>), dim3((unsigned)(ntx * nty)),
+                         dim3(L::NTH), (size_t)L::TOTAL * sizeof(B), s, op, c->g, c->prm, src,
+                         Tsrc, Usp, dt, out);
+    });
+  }
+  template <typename S>
+  static void pred2d(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
+                     const double* dt, B* out, hipStream_t s) {
+    if (c->cfg.riemann_sd != SPD_RS_LLF)
+      pred2d_rs<RS_HLLC>(c, op, src, Tsrc, Usp, dt, out, s);
+    else
+      pred2d_rs<RS_LLF>(c, op, src, Tsrc, Usp, dt, out, s);
+  }
+
   template <typename S>
   static void pred(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
                    const double* dt, B* out_ad, double* out_sp, int mode, hipStream_t s) {
@@ -1840,11 +1864,22 @@ template <int NDIM, int P, typename B> struct Stepper {
     if (!c->g.skip) c->g.skip = c->zero_flag;
     const int m = c->cfg.m;
     // --- ADER predictor (src/sdader_simulator.py:160-200)
-    faces(c, opb, (const double*)Usp, 1, s);
-    pred(c, opb, (const double*)Usp, 1, Usp, dt, Uad(c), nullptr, 0, s);
-    for (int it = 1; it < m; ++it) {
-      faces(c, opb, (const B*)Uad(c), T, s);
-      pred(c, opb, (const B*)Uad(c), T, Usp, dt, Uad(c), nullptr, 0, s);
+    if constexpr (NDIM == 2) {
+      // fused iterations alternate between the two predictor buffers; the
+      // face kernels run once, on the final state, for the corrector
+      B* buf[2] = {(B*)c->Uad, (B*)c->Uad2};
+      pred2d(c, opb, (const double*)Usp, 1, Usp, dt, buf[0], s);
+      for (int it = 1; it < m; ++it)
+        pred2d(c, opb, (const B*)buf[(it - 1) & 1], T, Usp, dt, buf[it & 1], s);
+      c->Uad_cur = buf[(m - 1) & 1];
+    } else {
+      c->Uad_cur = c->Uad;
+      faces(c, opb, (const double*)Usp, 1, s);
+      pred(c, opb, (const double*)Usp, 1, Usp, dt, Uad(c), nullptr, 0, s);
+      for (int it = 1; it < m; ++it) {
+        faces(c, opb, (const B*)Uad(c), T, s);
+        pred(c, opb, (const B*)Uad(c), T, Usp, dt, Uad(c), nullptr, 0, s);
+      }
     }
     faces(c, opb, (const B*)Uad(c), T, s);
     if (c->cfg.update == SPD_UPDATE_SD) {
@@ -2124,6 +2159,8 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
     return r;
   };
   if ((rc = alloc_bulk(&c->Uad, T * V * g.NE * NL)) != SPD_OK) return rc;
+  if (nd == 2 && (rc = alloc_bulk(&c->Uad2, T * V * g.NE * NL)) != SPD_OK) return rc;
+  c->Uad_cur = c->Uad;
   for (int a = 0; a < nd; ++a) {
     const long long Na = g.N[a];
     if ((rc = alloc_bulk(&c->Fface[a], T * V * (Na + 1) * (g.NE / Na) * NLP)) != SPD_OK)
