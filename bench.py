@@ -105,7 +105,7 @@ def algorithmic_bytes(N, p, launches):
 CLASS_KERNELS = {
     "k_fbface": ["k_fbcorrect2d<1, double>"],
     "k_face_sd": ["k_face_sd_x<2, 3, 1, double, double>", "k_face_sd<2, 3, 1, 1, double, double>"],
-    "k_pred": ["k_pred2d<3, 1, double, double>", "k_pred2d<3, 1, double, double>"],
+    "k_pred": ["k_pred2d<3, 1, double, double>"],
     "k_detect": ["k_detect2d"],
     "k_hoflux": ["k_ho2d<3, double>"],
     "k_apply": ["k_apply<2, double>"],
