@@ -40,8 +40,7 @@ template <int P> struct Layout {
   static constexpr int O_TR = O_DUDT + TE * T * SLAB;  // [TE][T][n][V] upper traces, then lower-face fluxes
   static constexpr int O_HL = O_TR + TE * LINE;      // [TM][T][n][V] upper traces of the lower halo
   static constexpr int O_HU = O_HL + TM * LINE;      // [TM][T][n][V] lower traces of the upper halo
-  static constexpr int O_FB = O_HU + TM * LINE;      // [TM][T][n][V] upper faces at the tile side
-  static constexpr int TOTAL = O_FB + TM * LINE;
+  static constexpr int TOTAL = O_HU + TM * LINE;
   static_assert(NTH <= 256, "one block per tile");
   static_assert(2 * TM * T * n <= NTH, "one halo line per thread");
 };
@@ -59,7 +58,6 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
   B* sTR = lds + L::O_TR;
   B* sHL = lds + L::O_HL;
   B* sHU = lds + L::O_HU;
-  B* sFB = lds + L::O_FB;
   const int Nx = g.N[0], Ny = g.N[1];
   const long long NE = g.NE;
   const int ntx = (Nx + TX - 1) / TX;
