@@ -1,0 +1,151 @@
+"""The reference's own test suite (test/test_create.py, test/test_update.py,
+test/test_transforms.py) restated against this API, plus closed-form physics
+checks: the Sod shock tube against the exact solution, convergence of the
+isentropic vortex, free-stream preservation, and the NaN path of the loop."""
+import numpy as np
+import pytest
+
+from exact_riemann import sample
+
+pytestmark = pytest.mark.gpu
+
+
+def sim(**kw):
+    from spd_amd.simulator import SDADER_Simulator
+    return SDADER_Simulator(device="cuda:0", verbose=False, **kw)
+
+
+@pytest.mark.parametrize("p", [1, 3])
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("mode", ["SD", "FV", "FB"])
+def test_create_and_update_against_oracle(p, ndim, mode):
+    """test/test_create.py and test/test_update.py: construction and one
+    iteration for p in {1,3}, 1-/2-/3-D, SD / FV / FV+FB (sine-wave IC), here
+    also compared with the oracle (states within 1e-12)."""
+    from oracle.spd_oracle import OracleSimulator
+    from spd_amd import initial_conditions as ic
+    print(f"[create/update] p={p} ndim={ndim} {mode}", flush=True)
+    kw = dict(p=p, N=(4,) * ndim, update="SD" if mode == "SD" else "FV", FB=(mode == "FB"),
+              BC=(("periodic", "periodic"),) * ndim)
+    s = sim(init_fct=ic.sine_wave, **kw)
+    o = OracleSimulator(ic.sine_wave, **kw)
+    s.perform_iterations(1)
+    o.compute_dt()
+    o.perform_update()
+    U = np.asarray(s.dm.U_sp)
+    assert np.all(np.isfinite(U))
+    scale = np.max(np.abs(o.U_sp))
+    assert np.max(np.abs(U - o.U_sp)) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("ndim,p", [(1, 1), (1, 5), (2, 2), (2, 5), (3, 1), (3, 3)])
+def test_transform_round_trips(ndim, p):
+    """test/test_transforms.py: cv -> sp -> cv and sp -> cv -> sp to 1e-13."""
+    from spd_amd import initial_conditions as ic
+    s = sim(init_fct=ic.sine_wave, p=p, N=(3,) * ndim)
+    rng = np.random.default_rng(ndim * 10 + p)
+    W = rng.uniform(0.5, 2.0, size=s.dm.U_sp.shape)
+    a = s.compute_cv_from_sp(s.compute_sp_from_cv(W))
+    b = s.compute_sp_from_cv(s.compute_cv_from_sp(W))
+    assert np.max(np.abs(a - W)) < 1e-13
+    assert np.max(np.abs(b - W)) < 1e-13
+
+
+def _sod_density_l1(W_cv, s):
+    """L1 distance of the subcell-mean density from the exact solution's
+    subcell means (exact solution averaged on 64 points per subcell)."""
+    from spd_amd.mesh import flux_points
+    fp = flux_points(s.p)
+    N, h = s.N["x"], s.h["x"]
+    faces = (np.arange(N)[:, None] + fp[None, :]) * h
+    lo, hi = faces[:, :-1].ravel(), faces[:, 1:].ravel()
+    q = (np.arange(64) + 0.5) / 64
+    pts = lo[:, None] + (hi - lo)[:, None] * q[None, :]
+    ex = sample(pts.ravel(), s.time, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1))[0].reshape(pts.shape)
+    exm = ex.mean(axis=1)
+    rho = np.asarray(W_cv)[0].ravel()
+    return float(np.sum(np.abs(rho - exm) * (hi - lo)))
+
+
+def test_sod_against_exact_solution():
+    """The 1-D Sod configuration (N=100, p=3, HLLC, NAD+PAD, MUSCL-Hancock,
+    outflow, t=0.2): density L1 error against the exact solution below 5e-3,
+    and the same as the oracle's to 1e-10."""
+    from oracle.spd_oracle import OracleSimulator
+    from spd_amd import initial_conditions as ic
+    kw = dict(p=3, N=(100,), BC=(("gradfree", "gradfree"),), update="FV", FB=True,
+              riemann_solver_sd="hllc", riemann_solver_fv="hllc", cfl_coeff=0.8)
+    s = sim(init_fct=ic.sod_shock_tube, **kw)
+    s.perform_time_evolution(0.2)
+    err = _sod_density_l1(s.dm.W_cv, s)
+    o = OracleSimulator(ic.sod_shock_tube, **kw)
+    o.perform_time_evolution(0.2)
+    o_err = _sod_density_l1(o.W_cv, s)
+    print(f"[sod] L1 density error {err:.3e} (oracle {o_err:.3e}), {s.n_step} steps", flush=True)
+    assert err < 5e-3
+    assert abs(err - o_err) <= 1e-10
+
+
+@pytest.mark.parametrize("p,mode", [(2, "FV"), (3, "SD")])
+def test_isentropic_vortex_convergence(p, mode):
+    """Smooth exact solution (Shu's vortex advected by (1,1) on [-5,5]^2,
+    t=0.5): the density L1 error of the subcell means falls at least like
+    h^(p+1-0.5) from N=8 to 16 to 32 elements (no fallback)."""
+    from spd_amd import initial_conditions as ic
+    errs = []
+    t_end = 0.5
+    for N in (8, 16, 32):
+        kw = dict(p=p, N=(N, N), xlim=(-5, 5), ylim=(-5, 5), update=mode,
+                  riemann_solver_sd="hllc")
+        s = sim(init_fct=ic.isentropic_vortex, **kw)
+        s.perform_time_evolution(t_end)
+        exact = sim(init_fct=lambda xyz, c: ic.isentropic_vortex(xyz, c, t=t_end), **kw)
+        err = np.mean(np.abs(np.asarray(s.dm.W_cv)[0] - exact.W_init_cv[0])) * 100.0
+        errs.append(err)
+    rates = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    print(f"[vortex p={p} {mode}] errors {errs} rates {rates}", flush=True)
+    assert min(rates) > p + 1 - 0.5
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("mode", ["SD", "FV", "FB"])
+def test_free_stream_preserved(ndim, mode):
+    """A uniform flow stays uniform to round-off (outflow sides)."""
+    state = {0: 1.3, 1: 0.4, 2: -0.7, 3: 0.25, "P": 2.1}
+
+    def uniform(xyz, case):
+        nd = xyz.shape[0]
+        v = state["P"] if case == nd + 1 else state[case]
+        return v + 0.0 * xyz[0]
+    kw = dict(p=2, N=(5,) * ndim, update="SD" if mode == "SD" else "FV", FB=(mode == "FB"),
+              BC=(("gradfree", "gradfree"),) * ndim, riemann_solver_sd="hllc",
+              riemann_solver_fv="hllc")
+    s = sim(init_fct=uniform, **kw)
+    W0 = np.asarray(s.W_init_cv).copy()
+    s.perform_iterations(5)
+    W = np.asarray(s.dm.W_cv)
+    assert np.max(np.abs(W - W0)) < 1e-13
+
+
+def test_nan_state_stops_both_loops():
+    """A NaN in the state makes dt NaN; the reference's loop then takes one
+    step (NaN time) and stops (src/sdader_simulator.py:401-413) -- the host
+    and the device loop alike, without faults."""
+    import torch
+    from spd_amd import initial_conditions as ic
+    out = []
+    for device_loop in (False, True):
+        s = sim(init_fct=ic.kelvin_helmholtz, p=3, N=(8, 8), update="FV", FB=True,
+                device_loop=device_loop)
+        s.init_sim()
+        U = s.dm.device_value("U_sp")
+        U[0, 3, 4, 1, 2] = float("nan")
+        s._dt_from_step = False
+        W = s.dm.device_value("W_cv")
+        W[0, 3, 4, 1, 2] = float("nan")
+        s.end_sim()
+        s.perform_time_evolution(1.0)
+        out.append((s.n_step, s.time))
+        torch.cuda.synchronize()
+    assert out[0][0] == out[1][0] == 1
+    assert np.isnan(out[0][1]) and np.isnan(out[1][1])

@@ -166,3 +166,48 @@ def test_ic_callables_accept_torch_tensors():
             b = torch.as_tensor(b, dtype=torch.float64).expand(pts.shape[1:]).numpy()
             np.testing.assert_allclose(b, a, rtol=1e-14, atol=1e-15,
                                        err_msg=f"{getattr(fct, '__name__', fct)} var {var}")
+
+
+def test_ctx_create_rejects_invalid_configs():
+    """Every configuration error is a status code and a message, returned before
+    any device call (no GPU needed); the reference raises on the same inputs."""
+    import ctypes
+    from spd_amd import _lib
+    lib = _lib.load()
+
+    def base():
+        c = _lib.SpdConfig()
+        c.ndim, c.p, c.m = 2, 3, 3
+        for a in range(3):
+            c.N[a] = 4
+            c.lim[a][0], c.lim[a][1] = 0.0, 1.0
+        c.gamma, c.cfl_coeff, c.min_c2 = 1.4, 0.8, 1e-10
+        c.update, c.FB = _lib.UPDATE["FV"], 1
+        c.riemann_sd = c.riemann_fv = _lib.RS["hllc"]
+        c.tolerance, c.SED, c.PAD, c.blending = 1e-5, 1, 1, 1
+        c.min_rho, c.max_rho, c.min_P = 1e-10, 1e10, 1e-10
+        return c
+
+    E_INVALID, E_UNSUPPORTED = -1, -2
+    cases = [
+        (lambda c: setattr(c, "ndim", 4), E_INVALID, "ndim"),
+        (lambda c: setattr(c, "precision", 3), E_INVALID, "precision"),
+        (lambda c: (setattr(c, "p", 9), setattr(c, "m", 9)), E_UNSUPPORTED, "no kernels"),
+        (lambda c: setattr(c, "m", 2), E_UNSUPPORTED, "m == p"),
+        (lambda c: setattr(c, "update", 5), E_INVALID, "update"),
+        (lambda c: setattr(c, "riemann_fv", 7), E_UNSUPPORTED, "riemann"),
+        (lambda c: setattr(c, "slope_limiter", 4), E_UNSUPPORTED, "limiter"),
+        (lambda c: c.N.__setitem__(1, 0), E_INVALID, "N must"),
+        (lambda c: c.bc[0].__setitem__(1, 9), E_UNSUPPORTED, "boundary"),
+        (lambda c: c.lim[1].__setitem__(1, -1.0), E_INVALID, "empty domain"),
+    ]
+    for mutate, code, msg in cases:
+        c = base()
+        mutate(c)
+        ctx = ctypes.c_void_p()
+        rc = lib.spd_ctx_create(ctypes.byref(c), ctypes.byref(ctx))
+        assert rc == code, (msg, rc)
+        assert msg in lib.spd_last_error().decode(), (msg, lib.spd_last_error())
+        assert not ctx.value
+    assert lib.spd_run_begin(None, 0.0, 1.0, 0, -1, 1, None) == E_INVALID
+    assert lib.spd_set_launch(None, 0, 1) == E_INVALID
