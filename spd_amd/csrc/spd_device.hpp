@@ -52,6 +52,13 @@ template <typename R> __device__ __forceinline__ R qsqrt(R x) {
     return sqrt(x);
 }
 
+// 1 / (gamma - 1): the same value at every call of a thread, so the compiler
+// evaluates it once; multiplying by it is within an ulp of the division of
+// src/hydro.py:118-124,159-166.
+template <typename R> __device__ __forceinline__ R inv_gm1(R gamma) {
+  return qrcp<R>(gamma - R(1));
+}
+
 // numpy.maximum / numpy.minimum: NaN in either operand propagates.
 template <typename R> __device__ __forceinline__ R nmax(R a, R b) {
   if (a != a) return a;
@@ -86,11 +93,14 @@ __device__ __forceinline__ void prims(const R* U, R* W, int a, R gamma) {
     }
     W[P] = (float)(((double)gamma - 1.0) * ((double)U[P] - 0.5 * K));
   } else {
+    // one reciprocal of the density for all velocity components (within an
+    // ulp of the per-component divisions of src/hydro.py:79-81)
+    const double rr = 1.0 / U[0];
     double K = U[0] * 0.0;
 #pragma unroll
     for (int k = 0; k < NDIM; ++k) {
       const int v = rolled<NDIM>(a, k);
-      W[v] = U[v] / U[0];
+      W[v] = U[v] * rr;
       K = K + W[v] * W[v];
     }
     K = K * (0.5 * U[0]);
@@ -111,7 +121,7 @@ __device__ __forceinline__ void cons(const R* W, R* U, int a, R gamma) {
     K = K + W[v] * W[v];
   }
   K = K * (R(0.5) * U[0]);
-  U[P] = qdiv<R>(W[P], gamma - R(1)) + K;
+  U[P] = W[P] * inv_gm1<R>(gamma) + K;
 }
 
 // Euler flux along dim a from primitives (src/hydro.py:127-170): the kinetic
@@ -129,7 +139,7 @@ __device__ __forceinline__ void flux(const R* W, R* F, int a, R gamma) {
     K = K + m * W[v];
     F[v] = m * W[v1];
   }
-  const R E = qdiv<R>(W[P], gamma - R(1)) + R(0.5) * K;
+  const R E = W[P] * inv_gm1<R>(gamma) + R(0.5) * K;
   F[0] = m;
   F[v1] = m * W[v1] + W[P];
   F[P] = W[v1] * (E + W[P]);
@@ -227,10 +237,14 @@ __device__ __forceinline__ void riemann(const R* WL, const R* WR, const R* UL, c
 
 // Slope limiters of src/muscl.py:12-59 applied to the one-sided slopes dL, dR
 // of a cell; xL, xR = centre spacings, xM = cell width (moncen only).
+// minmod is (R/L clipped to [., 1], then to [0, .]) * L (Q7).  Evaluated
+// without the division: same-sign slopes give R when |R| < |L| (the
+// reference's (R/L) * L, within an ulp) and L otherwise; opposite signs or a
+// zero give 0; a NaN R gives L and a NaN L gives NaN, as the ratio rules do.
 template <typename R> __device__ __forceinline__ R minmod(R dL, R dR) {
-  R r = qdiv<R>(dR, dL);
-  r = (r < R(1)) ? r : R(1);
-  return ((r > R(0)) ? r : R(0)) * dL;
+  if (dL != dL || dR != dR) return dL;
+  if (!(dL * dR > R(0))) return dL * R(0);
+  return (fabs(dR) < fabs(dL)) ? dR : dL;
 }
 template <typename R> __device__ __forceinline__ R moncen(R dL, R dR, R xL, R xR, R xM) {
   const R dC = qdiv<R>(xL * dL + xR * dR, xL + xR);

@@ -410,13 +410,14 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
       }
     }
     B dtW[4] = {B(0), B(0), B(0), B(0)};
+    const B rrho = qrcp<B>(Wc[0]);  // shared by both directions (within an ulp)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int vel = b + 1;
       const B* dW = dMh[b];
       dtW[0] = dtW[0] - (Wc[vel] * dW[0] + Wc[0] * dW[vel]);
       dtW[3] = dtW[3] - (Wc[vel] * dW[3] + gamma * Wc[3] * dW[vel]);
-      dtW[vel] = dtW[vel] - (Wc[vel] * dW[vel] + qdiv<B>(dW[3], Wc[0]));
+      dtW[vel] = dtW[vel] - (Wc[vel] * dW[vel] + dW[3] * rrho);
       dtW[2] = dtW[2] - Wc[vel] * dW[2];
     }
     dtW[3] = dtW[3] * B(0);
