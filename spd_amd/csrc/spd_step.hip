@@ -1789,8 +1789,10 @@ template <int NDIM, int P, typename B> struct Stepper {
   static int ho_epb_max() { return std::max(1, 49152 / (V * NL * (int)sizeof(B))); }
   static int ho_epb(const spd_ctx* c) {
     const int mx = ho_epb_max();
+    // default: the element count of a 24 KiB FP64 slab in both forms (the
+    // best of a sweep at Kelvin-Helmholtz 1024^2 p=3 in both precisions)
     return c->launch[SPD_LAUNCH_HO_EPB] > 0 ? std::min(mx, c->launch[SPD_LAUNCH_HO_EPB])
-                                            : std::max(1, 32768 / (V * NL * (int)sizeof(B)));
+                                            : std::min(mx, std::max(1, 24576 / (V * NL * 8)));
   }
   static int launch_value(const spd_ctx* c, int key) {
     return key == SPD_LAUNCH_PRED_EPB ? epb(c) : ho_epb(c);
