@@ -101,17 +101,20 @@ def algorithmic_bytes(N, p, launches):
     return out
 
 
-# rocprofv3 names of the kernels of each class in the FP64 2-D HLLC workload
+# rocprofv3 names of the kernels of each class in the FP64 2-D HLLC workload,
+# with their launches per step
 CLASS_KERNELS = {
-    "k_fbface": ["k_fbcorrect2d<1, double>"],
-    "k_face_sd": ["k_face_sd_x<2, 3, 1, double, double>", "k_face_sd<2, 3, 1, 1, double, double>"],
-    "k_pred": ["k_pred2d<3, 1, double, double>"],
-    "k_detect": ["k_detect2d"],
-    "k_hoflux": ["k_ho2d<3, double>"],
-    "k_apply": ["k_apply<2, double>"],
-    "k_theta1": ["k_theta2d"],
-    "k_cv2sp": ["k_cv2sp<2, 3>"],
-    "k_tensor": ["k_tensor<2, 3>"],
+    "k_fbface": [("k_fbcorrect2d<1, double>", 4)],
+    "k_face_sd": [("k_face_sd_x<2, 3, 1, double, double>", 1),
+                  ("k_face_sd<2, 3, 1, 1, double, double>", 1)],
+    "k_pred": [("k_pred2d<3, 1, double, double, true>", 1),
+               ("k_pred2d<3, 1, double, double, false>", 2)],
+    "k_detect": [("k_detect2d", 4)],
+    "k_hoflux": [("k_ho2d<3, double>", 4)],
+    "k_apply": [("k_apply<2, double>", 4)],
+    "k_theta1": [("k_theta2d", 4)],
+    "k_cv2sp": [("k_cv2sp<2, 3>", 1)],
+    "k_tensor": [("k_tensor<2, 3>", 1)],
 }
 TRAFFIC_SUMMARY = os.path.join(HERE, "profiles", "kh1024_p3_hbm_traffic.md")
 
@@ -119,10 +122,12 @@ TRAFFIC_SUMMARY = os.path.join(HERE, "profiles", "kh1024_p3_hbm_traffic.md")
 def measured_traffic(cls):
     """HBM bytes per launch of a kernel class from the committed counter
     summary (FETCH_SIZE / WRITE_SIZE passes scaled by the copy calibration),
-    averaged over the class's kernels; None when not recorded."""
-    names = CLASS_KERNELS.get(cls)
-    if not names or not os.path.exists(TRAFFIC_SUMMARY):
+    averaged over the class's kernels weighted by their launches per step;
+    None when not recorded."""
+    entries = CLASS_KERNELS.get(cls)
+    if not entries or not os.path.exists(TRAFFIC_SUMMARY):
         return None
+    names = dict(entries)
     vals = {}
     with open(TRAFFIC_SUMMARY) as f:
         for line in f:
@@ -134,7 +139,7 @@ def measured_traffic(cls):
                     pass
     if len(vals) != len(names):
         return None
-    return sum(vals.values()) / len(vals)
+    return sum(vals[k] * w for k, w in names.items()) / sum(names.values())
 
 
 def copy_bandwidth(torch, nbytes=2 << 30, reps=10):

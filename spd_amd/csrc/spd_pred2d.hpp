@@ -21,37 +21,50 @@
 namespace spd {
 namespace pred2d {
 
-template <int P> struct Tile;  // elements per block along x and y
+// Elements per block along x and y: all T time nodes per element (Tile), or
+// one (the first iteration, whose source U_sp is the same at every node).
+template <int P> struct Tile;
 template <> struct Tile<1> { static constexpr int X = 8, Y = 8; };
 template <> struct Tile<2> { static constexpr int X = 5, Y = 5; };
 template <> struct Tile<3> { static constexpr int X = 4, Y = 4; };
 template <> struct Tile<4> { static constexpr int X = 3, Y = 2; };
 template <> struct Tile<5> { static constexpr int X = 2, Y = 2; };
+template <int P> struct TileFirst;
+template <> struct TileFirst<1> { static constexpr int X = 16, Y = 8; };
+template <> struct TileFirst<2> { static constexpr int X = 9, Y = 9; };
+template <> struct TileFirst<3> { static constexpr int X = 8, Y = 8; };
+template <> struct TileFirst<4> { static constexpr int X = 7, Y = 6; };
+template <> struct TileFirst<5> { static constexpr int X = 6, Y = 5; };
 
-template <int P> struct Layout {
+template <int P, bool FIRST> struct Layout {
   static constexpr int V = 4, n = P + 1, nf = P + 2, T = P + 1, NL = n * n;
-  static constexpr int TX = Tile<P>::X, TY = Tile<P>::Y, TE = TX * TY;
-  static constexpr int NTH = TE * T * n;             // threads per block
+  static constexpr int TA = FIRST ? 1 : T;           // time nodes evaluated per element
+  static constexpr int TX = FIRST ? TileFirst<P>::X : Tile<P>::X;
+  static constexpr int TY = FIRST ? TileFirst<P>::Y : Tile<P>::Y;
+  static constexpr int TE = TX * TY;
+  static constexpr int NTH = TE * TA * n;            // threads per block
   static constexpr int TM = TX > TY ? TX : TY;       // rows of a halo side
   static constexpr int SLAB = V * NL + 1;            // dudt slab (padded)
-  static constexpr int LINE = T * n * V;             // traces of one element / halo row
+  static constexpr int LINE = TA * n * V;            // traces of one element / halo row
   // LDS offsets, in values of the bulk type
-  static constexpr int O_DUDT = 0;                   // [TE][T][SLAB]
-  static constexpr int O_TR = O_DUDT + TE * T * SLAB;  // [TE][T][n][V] upper traces, then lower-face fluxes
-  static constexpr int O_HL = O_TR + TE * LINE;      // [TM][T][n][V] upper traces of the lower halo
-  static constexpr int O_HU = O_HL + TM * LINE;      // [TM][T][n][V] lower traces of the upper halo
+  static constexpr int O_DUDT = 0;                   // [TE][TA][SLAB]
+  static constexpr int O_TR = O_DUDT + TE * TA * SLAB;  // [TE][TA][n][V] upper traces, then lower-face fluxes
+  static constexpr int O_HL = O_TR + TE * LINE;      // [TM][TA][n][V] upper traces of the lower halo
+  static constexpr int O_HU = O_HL + TM * LINE;      // [TM][TA][n][V] lower traces of the upper halo
   static constexpr int TOTAL = O_HU + TM * LINE;
   static_assert(NTH <= 256, "one block per tile");
-  static_assert(2 * TM * T * n <= NTH, "one halo line per thread");
+  static_assert(2 * TM * TA * n <= NTH, "one halo line per thread");
+  static_assert(TOTAL * 8 <= 49152, "LDS of the FP64 form");
 };
 
-template <int P, int RS, typename S, typename B>
+// FIRST: the first iteration, src = U_sp (Tsrc = 1); otherwise Tsrc = T.
+template <int P, int RS, typename S, typename B, bool FIRST>
 __global__ void __launch_bounds__(256)
 k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
          const double* __restrict__ Usp, const double* __restrict__ dt_dev, B* __restrict__ out) {
   SPD_SKIP(g);
-  using L = Layout<P>;
-  constexpr int V = 4, n = L::n, nf = L::nf, T = L::T, NL = L::NL;
+  using L = Layout<P, FIRST>;
+  constexpr int V = 4, n = L::n, nf = L::nf, T = L::TA, NL = L::NL;
   constexpr int TX = L::TX, TY = L::TY, SLAB = L::SLAB;
   B* lds = reinterpret_cast<B*>(spd_dyn_lds);
   B* dudt = lds + L::O_DUDT;
@@ -248,10 +261,10 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
     const double u0 = Usp[off];
     const B* dd = dudt + (long long)le * T * SLAB + v * NL + l;
 #pragma unroll
-    for (int tn = 0; tn < T; ++tn) {
+    for (int tn = 0; tn < L::T; ++tn) {
       B acc = B(0);
 #pragma unroll
-      for (int p = 0; p < T; ++p) {
+      for (int p = 0; p < L::T; ++p) {
         const int ps = (p < Tsrc) ? p : (Tsrc - 1);
         acc += op.inv[tn][p] * dd[(long long)ps * SLAB];
       }

@@ -1717,16 +1717,24 @@ template <int NDIM, int P, typename B> struct Stepper {
       faces_rs<RS_LLF>(c, op, src, Tsrc, s);
   }
   // fused 2-D predictor iteration (element faces solved in the kernel)
-  template <int RS, typename S>
-  static void pred2d_rs(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
-                        const double* dt, B* out, hipStream_t s) {
-    using L = pred2d::Layout<P>;
+  template <int RS, bool FIRST, typename S>
+  static void pred2d_launch(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
+                            const double* dt, B* out, hipStream_t s) {
+    using L = pred2d::Layout<P, FIRST>;
     const int ntx = (c->g.N[0] + L::TX - 1) / L::TX, nty = (c->g.N[1] + L::TY - 1) / L::TY;
     c->timed(KC_PRED, s, [&] {
-      hipLaunchKernelGGL((pred2d::k_pred2d<P, RS, S, B>), dim3((unsigned)(ntx * nty)),
+      hipLaunchKernelGGL((pred2d::k_pred2d<P, RS, S, B, FIRST>), dim3((unsigned)(ntx * nty)),
                          dim3(L::NTH), (size_t)L::TOTAL * sizeof(B), s, op, c->g, c->prm, src,
                          Tsrc, Usp, dt, out);
     });
+  }
+  template <int RS, typename S>
+  static void pred2d_rs(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
+                        const double* dt, B* out, hipStream_t s) {
+    if (Tsrc == 1)
+      pred2d_launch<RS, true>(c, op, src, Tsrc, Usp, dt, out, s);
+    else
+      pred2d_launch<RS, false>(c, op, src, Tsrc, Usp, dt, out, s);
   }
   template <typename S>
   static void pred2d(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
