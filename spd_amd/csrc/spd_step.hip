@@ -653,6 +653,117 @@ k_tensor(Ops<P> op, Geom g, int to_cv, const double* __restrict__ in, double* __
 }
 
 // ---------------------------------------------------------------------------
+// The same transforms for 1-D and 2-D with one thread per (variable,
+// element): the element's values are loaded once (contiguous), transformed
+// dimension by dimension in registers (the separable form of the tensor
+// product; the summation order differs from the single sum above by
+// round-off) and written once.  in_fv / out_fv select the FV layout for the
+// input / output; in_copy (optional) receives the input in element layout.
+// mode: 1 sp -> cv, 0 cv -> sp, 2 sp -> regular subcell centres.
+// ---------------------------------------------------------------------------
+template <int NDIM, int P>
+__global__ void __launch_bounds__(256)
+k_tensor_el(Ops<P> op, Geom g, int mode, const double* __restrict__ in, int in_fv,
+            double* __restrict__ out, int out_fv, double* __restrict__ in_copy) {
+  static_assert(NDIM <= 2, "1-D and 2-D");
+  SPD_SKIP(g);
+  constexpr int V = Dims<NDIM>::V, n = P + 1;
+  constexpr int NY = (NDIM == 2) ? n : 1;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long long)V * g.NE) return;
+  const long long e = tid % g.NE;
+  const int v = (int)(tid / g.NE);
+  const double(*M)[n] = (mode == 1) ? op.s2c : ((mode == 2) ? op.s2r : op.c2s);
+  const int ex = (int)(e % g.N[0]);
+  const long long ey = e / g.N[0];
+  // point (j, i) of the element: element layout e*NL + j*n + i; FV layout
+  // (ey*n + j) * Ns_x + ex*n + i
+  auto idx = [&](int fv, int j, int i) -> long long {
+    if (fv) return (long long)v * g.Nc + (ey * NY + j) * g.Ns[0] + (long long)ex * n + i;
+    return ((long long)v * g.NE + e) * (NY * n) + j * n + i;
+  };
+  double u[NY][n];
+#pragma unroll
+  for (int j = 0; j < NY; ++j)
+#pragma unroll
+    for (int i = 0; i < n; ++i) u[j][i] = in[idx(in_fv, j, i)];
+  if (in_copy) {
+#pragma unroll
+    for (int j = 0; j < NY; ++j)
+#pragma unroll
+      for (int i = 0; i < n; ++i) in_copy[idx(0, j, i)] = u[j][i];
+  }
+  double t[NY][n];
+#pragma unroll
+  for (int j = 0; j < NY; ++j)
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < n; ++k) acc += M[i][k] * u[j][k];
+      t[j][i] = acc;
+    }
+#pragma unroll
+  for (int jo = 0; jo < NY; ++jo)
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+      double acc;
+      if constexpr (NDIM == 2) {
+        acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < n; ++k) acc += M[jo][k] * t[k][i];
+      } else {
+        acc = t[jo][i];
+      }
+      out[idx(out_fv, jo, i)] = acc;
+    }
+}
+
+// Maximum wave speed of the new subcell-mean state (src/sd_simulator.py:
+// 214-227) per block, and the optional W_cv snapshot in element layout; one
+// thread per cell of the FV layout.
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_wave_cells(Geom g, Params prm, int n, const double* __restrict__ Ucv,
+             double* __restrict__ Wcv_sd, double* __restrict__ partial) {
+  SPD_SKIP(g);
+  constexpr int V = Dims<NDIM>::V, P_ = Dims<NDIM>::P;
+  const long long ci = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double cw = 0.0;
+  if (ci < g.Nc) {
+    double u[V], W[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) u[v] = Ucv[v * g.Nc + ci];
+    prims<NDIM>(u, W, 0, prm.gamma);
+    if (Wcv_sd) {
+      int c[3] = {0, 0, 0};
+      long long x = ci;
+#pragma unroll
+      for (int a = 0; a < NDIM; ++a) {
+        c[a] = (int)(x % g.Ns[a]);
+        x /= g.Ns[a];
+      }
+      long long e = 0, l = 0;
+#pragma unroll
+      for (int a = NDIM - 1; a >= 0; --a) {
+        e = e * g.N[a] + c[a] / n;
+        l = l * n + c[a] % n;
+      }
+      long long NL = 1;
+#pragma unroll
+      for (int a = 0; a < NDIM; ++a) NL *= n;
+#pragma unroll
+      for (int v = 0; v < V; ++v) Wcv_sd[((long long)v * g.NE + e) * NL + l] = W[v];
+    }
+    cw = sound(W[P_], W[0], prm.gamma, prm.min_c2) * NDIM;
+#pragma unroll
+    for (int k = 1; k <= NDIM; ++k) cw = cw + fabs(W[k]);
+  }
+  cw = block_max(cw);
+  if (threadIdx.x == 0) partial[blockIdx.x] = cw;
+}
+
+// ---------------------------------------------------------------------------
 // FV update U_out = U_in - dt_i * sum_a (F_a[f+1] - F_a[f]) / dx_a
 // (src/fv_simulator.py:189-205); in place when U_out == U_in.
 // ---------------------------------------------------------------------------
@@ -1898,8 +2009,12 @@ template <int NDIM, int P, typename B> struct Stepper {
     }
     // --- FV corrector (src/sdader_simulator.py:288-349)
     c->timed(KC_TENSOR, s, [&] {
-      hipLaunchKernelGGL((k_tensor<NDIM, P>), dim3(grid_for(c->g.NE * V * NL)), dim3(256), 0, s,
-                         op, c->g, 1, Usp, c->Ucv, 1);
+      if constexpr (NDIM <= 2)
+        hipLaunchKernelGGL((k_tensor_el<NDIM, P>), dim3(grid_for(c->g.NE * V)), dim3(256), 0, s,
+                           op, c->g, 1, Usp, 0, c->Ucv, 1, (double*)nullptr);
+      else
+        hipLaunchKernelGGL((k_tensor<NDIM, P>), dim3(grid_for(c->g.NE * V * NL)), dim3(256), 0, s,
+                           op, c->g, 1, Usp, c->Ucv, 1);
     });
     for (int i = 0; i < T; ++i) {
       const double w = c->ops->w_tp[i];
@@ -1937,11 +2052,22 @@ template <int NDIM, int P, typename B> struct Stepper {
       apply(c, dt, w, c->Ucv, s);
     }
     const unsigned nb = grid_for(c->g.NE * NL);
-    c->timed(KC_CV2SP, s, [&] {
-      hipLaunchKernelGGL((k_cv2sp<NDIM, P>), dim3(nb), dim3(256), 0, s, op, c->g, c->prm, c->Ucv,
-                         Usp, snapshot ? Ucv_sd : nullptr, snapshot ? Wcv_sd : nullptr,
-                         c->partial);
-    });
+    if constexpr (NDIM <= 2) {
+      c->timed(KC_CV2SP, s, [&] {
+        hipLaunchKernelGGL((k_tensor_el<NDIM, P>), dim3(grid_for(c->g.NE * V)), dim3(256), 0, s,
+                           op, c->g, 0, c->Ucv, 1, Usp, 0, snapshot ? Ucv_sd : nullptr);
+      });
+      c->timed(KC_CV2SP, s, [&] {
+        hipLaunchKernelGGL((k_wave_cells<NDIM>), dim3(nb), dim3(256), 0, s, c->g, c->prm, n,
+                           c->Ucv, snapshot ? Wcv_sd : nullptr, c->partial);
+      });
+    } else {
+      c->timed(KC_CV2SP, s, [&] {
+        hipLaunchKernelGGL((k_cv2sp<NDIM, P>), dim3(nb), dim3(256), 0, s, op, c->g, c->prm,
+                           c->Ucv, Usp, snapshot ? Ucv_sd : nullptr, snapshot ? Wcv_sd : nullptr,
+                           c->partial);
+      });
+    }
     hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, c->partial, (long long)nb, c->cmax,
                        c->g.skip);
     hipError_t e = hipGetLastError();
@@ -1951,8 +2077,12 @@ template <int NDIM, int P, typename B> struct Stepper {
 
   static int tensor(spd_ctx* c, int to_cv, const double* in, double* out, hipStream_t s) {
     const Ops<P> op = make_ops<P>(*c->ops);
-    hipLaunchKernelGGL((k_tensor<NDIM, P>), dim3(grid_for(c->g.NE * V * NL)), dim3(256), 0, s, op,
-                       c->g, to_cv, in, out, 0);
+    if constexpr (NDIM <= 2)
+      hipLaunchKernelGGL((k_tensor_el<NDIM, P>), dim3(grid_for(c->g.NE * V)), dim3(256), 0, s, op,
+                         c->g, to_cv, in, 0, out, 0, (double*)nullptr);
+    else
+      hipLaunchKernelGGL((k_tensor<NDIM, P>), dim3(grid_for(c->g.NE * V * NL)), dim3(256), 0, s, op,
+                         c->g, to_cv, in, out, 0);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
   }
