@@ -21,13 +21,15 @@ __device__ __forceinline__ bool in_ghost_range(const Geom& g, int x, int y) {
 }
 
 // ---------------------------------------------------------------------------
-// Trouble detection (src/trouble_detection.py:5-71) on a TX x TY tile.
+// Trouble detection (src/trouble_detection.py:5-71) on a DTX x DTY tile
+// (512 threads: the taller tile stages fewer halo cells per cell).
 //   wo: old primitives on the tile + 1 halo (all variables)
 //   wn: new (rho, P) on the tile + 3 halo (smooth-extrema stencil)
 // ---------------------------------------------------------------------------
+constexpr int DTX = 32, DTY = 16, DNT = DTX * DTY;
 constexpr int DH = 3;
-constexpr int DNX = TX + 2 * DH, DNY = TY + 2 * DH;
-constexpr int ONX = TX + 2, ONY = TY + 2;
+constexpr int DNX = DTX + 2 * DH, DNY = DTY + 2 * DH;
+constexpr int ONX = DTX + 2, ONY = DTY + 2;
 
 // Smooth-extrema quantities along one axis from LDS (src/trouble_detection.py:
 // 104-137).  w points at the cell of coordinate x; w[k*stride] is its k-th
@@ -81,19 +83,19 @@ __device__ __forceinline__ double sed_alpha_lds(const double* w, int stride, int
 // only; their space then holds the smooth-extrema A values of the tile (+1
 // along the axis) for rho and P.
 constexpr int DWO = 4 * ONY * ONX;
-constexpr int DAX = 2 * TY * (TX + 2), DAY = 2 * (TY + 2) * TX;
+constexpr int DAX = 2 * DTY * (DTX + 2), DAY = 2 * (DTY + 2) * DTX;
 static_assert(DAX + DAY <= DWO, "smooth-extrema table must fit the old-state tile");
 
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(DNT)
 k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ Unew,
            uint8_t* __restrict__ flags, int* __restrict__ any_flag) {
   SPD_SKIP(g);
   __shared__ double wn[2][DNY][DNX];  // new rho, P
   __shared__ double wbuf[DWO];        // old primitives, then the A table
   double(*wo)[ONY][ONX] = reinterpret_cast<double(*)[ONY][ONX]>(wbuf);
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+  const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
   const int Nx = g.Ns[0], Ny = g.Ns[1];
-  for (int k = threadIdx.x; k < DNX * DNY; k += NT) {
+  for (int k = threadIdx.x; k < DNX * DNY; k += DNT) {
     const int lx = k % DNX, ly = k / DNX;
     const int x = x0 + lx - DH, y = y0 + ly - DH;
     double r = qnan(), p = qnan();
@@ -107,7 +109,7 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
     wn[0][ly][lx] = r;
     wn[1][ly][lx] = p;
   }
-  for (int k = threadIdx.x; k < ONX * ONY; k += NT) {
+  for (int k = threadIdx.x; k < ONX * ONY; k += DNT) {
     const int lx = k % ONX, ly = k / ONX;
     const int x = x0 + lx - 1, y = y0 + ly - 1;
     double W[4] = {qnan(), qnan(), qnan(), qnan()};
@@ -119,7 +121,7 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
     for (int v = 0; v < 4; ++v) wo[v][ly][lx] = W[v];
   }
   __syncthreads();
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int tx = threadIdx.x % DTX, ty = threadIdx.x / DTX;
   const int x = x0 + tx, y = y0 + ty;
   const bool inside = x < Nx && y < Ny;
   bool cand = false;
@@ -168,23 +170,23 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
     // built only for tiles that hold one
     const int need = __syncthreads_or((pt[0] || pt[3]) ? 1 : 0);
     if (need) {
-      double* Ax = wbuf;        // [2][TY][TX+2], x in [x0-1, x0+TX]
-      double* Ay = wbuf + DAX;  // [2][TY+2][TX], y in [y0-1, y0+TY]
+      double* Ax = wbuf;        // [2][DTY][DTX+2], x in [x0-1, x0+DTX]
+      double* Ay = wbuf + DAX;  // [2][DTY+2][DTX], y in [y0-1, y0+DTY]
       const double* rdcx = g.rdc[0] + NGH;
       const double* rdcy = g.rdc[1] + NGH;
       const double* hfx = g.hfp[0] + NGH;
       const double* hfy = g.hfp[1] + NGH;
-      for (int k = threadIdx.x; k < DAX + DAY; k += NT) {
+      for (int k = threadIdx.x; k < DAX + DAY; k += DNT) {
         double aL;
         if (k < DAX) {
-          const int var = k / (TY * (TX + 2)), r = k % (TY * (TX + 2));
-          const int ly = r / (TX + 2), lx = r % (TX + 2);
+          const int var = k / (DTY * (DTX + 2)), r = k % (DTY * (DTX + 2));
+          const int ly = r / (DTX + 2), lx = r % (DTX + 2);
           const double* w = &wn[var][ly + DH][lx - 1 + DH];
           Ax[k] = sed_A(w, 1, x0 + lx - 1, Nx, rdcx, hfx, &aL);
         } else {
           const int kk = k - DAX;
-          const int var = kk / ((TY + 2) * TX), r = kk % ((TY + 2) * TX);
-          const int ly = r / TX, lx = r % TX;
+          const int var = kk / ((DTY + 2) * DTX), r = kk % ((DTY + 2) * DTX);
+          const int ly = r / DTX, lx = r % DTX;
           const double* w = &wn[var][ly - 1 + DH][lx + DH];
           Ay[kk] = sed_A(w, DNX, y0 + ly - 1, Ny, rdcy, hfy, &aL);
         }
@@ -196,8 +198,8 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
           const int var = (k == 0) ? 0 : 3;
           if (!pt[var]) continue;
           const double* w = &wn[k][ty + DH][tx + DH];
-          const double* ax = Ax + (k * TY + ty) * (TX + 2) + tx + 1;   // A at x
-          const double* ay = Ay + (k * (TY + 2) + ty + 1) * TX + tx;   // A at y
+          const double* ax = Ax + (k * DTY + ty) * (DTX + 2) + tx + 1;   // A at x
+          const double* ay = Ay + (k * (DTY + 2) + ty + 1) * DTX + tx;   // A at y
           double alpha = 1.0;
           double a1;
           if (Nx == 1 || x == Nx - 1)
@@ -211,9 +213,9 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
           if (Ny == 1 || y == Ny - 1)
             a2 = sed_alpha_lds(w, DNX, y, Ny, rdcy, hfy);
           else if (y == 0)
-            a2 = nmin(ay[0], ay[TX]);
+            a2 = nmin(ay[0], ay[DTX]);
           else
-            a2 = nmin(ay[-TX], nmin(ay[0], ay[TX]));
+            a2 = nmin(ay[-DTX], nmin(ay[0], ay[DTX]));
           alpha = (a2 < alpha) ? a2 : alpha;
           pt[var] = pt[var] && (alpha < 1.0);
         }
@@ -239,21 +241,24 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
 // halo, then the x and y 0.25 rings.  Tiles cover the ghosted grid; (gx, gy)
 // are ghosted indices.  Without blending theta = trouble.
 // ---------------------------------------------------------------------------
+// 64 x 16 outputs per block of 256 threads (four per thread): the kernel is
+// latency-bound, so fewer, larger tiles.
+constexpr int HTX = 64, HTY = 16, HNT = 256;
 constexpr int TH = 2;
-constexpr int TNX = TX + 2 * TH, TNY = TY + 2 * TH;
+constexpr int TNX = HTX + 2 * TH, TNY = HTY + 2 * TH;
 
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(HNT)
 k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* __restrict__ Ucv,
           const double* __restrict__ Unew, const int* __restrict__ any_flag,
           double* __restrict__ theta) {
   SPD_SKIP(g);
   __shared__ double tr[TNY][TNX];
-  __shared__ double t1[TY + 2][TX + 2];
-  const int gx0 = blockIdx.x * TX, gy0 = blockIdx.y * TY;
+  __shared__ double t1[HTY + 2][HTX + 2];
+  const int gx0 = blockIdx.x * HTX, gy0 = blockIdx.y * HTY;
   const int NGx = g.Ng[0], NGy = g.Ng[1];
   const bool gate = (*any_flag != 0) && prm.p > 1 && prm.sed;
   const double* Ulast = gate ? Unew : Ucv;
-  for (int k = threadIdx.x; k < TNX * TNY; k += NT) {
+  for (int k = threadIdx.x; k < TNX * TNY; k += HNT) {
     const int lx = k % TNX, ly = k / TNX;
     const int gx = gx0 + lx - TH, gy = gy0 + ly - TH;
     double v = 0.0;
@@ -265,13 +270,15 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
   }
   __syncthreads();
   if (!prm.blending) {
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int gx = gx0 + tx, gy = gy0 + ty;
-    if (gx < NGx && gy < NGy) theta[(long long)gy * NGx + gx] = tr[ty + TH][tx + TH];
+    for (int k = threadIdx.x; k < HTX * HTY; k += HNT) {
+      const int tx = k % HTX, ty = k / HTX;
+      const int gx = gx0 + tx, gy = gy0 + ty;
+      if (gx < NGx && gy < NGy) theta[(long long)gy * NGx + gx] = tr[ty + TH][tx + TH];
+    }
     return;
   }
-  for (int k = threadIdx.x; k < (TX + 2) * (TY + 2); k += NT) {
-    const int lx = k % (TX + 2), ly = k / (TX + 2);
+  for (int k = threadIdx.x; k < (HTX + 2) * (HTY + 2); k += HNT) {
+    const int lx = k % (HTX + 2), ly = k / (HTX + 2);
     const int gx = gx0 + lx - 1, gy = gy0 + ly - 1;
     double th = 0.0;
     if (gx >= 0 && gx < NGx && gy >= 0 && gy < NGy) {
@@ -290,18 +297,20 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
     t1[ly][lx] = th;
   }
   __syncthreads();
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  const int gx = gx0 + tx, gy = gy0 + ty;
-  if (gx >= NGx || gy >= NGy) return;
-  const int lx = tx + 1, ly = ty + 1;
-  auto rx = [&](int dy) -> double {
-    const double lo = (gx >= 1) ? t1[ly + dy][lx - 1] : 0.0;
-    const double hi = (gx <= NGx - 2) ? t1[ly + dy][lx + 1] : 0.0;
-    return ring1(gx, NGx, lo, t1[ly + dy][lx], hi);
-  };
-  const double lo = (gy >= 1) ? rx(-1) : 0.0;
-  const double hi = (gy <= NGy - 2) ? rx(+1) : 0.0;
-  theta[(long long)gy * NGx + gx] = ring1(gy, NGy, lo, rx(0), hi);
+  for (int k = threadIdx.x; k < HTX * HTY; k += HNT) {
+    const int tx = k % HTX, ty = k / HTX;
+    const int gx = gx0 + tx, gy = gy0 + ty;
+    if (gx >= NGx || gy >= NGy) continue;
+    const int lx = tx + 1, ly = ty + 1;
+    auto rx = [&](int dy) -> double {
+      const double lo = (gx >= 1) ? t1[ly + dy][lx - 1] : 0.0;
+      const double hi = (gx <= NGx - 2) ? t1[ly + dy][lx + 1] : 0.0;
+      return ring1(gx, NGx, lo, t1[ly + dy][lx], hi);
+    };
+    const double lo = (gy >= 1) ? rx(-1) : 0.0;
+    const double hi = (gy <= NGy - 2) ? rx(+1) : 0.0;
+    theta[(long long)gy * NGx + gx] = ring1(gy, NGy, lo, rx(0), hi);
+  }
 }
 
 // ---------------------------------------------------------------------------

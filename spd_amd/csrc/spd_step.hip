@@ -1936,13 +1936,16 @@ template <int NDIM, int P, typename B> struct Stepper {
     apply(c, dt, w, c->Unew, s);
     if (c->need_gate) hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
     const dim3 tiles((c->g.Ns[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ns[1] + fv2d::TY - 1) / fv2d::TY);
+    const dim3 dtiles((c->g.Ns[0] + fv2d::DTX - 1) / fv2d::DTX,
+                      (c->g.Ns[1] + fv2d::DTY - 1) / fv2d::DTY);
     c->timed(KC_DETECT, s, [&] {
-      hipLaunchKernelGGL(fv2d::k_detect2d, tiles, dim3(fv2d::NT), 0, s, c->g, c->prm, c->Ucv,
+      hipLaunchKernelGGL(fv2d::k_detect2d, dtiles, dim3(fv2d::DNT), 0, s, c->g, c->prm, c->Ucv,
                          c->Unew, c->flags, c->need_gate ? c->any_flag : nullptr);
     });
-    const dim3 gtiles((c->g.Ng[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ng[1] + fv2d::TY - 1) / fv2d::TY);
+    const dim3 gtiles((c->g.Ng[0] + fv2d::HTX - 1) / fv2d::HTX,
+                      (c->g.Ng[1] + fv2d::HTY - 1) / fv2d::HTY);
     c->timed(KC_THETA1, s, [&] {
-      hipLaunchKernelGGL(fv2d::k_theta2d, gtiles, dim3(fv2d::NT), 0, s, c->g, c->prm, c->flags,
+      hipLaunchKernelGGL(fv2d::k_theta2d, gtiles, dim3(fv2d::HNT), 0, s, c->g, c->prm, c->flags,
                          c->Ucv, c->Unew, c->any_flag, c->theta);
     });
     tap(c, i, s);
