@@ -67,8 +67,9 @@ def algorithmic_bytes(N, p, launches):
       k_apply    T launches: read U_cv + both face arrays, write the candidate
       k_detect   T launches: read U_cv and the candidate (2 V C), write C flags
       k_theta1   T launches (k_theta2d): read C flags, write G
-      k_fbface   T launches (k_fbcorrect2d): read U_cv, theta, both face arrays,
-                 write V C
+      k_fbface   T launches (k_fbmarch2d): read U_cv, theta, both face arrays,
+                 write V C (at most: cells with no blended face keep the
+                 candidate)
       k_cv2sp    twice (cv->sp transform; wave-speed maximum of the cells):
                  read V C and write V C, then read V C
     """
@@ -105,7 +106,7 @@ def algorithmic_bytes(N, p, launches):
 # rocprofv3 names of the kernels of each class in the FP64 2-D HLLC workload,
 # with their launches per step
 CLASS_KERNELS = {
-    "k_fbface": [("k_fbcorrect2d<1, double>", 4)],
+    "k_fbface": [("k_fbmarch2d<1, double>", 4)],
     "k_face_sd": [("k_face_sd_x<2, 3, 1, double, double>", 1),
                   ("k_face_sd<2, 3, 1, 1, double, double>", 1)],
     "k_pred": [("k_pred2d<3, 1, double, double, true>", 1),

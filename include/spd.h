@@ -150,10 +150,18 @@ int spd_run_status(spd_ctx* ctx, double* time, long long* n_step, int* done, dou
 /* Launch shapes that the kernels take at run time (an extension with no
  * reference counterpart; the autotuner of spd_amd.ensemble sets them):
  * elements per block of the predictor update and of the 2-D high-order
- * face-flux kernel.  value 0 restores the default; values above the maximum
- * (256 threads / 48 KiB of LDS per block) are rejected.  spd_get_launch reports
- * the value in effect and the maximum. */
-enum spd_launch_key { SPD_LAUNCH_PRED_EPB = 0, SPD_LAUNCH_HO_EPB = 1, SPD_LAUNCH_COUNT = 2 };
+ * face-flux kernel, and the form of the 2-D fallback correction (1: cell
+ * tiles staged in LDS, 2: column strips marched in registers, the default;
+ * the two agree to round-off, not bit for bit, so the autotuner leaves the
+ * form alone).  value 0 restores the default; values above the maximum
+ * (256 threads / 48 KiB of LDS per block; 2 forms) are rejected.
+ * spd_get_launch reports the value in effect and the maximum. */
+enum spd_launch_key {
+  SPD_LAUNCH_PRED_EPB = 0,
+  SPD_LAUNCH_HO_EPB = 1,
+  SPD_LAUNCH_FB_FORM = 2,
+  SPD_LAUNCH_COUNT = 3
+};
 int spd_set_launch(spd_ctx* ctx, int key, int value);
 int spd_get_launch(const spd_ctx* ctx, int key, int* value, int* max_value);
 

@@ -31,21 +31,24 @@ constexpr int DH = 3;
 constexpr int DNX = DTX + 2 * DH, DNY = DTY + 2 * DH;
 constexpr int ONX = DTX + 2, ONY = DTY + 2;
 
-// Smooth-extrema quantities along one axis from LDS (src/trouble_detection.py:
+// Smooth-extrema test along one axis from LDS (src/trouble_detection.py:
 // 104-137).  w points at the cell of coordinate x; w[k*stride] is its k-th
 // neighbour along the axis.  dU is zero outside [-1, Ns].
-__device__ __forceinline__ double sed_dU(const double* w, int stride, int x, int Ns,
-                                         const double* rdc) {
+__device__ __forceinline__ double sed_dU(const double* w, int stride, int x, int Ns, const double* rdc) {
   return (x >= -1 && x <= Ns) ? (w[stride] - w[-stride]) * rdc[x] : 0.0;
 }
 
-// A = min(alpha_R, alpha_L) of the cell at x (0 outside [0, Ns)); *aL = alpha_L.
-__device__ __forceinline__ double sed_A(const double* w, int stride, int x, int Ns,
-                                        const double* rdc, const double* hf, double* aLout) {
-  if (x < 0 || x >= Ns) {
-    *aLout = 1.0;
-    return 0.0;
-  }
+// The reference forms alpha_L = -c_L / dv and alpha_R = c_R / dv (c the part
+// of v_L, v_R of the sign that limits; 1 where dv == 0; clipped to 1) and
+// keeps a candidate flag where the neighbourhood minimum of
+// A = min(alpha_R, alpha_L) is below 1.  Every clipped quotient is |c| / |dv|
+// in [0, 1], and a correctly rounded |c| / |dv| is below 1 exactly when
+// |c| < |dv| (NaN and dv == 0 give 1 there and false here), so the test needs
+// no division.  Returns bit 0: A < 1 (true outside [0, Ns), where the
+// reference's A is 0), bit 1: alpha_L < 1.
+__device__ __forceinline__ int sed_bits(const double* w, int stride, int x, int Ns, const double* rdc,
+                                        const double* hf) {
+  if (x < 0 || x >= Ns) return 1;
   const double dUm = sed_dU(w - stride, stride, x - 1, Ns, rdc);
   const double dU0 = sed_dU(w, stride, x, Ns, rdc);
   const double dUp = sed_dU(w + stride, stride, x + 1, Ns, rdc);
@@ -53,45 +56,41 @@ __device__ __forceinline__ double sed_A(const double* w, int stride, int x, int 
   const double dv = 0.5 * hf[x] * d2U;
   const double vL = dUm - dU0;
   const double vR = dUp - dU0;
-  double aL = -((dv < 0.0) ? ((vL > 0.0) ? vL : 0.0) : ((vL < 0.0) ? vL : 0.0)) / dv;
-  aL = (fabs(dv) <= 0.0) ? 1.0 : aL;
-  aL = (aL < 1.0) ? aL : 1.0;
-  double aR = ((dv > 0.0) ? ((vR > 0.0) ? vR : 0.0) : ((vR < 0.0) ? vR : 0.0)) / dv;
-  aR = (fabs(dv) <= 0.0) ? 1.0 : aR;
-  aR = (aR < 1.0) ? aR : 1.0;
-  *aLout = aL;
-  return (aR < aL) ? aR : aL;
+  const double cL = (dv < 0.0) ? ((vL > 0.0) ? vL : 0.0) : ((vL < 0.0) ? vL : 0.0);
+  const double cR = (dv > 0.0) ? ((vR > 0.0) ? vR : 0.0) : ((vR < 0.0) ? vR : 0.0);
+  const double adv = fabs(dv);
+  const bool l = fabs(cL) < adv, r = fabs(cR) < adv;
+  return ((l || r) ? 1 : 0) | (l ? 2 : 0);
 }
 
-// Smooth-extrema coefficient of the cell at ca with the compute_min edge rules
+// alpha < 1 along one axis for the cell at ca, with the compute_min edge rules
 // (first cell: min(A[0], A[1]); last cell: min(A[-2], alpha_L[-1])).
-__device__ __forceinline__ double sed_alpha_lds(const double* w, int stride, int ca, int Ns,
-                                                const double* rdc, const double* hf) {
-  double A[3], aLc = 1.0, aL;
-#pragma unroll
-  for (int k = -1; k <= 1; ++k) {
-    A[k + 1] = sed_A(w + k * stride, stride, ca + k, Ns, rdc, hf, &aL);
-    if (k == 0) aLc = aL;
-  }
-  if (Ns == 1) return aLc;
-  if (ca == 0) return nmin(A[1], A[2]);
-  if (ca == Ns - 1) return nmin(A[0], aLc);
-  return nmin(A[0], nmin(A[1], A[2]));
+__device__ __forceinline__ bool sed_relax_lds(const double* w, int stride, int ca, int Ns,
+                                              const double* rdc, const double* hf) {
+  const int am = sed_bits(w - stride, stride, ca - 1, Ns, rdc, hf);
+  const int a0 = sed_bits(w, stride, ca, Ns, rdc, hf);
+  const int ap = sed_bits(w + stride, stride, ca + 1, Ns, rdc, hf);
+  if (Ns == 1) return (a0 & 2) != 0;
+  if (ca == 0) return ((a0 | ap) & 1) != 0;
+  if (ca == Ns - 1) return ((am & 1) | ((a0 >> 1) & 1)) != 0;
+  return ((am | a0 | ap) & 1) != 0;
 }
 
 // LDS of k_detect2d: the old primitives (tile + 1) are needed by the NAD phase
-// only; their space then holds the smooth-extrema A values of the tile (+1
-// along the axis) for rho and P.
+// only; their space then holds the smooth-extrema bits of the tile (+1 along
+// the axis) for rho and P.
 constexpr int DWO = 4 * ONY * ONX;
 constexpr int DAX = 2 * DTY * (DTX + 2), DAY = 2 * (DTY + 2) * DTX;
-static_assert(DAX + DAY <= DWO, "smooth-extrema table must fit the old-state tile");
+static_assert(DAX + DAY <= DWO * 8, "smooth-extrema table must fit the old-state tile");
 
+// FP64 in both precision forms: the smooth-extrema test takes second
+// differences of the new state, ~1e-8 of it at 4096 subcells per side.
 __global__ void __launch_bounds__(DNT)
 k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ Unew,
            uint8_t* __restrict__ flags, int* __restrict__ any_flag) {
   SPD_SKIP(g);
   __shared__ double wn[2][DNY][DNX];  // new rho, P
-  __shared__ double wbuf[DWO];        // old primitives, then the A table
+  __shared__ double wbuf[DWO];        // old primitives, then the smooth-extrema bits
   double(*wo)[ONY][ONX] = reinterpret_cast<double(*)[ONY][ONX]>(wbuf);
   const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
   const int Nx = g.Ns[0], Ny = g.Ns[1];
@@ -140,6 +139,7 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
       Wn[1] = W[1];
       Wn[2] = W[2];
     }
+    const double tol = prm.tol;
     for (int k = 0; k < nvar_nad; ++k) {
       const int v = (k == 0) ? 0 : (k == 1 ? 3 : k - 1);
       double mx = wo[v][oy][ox], mn = mx;
@@ -153,12 +153,12 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
       double lo = mn, hi = mx;
       if (prm.p > 0) {
         if (prm.nad_delta) {
-          const double eps = prm.tol * (hi - lo);
+          const double eps = tol * (hi - lo);
           lo = lo - eps;
           hi = hi + eps;
         } else {
-          lo = lo - fabs(lo) * prm.tol;
-          hi = hi + fabs(hi) * prm.tol;
+          lo = lo - fabs(lo) * tol;
+          hi = hi + fabs(hi) * tol;
         }
       }
       pt[v] = !(Wn[v] >= lo) || !(Wn[v] <= hi);
@@ -166,29 +166,28 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
     }
   }
   if (prm.p > 1 && prm.sed) {
-    // smooth-extrema relaxation of the rho / P candidates; the A table is
-    // built only for tiles that hold one
+    // smooth-extrema relaxation of the rho / P candidates; the table is built
+    // only for tiles that hold one
     const int need = __syncthreads_or((pt[0] || pt[3]) ? 1 : 0);
     if (need) {
-      double* Ax = wbuf;        // [2][DTY][DTX+2], x in [x0-1, x0+DTX]
-      double* Ay = wbuf + DAX;  // [2][DTY+2][DTX], y in [y0-1, y0+DTY]
+      uint8_t* Ax = reinterpret_cast<uint8_t*>(wbuf);  // [2][DTY][DTX+2], x in [x0-1, x0+DTX]
+      uint8_t* Ay = Ax + DAX;                          // [2][DTY+2][DTX], y in [y0-1, y0+DTY]
       const double* rdcx = g.rdc[0] + NGH;
       const double* rdcy = g.rdc[1] + NGH;
       const double* hfx = g.hfp[0] + NGH;
       const double* hfy = g.hfp[1] + NGH;
       for (int k = threadIdx.x; k < DAX + DAY; k += DNT) {
-        double aL;
         if (k < DAX) {
           const int var = k / (DTY * (DTX + 2)), r = k % (DTY * (DTX + 2));
           const int ly = r / (DTX + 2), lx = r % (DTX + 2);
           const double* w = &wn[var][ly + DH][lx - 1 + DH];
-          Ax[k] = sed_A(w, 1, x0 + lx - 1, Nx, rdcx, hfx, &aL);
+          Ax[k] = (uint8_t)sed_bits(w, 1, x0 + lx - 1, Nx, rdcx, hfx);
         } else {
           const int kk = k - DAX;
           const int var = kk / ((DTY + 2) * DTX), r = kk % ((DTY + 2) * DTX);
           const int ly = r / DTX, lx = r % DTX;
           const double* w = &wn[var][ly - 1 + DH][lx + DH];
-          Ay[kk] = sed_A(w, DNX, y0 + ly - 1, Ny, rdcy, hfy, &aL);
+          Ay[kk] = (uint8_t)sed_bits(w, DNX, y0 + ly - 1, Ny, rdcy, hfy);
         }
       }
       __syncthreads();
@@ -198,26 +197,23 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
           const int var = (k == 0) ? 0 : 3;
           if (!pt[var]) continue;
           const double* w = &wn[k][ty + DH][tx + DH];
-          const double* ax = Ax + (k * DTY + ty) * (DTX + 2) + tx + 1;   // A at x
-          const double* ay = Ay + (k * (DTY + 2) + ty + 1) * DTX + tx;   // A at y
-          double alpha = 1.0;
-          double a1;
+          const uint8_t* ax = Ax + (k * DTY + ty) * (DTX + 2) + tx + 1;  // bits at x
+          const uint8_t* ay = Ay + (k * (DTY + 2) + ty + 1) * DTX + tx;  // bits at y
+          bool r1, r2;
           if (Nx == 1 || x == Nx - 1)
-            a1 = sed_alpha_lds(w, 1, x, Nx, rdcx, hfx);
+            r1 = sed_relax_lds(w, 1, x, Nx, rdcx, hfx);
           else if (x == 0)
-            a1 = nmin(ax[0], ax[1]);
+            r1 = ((ax[0] | ax[1]) & 1) != 0;
           else
-            a1 = nmin(ax[-1], nmin(ax[0], ax[1]));
-          alpha = (a1 < alpha) ? a1 : alpha;
-          double a2;
+            r1 = ((ax[-1] | ax[0] | ax[1]) & 1) != 0;
           if (Ny == 1 || y == Ny - 1)
-            a2 = sed_alpha_lds(w, DNX, y, Ny, rdcy, hfy);
+            r2 = sed_relax_lds(w, DNX, y, Ny, rdcy, hfy);
           else if (y == 0)
-            a2 = nmin(ay[0], ay[DTX]);
+            r2 = ((ay[0] | ay[DTX]) & 1) != 0;
           else
-            a2 = nmin(ay[-DTX], nmin(ay[0], ay[DTX]));
-          alpha = (a2 < alpha) ? a2 : alpha;
-          pt[var] = pt[var] && (alpha < 1.0);
+            r2 = ((ay[-DTX] | ay[0] | ay[DTX]) & 1) != 0;
+          // alpha = min(1, alpha_x, alpha_y) < 1
+          pt[var] = pt[var] && (r1 || r2);
         }
       }
     }
@@ -345,7 +341,6 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
   __shared__ B sW[SW];
   __shared__ B sH[12][CY][CX];  // Mp(4), Sx(4), Sy(4)
   __shared__ double sT[CY][CX];
-  __shared__ unsigned char sNeed[CY][CX];
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
   const int Nx = g.Ns[0], Ny = g.Ns[1];
   const int NGx = g.Ng[0];
@@ -395,7 +390,6 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
       if (ly <= TY && y + 1 <= Ny) need |= godunov || !(nmax(sT[ly][lx], sT[ly + 1][lx]) == 0.0);
       if (ly >= 1 && y <= Ny) need |= godunov || !(nmax(sT[ly - 1][lx], sT[ly][lx]) == 0.0);
     }
-    sNeed[ly][lx] = need ? 1 : 0;
     if (!need) continue;
     const int wx = lx + 1, wy = ly + 1;  // position in W
     B Wc[4], dMh[2][4];
@@ -509,6 +503,236 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
              (double)bFx[(ty * (TX + 1) + tx) * 4 + v]) * rdx;
     d = d + ((double)bFy[((ty + 1) * TX + tx) * 4 + v] - (double)bFy[(ty * TX + tx) * 4 + v]) * rdy;
     Uout[v * g.Nc + ci] = u - d * dti;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same correction by column strips (the default 2-D form): a wave owns
+// MOUT columns of cells (its 64 lanes cover two more on each side) and
+// marches up a chunk of MR rows (march_rows), keeping in registers a three-row window of
+// old primitives, the Hancock state of the row below and the fluxes the next
+// row's update needs; x-neighbours come from the adjacent lanes.  Every
+// cell's primitives and Hancock state and every face's fallback flux are
+// evaluated once per strip: no tile halo, no LDS, no barriers.  The loads of
+// the next row are issued one row ahead.  The arithmetic of each cell and face
+// is that of k_fbcorrect2d; only the cells that border a face with a non-zero
+// coefficient are written (the others keep the candidate, the same
+// expression).
+// ---------------------------------------------------------------------------
+constexpr int MW = 64, MOUT = MW - 4;  // lanes of a wave, output columns of a strip
+constexpr int MWAVES = 4;              // waves (chunks of one strip) per block
+
+// Rows per chunk: 32 (about 8% of the rows evaluated twice, at chunk edges),
+// fewer on small grids so that the launch still has some 4 waves per CU.
+inline int march_rows(int Nx, int Ny) {
+  const long long strips = (Nx + MOUT - 1) / MOUT;
+  int r = 32;
+  while (r > 4 && strips * ((Ny + r - 1) / r) < 4096) r /= 2;
+  return r;
+}
+
+template <typename B>
+__device__ __forceinline__ B from_right(B v) {  // value of lane + 1 (lane 63: its own)
+  return __shfl_down(v, 1, MW);
+}
+template <typename B>
+__device__ __forceinline__ B from_left(B v) {  // value of lane - 1 (lane 0: its own)
+  return __shfl_up(v, 1, MW);
+}
+
+template <int RS, typename B>
+__global__ void __launch_bounds__(MW * MWAVES)
+k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
+            const double* __restrict__ theta, const B* __restrict__ Fx,
+            const B* __restrict__ Fy, const double* __restrict__ dt_dev, double w_i,
+            int MR, double* __restrict__ Uout) {
+  SPD_SKIP(g);
+  const int lane = threadIdx.x % MW;
+  const int Nx = g.Ns[0], Ny = g.Ns[1];
+  const int NGx = g.Ng[0];
+  const int x = blockIdx.x * MOUT - 2 + lane;
+  const int y0 = (blockIdx.y * MWAVES + threadIdx.x / MW) * MR;
+  if (y0 >= Ny) return;  // a whole wave
+  const int y1 = (y0 + MR < Ny) ? y0 + MR : Ny;
+  const B gamma = (B)prm.gamma, mc2 = (B)prm.min_c2;
+  const double dti = (*dt_dev) * w_i;
+  const B dtib = (B)dti;
+  const bool godunov = prm.godunov != 0;
+  const bool out = lane >= 2 && lane < MW - 2 && x < Nx;  // lanes that update a cell
+  const bool xghost = x >= -NGH && x < Nx + NGH;
+  const bool xth = x >= -1 && x <= Nx;                     // theta read (else 0)
+  // x geometry of the lane's cell (halo lanes outside [-1, Nx] clamp: unused)
+  const int Gx = ((x < -1) ? -1 : (x > Nx ? Nx : x)) + NGH;
+  const B rhlx = (B)g.rhcv[0][Gx - 1], rhrx = (B)g.rhcv[0][Gx], hfx = (B)g.hfp[0][Gx];
+  const B hclx = (B)g.hcv[0][Gx - 1], hcrx = (B)g.hcv[0][Gx];
+  const double rdx = g.rhfp[0][Gx];
+  const long long nfx = (long long)Ny * (Nx + 1), nfy = (long long)(Ny + 1) * Nx;
+  const bool fxin = x >= -1 && x < Nx;  // the lane's x-face (x + 1/2) exists
+  const bool fyin = x >= 0 && x < Nx;   // the lane's y-faces exist
+
+  // conservative state of the ghosted cell (x, y), fetched ahead of its use
+  double un[4];
+  bool okn = false, flxn = false, flyn = false;
+  auto fetch = [&](int y) {
+    okn = xghost && y >= -NGH && y < Ny + NGH;
+    flxn = flyn = false;
+    if (okn) {
+      const int mx = ghost_map(x, Nx, g.bc[0], &flxn);
+      const int my = ghost_map(y, Ny, g.bc[1], &flyn);
+      const long long idx = (long long)my * Nx + mx;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) un[v] = Ucv[v * g.Nc + idx];
+    }
+  };
+  auto to_W = [&](B* W) {
+    double Wv[4] = {qnan(), qnan(), qnan(), qnan()};
+    if (okn) {
+      prims<2>(un, Wv, 0, prm.gamma);
+      if (flxn) Wv[1] = -Wv[1];
+      if (flyn) Wv[2] = -Wv[2];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) W[v] = (B)Wv[v];
+  };
+  auto theta_at = [&](int y) -> double {
+    return (xth && y >= -1 && y <= Ny) ? theta[(long long)(y + NGH) * NGx + (x + NGH)] : 0.0;
+  };
+
+  B Wm[4], W0[4], Wp[4];
+  fetch(y0 - 2);
+  to_W(Wm);
+  fetch(y0 - 1);
+  to_W(W0);
+  fetch(y0);
+  double th = theta_at(y0 - 1), thn = theta_at(y0);
+  double thm = 0.0;                  // theta of row k - 1
+  double tfxR = 0.0, tfyB = 0.0;     // coefficients of FxR and FyB
+  B Qm[4] = {B(0), B(0), B(0), B(0)};  // Hancock state + half y-slope of row k - 1
+  B FxR[4] = {B(0), B(0), B(0), B(0)};  // blended flux at (k - 1, x + 1/2)
+  B FyB[4] = {B(0), B(0), B(0), B(0)};  // blended flux at y-face k - 1
+  for (int k = y0 - 1; k <= y1; ++k) {
+    to_W(Wp);  // row k + 1
+    fetch(k + 2);
+    const double thk = th;
+    th = thn;
+    thn = theta_at(k + 2);
+    // high-order fluxes this row needs: y-face k, x-face (k, x + 1/2)
+    B Fyk[4], Fxk[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Fyk[v] = (k >= y0 && fyin) ? Fy[v * nfy + (long long)k * Nx + x] : B(0);
+      Fxk[v] = (k >= y0 && k < y1 && fxin) ? Fx[v * nfx + (long long)k * (Nx + 1) + x + 1] : B(0);
+    }
+
+    // Hancock state and half slopes of row k (as k_fbcorrect2d step 2)
+    const int Gy = k + NGH;  // k in [-1, Ny]
+    const B rhly = (B)g.rhcv[1][Gy - 1], rhry = (B)g.rhcv[1][Gy], hfy = (B)g.hfp[1][Gy];
+    B Mp[4], Sx[4], Sy[4], dMx[4], dMy[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const B Wl = from_left(W0[v]), Wr = from_right(W0[v]);
+      const B Lx = (W0[v] - Wl) * rhlx, Rx = (Wr - W0[v]) * rhrx;
+      const B Ly = (W0[v] - Wm[v]) * rhly, Ry = (Wp[v] - W0[v]) * rhry;
+      if (prm.moncen) {
+        dMx[v] = moncen<B>(Lx, Rx, hclx, hcrx, hfx);
+        dMy[v] = moncen<B>(Ly, Ry, (B)g.hcv[1][Gy - 1], (B)g.hcv[1][Gy], hfy);
+      } else {
+        dMx[v] = minmod<B>(Lx, Rx);
+        dMy[v] = minmod<B>(Ly, Ry);
+      }
+      Sx[v] = B(0.5) * dMx[v] * hfx;
+      Sy[v] = B(0.5) * dMy[v] * hfy;
+    }
+    {
+      B dtW[4] = {B(0), B(0), B(0), B(0)};
+      const B rrho = qrcp<B>(W0[0]);
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int vel = b + 1;
+        const B* dW = (b == 0) ? dMx : dMy;
+        dtW[0] = dtW[0] - (W0[vel] * dW[0] + W0[0] * dW[vel]);
+        dtW[3] = dtW[3] - (W0[vel] * dW[3] + gamma * W0[3] * dW[vel]);
+        dtW[vel] = dtW[vel] - (W0[vel] * dW[vel] + dW[3] * rrho);
+        dtW[2] = dtW[2] - W0[vel] * dW[2];
+      }
+      dtW[3] = dtW[3] * B(0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Mp[v] = W0[v] + B(0.5) * dtW[v] * dtib;
+    }
+    // lane + 1's lower x-face state, its theta
+    B WRx[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) WRx[v] = from_right(Mp[v] - Sx[v]);
+    const double thr = from_right(thk);
+
+    if (k >= y0) {
+      // y-face k, between rows k - 1 and k
+      double tf = godunov ? 1.0 : nmax(thm, thk);
+      if (fyin && !(tf == 0.0)) {
+        B WL[4], WR[4], UL[4], UR[4], Ff[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          WL[v] = Qm[v];
+          WR[v] = Mp[v] - Sy[v];
+        }
+        cons<2>(WL, UL, 1, gamma);
+        cons<2>(WR, UR, 1, gamma);
+        riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, mc2, prm.lowmach_fv);
+        const B tb = (B)tf;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Fyk[v] = tb * Ff[v] + (B(1) - tb) * Fyk[v];
+      }
+      // update of row k - 1
+      B FxL[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) FxL[v] = from_left(FxR[v]);
+      const double tfxL = from_left(tfxR);
+      if (k - 1 >= y0 && out) {
+        const bool touched =
+            godunov || !(tfxL == 0.0) || !(tfxR == 0.0) || !(tfyB == 0.0) || !(tf == 0.0);
+        if (touched) {
+          const int y = k - 1;
+          const double rdy = g.rhfp[1][y + NGH];
+          const long long ci = (long long)y * Nx + x;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const double u = Ucv[v * g.Nc + ci];
+            double d = u * 0.0;
+            d = d + ((double)FxR[v] - (double)FxL[v]) * rdx;
+            d = d + ((double)Fyk[v] - (double)FyB[v]) * rdy;
+            Uout[v * g.Nc + ci] = u - d * dti;
+          }
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) FyB[v] = Fyk[v];
+      tfyB = tf;
+    }
+    if (k >= y0 && k < y1) {
+      // x-face (k, x + 1/2)
+      double tf = godunov ? 1.0 : nmax(thk, thr);
+      if (fxin && !(tf == 0.0)) {
+        B WL[4], UL[4], UR[4], Ff[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) WL[v] = Mp[v] + Sx[v];
+        cons<2>(WL, UL, 0, gamma);
+        cons<2>(WRx, UR, 0, gamma);
+        riemann<2, RS>(WL, WRx, UL, UR, Ff, 0, gamma, mc2, prm.lowmach_fv);
+        const B tb = (B)tf;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Fxk[v] = tb * Ff[v] + (B(1) - tb) * Fxk[v];
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) FxR[v] = Fxk[v];
+      tfxR = tf;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Qm[v] = Mp[v] + Sy[v];
+      Wm[v] = W0[v];
+      W0[v] = Wp[v];
+    }
+    thm = thk;
   }
 }
 

@@ -117,6 +117,16 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
     riemann<2, RS>(WL, WR, UL, UR, F, a, gam, mc2, prm.lowmach_sd);
   };
 
+  // Both lines of the thread are loaded up front where the extra registers
+  // leave the occupancy as it is (FP64 at p = 1 and 3, whose LDS or registers
+  // already hold it at 4 and 3 waves per SIMD), so that the y line's latency
+  // hides behind the x phase; elsewhere each line is loaded at its phase.
+  constexpr bool PREFETCH = sizeof(B) == 8 && (P == 1 || P == 3);
+  B ux[V][n], uy[V][n];
+  if (active) {
+    load_line(gx, gy, t, 0, lp, ux);
+    if (PREFETCH) load_line(gx, gy, t, 1, lp, uy);
+  }
 #pragma unroll
   for (int a = 0; a < 2; ++a) {
     const int ca = (a == 0) ? gx : gy;        // element coordinate along a
@@ -125,9 +135,10 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
     const int TA = (a == 0) ? TX : TY;
     const int Na = (a == 0) ? Nx : Ny;
     const int nbr = (a == 0) ? 1 : TX;        // element offset of the upper neighbour in the tile
-    B u[V][n], qL[V], qR[V];
+    B(&u)[V][n] = (a == 0) ? ux : uy;
+    B qL[V], qR[V];
     if (active) {
-      load_line(gx, gy, t, a, lp, u);
+      if (!PREFETCH && a == 1) load_line(gx, gy, t, 1, lp, uy);
       trace(u, 0, qL);
       trace(u, nf - 1, qR);
       B* d = sTR + ((el * T + t) * n + lp) * V;

@@ -1716,7 +1716,7 @@ struct spd_ctx {
   double* partial = nullptr;  // per-block partial maxima
   long long npartial = 0;
   bool need_gate = false;     // some side is not periodic (global NAD gate needed)
-  int launch[SPD_LAUNCH_COUNT] = {0, 0};  // tuned launch shapes (0 = default)
+  int launch[SPD_LAUNCH_COUNT] = {0, 0, 0};  // tuned launch shapes (0 = default)
   // debug tap
   int dbg_sub = -1;
   double* dbg_tr = nullptr;
@@ -1913,10 +1913,16 @@ template <int NDIM, int P, typename B> struct Stepper {
     return c->launch[SPD_LAUNCH_HO_EPB] > 0 ? std::min(mx, c->launch[SPD_LAUNCH_HO_EPB])
                                             : std::min(mx, std::max(1, 24576 / (V * NL * 8)));
   }
-  static int launch_value(const spd_ctx* c, int key) {
-    return key == SPD_LAUNCH_PRED_EPB ? epb(c) : ho_epb(c);
+  // Form of the 2-D fallback correction: 1 cell tiles, 2 column strips (default).
+  static int fb_form(const spd_ctx* c) {
+    return c->launch[SPD_LAUNCH_FB_FORM] > 0 ? c->launch[SPD_LAUNCH_FB_FORM] : 2;
   }
-  static int launch_max(int key) { return key == SPD_LAUNCH_PRED_EPB ? epb_max() : ho_epb_max(); }
+  static int launch_value(const spd_ctx* c, int key) {
+    return key == SPD_LAUNCH_PRED_EPB ? epb(c) : key == SPD_LAUNCH_HO_EPB ? ho_epb(c) : fb_form(c);
+  }
+  static int launch_max(int key) {
+    return key == SPD_LAUNCH_PRED_EPB ? epb_max() : key == SPD_LAUNCH_HO_EPB ? ho_epb_max() : 2;
+  }
 
   // One FV substep with the 2-D tiled kernels; the corrected state goes to the
   // other state buffer and the buffers are swapped.
@@ -1949,14 +1955,29 @@ template <int NDIM, int P, typename B> struct Stepper {
                          c->Ucv, c->Unew, c->any_flag, c->theta);
     });
     tap(c, i, s);
-    // the corrected state replaces the candidate in place, tile by tile
+    // the corrected state replaces the candidate in place
     c->timed(KC_FBFLUX, s, [&] {
-      if (c->cfg.riemann_fv != SPD_RS_LLF)
-        hipLaunchKernelGGL((fv2d::k_fbcorrect2d<RS_HLLC, B>), tiles, dim3(fv2d::NT), 0, s, c->g,
-                           c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, c->Unew);
-      else
-        hipLaunchKernelGGL((fv2d::k_fbcorrect2d<RS_LLF, B>), tiles, dim3(fv2d::NT), 0, s, c->g,
-                           c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, c->Unew);
+      const bool hllc = c->cfg.riemann_fv != SPD_RS_LLF;
+      if (fb_form(c) == 1) {
+        if (hllc)
+          hipLaunchKernelGGL((fv2d::k_fbcorrect2d<RS_HLLC, B>), tiles, dim3(fv2d::NT), 0, s,
+                             c->g, c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, c->Unew);
+        else
+          hipLaunchKernelGGL((fv2d::k_fbcorrect2d<RS_LLF, B>), tiles, dim3(fv2d::NT), 0, s,
+                             c->g, c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, c->Unew);
+      } else {
+        const int rows = fv2d::march_rows(c->g.Ns[0], c->g.Ns[1]);
+        const dim3 strips((c->g.Ns[0] + fv2d::MOUT - 1) / fv2d::MOUT,
+                          (c->g.Ns[1] + rows * fv2d::MWAVES - 1) / (rows * fv2d::MWAVES));
+        if (hllc)
+          hipLaunchKernelGGL((fv2d::k_fbmarch2d<RS_HLLC, B>), strips, dim3(fv2d::MW * fv2d::MWAVES),
+                             0, s, c->g, c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, rows,
+                             c->Unew);
+        else
+          hipLaunchKernelGGL((fv2d::k_fbmarch2d<RS_LLF, B>), strips, dim3(fv2d::MW * fv2d::MWAVES),
+                             0, s, c->g, c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, rows,
+                             c->Unew);
+      }
     });
     std::swap(c->Ucv, c->Unew);
   }

@@ -55,7 +55,10 @@ def test_autotuner_shapes_do_not_change_results():
     kw = members(1, 5, (12,), (3,))[0]
     ref = SDADER_Simulator(device="cuda:0", **kw)
     ref.perform_iterations(2)
+    from spd_amd.ensemble import TUNED
     for name, key in _lib.LAUNCH.items():
+        if name not in TUNED:
+            continue
         s0 = SDADER_Simulator(device="cuda:0", **kw)
         cur, mx = ctypes.c_int(), ctypes.c_int()
         _lib.check(s0._lib.spd_get_launch(s0._ctx, key, ctypes.byref(cur), ctypes.byref(mx)))

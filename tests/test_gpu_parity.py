@@ -76,6 +76,31 @@ def test_steps_vs_golden_and_oracle(name):
                     assert relerr(rg[f"aff_{d}"], ro[f"aff_{d}"]) < TOL, (k, i, d)
 
 
+def _fb2d(name):
+    kw = load(name)["cfg"]["kw"]
+    return len(kw["N"]) == 2 and kw.get("update", "SD") == "FV" and kw.get("FB", False)
+
+
+@pytest.mark.parametrize("name", [n for n in NAMES if _fb2d(n)])
+def test_fallback_tile_form_vs_oracle(name):
+    """The 2-D fallback correction has two kernel forms (cell tiles in LDS,
+    column strips in registers; the strips are the default): the tile form
+    meets the same bar against the oracle."""
+    from spd_amd import _lib
+    print(f"[gpu parity tiles] {name}", flush=True)
+    fx = load(name)
+    s = gpu_sim(fx)
+    _lib.check(s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], 1), "spd_set_launch")
+    assert s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], 3) < 0
+    o = oracle_sim(fx)
+    for k in range(1, int(fx["nsteps"]) + 1):
+        s.compute_dt()
+        o.compute_dt()
+        o.perform_update()
+        s.perform_update()
+        assert relerr_fields(s.dm.U_sp, o.U_sp) < TOL, ("oracle", k, relerr_fields(s.dm.U_sp, o.U_sp))
+
+
 @pytest.mark.parametrize("name", [n for n in NAMES if "final_U_sp" in load(n)])
 def test_time_evolution_vs_golden(name):
     print(f"[gpu evolution] {name}", flush=True)
