@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define SPD_ABI_VERSION 3
+#define SPD_ABI_VERSION 4
 
 enum spd_status {
   SPD_OK = 0,
@@ -204,6 +204,32 @@ int spd_prims_to_cons(spd_ctx* ctx, const double* W, double* U, long long count,
 #define SPD_TAP_ALL (-2)
 int spd_set_debug_tap(spd_ctx* ctx, int substep, double* troubles, double* theta,
                       double* aff_x, double* aff_y, double* aff_z);
+
+/* Reverse-mode derivative (adjoint) of one step, FP64 form only.  No entry
+ * of the reference corresponds: the reference has no derivative.  These
+ * differentiate perform_update (src/sdader_simulator.py:361-378: the predictor
+ * src/sdader_simulator.py:160-200,213-240, the FV corrector :288-349 with
+ * src/fv_simulator.py:167-205, src/muscl.py:129-221 and the blend :325-331)
+ * with the trouble flags, blending coefficients and dt frozen.
+ *
+ * spd_adj_step runs one step from U_in into U_out (U_in is not modified) and
+ * records its tape in the context: the m Picard iterates, the FV state before
+ * each substep and each substep's blending coefficients.  theta_in != NULL
+ * replays coefficients recorded earlier (spd_adj_theta_count doubles: m+1
+ * substeps of the ghosted grid (Ns+4)^ndim) instead of detecting them;
+ * theta_out != NULL receives the coefficients used.  Both need update FV with
+ * the fallback.
+ *
+ * spd_adj_backward takes the cotangent of the step's output in U_bar and
+ * replaces it with the cotangent of its input, U_bar^T d(step)/d(U_in), by the
+ * reverse sweep over the tape of the last spd_adj_step, which must have been
+ * run from the same U_in and dt.  The tape (m (m+1) V E L + (m+1)(V Nc + Ngc)
+ * doubles) and the adjoint scratch are allocated on first use. */
+int spd_adj_step(spd_ctx* ctx, const double* U_in, double* U_out, const double* dt_dev,
+                 const double* theta_in, double* theta_out, void* stream);
+int spd_adj_backward(spd_ctx* ctx, const double* U_in, const double* dt_dev, double* U_bar,
+                     void* stream);
+size_t spd_adj_theta_count(const spd_ctx* ctx);
 
 /* Optional HIP-event timing of the step's kernels (no counterpart in the
  * reference, which only times whole runs with a wall clock,

@@ -30,6 +30,7 @@ EXPORTS = [
     "spd_profile_begin", "spd_profile_end", "spd_kernel_class_count", "spd_kernel_class_name",
     "spd_run_begin", "spd_run_steps", "spd_run_status", "spd_set_launch", "spd_get_launch",
     "spd_ic_points", "spd_ic_mean", "spd_sp_to_regular",
+    "spd_adj_step", "spd_adj_backward", "spd_adj_theta_count",
 ]
 
 
@@ -134,6 +135,12 @@ def load():
     L.spd_set_launch.restype = i
     L.spd_get_launch.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i)]
     L.spd_get_launch.restype = i
+    L.spd_adj_step.argtypes = [vp, dp, dp, dp, dp, dp, vp]
+    L.spd_adj_step.restype = i
+    L.spd_adj_backward.argtypes = [vp, dp, dp, dp, vp]
+    L.spd_adj_backward.restype = i
+    L.spd_adj_theta_count.argtypes = [vp]
+    L.spd_adj_theta_count.restype = ctypes.c_size_t
     _lib = L
     return L
 

@@ -1,0 +1,163 @@
+"""Reverse-mode derivative (adjoint) of SD-ADER steps behind ``torch.autograd``.
+
+The step is differentiated with its trouble flags, blending coefficients and
+dt frozen: they are taken from the forward pass and held fixed (the flags and
+the CFL dt are piecewise-constant / non-smooth functions of the state and have
+no useful derivative).  With them fixed, one step of
+``SDADER_Simulator.perform_update`` (``src/sdader_simulator.py:361-378``) is a
+smooth map of ``U_sp`` almost everywhere, and the library's hand-written
+reverse sweep (``spd_adj_step`` / ``spd_adj_backward``, ``include/spd.h``)
+returns its vector-Jacobian product on the device.
+
+``Adjoint.run`` advances ``nsteps`` steps as a ``torch.autograd.Function``:
+the forward pass keeps the state every ``checkpoint_every`` steps and the dt
+of every step; the backward pass re-runs each interval from its checkpoint
+(the step is deterministic, so the recomputed flags are the forward pass's),
+then walks the interval backwards, recording one step's tape (Picard
+iterates, FV substep states, blending coefficients) and running its reverse
+sweep.  Memory: ``nsteps / k + k`` states plus one step's tape.
+
+Only the FP64 form is differentiated.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+class Adjoint:
+    """Adjoint of the steps of one ``SDADER_Simulator`` (its context, options
+    and mesh).  The simulator's own state is not touched."""
+
+    def __init__(self, sim):
+        if sim.precision != "fp64":
+            raise NotImplementedError("the adjoint is built for precision='fp64' only")
+        self.sim = sim
+        self.lib = _lib.load()
+        self.device = sim.device
+        self.fb = sim.update == "FV" and bool(sim.FB)
+        self.n_theta = int(self.lib.spd_adj_theta_count(sim._ctx))
+
+    # -- one step ------------------------------------------------------------------
+    @property
+    def _stream(self):
+        return _lib.stream_handle(self.device)
+
+    def _check(self, U):
+        shape = tuple(self.sim._shape_sd())
+        if tuple(U.shape) != shape or U.dtype != torch.float64 or not U.is_cuda:
+            raise ValueError(f"expected a float64 device tensor of shape {shape}, "
+                             f"got {tuple(U.shape)} {U.dtype} {U.device}")
+        return U.contiguous()
+
+    def dt_of(self, U_sp, out=None):
+        """CFL dt of a state (``src/sd_simulator.py:214-227``): from the
+        subcell-mean primitives of ``U_sp``.  Returns a 1-element device tensor."""
+        U_sp = self._check(U_sp)
+        W = self.sim._pointwise("spd_cons_to_prims", self.sim._tensor_op("spd_sp_to_cv", U_sp))
+        out = torch.empty(1, dtype=torch.float64, device=self.device) if out is None else out
+        _lib.check(self.lib.spd_compute_dt(self.sim._ctx, _lib.ptr(W.contiguous()), _lib.ptr(out),
+                                           self._stream), "spd_compute_dt")
+        return out
+
+    def step(self, U_sp, dt, theta=None, record_theta=False):
+        """One step out of place, recording the tape.  ``dt``: 1-element device
+        tensor.  ``theta`` (optional): blending coefficients of every substep
+        (as returned with ``record_theta``) to use instead of the detection.
+        Returns ``U_out`` or ``(U_out, theta)``."""
+        U_sp = self._check(U_sp)
+        out = torch.empty_like(U_sp)
+        th_out = None
+        if record_theta:
+            if not self.fb:
+                raise ValueError("blending coefficients exist only with update='FV', FB=True")
+            th_out = torch.empty(self.n_theta, dtype=torch.float64, device=self.device)
+        th_in = None if theta is None else theta.contiguous()
+        if th_in is not None and th_in.numel() != self.n_theta:
+            raise ValueError(f"theta must hold {self.n_theta} values")
+        _lib.check(self.lib.spd_adj_step(self.sim._ctx, _lib.ptr(U_sp), _lib.ptr(out),
+                                         _lib.ptr(dt), _lib.ptr(th_in), _lib.ptr(th_out),
+                                         self._stream), "spd_adj_step")
+        return (out, th_out) if record_theta else out
+
+    def step_vjp(self, U_sp, dt, U_bar, theta=None):
+        """Vector-Jacobian product of one step at ``U_sp`` (flags frozen as
+        detected at ``U_sp``, or ``theta``): returns ``U_bar^T d step / d U_sp``."""
+        U_sp = self._check(U_sp)
+        self.step(U_sp, dt, theta=theta)
+        Ub = self._check(U_bar).clone()
+        _lib.check(self.lib.spd_adj_backward(self.sim._ctx, _lib.ptr(U_sp), _lib.ptr(dt),
+                                             _lib.ptr(Ub), self._stream), "spd_adj_backward")
+        return Ub
+
+    # -- many steps ----------------------------------------------------------------
+    def _advance(self, U, dt):
+        _lib.check(self.lib.spd_step(self.sim._ctx, _lib.ptr(U), None, None, _lib.ptr(dt), 0,
+                                     self._stream), "spd_step")
+
+    def _forward(self, U0, nsteps, k, dt):
+        """Run nsteps from U0; returns (U_N, checkpoints, dts)."""
+        U = self._check(U0).clone()
+        dts = torch.empty(max(nsteps, 1), dtype=torch.float64, device=self.device)
+        ckpt = {}
+        for s in range(nsteps):
+            if s % k == 0:
+                ckpt[s] = U.clone()
+            d = dts[s:s + 1]
+            if dt is not None:
+                d.fill_(float(dt))
+            elif s == 0 or self.sim.update == "SD":
+                # the SD update never refreshes W_cv, so its dt stays the first
+                # one (src/sdader_simulator.py:376)
+                if s == 0:
+                    self.dt_of(U, out=d)
+                else:
+                    d.copy_(dts[0:1])
+            else:
+                _lib.check(self.lib.spd_compute_dt(self.sim._ctx, None, _lib.ptr(d),
+                                                   self._stream), "spd_compute_dt")
+            self._advance(U, d)
+        return U, ckpt, dts
+
+    def _backward(self, U_bar, nsteps, k, ckpt, dts):
+        Ub = self._check(U_bar).clone()
+        starts = sorted(ckpt)
+        for j0 in reversed(starts):
+            j1 = min(j0 + k, nsteps)
+            states = [ckpt[j0]]
+            U = ckpt[j0].clone()
+            for s in range(j0, j1 - 1):
+                self._advance(U, dts[s:s + 1])
+                states.append(U.clone())
+            for s in range(j1 - 1, j0 - 1, -1):
+                Ub = self.step_vjp(states[s - j0], dts[s:s + 1], Ub)
+        return Ub
+
+    def run(self, U0, nsteps, checkpoint_every=20, dt=None):
+        """``nsteps`` steps from ``U0`` (element layout, FP64, on the device),
+        differentiable with respect to ``U0`` through ``torch.autograd``.
+        ``dt``: a fixed step, or None for the CFL dt of each state (frozen in
+        the derivative).  Returns the final state."""
+        if nsteps < 0 or checkpoint_every < 1:
+            raise ValueError("nsteps >= 0 and checkpoint_every >= 1 required")
+        return _Run.apply(U0, self, int(nsteps), int(checkpoint_every), dt)
+
+
+class _Run(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, U0, adj, nsteps, k, dt):
+        with torch.no_grad():
+            U, ckpt, dts = adj._forward(U0.detach(), nsteps, k, dt)
+        ctx.adj, ctx.nsteps, ctx.k = adj, nsteps, k
+        ctx.ckpt, ctx.dts = ckpt, dts
+        return U
+
+    @staticmethod
+    def backward(ctx, U_bar):
+        with torch.no_grad():
+            g = ctx.adj._backward(U_bar.contiguous(), ctx.nsteps, ctx.k, ctx.ckpt, ctx.dts)
+        ctx.ckpt = None
+        return g, None, None, None, None

@@ -1,0 +1,1126 @@
+// Reverse-mode derivative (adjoint) of the SD-ADER step, FP64.
+//
+// The step is differentiated with its trouble flags, blending coefficients
+// and dt frozen (taken from a recorded forward pass): with those fixed the
+// step is a composition of linear operators (Lagrange interpolation and
+// differentiation, the ADER time matrix, sp <-> cv transforms, transverse
+// subcell averages, the FV update and the blend) and pointwise or stencil
+// maps (Euler flux, prims / cons, the element-face and fallback Riemann
+// solvers, the MUSCL-Hancock state).  The linear parts are transposed by
+// hand; the Jacobian-transpose products of the pointwise maps are formed
+// from forward-mode dual numbers run through the same device functions the
+// step uses (spd_device.hpp), one pass per input, so every branch (HLLC
+// state selection, limiter case, sound-speed clamp) is the one the forward
+// pass took.
+//
+// Forward (src/sdader_simulator.py:160-200, 288-349, src/fv_simulator.py:
+// 167-205), with U_ad^(k) the k-th Picard iterate (U_ad^(0) = U_sp at every
+// node), D(.) the SD spatial operator (element-face Riemann solves + flux-point
+// derivative, src/sdader_simulator.py:148-158,213-240):
+//   U_ad^(k)   = U_sp - dt * invader . D(U_ad^(k-1))               k = 1..m
+//   FV:  U^0   = sp_to_cv(U_sp);   for substep i (dt_i = w_i dt):
+//        U^{i+1} = U^i - dt_i sum_a Delta_a[th F_FB,a(U^i) + (1-th) F_HO,a(U_ad^(m), i)] / dx
+//        U_sp' = cv_to_sp(U^T)
+//   SD:  U_sp' = U_sp - dt sum_t w_t D(U_ad^(m))[t]
+// The reverse sweep runs these equations backwards; the tape holds the
+// iterates U_ad^(1..m), the FV states U^i and the coefficients th of every
+// substep (recorded by spd_step, spd_step.hip).
+#include "spd_ctx.hpp"
+
+namespace spd {
+namespace ad {
+
+// Dual number: value and one directional derivative.
+struct Dual {
+  double v, d;
+  Dual() = default;
+  __host__ __device__ constexpr Dual(double x) : v(x), d(0.0) {}
+  __host__ __device__ constexpr Dual(double x, double dx) : v(x), d(dx) {}
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__device__ __forceinline__ Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) {
+  return Dual(a.v * b.v, a.d * b.v + a.v * b.d);
+}
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+__device__ __forceinline__ bool operator<(Dual a, Dual b) { return a.v < b.v; }
+__device__ __forceinline__ bool operator>(Dual a, Dual b) { return a.v > b.v; }
+__device__ __forceinline__ bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+__device__ __forceinline__ bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+__device__ __forceinline__ bool operator==(Dual a, Dual b) { return a.v == b.v; }
+__device__ __forceinline__ bool operator!=(Dual a, Dual b) { return a.v != b.v; }
+__device__ __forceinline__ Dual fabs(Dual a) { return (a.v < 0.0) ? -a : a; }
+__device__ __forceinline__ Dual sqrt(Dual a) {
+  const double s = ::sqrt(a.v);
+  return Dual(s, a.d * 0.5 / s);
+}
+
+// xbar[k] = sum_j ybar[j] * d f_j / d x_k at x: one forward-mode pass per input.
+template <int NI, int NO, typename F>
+__device__ __forceinline__ void vjp(const double* x, const double* ybar, double* xbar, F&& f) {
+#pragma unroll 1
+  for (int k = 0; k < NI; ++k) {
+    Dual xd[NI], yd[NO];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) xd[i] = Dual(x[i], i == k ? 1.0 : 0.0);
+    f(xd, yd);
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < NO; ++j) acc += ybar[j] * yd[j].d;
+    xbar[k] = acc;
+  }
+}
+
+}  // namespace ad
+
+using ad::Dual;
+
+// ---------------------------------------------------------------------------
+// Transpose of the tensor-product transform with the n x n matrix M (which: 0
+// cv_to_sp, 1 sp_to_cv) on every element: out[qo] = sum_qi prod_b M[qi_b][qo_b]
+// in[qi]; in_fv / out_fv select the FV layout (src/transforms.py:27-45).
+// ---------------------------------------------------------------------------
+template <int NDIM, int P>
+__global__ void __launch_bounds__(256)
+k_adj_tensor(Ops<P> op, Geom g, int which, const double* __restrict__ in, int in_fv,
+             double* __restrict__ out, int out_fv) {
+  constexpr int V = Dims<NDIM>::V, n = P + 1;
+  constexpr int NL = IPow<n, NDIM>::v;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= g.NE * V * NL) return;
+  const int lo = (int)(tid % NL);
+  const long long r = tid / NL;
+  const long long e = r % g.NE;
+  const int v = (int)(r / g.NE);
+  const double(*M)[n] = which ? op.s2c : op.c2s;
+  int ce[3] = {0, 0, 0};
+  elem_coords<NDIM>(g, e, ce);
+  auto index = [&](int fv, int l) -> long long {
+    if (!fv) return ((long long)v * g.NE + e) * NL + l;
+    int cc[3] = {0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      cc[b] = ce[b] * n + l % n;
+      l /= n;
+    }
+    return (long long)v * g.Nc + cell_index<NDIM>(g, cc);
+  };
+  int qo[3] = {0, 0, 0};
+  {
+    int x = lo;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      qo[b] = x % n;
+      x /= n;
+    }
+  }
+  double acc = 0.0;
+  for (int li = 0; li < NL; ++li) {
+    int x = li;
+    double w = 1.0;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      w *= M[x % n][qo[b]];
+      x /= n;
+    }
+    acc += w * in[index(in_fv, li)];
+  }
+  out[index(out_fv, lo)] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Hancock states and half slopes of the cells the fallback needs (the cells
+// next to a face with a non-zero coefficient), as in the step; layout
+// [Mp (V), S_0 (V), ..., S_{d-1} (V)] per cell of the (Ns+2)^d grid.
+// ---------------------------------------------------------------------------
+template <int NDIM>
+__device__ __forceinline__ double face_theta(const Geom& g, const Params& prm,
+                                             const double* __restrict__ theta, int a,
+                                             const int* f) {
+  if (prm.godunov) return 1.0;
+  long long gl = 0, gr = 0;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b) {
+    gl = gl * g.Ng[b] + f[b] + NGH - (b == a ? 1 : 0);
+    gr = gr * g.Ng[b] + f[b] + NGH;
+  }
+  return nmax(theta[gl], theta[gr]);
+}
+
+// Cell c of the one-ghost grid borders a face of dimension a with a non-zero
+// coefficient (or every face with godunov).
+template <int NDIM>
+__device__ __forceinline__ bool hs_needed(const Geom& g, const Params& prm,
+                                          const double* __restrict__ theta, const int* c) {
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    bool trans_ok = true;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b)
+      if (b != a && (c[b] < 0 || c[b] >= g.Ns[b])) trans_ok = false;
+    if (!trans_ok) continue;
+    int f[3] = {c[0], c[1], c[2]};
+    if (c[a] >= 0 && c[a] <= g.Ns[a] && !(face_theta<NDIM>(g, prm, theta, a, f) == 0.0))
+      return true;
+    f[a] = c[a] + 1;
+    if (f[a] <= g.Ns[a] && !(face_theta<NDIM>(g, prm, theta, a, f) == 0.0)) return true;
+  }
+  return false;
+}
+
+template <int NDIM>
+__device__ __forceinline__ void load_stencil(const double* __restrict__ U, const Geom& g,
+                                             double gamma, const int* c, double* x) {
+  constexpr int V = Dims<NDIM>::V;
+  load_W<NDIM>(U, g, c, x, gamma);
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    int cm[3] = {c[0], c[1], c[2]}, cp[3] = {c[0], c[1], c[2]};
+    cm[b] -= 1;
+    cp[b] += 1;
+    load_W<NDIM>(U, g, cm, x + (1 + b) * V, gamma);
+    load_W<NDIM>(U, g, cp, x + (1 + NDIM + b) * V, gamma);
+  }
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_adj_hs(Geom g, Params prm, const double* __restrict__ U, const double* __restrict__ theta,
+         const double* __restrict__ dt_dev, double w_i, double* __restrict__ HS) {
+  constexpr int V = Dims<NDIM>::V;
+  long long n1 = 1;
+  int e1[3] = {1, 1, 1};
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    e1[a] = g.Ns[a] + 2;
+    n1 *= e1[a];
+  }
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n1) return;
+  int c[3] = {0, 0, 0};
+  {
+    long long x = tid;
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+      c[a] = (int)(x % e1[a]) - 1;
+      x /= e1[a];
+    }
+  }
+  if (!hs_needed<NDIM>(g, prm, theta, c)) return;
+  double x[(1 + 2 * NDIM) * V];
+  load_stencil<NDIM>(U, g, prm.gamma, c, x);
+  double Mp[V], S[NDIM][V];
+  hancock_state<NDIM, double>(g, prm, c, x, (const double(*)[V])(x + V),
+                              (const double(*)[V])(x + (1 + NDIM) * V), (*dt_dev) * w_i, Mp, S);
+#pragma unroll
+  for (int v = 0; v < V; ++v) HS[(long long)v * n1 + tid] = Mp[v];
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b)
+#pragma unroll
+    for (int v = 0; v < V; ++v) HS[(long long)((1 + b) * V + v) * n1 + tid] = S[b][v];
+}
+
+// ---------------------------------------------------------------------------
+// Substep faces of dimension A: the flux cotangent from the substep's output
+// cotangent Ub1 (transpose of U^{i+1} = U^i - dt_i sum_a Delta_a F / dx), split
+// into the high-order part (1 - th) and, on faces with th != 0, the
+// cotangents of the fallback face states (WL, WR) = (Mp_L + S_L, Mp_R - S_R).
+// ---------------------------------------------------------------------------
+template <int NDIM, int RS, int A>
+__global__ void __launch_bounds__(256)
+k_adj_fvface(Geom g, Params prm, int fb, const double* __restrict__ Ub1,
+             const double* __restrict__ theta, const double* __restrict__ HS,
+             const double* __restrict__ dt_dev, double w_i, double* __restrict__ Fhob,
+             double* __restrict__ WbL, double* __restrict__ WbR) {
+  constexpr int a = A;
+  constexpr int V = Dims<NDIM>::V;
+  long long ext[3] = {1, 1, 1}, nface = 1;
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    ext[b] = g.Ns[b] + (b == a ? 1 : 0);
+    nface *= ext[b];
+  }
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nface) return;
+  int c[3] = {0, 0, 0};
+  {
+    long long x = tid;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      c[b] = (int)(x % ext[b]);
+      x /= ext[b];
+    }
+  }
+  const double dti = (*dt_dev) * w_i;
+  double Fb[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) Fb[v] = 0.0;
+  if (c[a] < g.Ns[a]) {
+    const long long ci = cell_index<NDIM>(g, c);
+    const double r = dti * g.rhfp[a][c[a] + NGH];
+#pragma unroll
+    for (int v = 0; v < V; ++v) Fb[v] += Ub1[v * g.Nc + ci] * r;
+  }
+  if (c[a] > 0) {
+    int cl[3] = {c[0], c[1], c[2]};
+    cl[a] -= 1;
+    const long long ci = cell_index<NDIM>(g, cl);
+    const double r = dti * g.rhfp[a][c[a] - 1 + NGH];
+#pragma unroll
+    for (int v = 0; v < V; ++v) Fb[v] -= Ub1[v * g.Nc + ci] * r;
+  }
+  const double th = fb ? face_theta<NDIM>(g, prm, theta, a, c) : 0.0;
+#pragma unroll
+  for (int v = 0; v < V; ++v) Fhob[v * nface + tid] = (th == 0.0) ? Fb[v] : (1.0 - th) * Fb[v];
+  if (th == 0.0) return;
+  long long hl = 0, hr = 0, n1 = 1;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b) {
+    hl = hl * (g.Ns[b] + 2) + c[b] + 1 - (b == a ? 1 : 0);
+    hr = hr * (g.Ns[b] + 2) + c[b] + 1;
+  }
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) n1 *= g.Ns[b] + 2;
+  double x[2 * V], yb[V], xb[2 * V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    x[v] = HS[(long long)v * n1 + hl] + HS[(long long)((1 + a) * V + v) * n1 + hl];
+    x[V + v] = HS[(long long)v * n1 + hr] - HS[(long long)((1 + a) * V + v) * n1 + hr];
+    yb[v] = th * Fb[v];
+  }
+  const Dual gam(prm.gamma), mc2(prm.min_c2);
+  const int lm = prm.lowmach_fv;
+  ad::vjp<2 * V, V>(x, yb, xb, [&](const Dual* xd, Dual* yd) {
+    Dual UL[V], UR[V];
+    cons<NDIM, Dual>(xd, UL, a, gam);
+    cons<NDIM, Dual>(xd + V, UR, a, gam);
+    riemann<NDIM, RS, Dual>(xd, xd + V, UL, UR, yd, a, gam, mc2, lm);
+  });
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    WbL[v * nface + tid] = xb[v];
+    WbR[v * nface + tid] = xb[V + v];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fallback states -> FV state: per cell of the one-ghost grid, the cotangent
+// of its Hancock state (gathered from its faces), through the MUSCL-Hancock
+// map of its (1 + 2d)-cell stencil, the ghost maps (periodic wrap,
+// non-mirrored copies, reflective sign flips) and prims, accumulated into Ub0.
+// ---------------------------------------------------------------------------
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_adj_hancock(Geom g, Params prm, const double* __restrict__ U, const double* __restrict__ theta,
+              const double* __restrict__ dt_dev, double w_i, const double* __restrict__ WbL0,
+              const double* __restrict__ WbL1, const double* __restrict__ WbL2,
+              const double* __restrict__ WbR0, const double* __restrict__ WbR1,
+              const double* __restrict__ WbR2, double* __restrict__ Ub0) {
+  constexpr int V = Dims<NDIM>::V;
+  constexpr int NI = (1 + 2 * NDIM) * V, NO = (1 + NDIM) * V;
+  long long n1 = 1;
+  int e1[3] = {1, 1, 1};
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    e1[a] = g.Ns[a] + 2;
+    n1 *= e1[a];
+  }
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n1) return;
+  int c[3] = {0, 0, 0};
+  {
+    long long x = tid;
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+      c[a] = (int)(x % e1[a]) - 1;
+      x /= e1[a];
+    }
+  }
+  const double* WbL[3] = {WbL0, WbL1, WbL2};
+  const double* WbR[3] = {WbR0, WbR1, WbR2};
+  double yb[NO];
+#pragma unroll
+  for (int k = 0; k < NO; ++k) yb[k] = 0.0;
+  bool any = false;
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) {
+    bool trans_ok = true;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b)
+      if (b != a && (c[b] < 0 || c[b] >= g.Ns[b])) trans_ok = false;
+    if (!trans_ok) continue;
+    long long nface = 1, ext[3] = {1, 1, 1};
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      ext[b] = g.Ns[b] + (b == a ? 1 : 0);
+      nface *= ext[b];
+    }
+    auto face_idx = [&](const int* f) {
+      long long idx = 0;
+#pragma unroll
+      for (int b = NDIM - 1; b >= 0; --b) idx = idx * ext[b] + f[b];
+      return idx;
+    };
+    int f[3] = {c[0], c[1], c[2]};
+    // the face on the left of c: c is its right state, Mp - S_a
+    if (c[a] >= 0 && c[a] <= g.Ns[a] && !(face_theta<NDIM>(g, prm, theta, a, f) == 0.0)) {
+      const long long fi = face_idx(f);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const double w = WbR[a][v * nface + fi];
+        yb[v] += w;
+        yb[(1 + a) * V + v] -= w;
+      }
+      any = true;
+    }
+    // the face on the right of c: c is its left state, Mp + S_a
+    f[a] = c[a] + 1;
+    if (f[a] <= g.Ns[a] && !(face_theta<NDIM>(g, prm, theta, a, f) == 0.0)) {
+      const long long fi = face_idx(f);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const double w = WbL[a][v * nface + fi];
+        yb[v] += w;
+        yb[(1 + a) * V + v] += w;
+      }
+      any = true;
+    }
+  }
+  if (!any) return;
+  double x[NI], xb[NI];
+  load_stencil<NDIM>(U, g, prm.gamma, c, x);
+  const Dual dti((*dt_dev) * w_i);
+  ad::vjp<NI, NO>(x, yb, xb, [&](const Dual* xd, Dual* yd) {
+    hancock_state<NDIM, Dual>(g, prm, c, xd, (const Dual(*)[V])(xd + V),
+                              (const Dual(*)[V])(xd + (1 + NDIM) * V), dti, yd,
+                              (Dual(*)[V])(yd + V));
+  });
+  // W of each stencil cell = flip(prims(U[map(cell)])): transpose into Ub0
+  const Dual gam(prm.gamma);
+#pragma unroll 1
+  for (int k = 0; k < 1 + 2 * NDIM; ++k) {
+    int cc[3] = {c[0], c[1], c[2]};
+    if (k >= 1 && k <= NDIM) cc[k - 1] -= 1;
+    if (k > NDIM) cc[k - 1 - NDIM] += 1;
+    int mm[3];
+    bool flip[3] = {false, false, false};
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) mm[a] = ghost_map(cc[a], g.Ns[a], g.bc[a], &flip[a]);
+    const long long idx = cell_index<NDIM>(g, mm);
+    double wb[V], u[V], ub[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      wb[v] = xb[k * V + v];
+      u[v] = U[v * g.Nc + idx];
+    }
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a)
+      if (flip[a]) wb[a + 1] = -wb[a + 1];
+    ad::vjp<V, V>(u, wb, ub, [&](const Dual* ud, Dual* wd) { prims<NDIM, Dual>(ud, wd, 0, gam); });
+#pragma unroll
+    for (int v = 0; v < V; ++v) atomicAdd(&Ub0[v * g.Nc + idx], ub[v]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// High-order FV fluxes of node i -> predictor: transpose of the transverse
+// subcell averages and of the flux-point evaluation (k_hoflux / k_ho2d).
+// Interior flux points: Euler-flux cotangent into the node's iterate
+// cotangent Uadb; end points: the element-face flux cotangent Ffb (every
+// element face of the node is written exactly once).
+// ---------------------------------------------------------------------------
+template <int NDIM, int P, int A>
+__global__ void __launch_bounds__(256)
+k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
+             const double* __restrict__ Fhob, double* __restrict__ Uadb, double* __restrict__ Ffb) {
+  constexpr int a = A;
+  constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
+  constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= g.NE * nf) return;
+  const int f = (int)(tid % nf);
+  const long long e = tid / nf;
+  int c[3] = {0, 0, 0};
+  elem_coords<NDIM>(g, e, c);
+  const int Na = g.N[a];
+  if (f == nf - 1 && c[a] != Na - 1) return;
+  long long ext[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) ext[b] = (b < NDIM) ? (g.Ns[b] + (b == a ? 1 : 0)) : 1;
+  long long nface = 1;
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) nface *= ext[b];
+  double Fa[NLP][V];
+#pragma unroll
+  for (int lp = 0; lp < NLP; ++lp) {
+    int q[3] = {0, 0, 0};
+    int r = lp;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      if (b == a) continue;
+      q[b] = r % n;
+      r /= n;
+    }
+    long long idx = 0;
+#pragma unroll
+    for (int b = NDIM - 1; b >= 0; --b) {
+      const long long cb = (b == a) ? (long long)c[a] * n + f : (long long)c[b] * n + q[b];
+      idx = idx * ext[b] + cb;
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) Fa[lp][v] = Fhob[v * nface + idx];
+  }
+  double Fl[NLP][V];
+  if constexpr (NDIM == 1) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) Fl[0][v] = Fa[0][v];
+  } else if constexpr (NDIM == 2) {
+#pragma unroll
+    for (int s = 0; s < n; ++s)
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < n; ++q) acc += op.s2c[q][s] * Fa[q][v];
+        Fl[s][v] = acc;
+      }
+  } else {
+    double G[NLP][V];
+#pragma unroll
+    for (int s2 = 0; s2 < n; ++s2)
+#pragma unroll
+      for (int q1 = 0; q1 < n; ++q1)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          double acc = 0.0;
+#pragma unroll
+          for (int q2 = 0; q2 < n; ++q2) acc += op.s2c[q2][s2] * Fa[q1 + n * q2][v];
+          G[q1 + n * s2][v] = acc;
+        }
+#pragma unroll
+    for (int c2 = 0; c2 < n; ++c2)
+#pragma unroll
+      for (int s = 0; s < n; ++s)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          double acc = 0.0;
+#pragma unroll
+          for (int q1 = 0; q1 < n; ++q1) acc += op.s2c[q1][s] * G[q1 + n * c2][v];
+          Fl[s + n * c2][v] = acc;
+        }
+  }
+  const int sa = stride_of<n>(a);
+  const Dual gam(prm.gamma);
+#pragma unroll 1
+  for (int lp = 0; lp < NLP; ++lp) {
+    if (f == 0 || f == nf - 1) {
+      const int fa = c[a] + (f == 0 ? 0 : 1);
+      int cl[3] = {0, 0, 0};
+      {
+        int r = lp;
+#pragma unroll
+        for (int b = 0; b < NDIM; ++b) {
+          if (b == a) continue;
+          cl[b] = r % n;
+          r /= n;
+        }
+      }
+      long long plane;
+      const long long i0 = face_grid_base<NDIM, n>(g, a, c, cl, fa, &plane);
+#pragma unroll
+      for (int v = 0; v < V; ++v) Ffb[(long long)(t * V + v) * plane + i0] = Fl[lp][v];
+    } else {
+      const int lb = line_base<NDIM, n>(a, lp);
+      double M[V], Mb[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const double* u = Uad + ((long long)(t * V + v) * g.NE + e) * NL + lb;
+        double acc = 0.0;
+#pragma unroll
+        for (int s = 0; s < n; ++s) acc += op.s2f[f][s] * u[s * sa];
+        M[v] = acc;
+      }
+      ad::vjp<V, V>(M, Fl[lp], Mb, [&](const Dual* xd, Dual* yd) {
+        Dual W[V];
+        prims<NDIM, Dual>(xd, W, a, gam);
+        flux<NDIM, Dual>(W, yd, a, gam);
+      });
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        double* ub = Uadb + ((long long)(t * V + v) * g.NE + e) * NL + lb;
+#pragma unroll
+        for (int s = 0; s < n; ++s) atomicAdd(&ub[s * sa], op.s2f[f][s] * Mb[v]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element faces of dimension A (k_face_sd): the flux cotangent Ffb through the
+// Riemann solver and prims to the two traces, then through the boundary
+// traces (periodic wrap, own trace, reflective flip) and the sp -> fp
+// interpolation into the source cotangent Sb (Tsrc time nodes).
+// ---------------------------------------------------------------------------
+template <int NDIM, int P, int RS, int A>
+__global__ void __launch_bounds__(256)
+k_adj_face(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
+           const double* __restrict__ Ffb, double* __restrict__ Sb) {
+  constexpr int a = A;
+  constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
+  constexpr int NL = IPow<n, NDIM>::v;
+  const int Na = g.N[a];
+  long long ext[3] = {1, 1, 1}, plane = 1;
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    ext[b] = (b == a) ? (long long)Na + 1 : (long long)g.N[b] * n;
+    plane *= ext[b];
+  }
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= plane * Tsrc) return;
+  const int t = (int)(tid / plane);
+  const long long pidx = tid % plane;
+  int ce[3] = {0, 0, 0}, cl[3] = {0, 0, 0};
+  int f = 0;
+  {
+    long long x = pidx;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      const int q = (int)(x % ext[b]);
+      x /= ext[b];
+      if (b == a) {
+        f = q;
+      } else {
+        ce[b] = q / n;
+        cl[b] = q % n;
+      }
+    }
+  }
+  int lb = 0;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b) lb = lb * n + ((b == a) ? 0 : cl[b]);
+  const int sa = stride_of<n>(a);
+  const long long tbase = (long long)t * V * g.NE * NL;
+  auto elem = [&](int ea) {
+    int cc[3] = {ce[0], ce[1], ce[2]};
+    cc[a] = ea;
+    return elem_index<NDIM>(g, cc);
+  };
+  auto trace = [&](int ea, int fp, double* q) {
+    const long long e = elem(ea);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const double* u = src + tbase + ((long long)v * g.NE + e) * NL + lb;
+      double acc = 0.0;
+#pragma unroll
+      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[s * sa];
+      q[v] = acc;
+    }
+  };
+  auto scatter = [&](int ea, int fp, const double* qb) {
+    const long long e = elem(ea);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double* ub = Sb + tbase + ((long long)v * g.NE + e) * NL + lb;
+#pragma unroll
+      for (int s = 0; s < n; ++s) atomicAdd(&ub[s * sa], op.s2f[fp][s] * qb[v]);
+    }
+  };
+  const int bl = g.bc[a][0], br = g.bc[a][1];
+  double x[2 * V];
+  double* qL = x;
+  double* qR = x + V;
+  if (f > 0) trace(f - 1, nf - 1, qL);
+  if (f < Na) trace(f, 0, qR);
+  if (f == 0) {
+    if (bl == BC_PERIODIC) {
+      trace(Na - 1, nf - 1, qL);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) qL[v] = qR[v];
+      if (bl == BC_REFLECTIVE) qL[a + 1] = -qL[a + 1];
+    }
+  }
+  if (f == Na) {
+    if (br == BC_PERIODIC) {
+      trace(0, 0, qR);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) qR[v] = qL[v];
+      if (br == BC_REFLECTIVE) qR[a + 1] = -qR[a + 1];
+    }
+  }
+  double yb[V], xb[2 * V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) yb[v] = Ffb[(long long)(t * V + v) * plane + pidx];
+  const Dual gam(prm.gamma), mc2(prm.min_c2);
+  const int lm = prm.lowmach_sd;
+  ad::vjp<2 * V, V>(x, yb, xb, [&](const Dual* xd, Dual* yd) {
+    Dual WL[V], WR[V];
+    prims<NDIM, Dual>(xd, WL, a, gam);
+    prims<NDIM, Dual>(xd + V, WR, a, gam);
+    riemann<NDIM, RS, Dual>(WL, WR, xd, xd + V, yd, a, gam, mc2, lm);
+  });
+  double* qLb = xb;
+  double* qRb = xb + V;
+  // a copied trace hands its cotangent (flipped back) to the trace it copies
+  if (f == 0 && bl != BC_PERIODIC) {
+    if (bl == BC_REFLECTIVE) qLb[a + 1] = -qLb[a + 1];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      qRb[v] += qLb[v];
+      qLb[v] = 0.0;
+    }
+  }
+  if (f == Na && br != BC_PERIODIC) {
+    if (br == BC_REFLECTIVE) qRb[a + 1] = -qRb[a + 1];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      qLb[v] += qRb[v];
+      qRb[v] = 0.0;
+    }
+  }
+  if (f > 0)
+    scatter(f - 1, nf - 1, qLb);
+  else if (bl == BC_PERIODIC)
+    scatter(Na - 1, nf - 1, qLb);
+  if (f < Na)
+    scatter(f, 0, qRb);
+  else if (br == BC_PERIODIC)
+    scatter(0, 0, qRb);
+}
+
+// ---------------------------------------------------------------------------
+// SD spatial operator along dimension A (k_pred's derivative phase): the
+// derivative cotangent Db -> flux-point cotangents; interior points through
+// the Euler flux into Sb, end points into the element-face cotangent Ffb.
+// One thread per (node, element, line along A).
+// ---------------------------------------------------------------------------
+template <int NDIM, int P, int A>
+__global__ void __launch_bounds__(256)
+k_adj_pred(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
+           const double* __restrict__ Db, double* __restrict__ Sb, double* __restrict__ Ffb) {
+  constexpr int a = A;
+  constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
+  constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long long)Tsrc * g.NE * NLP) return;
+  const int lp = (int)(tid % NLP);
+  const long long e = (tid / NLP) % g.NE;
+  const int t = (int)(tid / ((long long)NLP * g.NE));
+  int c[3] = {0, 0, 0};
+  elem_coords<NDIM>(g, e, c);
+  const int lb = line_base<NDIM, n>(a, lp);
+  const int sa = stride_of<n>(a);
+  const long long tbase = (long long)t * V * g.NE * NL;
+  double db[V][n];
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+#pragma unroll
+    for (int s = 0; s < n; ++s) db[v][s] = Db[tbase + ((long long)v * g.NE + e) * NL + lb + s * sa];
+  const double rha = g.rh[a];
+  double Fb[nf][V];
+#pragma unroll
+  for (int fp = 0; fp < nf; ++fp)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double acc = 0.0;
+#pragma unroll
+      for (int s = 0; s < n; ++s) acc += op.d2s[s][fp] * db[v][s];
+      Fb[fp][v] = acc * rha;
+    }
+  {
+    int cl[3] = {0, 0, 0};
+    int r = lp;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      if (b == a) continue;
+      cl[b] = r % n;
+      r /= n;
+    }
+    long long plane;
+    const long long i0 = face_grid_base<NDIM, n>(g, a, c, cl, c[a], &plane);
+    const long long sfa = stride_face<NDIM, n>(g, a);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double* fb = Ffb + (long long)(t * V + v) * plane + i0;
+      atomicAdd(&fb[0], Fb[0][v]);
+      atomicAdd(&fb[sfa], Fb[nf - 1][v]);
+    }
+  }
+  double u[V][n], ub[V][n];
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+#pragma unroll
+    for (int s = 0; s < n; ++s) {
+      u[v][s] = src[tbase + ((long long)v * g.NE + e) * NL + lb + s * sa];
+      ub[v][s] = 0.0;
+    }
+  const Dual gam(prm.gamma);
+#pragma unroll 1
+  for (int fp = 1; fp < nf - 1; ++fp) {
+    double M[V], Mb[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double acc = 0.0;
+#pragma unroll
+      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[v][s];
+      M[v] = acc;
+    }
+    ad::vjp<V, V>(M, Fb[fp], Mb, [&](const Dual* xd, Dual* yd) {
+      Dual W[V];
+      prims<NDIM, Dual>(xd, W, a, gam);
+      flux<NDIM, Dual>(W, yd, a, gam);
+    });
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+#pragma unroll
+      for (int s = 0; s < n; ++s) ub[v][s] += op.s2f[fp][s] * Mb[v];
+  }
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+#pragma unroll
+    for (int s = 0; s < n; ++s)
+      atomicAdd(&Sb[tbase + ((long long)v * g.NE + e) * NL + lb + s * sa], ub[v][s]);
+}
+
+// ---------------------------------------------------------------------------
+// Picard update U_ad^(k)[tn] = U_sp - dt sum_p invader[tn][p] dudt[p]
+// (src/sdader_simulator.py:187-200) transposed: Usb += sum_tn Uadb[tn],
+// Db[p] = -dt sum_tn invader[tn][p] Uadb[tn]; with Tsrc == 1 (first iteration,
+// one source node) Db[0] = sum_p Db[p].  mode 1: SD corrector
+// U_sp' = U_sp - dt sum_t w_t dudt[t]: Db[t] = -dt w_t Usb (Usb unchanged).
+// ---------------------------------------------------------------------------
+template <int NDIM, int P>
+__global__ void __launch_bounds__(256)
+k_adj_picard(Ops<P> op, Geom g, int mode, int Tsrc, const double* __restrict__ dt_dev,
+             const double* __restrict__ Uadb, double* __restrict__ Usb, double* __restrict__ Db) {
+  constexpr int V = Dims<NDIM>::V, n = P + 1, T = P + 1;
+  constexpr int NL = IPow<n, NDIM>::v;
+  const long long npt = (long long)V * g.NE * NL;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= npt) return;
+  const double dt = *dt_dev;
+  if (mode == 1) {
+    const double ub = Usb[tid];
+#pragma unroll
+    for (int t = 0; t < T; ++t) Db[t * npt + tid] = -dt * (op.w[t] * ub);
+    return;
+  }
+  double ub[T];
+  double s = Usb[tid];
+#pragma unroll
+  for (int tn = 0; tn < T; ++tn) {
+    ub[tn] = Uadb[tn * npt + tid];
+    s += ub[tn];
+  }
+  Usb[tid] = s;
+  double tot = 0.0;
+#pragma unroll
+  for (int p = 0; p < T; ++p) {
+    double acc = 0.0;
+#pragma unroll
+    for (int tn = 0; tn < T; ++tn) acc += op.inv[tn][p] * ub[tn];
+    if (Tsrc == 1)
+      tot += acc;
+    else
+      Db[p * npt + tid] = -dt * acc;
+  }
+  if (Tsrc == 1) Db[tid] = -dt * tot;
+}
+
+}  // namespace spd
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+// Adjoint scratch of one context (FP64).
+struct AdjScratch {
+  double* Uadb = nullptr;       // (T, V, E, L) cotangent of a predictor iterate
+  double* Db = nullptr;         // (T, V, E, L) cotangent of dudt
+  double* Ffb[3] = {nullptr, nullptr, nullptr};  // (T, V, plane_a) element-face cotangents
+  double* Ucvb[2] = {nullptr, nullptr};          // (V, Nc) FV-state cotangents
+  double* Fhob[3] = {nullptr, nullptr, nullptr};  // (V, nface_a) HO-flux cotangents
+  double* WbL[3] = {nullptr, nullptr, nullptr};   // (V, nface_a) fallback left-state cotangents
+  double* WbR[3] = {nullptr, nullptr, nullptr};
+  double* HS = nullptr;         // ((1+d) V, (Ns+2)^d) Hancock states
+  bool ready = false;
+};
+
+void spd_ctx::adj_release() {
+  delete adj;
+  adj = nullptr;
+}
+
+namespace {
+
+using namespace spd;
+
+long long plane_of(const spd_ctx* c, int a) {
+  return (long long)(c->g.N[a] + 1) * (c->g.NE / c->g.N[a]) * c->NLP;
+}
+long long nface_of(const spd_ctx* c, int a) {
+  long long nf = 1;
+  for (int b = 0; b < c->ndim; ++b) nf *= c->g.Ns[b] + (b == a ? 1 : 0);
+  return nf;
+}
+long long n1_of(const spd_ctx* c) {
+  long long n1 = 1;
+  for (int b = 0; b < c->ndim; ++b) n1 *= c->g.Ns[b] + 2;
+  return n1;
+}
+
+int ensure_tape(spd_ctx* c) {
+  if (c->tape_uad) return SPD_OK;
+  const long long it = (long long)c->T * c->V * c->g.NE * c->NL;
+  int rc;
+  double* uad = nullptr;
+  if ((rc = c->alloc(&uad, (size_t)(c->cfg.m * it))) != SPD_OK) return rc;
+  c->tape_uad = uad;
+  if (c->cfg.update == SPD_UPDATE_FV) {
+    if ((rc = c->alloc(&c->tape_ucv, (size_t)(c->T * c->V * c->g.Nc))) != SPD_OK) return rc;
+    if (c->cfg.FB && (rc = c->alloc(&c->tape_theta, (size_t)(c->T * c->g.Ngc))) != SPD_OK)
+      return rc;
+  }
+  return SPD_OK;
+}
+
+int ensure_scratch(spd_ctx* c) {
+  if (!c->adj) c->adj = new AdjScratch();
+  AdjScratch* A = c->adj;
+  if (A->ready) return SPD_OK;
+  const long long it = (long long)c->T * c->V * c->g.NE * c->NL;
+  int rc;
+  if ((rc = c->alloc(&A->Uadb, (size_t)it)) != SPD_OK) return rc;
+  if ((rc = c->alloc(&A->Db, (size_t)it)) != SPD_OK) return rc;
+  for (int a = 0; a < c->ndim; ++a)
+    if ((rc = c->alloc(&A->Ffb[a], (size_t)(c->T * c->V * plane_of(c, a)))) != SPD_OK) return rc;
+  if (c->cfg.update == SPD_UPDATE_FV) {
+    for (int k = 0; k < 2; ++k)
+      if ((rc = c->alloc(&A->Ucvb[k], (size_t)(c->V * c->g.Nc))) != SPD_OK) return rc;
+    for (int a = 0; a < c->ndim; ++a) {
+      const size_t nfc = (size_t)(c->V * nface_of(c, a));
+      if ((rc = c->alloc(&A->Fhob[a], nfc)) != SPD_OK) return rc;
+      if (c->cfg.FB) {
+        if ((rc = c->alloc(&A->WbL[a], nfc)) != SPD_OK) return rc;
+        if ((rc = c->alloc(&A->WbR[a], nfc)) != SPD_OK) return rc;
+      }
+    }
+    if (c->cfg.FB &&
+        (rc = c->alloc(&A->HS, (size_t)((1 + c->ndim) * c->V * n1_of(c)))) != SPD_OK)
+      return rc;
+  }
+  for (int a = c->ndim; a < 3; ++a) {
+    A->Ffb[a] = A->Ffb[0];
+    A->Fhob[a] = A->Fhob[0];
+    A->WbL[a] = A->WbL[0];
+    A->WbR[a] = A->WbR[0];
+  }
+  A->ready = true;
+  return SPD_OK;
+}
+
+template <int NDIM, int P> struct Adjoint {
+  static constexpr int V = NDIM + 2, n = P + 1, nf = P + 2, T = P + 1;
+  static constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
+
+  template <int A>
+  static void face(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, double* Sb,
+                   hipStream_t s) {
+    const long long tot = (long long)Tsrc * plane_of(c, A);
+    if (c->cfg.riemann_sd != SPD_RS_LLF)
+      hipLaunchKernelGGL((k_adj_face<NDIM, P, RS_HLLC, A>), dim3(grid_for(tot)), dim3(256), 0, s,
+                         op, c->g, c->prm, src, Tsrc, c->adj->Ffb[A], Sb);
+    else
+      hipLaunchKernelGGL((k_adj_face<NDIM, P, RS_LLF, A>), dim3(grid_for(tot)), dim3(256), 0, s,
+                         op, c->g, c->prm, src, Tsrc, c->adj->Ffb[A], Sb);
+  }
+  static void faces(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, double* Sb,
+                    hipStream_t s) {
+    face<0>(c, op, src, Tsrc, Sb, s);
+    if constexpr (NDIM >= 2) face<1>(c, op, src, Tsrc, Sb, s);
+    if constexpr (NDIM >= 3) face<2>(c, op, src, Tsrc, Sb, s);
+  }
+  template <int A>
+  static void pred_one(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, double* Sb,
+                       hipStream_t s) {
+    const long long tot = (long long)Tsrc * c->g.NE * NLP;
+    hipLaunchKernelGGL((k_adj_pred<NDIM, P, A>), dim3(grid_for(tot)), dim3(256), 0, s, op, c->g,
+                       c->prm, src, Tsrc, c->adj->Db, Sb, c->adj->Ffb[A]);
+  }
+  // cotangent of D(src) (held in Db) -> Sb
+  static void spatial(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, double* Sb,
+                      hipStream_t s) {
+    for (int a = 0; a < NDIM; ++a)
+      (void)hipMemsetAsync(c->adj->Ffb[a], 0, (size_t)Tsrc * V * plane_of(c, a) * sizeof(double),
+                           s);
+    pred_one<0>(c, op, src, Tsrc, Sb, s);
+    if constexpr (NDIM >= 2) pred_one<1>(c, op, src, Tsrc, Sb, s);
+    if constexpr (NDIM >= 3) pred_one<2>(c, op, src, Tsrc, Sb, s);
+    faces(c, op, src, Tsrc, Sb, s);
+  }
+  template <int RS, int A>
+  static void fvface(spd_ctx* c, const double* Ub1, const double* th, const double* dt, double w,
+                     hipStream_t s) {
+    AdjScratch* Ad = c->adj;
+    hipLaunchKernelGGL((k_adj_fvface<NDIM, RS, A>), dim3(grid_for(nface_of(c, A))), dim3(256), 0,
+                       s, c->g, c->prm, c->cfg.FB, Ub1, th, Ad->HS, dt, w, Ad->Fhob[A],
+                       Ad->WbL[A], Ad->WbR[A]);
+  }
+  template <int RS>
+  static void fvfaces(spd_ctx* c, const double* Ub1, const double* th, const double* dt, double w,
+                      hipStream_t s) {
+    fvface<RS, 0>(c, Ub1, th, dt, w, s);
+    if constexpr (NDIM >= 2) fvface<RS, 1>(c, Ub1, th, dt, w, s);
+    if constexpr (NDIM >= 3) fvface<RS, 2>(c, Ub1, th, dt, w, s);
+  }
+  template <int A>
+  static void hoflux(spd_ctx* c, const Ops<P>& op, int i, const double* Uad, hipStream_t s) {
+    hipLaunchKernelGGL((k_adj_hoflux<NDIM, P, A>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s,
+                       op, c->g, c->prm, i, Uad, c->adj->Fhob[A], c->adj->Uadb, c->adj->Ffb[A]);
+  }
+
+  static int backward(spd_ctx* c, const double* Usp, const double* dt, double* Ub,
+                      hipStream_t s) {
+    const Ops<P> op = make_ops<P>(*c->ops);
+    AdjScratch* Ad = c->adj;
+    const int m = c->cfg.m;
+    const long long it_size = (long long)T * V * c->g.NE * NL;
+    const long long npt = (long long)V * c->g.NE * NL;
+    const double* tape = (const double*)c->tape_uad;
+    auto iterate = [&](int k) { return tape + (long long)(k - 1) * it_size; };  // U_ad^(k)
+    (void)hipMemsetAsync(Ad->Uadb, 0, (size_t)it_size * sizeof(double), s);
+    if (c->cfg.update == SPD_UPDATE_FV) {
+      const long long Nc = c->g.Nc;
+      // U_sp' = cv_to_sp(U^T)
+      hipLaunchKernelGGL((k_adj_tensor<NDIM, P>), dim3(grid_for(npt)), dim3(256), 0, s, op, c->g,
+                         0, (const double*)Ub, 0, Ad->Ucvb[0], 1);
+      int cur = 0;
+      for (int i = T - 1; i >= 0; --i) {
+        const double w = c->ops->w_tp[i];
+        double* Ub1 = Ad->Ucvb[cur];
+        double* Ub0 = Ad->Ucvb[1 - cur];
+        const double* Ui = c->tape_ucv + (long long)i * V * Nc;
+        const double* th = c->cfg.FB ? c->tape_theta + (long long)i * c->g.Ngc : nullptr;
+        (void)hipMemcpyAsync(Ub0, Ub1, (size_t)V * Nc * sizeof(double), hipMemcpyDeviceToDevice,
+                             s);
+        if (c->cfg.FB)
+          hipLaunchKernelGGL((k_adj_hs<NDIM>), dim3(grid_for(n1_of(c))), dim3(256), 0, s, c->g,
+                             c->prm, Ui, th, dt, w, Ad->HS);
+        if (c->cfg.riemann_fv != SPD_RS_LLF)
+          fvfaces<RS_HLLC>(c, Ub1, th, dt, w, s);
+        else
+          fvfaces<RS_LLF>(c, Ub1, th, dt, w, s);
+        if (c->cfg.FB)
+          hipLaunchKernelGGL((k_adj_hancock<NDIM>), dim3(grid_for(n1_of(c))), dim3(256), 0, s,
+                             c->g, c->prm, Ui, th, dt, w, Ad->WbL[0], Ad->WbL[1], Ad->WbL[2],
+                             Ad->WbR[0], Ad->WbR[1], Ad->WbR[2], Ub0);
+        hoflux<0>(c, op, i, iterate(m), s);
+        if constexpr (NDIM >= 2) hoflux<1>(c, op, i, iterate(m), s);
+        if constexpr (NDIM >= 3) hoflux<2>(c, op, i, iterate(m), s);
+        cur = 1 - cur;
+      }
+      // U^0 = sp_to_cv(U_sp)
+      hipLaunchKernelGGL((k_adj_tensor<NDIM, P>), dim3(grid_for(npt)), dim3(256), 0, s, op, c->g,
+                         1, (const double*)Ad->Ucvb[cur], 1, Ub, 0);
+      // element faces of the final iterate (every node, written by k_adj_hoflux)
+      faces(c, op, iterate(m), T, Ad->Uadb, s);
+    } else {
+      hipLaunchKernelGGL((k_adj_picard<NDIM, P>), dim3(grid_for(npt)), dim3(256), 0, s, op, c->g,
+                         1, T, dt, (const double*)Ad->Uadb, Ub, Ad->Db);
+      spatial(c, op, iterate(m), T, Ad->Uadb, s);
+    }
+    // Picard chain
+    for (int k = m; k >= 1; --k) {
+      const int Tsrc = (k == 1) ? 1 : T;
+      hipLaunchKernelGGL((k_adj_picard<NDIM, P>), dim3(grid_for(npt)), dim3(256), 0, s, op, c->g,
+                         0, Tsrc, dt, (const double*)Ad->Uadb, Ub, Ad->Db);
+      if (k == 1) {
+        spatial(c, op, Usp, 1, Ub, s);
+      } else {
+        (void)hipMemsetAsync(Ad->Uadb, 0, (size_t)it_size * sizeof(double), s);
+        spatial(c, op, iterate(k - 1), T, Ad->Uadb, s);
+      }
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, std::string("adjoint launch: ") +
+                                                         hipGetErrorString(e));
+  }
+};
+
+using BackFn = int (*)(spd_ctx*, const double*, const double*, double*, hipStream_t);
+struct AdjEntry {
+  int ndim, p;
+  BackFn back;
+};
+const AdjEntry kAdj[] = {
+    {1, 1, &Adjoint<1, 1>::backward}, {1, 2, &Adjoint<1, 2>::backward},
+    {1, 3, &Adjoint<1, 3>::backward}, {1, 4, &Adjoint<1, 4>::backward},
+    {1, 5, &Adjoint<1, 5>::backward}, {2, 1, &Adjoint<2, 1>::backward},
+    {2, 2, &Adjoint<2, 2>::backward}, {2, 3, &Adjoint<2, 3>::backward},
+    {2, 4, &Adjoint<2, 4>::backward}, {2, 5, &Adjoint<2, 5>::backward},
+    {3, 1, &Adjoint<3, 1>::backward}, {3, 2, &Adjoint<3, 2>::backward},
+    {3, 3, &Adjoint<3, 3>::backward},
+};
+
+int adj_check(const spd_ctx* c) {
+  if (!c) return fail(SPD_E_INVALID, "null context");
+  if (c->cfg.precision != SPD_PRECISION_FP64)
+    return fail(SPD_E_UNSUPPORTED, "the adjoint is built for the FP64 form only");
+  return SPD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spd_adj_step(spd_ctx* ctx, const double* U_in, double* U_out, const double* dt_dev,
+                 const double* theta_in, double* theta_out, void* stream) {
+  int rc = adj_check(ctx);
+  if (rc != SPD_OK) return rc;
+  if (!U_in || !U_out || !dt_dev) return fail(SPD_E_INVALID, "null argument");
+  if (U_in == U_out) return fail(SPD_E_INVALID, "U_in and U_out must differ");
+  const bool fb = ctx->cfg.update == SPD_UPDATE_FV && ctx->cfg.FB;
+  if ((theta_in || theta_out) && !fb)
+    return fail(SPD_E_INVALID, "blending coefficients exist only with the FV fallback");
+  if ((rc = ensure_tape(ctx)) != SPD_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t th_bytes = (size_t)ctx->T * ctx->g.Ngc * sizeof(double);
+  SPD_HIP_CHECK(hipMemcpyAsync(U_out, U_in, (size_t)ctx->V * ctx->g.NE * ctx->NL * sizeof(double),
+                               hipMemcpyDeviceToDevice, s));
+  if (theta_in)
+    SPD_HIP_CHECK(hipMemcpyAsync(ctx->tape_theta, theta_in, th_bytes, hipMemcpyDeviceToDevice, s));
+  ctx->tape_on = 1;
+  ctx->tape_replay = theta_in ? 1 : 0;
+  rc = spd_step(ctx, U_out, nullptr, nullptr, dt_dev, 0, stream);
+  ctx->tape_on = 0;
+  ctx->tape_replay = 0;
+  if (rc != SPD_OK) return rc;
+  if (theta_out)
+    SPD_HIP_CHECK(hipMemcpyAsync(theta_out, ctx->tape_theta, th_bytes, hipMemcpyDeviceToDevice, s));
+  return SPD_OK;
+}
+
+int spd_adj_backward(spd_ctx* ctx, const double* U_in, const double* dt_dev, double* U_bar,
+                     void* stream) {
+  int rc = adj_check(ctx);
+  if (rc != SPD_OK) return rc;
+  if (!U_in || !dt_dev || !U_bar) return fail(SPD_E_INVALID, "null argument");
+  if (!ctx->tape_uad) return fail(SPD_E_INVALID, "no tape: call spd_adj_step first");
+  if ((rc = ensure_scratch(ctx)) != SPD_OK) return rc;
+  for (const AdjEntry& e : kAdj)
+    if (e.ndim == ctx->ndim && e.p == ctx->p)
+      return e.back(ctx, U_in, dt_dev, U_bar, (hipStream_t)stream);
+  return fail(SPD_E_UNSUPPORTED, "no adjoint kernels for this dimension and order");
+}
+
+size_t spd_adj_theta_count(const spd_ctx* ctx) {
+  if (!ctx || ctx->cfg.update != SPD_UPDATE_FV || !ctx->cfg.FB) return 0;
+  return (size_t)ctx->T * ctx->g.Ngc;
+}
+
+}  // extern "C"

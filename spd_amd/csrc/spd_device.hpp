@@ -95,16 +95,16 @@ __device__ __forceinline__ void prims(const R* U, R* W, int a, R gamma) {
   } else {
     // one reciprocal of the density for all velocity components (within an
     // ulp of the per-component divisions of src/hydro.py:79-81)
-    const double rr = 1.0 / U[0];
-    double K = U[0] * 0.0;
+    const R rr = R(1.0) / U[0];
+    R K = U[0] * R(0.0);
 #pragma unroll
     for (int k = 0; k < NDIM; ++k) {
       const int v = rolled<NDIM>(a, k);
       W[v] = U[v] * rr;
       K = K + W[v] * W[v];
     }
-    K = K * (0.5 * U[0]);
-    W[P] = (gamma - 1.0) * (U[P] - K);
+    K = K * (R(0.5) * U[0]);
+    W[P] = (gamma - R(1.0)) * (U[P] - K);
   }
 }
 
@@ -339,6 +339,52 @@ __device__ __forceinline__ void load_W(const double* __restrict__ U, const Geom&
 #pragma unroll
   for (int a = 0; a < NDIM; ++a)
     if (flip[a]) W[a + 1] = -W[a + 1];
+}
+
+// MUSCL-Hancock state of one cell of the one-layer-ghosted grid
+// (src/muscl.py:61-75,129-221; Q1, Q7, Q19): limited slopes of the primitives
+// W from the neighbours Wm[b] (c - e_b) and Wp[b] (c + e_b), the half slopes
+// S[b] = slope * width / 2 and the half-step predicted state Mp with the
+// pressure row of the time derivative zeroed.  `c` is the cell's coordinate
+// (active range [0, Ns)).
+template <int NDIM, typename R>
+__device__ __forceinline__ void hancock_state(const Geom& g, const Params& prm, const int* c,
+                                              const R* W, const R (*Wm)[Dims<NDIM>::V],
+                                              const R (*Wp)[Dims<NDIM>::V], R dti, R* Mp,
+                                              R (*S)[Dims<NDIM>::V]) {
+  constexpr int V = Dims<NDIM>::V, P = Dims<NDIM>::P;
+  const R gam = (R)prm.gamma;
+  R dMh[NDIM][V];
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    const int G = c[b] + NGH;
+    const R rhl = (R)g.rhcv[b][G - 1], rhr = (R)g.rhcv[b][G];
+    const R hf = (R)g.hfp[b][G];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const R L = (W[v] - Wm[b][v]) * rhl;
+      const R Rr = (Wp[b][v] - W[v]) * rhr;
+      dMh[b][v] = prm.moncen ? moncen<R>(L, Rr, (R)g.hcv[b][G - 1], (R)g.hcv[b][G], hf)
+                             : minmod<R>(L, Rr);
+      S[b][v] = R(0.5) * dMh[b][v] * hf;
+    }
+  }
+  R dtW[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) dtW[v] = R(0);
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    const int vel = b + 1;
+    const R* dW = dMh[b];
+    dtW[0] = dtW[0] - (W[vel] * dW[0] + W[0] * dW[vel]);
+    dtW[P] = dtW[P] - (W[vel] * dW[P] + gam * W[P] * dW[vel]);
+    dtW[vel] = dtW[vel] - (W[vel] * dW[vel] + qdiv<R>(dW[P], W[0]));
+#pragma unroll
+    for (int v2 = 2; v2 <= NDIM; ++v2) dtW[v2] = dtW[v2] - W[vel] * dW[v2];
+  }
+  dtW[P] = dtW[P] * R(0);
+#pragma unroll
+  for (int v = 0; v < V; ++v) Mp[v] = W[v] + R(0.5) * dtW[v] * dti;
 }
 
 }  // namespace spd

@@ -30,64 +30,13 @@
 #include "../../include/spd.h"
 #include "spd_device.hpp"
 #include "spd_host_ops.hpp"
+#include "spd_layout.hpp"
+#include "spd_ctx.hpp"
 
 namespace spd {
 
-// Operator matrices in the arithmetic type R of the kernels that use them;
-// the quadrature weights stay FP64 (they scale the FP64 update).
-template <int P, typename R = double> struct Ops {
-  static constexpr int n = P + 1, nf = P + 2, T = P + 1;
-  R s2f[nf][n];  // sp_to_fp
-  R d2s[n][nf];  // dfp_to_sp
-  R s2c[n][n];   // sp_to_cv
-  R c2s[n][n];   // cv_to_sp
-  R s2r[n][n];   // solution points -> regular subcell centres
-  R inv[T][T];   // invader
-  double w[T];   // w_tp
-};
-
 // Dynamic LDS of the templated kernels (typed at each use).
 extern __shared__ __attribute__((aligned(16))) unsigned char spd_dyn_lds[];
-
-// ---------------------------------------------------------------------------
-// index helpers
-// ---------------------------------------------------------------------------
-template <int NDIM>
-__device__ __forceinline__ void elem_coords(const Geom& g, long long e, int* c) {
-#pragma unroll
-  for (int a = 0; a < NDIM; ++a) {
-    c[a] = (int)(e % g.N[a]);
-    e /= g.N[a];
-  }
-}
-template <int NDIM>
-__device__ __forceinline__ long long elem_index(const Geom& g, const int* c) {
-  long long e = 0;
-#pragma unroll
-  for (int a = NDIM - 1; a >= 0; --a) e = e * g.N[a] + c[a];
-  return e;
-}
-// local point index of the line `lp` (local coordinates other than a) at
-// position 0 along a; the stride along a is n^a.
-template <int NDIM, int n>
-__device__ __forceinline__ int line_base(int a, int lp) {
-  int c[3] = {0, 0, 0};
-#pragma unroll
-  for (int b = 0; b < NDIM; ++b) {
-    if (b == a) continue;
-    c[b] = lp % n;
-    lp /= n;
-  }
-  int l = 0;
-#pragma unroll
-  for (int b = NDIM - 1; b >= 0; --b) l = l * n + c[b];
-  return l;
-}
-template <int n> __device__ __forceinline__ int stride_of(int a) {
-  int s = 1;
-  for (int b = 0; b < a; ++b) s *= n;
-  return s;
-}
 
 // Maximum over the block (NaN-propagating); the result is valid in thread 0.
 __device__ __forceinline__ double block_max(double v) {
@@ -102,36 +51,6 @@ __device__ __forceinline__ double block_max(double v) {
     for (int off = warpSize / 2; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off));
   }
   return v;
-}
-
-// Element-face arrays use a "face grid" per dimension a: coordinates q_b =
-// E_b * n + l_b for b != a (transverse solution point) and q_a = face index in
-// [0, N_a]; x fastest, then the variable, then the time node:
-//   idx = ((t * V + v) * prod_b ext_b) + sum_b q_b * stride_b
-template <int NDIM, int n>
-__device__ __forceinline__ long long face_grid_base(const Geom& g, int a, const int* ce,
-                                                    const int* cl, int f, long long* plane) {
-  long long idx = 0, tot = 1;
-#pragma unroll
-  for (int b = NDIM - 1; b >= 0; --b) {
-    const long long ext = (b == a) ? (long long)g.N[a] + 1 : (long long)g.N[b] * n;
-    const long long q = (b == a) ? f : (long long)ce[b] * n + cl[b];
-    idx = idx * ext + q;
-    tot *= ext;
-  }
-  *plane = tot;
-  return idx;
-}
-
-template <int NDIM, int n>
-__device__ __forceinline__ long long stride_face(const Geom& g, int a) {
-  long long st = 1;
-#pragma unroll
-  for (int b = 0; b < NDIM; ++b) {
-    if (b == a) break;
-    st *= (long long)g.N[b] * n;
-  }
-  return st;
 }
 
 // ---------------------------------------------------------------------------
@@ -1216,10 +1135,8 @@ k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __re
   }
   if (!need) return;
   const B dti = (B)((*dt_dev) * w_i);
-  const B gam = (B)prm.gamma;
   B Mp[V], S[NDIM][V];
   {
-    constexpr int P = Dims<NDIM>::P;
     // primitives from the FP64 state (FP64 pressure recovery)
     auto load = [&](const int* cc, B* Wb) {
       double Wd[V];
@@ -1227,46 +1144,17 @@ k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __re
 #pragma unroll
       for (int v = 0; v < V; ++v) Wb[v] = (B)Wd[v];
     };
-    B W[V];
+    B W[V], Wm[NDIM][V], Wp[NDIM][V];
     load(c, W);
-    B dMh[NDIM][V];
 #pragma unroll
     for (int b = 0; b < NDIM; ++b) {
       int cp[3] = {c[0], c[1], c[2]}, cm[3] = {c[0], c[1], c[2]};
       cp[b] += 1;
       cm[b] -= 1;
-      B Wp[V], Wm[V];
-      load(cp, Wp);
-      load(cm, Wm);
-      const int G = c[b] + NGH;
-      const B rhl = (B)g.rhcv[b][G - 1], rhr = (B)g.rhcv[b][G];
-      const B hf = (B)g.hfp[b][G];
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        const B L = (W[v] - Wm[v]) * rhl;
-        const B R = (Wp[v] - W[v]) * rhr;
-        dMh[b][v] = prm.moncen
-                        ? moncen<B>(L, R, (B)g.hcv[b][G - 1], (B)g.hcv[b][G], hf)
-                        : minmod<B>(L, R);
-        S[b][v] = B(0.5) * dMh[b][v] * hf;
-      }
+      load(cp, Wp[b]);
+      load(cm, Wm[b]);
     }
-    B dtW[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) dtW[v] = B(0);
-#pragma unroll
-    for (int b = 0; b < NDIM; ++b) {
-      const int vel = b + 1;
-      const B* dW = dMh[b];
-      dtW[0] = dtW[0] - (W[vel] * dW[0] + W[0] * dW[vel]);
-      dtW[P] = dtW[P] - (W[vel] * dW[P] + gam * W[P] * dW[vel]);
-      dtW[vel] = dtW[vel] - (W[vel] * dW[vel] + qdiv<B>(dW[P], W[0]));
-#pragma unroll
-      for (int v2 = 2; v2 <= NDIM; ++v2) dtW[v2] = dtW[v2] - W[vel] * dW[v2];
-    }
-    dtW[P] = dtW[P] * B(0);
-#pragma unroll
-    for (int v = 0; v < V; ++v) Mp[v] = W[v] + B(0.5) * dtW[v] * dti;
+    hancock_state<NDIM, B>(g, prm, c, W, Wm, Wp, dti, Mp, S);
   }
   constexpr int REC = (1 + NDIM) * V;
   B* out = HS + tid;
@@ -1449,16 +1337,6 @@ __global__ void k_dt_final(const double* cmax, double h, double cfl, double* dt)
   if (threadIdx.x == 0 && blockIdx.x == 0) dt[0] = cfl * (h / *cmax);
 }
 
-// Device-side time loop (src/sdader_simulator.py:394-413): time, dt, step
-// count and the stop condition live on the device so that chunks of steps can
-// be captured in a graph and replayed without host synchronisation.
-struct RunState {
-  double time, t_end, dt;
-  long long n_step;
-  long long n_left;  // steps still allowed (< 0: unlimited)
-  int done;
-  int clamp;         // clamp dt to reach t_end and stop when dt < 1e-14
-};
 
 __global__ void k_run_init(RunState* st, int* skip, double time, double t_end, long long n_step,
                            long long n_left, int clamp) {
@@ -1648,8 +1526,6 @@ __global__ void k_tap(Geom g, const uint8_t* __restrict__ flags, const double* _
 // ===========================================================================
 using namespace spd;
 
-namespace {
-
 thread_local std::string g_last_error;
 
 int fail(int code, const std::string& msg) {
@@ -1657,122 +1533,12 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define SPD_HIP_CHECK(expr)                                                          \
-  do {                                                                               \
-    hipError_t _e = (expr);                                                          \
-    if (_e != hipSuccess)                                                            \
-      return fail(SPD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
-  } while (0)
-
-inline unsigned grid_for(long long n, int bs = 256) {
-  return (unsigned)((n + bs - 1) / bs);
-}
-
-}  // namespace
-
-// Kernel classes timed by the optional HIP-event profiler.
-enum KernelClass : int {
-  KC_FACE_SD = 0, KC_PRED, KC_HOFLUX, KC_TENSOR, KC_APPLY, KC_DETECT, KC_THETA1, KC_RING,
-  KC_FBFLUX, KC_CV2SP, KC_HANCOCK, KC_OTHER, KC_COUNT
-};
 static const char* const kKernelClassNames[KC_COUNT] = {
     "k_face_sd", "k_pred", "k_hoflux", "k_tensor", "k_apply", "k_detect", "k_theta1", "k_ring",
     "k_fbface", "k_cv2sp", "k_hancock", "other"};
 
-struct spd_ctx {
-  spd_config cfg{};
-  // HIP-event profiler: one event pair per recorded launch
-  bool prof_on = false;
-  std::vector<hipEvent_t> prof_ev;
-  std::vector<int> prof_cls;
-  size_t prof_used = 0;
-  int ndim = 0, p = 0, n = 0, nf = 0, T = 0, V = 0, NL = 0, NLP = 0;
-  Geom g{};
-  Params prm{};
-  std::unique_ptr<host::OpsHost> ops;
-  double h_dt = 0.0;  // h_min / (p+1)
-  int device = 0;
-  std::vector<void*> allocs;
-  size_t scratch_bytes = 0;
-  double* geom_buf = nullptr;
-  // bulk-type buffers (FP64, or FP32 in the FP32-bulk form)
-  void* Uad = nullptr;
-  void* Uad2 = nullptr;       // second predictor buffer of the fused 2-D iteration
-  void* Uad_cur = nullptr;    // predictor state of the current step's last iteration
-  void* Fface[3] = {nullptr, nullptr, nullptr};
-  void* Fho[3] = {nullptr, nullptr, nullptr};
-  double* Ucv = nullptr;
-  double* Unew = nullptr;
-  uint8_t* flags = nullptr;
-  double* theta1 = nullptr;
-  double* theta = nullptr;
-  void* HS = nullptr;         // Hancock states and half slopes, (1+ndim)*V per cell (bulk)
-  int* any_flag = nullptr;
-  int* zero_flag = nullptr;   // always 0: the skip flag of ordinary calls
-  int* run_skip = nullptr;    // skip flag of device-loop steps
-  RunState* run = nullptr;
-  double* run_dt = nullptr;
-  double* cmax = nullptr;     // max wave speed of the last state (for dt)
-  double* partial = nullptr;  // per-block partial maxima
-  long long npartial = 0;
-  bool need_gate = false;     // some side is not periodic (global NAD gate needed)
-  int launch[SPD_LAUNCH_COUNT] = {0, 0, 0};  // tuned launch shapes (0 = default)
-  // debug tap
-  int dbg_sub = -1;
-  double* dbg_tr = nullptr;
-  double* dbg_th = nullptr;
-  double* dbg_aff[3] = {nullptr, nullptr, nullptr};
-
-  ~spd_ctx() {
-    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
-    for (void* p : allocs) (void)hipFree(p);
-  }
-  // Launch `fn` on stream s, bracketed by an event pair when profiling.
-  template <typename F> void timed(int cls, hipStream_t s, F&& fn) {
-    const bool rec = prof_on && prof_used < prof_cls.size();
-    if (rec) (void)hipEventRecord(prof_ev[2 * prof_used], s);
-    fn();
-    if (rec) {
-      (void)hipEventRecord(prof_ev[2 * prof_used + 1], s);
-      prof_cls[prof_used] = cls;
-      ++prof_used;
-    }
-  }
-  template <typename X> int alloc(X** ptr, size_t count) {
-    void* p = nullptr;
-    const size_t bytes = std::max<size_t>(count * sizeof(X), 256);
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess)
-      return fail(SPD_E_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
-                                   "): " + hipGetErrorString(e));
-    allocs.push_back(p);
-    scratch_bytes += bytes;
-    *ptr = (X*)p;
-    return SPD_OK;
-  }
-};
-
 namespace {
 
-template <int P, typename R = double> Ops<P, R> make_ops(const host::OpsHost& h) {
-  Ops<P, R> o{};
-  constexpr int n = P + 1, nf = P + 2, T = P + 1;
-  for (int a = 0; a < nf; ++a)
-    for (int b = 0; b < n; ++b) o.s2f[a][b] = (R)h.sp_to_fp[a][b];
-  for (int a = 0; a < n; ++a)
-    for (int b = 0; b < nf; ++b) o.d2s[a][b] = (R)h.dfp_to_sp[a][b];
-  for (int a = 0; a < n; ++a)
-    for (int b = 0; b < n; ++b) {
-      o.s2c[a][b] = (R)h.sp_to_cv[a][b];
-      o.c2s[a][b] = (R)h.cv_to_sp[a][b];
-      o.s2r[a][b] = (R)h.sp_to_reg[a][b];
-    }
-  for (int a = 0; a < T; ++a) {
-    for (int b = 0; b < T; ++b) o.inv[a][b] = (R)h.invader[a][b];
-    o.w[a] = h.w_tp[a];
-  }
-  return o;
-}
 
 // Typed launcher of one (ndim, p, bulk type) instantiation.
 template <int NDIM, int P, typename B> struct Stepper {
@@ -1928,6 +1694,7 @@ template <int NDIM, int P, typename B> struct Stepper {
   // other state buffer and the buffers are swapped.
   static void substep2d(spd_ctx* c, const OpsB& op, int i, double w, const double* dt,
                         hipStream_t s) {
+    tape_state(c, i, s);
     const int E = ho_epb(c);
     const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
     c->timed(KC_HOFLUX, s, [&] {
@@ -1940,20 +1707,24 @@ template <int NDIM, int P, typename B> struct Stepper {
       return;
     }
     apply(c, dt, w, c->Unew, s);
-    if (c->need_gate) hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
     const dim3 tiles((c->g.Ns[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ns[1] + fv2d::TY - 1) / fv2d::TY);
-    const dim3 dtiles((c->g.Ns[0] + fv2d::DTX - 1) / fv2d::DTX,
-                      (c->g.Ns[1] + fv2d::DTY - 1) / fv2d::DTY);
-    c->timed(KC_DETECT, s, [&] {
-      hipLaunchKernelGGL(fv2d::k_detect2d, dtiles, dim3(fv2d::DNT), 0, s, c->g, c->prm, c->Ucv,
-                         c->Unew, c->flags, c->need_gate ? c->any_flag : nullptr);
-    });
-    const dim3 gtiles((c->g.Ng[0] + fv2d::HTX - 1) / fv2d::HTX,
-                      (c->g.Ng[1] + fv2d::HTY - 1) / fv2d::HTY);
-    c->timed(KC_THETA1, s, [&] {
-      hipLaunchKernelGGL(fv2d::k_theta2d, gtiles, dim3(fv2d::HNT), 0, s, c->g, c->prm, c->flags,
-                         c->Ucv, c->Unew, c->any_flag, c->theta);
-    });
+    if (!c->tape_replay) {
+      if (c->need_gate)
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
+      const dim3 dtiles((c->g.Ns[0] + fv2d::DTX - 1) / fv2d::DTX,
+                        (c->g.Ns[1] + fv2d::DTY - 1) / fv2d::DTY);
+      c->timed(KC_DETECT, s, [&] {
+        hipLaunchKernelGGL(fv2d::k_detect2d, dtiles, dim3(fv2d::DNT), 0, s, c->g, c->prm, c->Ucv,
+                           c->Unew, c->flags, c->need_gate ? c->any_flag : nullptr);
+      });
+      const dim3 gtiles((c->g.Ng[0] + fv2d::HTX - 1) / fv2d::HTX,
+                        (c->g.Ng[1] + fv2d::HTY - 1) / fv2d::HTY);
+      c->timed(KC_THETA1, s, [&] {
+        hipLaunchKernelGGL(fv2d::k_theta2d, gtiles, dim3(fv2d::HNT), 0, s, c->g, c->prm,
+                           c->flags, c->Ucv, c->Unew, c->any_flag, c->theta);
+      });
+    }
+    tape_theta(c, i, s);
     tap(c, i, s);
     // the corrected state replaces the candidate in place
     c->timed(KC_FBFLUX, s, [&] {
@@ -1980,6 +1751,25 @@ template <int NDIM, int P, typename B> struct Stepper {
       }
     });
     std::swap(c->Ucv, c->Unew);
+  }
+
+  // Tape of the reverse-mode derivative: the FV state entering substep i,
+  // and the substep's blending coefficients (recorded, or replayed into
+  // c->theta in place of the detection).
+  static void tape_state(spd_ctx* c, int i, hipStream_t s) {
+    if (!c->tape_on) return;
+    const size_t bytes = (size_t)V * c->g.Nc * sizeof(double);
+    (void)hipMemcpyAsync(c->tape_ucv + (long long)i * V * c->g.Nc, c->Ucv, bytes,
+                         hipMemcpyDeviceToDevice, s);
+  }
+  static void tape_theta(spd_ctx* c, int i, hipStream_t s) {
+    if (!c->tape_on && !c->tape_replay) return;
+    const size_t bytes = (size_t)c->g.Ngc * sizeof(double);
+    double* slot = c->tape_theta + (long long)i * c->g.Ngc;
+    if (c->tape_replay)
+      (void)hipMemcpyAsync(c->theta, slot, bytes, hipMemcpyDeviceToDevice, s);
+    else
+      (void)hipMemcpyAsync(slot, c->theta, bytes, hipMemcpyDeviceToDevice, s);
   }
 
   static bool tapping(const spd_ctx* c, int i) {
@@ -2009,23 +1799,28 @@ template <int NDIM, int P, typename B> struct Stepper {
     if (!c->g.skip) c->g.skip = c->zero_flag;
     const int m = c->cfg.m;
     // --- ADER predictor (src/sdader_simulator.py:160-200)
+    // Output buffer of Picard iteration `it` (0-based): the tape keeps every
+    // iterate; otherwise the fused 2-D iterations alternate between two
+    // buffers and the generic update works in place.
+    const long long it_size = (long long)T * V * c->g.NE * NL;
+    auto dst = [&](int it) -> B* {
+      if (c->tape_on) return (B*)c->tape_uad + it * it_size;
+      if constexpr (NDIM == 2) return (it & 1) ? (B*)c->Uad2 : (B*)c->Uad;
+      return (B*)c->Uad;
+    };
     if constexpr (NDIM == 2) {
-      // fused iterations alternate between the two predictor buffers; the
-      // face kernels run once, on the final state, for the corrector
-      B* buf[2] = {(B*)c->Uad, (B*)c->Uad2};
-      pred2d(c, opb, (const double*)Usp, 1, Usp, dt, buf[0], s);
-      for (int it = 1; it < m; ++it)
-        pred2d(c, opb, (const B*)buf[(it - 1) & 1], T, Usp, dt, buf[it & 1], s);
-      c->Uad_cur = buf[(m - 1) & 1];
+      // the face kernels run once, on the final state, for the corrector
+      pred2d(c, opb, (const double*)Usp, 1, Usp, dt, dst(0), s);
+      for (int it = 1; it < m; ++it) pred2d(c, opb, (const B*)dst(it - 1), T, Usp, dt, dst(it), s);
     } else {
-      c->Uad_cur = c->Uad;
       faces(c, opb, (const double*)Usp, 1, s);
-      pred(c, opb, (const double*)Usp, 1, Usp, dt, Uad(c), nullptr, 0, s);
+      pred(c, opb, (const double*)Usp, 1, Usp, dt, dst(0), nullptr, 0, s);
       for (int it = 1; it < m; ++it) {
-        faces(c, opb, (const B*)Uad(c), T, s);
-        pred(c, opb, (const B*)Uad(c), T, Usp, dt, Uad(c), nullptr, 0, s);
+        faces(c, opb, (const B*)dst(it - 1), T, s);
+        pred(c, opb, (const B*)dst(it - 1), T, Usp, dt, dst(it), nullptr, 0, s);
       }
     }
+    c->Uad_cur = dst(m - 1);
     faces(c, opb, (const B*)Uad(c), T, s);
     if (c->cfg.update == SPD_UPDATE_SD) {
       pred(c, opb, (const B*)Uad(c), T, Usp, dt, nullptr, Usp, 1, s);
@@ -2049,24 +1844,29 @@ template <int NDIM, int P, typename B> struct Stepper {
       hoflux_generic<0>(c, opb, i, s);
       if constexpr (NDIM >= 2) hoflux_generic<1>(c, opb, i, s);
       if constexpr (NDIM >= 3) hoflux_generic<2>(c, opb, i, s);
+      tape_state(c, i, s);
       if (c->cfg.FB) {
         apply(c, dt, w, c->Unew, s);
-        if (c->need_gate) hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
-        c->timed(KC_DETECT, s, [&] {
-          hipLaunchKernelGGL((k_detect<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
-                             c->prm, c->Ucv, c->Unew, c->flags,
-                             c->need_gate ? c->any_flag : nullptr);
-        });
-        double* th_out = c->prm.blending ? c->theta1 : c->theta;
-        c->timed(KC_THETA1, s, [&] {
-          hipLaunchKernelGGL((k_theta1<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
-                             c->prm, c->flags, c->Ucv, c->Unew, c->any_flag, th_out);
-        });
-        if (c->prm.blending)
-          c->timed(KC_RING, s, [&] {
-            hipLaunchKernelGGL((k_ring<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
-                               c->theta1, c->theta);
+        if (!c->tape_replay) {
+          if (c->need_gate)
+            hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
+          c->timed(KC_DETECT, s, [&] {
+            hipLaunchKernelGGL((k_detect<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
+                               c->prm, c->Ucv, c->Unew, c->flags,
+                               c->need_gate ? c->any_flag : nullptr);
           });
+          double* th_out = c->prm.blending ? c->theta1 : c->theta;
+          c->timed(KC_THETA1, s, [&] {
+            hipLaunchKernelGGL((k_theta1<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
+                               c->prm, c->flags, c->Ucv, c->Unew, c->any_flag, th_out);
+          });
+          if (c->prm.blending)
+            c->timed(KC_RING, s, [&] {
+              hipLaunchKernelGGL((k_ring<NDIM>), dim3(grid_for(c->g.Ngc)), dim3(256), 0, s, c->g,
+                                 c->theta1, c->theta);
+            });
+        }
+        tape_theta(c, i, s);
         tap(c, i, s);
         if (c->cfg.riemann_fv != SPD_RS_LLF)
           fb_rs<RS_HLLC>(c, dt, w, s);
