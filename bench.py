@@ -22,6 +22,12 @@ before the timed region.  The JSON line carries:
     (N in {64,128,256,384} resp. {16,32,64}, p in {2..5}, outflow), a fixed
     number of steps per member, batched (multi-branch HIP graph, autotuned
     launch shapes) against the members run one after the other;
+  * adjoint: the reverse-mode derivative on the 3-D Taylor-Green flow
+    (p = 3, FV+FB, HLLC, periodic [0, 2 pi]^3, seeded density perturbation of
+    amplitude 1e-2): gradient of the final kinetic energy with respect to the
+    initial density field through the checkpointed torch.autograd.Function,
+    its time against the forward run, and central differences of the
+    frozen-flag map on random directions;
   * cpu_baseline: the NumPy oracle on a bounded sample, one host core.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
@@ -243,6 +249,120 @@ def run_ensemble(torch, n=64, steps=10, sizes=(64, 128, 256, 384), seed=0):
     return out
 
 
+def smooth_direction(torch, s, rng, modes=6, kmax=2):
+    """A random smooth field at the solution points of ``s`` (element layout):
+    a sum of ``modes`` Fourier modes with integer wave numbers up to ``kmax``
+    over the domain and random phases and amplitudes."""
+    import numpy as np
+    nd, n = s.ndim, s.p + 1
+    X = []
+    for a, d in enumerate(s.dims):
+        lo = s.lim[d][0]
+        e = np.arange(s.N[d])[:, None]
+        x = lo + (e + np.asarray(s.x_sp)[None, :]) * s.h[d]          # (N_a, n)
+        shape = [1] * (2 * nd)
+        shape[nd - 1 - a] = s.N[d]
+        shape[2 * nd - 1 - a] = n
+        X.append(torch.as_tensor(x.reshape(shape), device="cuda"))
+    out = 0.0
+    for _ in range(modes):
+        kv = rng.integers(-kmax, kmax + 1, size=nd)
+        ph = rng.uniform(0, 2 * np.pi)
+        arg = sum(float(kv[a]) * 2 * np.pi / s.len[d] * (X[a] - s.lim[d][0])
+                  for a, d in enumerate(s.dims)) + ph
+        out = out + rng.standard_normal() * torch.cos(arg)
+    return out.contiguous()
+
+
+def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2):
+    """Gradient of the final kinetic energy of the 3-D Taylor-Green flow with
+    respect to the initial density field (``steps`` steps, a checkpoint every
+    ``k``), timed against the forward run; central differences (steps 1e-7, 1e-8
+    and 1e-9 of the density scale, the closest one kept) of the same frozen-flag
+    map (the forward's dt and blending coefficients replayed) on ``n_fd``
+    random smooth directions (white-noise directions put many limiter
+    arguments within the difference step of a kink at this size)."""
+    import math
+    import numpy as np
+    from spd_amd import initial_conditions as ic
+    from spd_amd.adjoint import Adjoint
+    from spd_amd.simulator import SDADER_Simulator
+    L = 2 * math.pi
+    s = SDADER_Simulator(init_fct=ic.taylor_green, p=3, N=(N, N, N), xlim=(0, L), ylim=(0, L),
+                         zlim=(0, L), update="FV", FB=True, riemann_solver_sd="hllc",
+                         riemann_solver_fv="hllc", device="cuda:0", verbose=False)
+    adj = Adjoint(s)
+    U0 = s.dm.device_value("U_sp").clone().contiguous()
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    U0[0] *= 1.0 + 1e-2 * (2 * torch.rand(U0[0].shape, generator=gen, device="cuda",
+                                          dtype=torch.float64) - 1)
+    W0 = s._pointwise("spd_cons_to_prims", U0)
+    gam = s.gamma
+
+    def cons(rho):
+        vel = [W0[1 + a] for a in range(3)]
+        E = W0[4] / (gam - 1) + 0.5 * rho * sum(v * v for v in vel)
+        return torch.stack([rho] + [rho * v for v in vel] + [E]).contiguous()
+
+    def ke_pts(U):
+        return 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / U[0]
+
+    def ke(U):
+        return ke_pts(U).sum()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        UN, _, dts = adj._forward(cons(W0[0]), steps, k, None)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    rho = W0[0].clone().requires_grad_(True)
+    torch.cuda.reset_peak_memory_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = ke(adj.run(cons(rho), steps, checkpoint_every=k))
+    J.backward()
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    peak = torch.cuda.max_memory_allocated()
+    g = rho.grad
+    with torch.no_grad():
+        U = cons(W0[0])
+        thetas = []
+        for st in range(steps):
+            U, th = adj.step(U, dts[st:st + 1], record_theta=True)
+            thetas.append(th)
+
+        def final(r):
+            U = cons(r)
+            for st in range(steps):
+                U = adj.step(U, dts[st:st + 1], theta=thetas[st])
+            return U
+
+        def dJ(d, eps):
+            # pointwise difference first: the sum of 10^7 kinetic energies
+            # would bury the O(eps) difference in its own round-off
+            return float((ke_pts(final(W0[0] + eps * d)) - ke_pts(final(W0[0] - eps * d))).sum())
+
+        rng = np.random.default_rng(5)
+        errs = []
+        for _ in range(n_fd):
+            d = smooth_direction(torch, s, rng)
+            ad = float((g * d).sum())
+            e = [abs(dJ(d, eps) / (2 * eps) - ad) / abs(ad) for eps in (1e-7, 1e-8, 1e-9)]
+            errs.append(min(e))
+    out = {"workload": f"Taylor-Green 3-D {N}^3 p=3 FV+FB HLLC periodic, FP64, density "
+                       f"perturbation 1e-2, d(final kinetic energy)/d(initial density)",
+           "steps": steps, "checkpoint_every": k, "dof": (N * 4) ** 3,
+           "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
+           "peak_torch_GiB": peak / 2 ** 30,
+           "scratch_GiB": float(s._lib.spd_ctx_scratch_bytes(s._ctx)) / 2 ** 30,
+           "fd_directions": n_fd, "fd_rel_err_max": max(errs)}
+    del adj, s
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +377,7 @@ def main():
     ap.add_argument("--cpu-warmup", type=int, default=1)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
 
     if args.cpu_baseline_child:
@@ -365,6 +486,11 @@ def main():
         out["ensemble"] = run_ensemble(torch, steps=args.ensemble_steps)
         out["ensemble_small_members"] = run_ensemble(torch, steps=2 * args.ensemble_steps,
                                                      sizes=(16, 32, 64))
+    if not args.no_adjoint:
+        if "ensemble" not in out:
+            del s
+            torch.cuda.empty_cache()
+        out["adjoint"] = run_adjoint(torch)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline(args.cpu_N, args.cpu_steps, args.cpu_warmup)
     print(json.dumps(out))

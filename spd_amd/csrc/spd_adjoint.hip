@@ -59,6 +59,11 @@ __device__ __forceinline__ Dual sqrt(Dual a) {
   return Dual(s, a.d * 0.5 / s);
 }
 
+// Accumulate into device memory with the hardware FP64 atomic add: every
+// buffer the adjoint adds into is hipMalloc'd (coarse-grained) device memory,
+// for which the instruction is supported (not fine-grained host memory).
+__device__ __forceinline__ void acc_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
 // xbar[k] = sum_j ybar[j] * d f_j / d x_k at x: one forward-mode pass per input.
 template <int NI, int NO, typename F>
 __device__ __forceinline__ void vjp(const double* x, const double* ybar, double* xbar, F&& f) {
@@ -422,7 +427,7 @@ k_adj_hancock(Geom g, Params prm, const double* __restrict__ U, const double* __
       if (flip[a]) wb[a + 1] = -wb[a + 1];
     ad::vjp<V, V>(u, wb, ub, [&](const Dual* ud, Dual* wd) { prims<NDIM, Dual>(ud, wd, 0, gam); });
 #pragma unroll
-    for (int v = 0; v < V; ++v) atomicAdd(&Ub0[v * g.Nc + idx], ub[v]);
+    for (int v = 0; v < V; ++v) ad::acc_add(&Ub0[v * g.Nc + idx], ub[v]);
   }
 }
 
@@ -442,8 +447,9 @@ k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Ua
   constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= g.NE * nf) return;
-  const int f = (int)(tid % nf);
-  const long long e = tid / nf;
+  // adjacent lanes take adjacent elements (their atomic updates never collide)
+  const int f = (int)(tid / g.NE);
+  const long long e = tid % g.NE;
   int c[3] = {0, 0, 0};
   elem_coords<NDIM>(g, e, c);
   const int Na = g.N[a];
@@ -553,7 +559,7 @@ k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Ua
       for (int v = 0; v < V; ++v) {
         double* ub = Uadb + ((long long)(t * V + v) * g.NE + e) * NL + lb;
 #pragma unroll
-        for (int s = 0; s < n; ++s) atomicAdd(&ub[s * sa], op.s2f[f][s] * Mb[v]);
+        for (int s = 0; s < n; ++s) ad::acc_add(&ub[s * sa], op.s2f[f][s] * Mb[v]);
       }
     }
   }
@@ -582,20 +588,30 @@ k_adj_face(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Ts
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= plane * Tsrc) return;
   const int t = (int)(tid / plane);
-  const long long pidx = tid % plane;
+  // thread -> face: the transverse coordinates vary fastest and the face
+  // index along A slowest, so that adjacent lanes read adjacent lines of one
+  // element; pidx is the face's index in the face grid (x fastest)
   int ce[3] = {0, 0, 0}, cl[3] = {0, 0, 0};
   int f = 0;
+  long long pidx = 0;
   {
-    long long x = pidx;
+    long long x = tid % plane;
+    int q[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < NDIM; ++k) {
+      const int b = (k < NDIM - 1) ? (k < a ? k : k + 1) : a;
+      q[b] = (int)(x % ext[b]);
+      x /= ext[b];
+    }
+#pragma unroll
+    for (int b = NDIM - 1; b >= 0; --b) pidx = pidx * ext[b] + q[b];
 #pragma unroll
     for (int b = 0; b < NDIM; ++b) {
-      const int q = (int)(x % ext[b]);
-      x /= ext[b];
       if (b == a) {
-        f = q;
+        f = q[b];
       } else {
-        ce[b] = q / n;
-        cl[b] = q % n;
+        ce[b] = q[b] / n;
+        cl[b] = q[b] % n;
       }
     }
   }
@@ -626,7 +642,7 @@ k_adj_face(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Ts
     for (int v = 0; v < V; ++v) {
       double* ub = Sb + tbase + ((long long)v * g.NE + e) * NL + lb;
 #pragma unroll
-      for (int s = 0; s < n; ++s) atomicAdd(&ub[s * sa], op.s2f[fp][s] * qb[v]);
+      for (int s = 0; s < n; ++s) ad::acc_add(&ub[s * sa], op.s2f[fp][s] * qb[v]);
     }
   };
   const int bl = g.bc[a][0], br = g.bc[a][1];
@@ -747,8 +763,8 @@ k_adj_pred(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Ts
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       double* fb = Ffb + (long long)(t * V + v) * plane + i0;
-      atomicAdd(&fb[0], Fb[0][v]);
-      atomicAdd(&fb[sfa], Fb[nf - 1][v]);
+      ad::acc_add(&fb[0], Fb[0][v]);
+      ad::acc_add(&fb[sfa], Fb[nf - 1][v]);
     }
   }
   double u[V][n], ub[V][n];
@@ -784,7 +800,7 @@ k_adj_pred(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Ts
   for (int v = 0; v < V; ++v)
 #pragma unroll
     for (int s = 0; s < n; ++s)
-      atomicAdd(&Sb[tbase + ((long long)v * g.NE + e) * NL + lb + s * sa], ub[v][s]);
+      ad::acc_add(&Sb[tbase + ((long long)v * g.NE + e) * NL + lb + s * sa], ub[v][s]);
 }
 
 // ---------------------------------------------------------------------------
