@@ -357,7 +357,7 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2):
            "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
            "peak_torch_GiB": peak / 2 ** 30,
            "scratch_GiB": float(s._lib.spd_ctx_scratch_bytes(s._ctx)) / 2 ** 30,
-           "fd_directions": n_fd, "fd_rel_err_max": max(errs)}
+           "fd_directions": n_fd, "fd_rel_err_max": max(errs) if errs else None}
     del adj, s
     torch.cuda.empty_cache()
     return out
@@ -378,6 +378,12 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint leg (for profiling) and print its JSON")
+    ap.add_argument("--adjoint-N", type=int, default=64)
+    ap.add_argument("--adjoint-steps", type=int, default=20)
+    ap.add_argument("--adjoint-k", type=int, default=5)
+    ap.add_argument("--adjoint-fd", type=int, default=2)
     args = ap.parse_args()
 
     if args.cpu_baseline_child:
@@ -398,6 +404,10 @@ def main():
     N, p = args.N, args.p
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": run_adjoint(torch, args.adjoint_N, args.adjoint_steps,
+                                                 args.adjoint_k, args.adjoint_fd)}))
+        return
     dof = (N * (p + 1)) ** 2
     noise = ic.subcell_noise((N, N), p, [1, 2], 1e-3, seed=0)
 
@@ -490,7 +500,8 @@ def main():
         if "ensemble" not in out:
             del s
             torch.cuda.empty_cache()
-        out["adjoint"] = run_adjoint(torch)
+        out["adjoint"] = run_adjoint(torch, args.adjoint_N, args.adjoint_steps, args.adjoint_k,
+                                     args.adjoint_fd)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline(args.cpu_N, args.cpu_steps, args.cpu_warmup)
     print(json.dumps(out))
