@@ -84,6 +84,9 @@ __device__ __forceinline__ void vjp(const double* x, const double* ybar, double*
 
 using ad::Dual;
 
+// Dynamic LDS of the reverse kernels (typed at each use).
+extern __shared__ __attribute__((aligned(16))) unsigned char spd_adj_lds[];
+
 // ---------------------------------------------------------------------------
 // Transpose of the tensor-product transform with the n x n matrix M (which: 0
 // cv_to_sp, 1 sp_to_cv) on every element: out[qo] = sum_qi prod_b M[qi_b][qo_b]
@@ -398,12 +401,41 @@ k_adj_hancock(Geom g, Params prm, const double* __restrict__ U, const double* __
   if (!any) return;
   double x[NI], xb[NI];
   load_stencil<NDIM>(U, g, prm.gamma, c, x);
+  // The map factors as slopes dM[b][v] = slope(W[v], Wm[b][v], Wp[b][v]),
+  // S[b] = dM[b] hf_b / 2, Mp = predict(W, dM): the predictor's Jacobian
+  // from (1 + d) V dual passes, each slope's from three scalar passes.
   const Dual dti((*dt_dev) * w_i);
-  ad::vjp<NI, NO>(x, yb, xb, [&](const Dual* xd, Dual* yd) {
-    hancock_state<NDIM, Dual>(g, prm, c, xd, (const Dual(*)[V])(xd + V),
-                              (const Dual(*)[V])(xd + (1 + NDIM) * V), dti, yd,
-                              (Dual(*)[V])(yd + V));
+  double pin[NO], pb[NO];
+#pragma unroll
+  for (int v = 0; v < V; ++v) pin[v] = x[v];
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b)
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      pin[(1 + b) * V + v] = hancock_slope<double>(g, prm, b, c[b], x[v], x[(1 + b) * V + v],
+                                                   x[(1 + NDIM + b) * V + v]);
+  ad::vjp<NO, V>(pin, yb, pb, [&](const Dual* xd, Dual* yd) {
+    hancock_predict<NDIM, Dual>(prm, xd, (const Dual(*)[V])(xd + V), dti, yd);
   });
+#pragma unroll
+  for (int v = 0; v < V; ++v) xb[v] = pb[v];
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    const double hf = g.hfp[b][c[b] + NGH];
+    const int cb = c[b];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const double mb = pb[(1 + b) * V + v] + 0.5 * hf * yb[(1 + b) * V + v];
+      const double s3[3] = {x[v], x[(1 + b) * V + v], x[(1 + NDIM + b) * V + v]};
+      double g3[3];
+      ad::vjp<3, 1>(s3, &mb, g3, [&](const Dual* xd, Dual* yd) {
+        yd[0] = hancock_slope<Dual>(g, prm, b, cb, xd[0], xd[1], xd[2]);
+      });
+      xb[v] += g3[0];
+      xb[(1 + b) * V + v] = g3[1];
+      xb[(1 + NDIM + b) * V + v] = g3[2];
+    }
+  }
   // W of each stencil cell = flip(prims(U[map(cell)])): transpose into Ub0
   const Dual gam(prm.gamma);
 #pragma unroll 1
@@ -432,99 +464,49 @@ k_adj_hancock(Geom g, Params prm, const double* __restrict__ U, const double* __
 }
 
 // ---------------------------------------------------------------------------
-// High-order FV fluxes of node i -> predictor: transpose of the transverse
-// subcell averages and of the flux-point evaluation (k_hoflux / k_ho2d).
-// Interior flux points: Euler-flux cotangent into the node's iterate
-// cotangent Uadb; end points: the element-face flux cotangent Ffb (every
-// element face of the node is written exactly once).
+// High-order FV fluxes of node t -> predictor: transpose of the transverse
+// subcell averages and of the flux-point evaluation (k_hoflux / k_ho2d), all
+// dimensions in one launch.  A block owns EPB elements and accumulates their
+// iterate cotangent in an LDS slab; one work item per (element, line along a)
+// walks the line's flux points: interior points through the Euler flux (one
+// dual pass per input) into the line's solution points, end points into the
+// element-face cotangent Ffb (every element face of the node is written
+// exactly once).  Dimensions are processed one after the other, so the items
+// of one dimension own disjoint points: no atomics.  The node's slice of
+// Uadb is written once, at the end (nothing else writes it in the substep
+// sweep).
 // ---------------------------------------------------------------------------
-template <int NDIM, int P, int A>
+template <int NDIM, int P>
 __global__ void __launch_bounds__(256)
 k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
-             const double* __restrict__ Fhob, double* __restrict__ Uadb, double* __restrict__ Ffb) {
-  constexpr int a = A;
+             const double* __restrict__ Fhob0, const double* __restrict__ Fhob1,
+             const double* __restrict__ Fhob2, double* __restrict__ Uadb,
+             double* __restrict__ Ffb0, double* __restrict__ Ffb1, double* __restrict__ Ffb2,
+             int EPB) {
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
   constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= g.NE * nf) return;
-  // adjacent lanes take adjacent elements (their atomic updates never collide)
-  const int f = (int)(tid / g.NE);
-  const long long e = tid % g.NE;
-  int c[3] = {0, 0, 0};
-  elem_coords<NDIM>(g, e, c);
-  const int Na = g.N[a];
-  if (f == nf - 1 && c[a] != Na - 1) return;
-  long long ext[3];
-#pragma unroll
-  for (int b = 0; b < 3; ++b) ext[b] = (b < NDIM) ? (g.Ns[b] + (b == a ? 1 : 0)) : 1;
-  long long nface = 1;
-#pragma unroll
-  for (int b = 0; b < NDIM; ++b) nface *= ext[b];
-  double Fa[NLP][V];
-#pragma unroll
-  for (int lp = 0; lp < NLP; ++lp) {
-    int q[3] = {0, 0, 0};
-    int r = lp;
-#pragma unroll
-    for (int b = 0; b < NDIM; ++b) {
-      if (b == a) continue;
-      q[b] = r % n;
-      r /= n;
-    }
-    long long idx = 0;
-#pragma unroll
-    for (int b = NDIM - 1; b >= 0; --b) {
-      const long long cb = (b == a) ? (long long)c[a] * n + f : (long long)c[b] * n + q[b];
-      idx = idx * ext[b] + cb;
-    }
-#pragma unroll
-    for (int v = 0; v < V; ++v) Fa[lp][v] = Fhob[v * nface + idx];
-  }
-  double Fl[NLP][V];
-  if constexpr (NDIM == 1) {
-#pragma unroll
-    for (int v = 0; v < V; ++v) Fl[0][v] = Fa[0][v];
-  } else if constexpr (NDIM == 2) {
-#pragma unroll
-    for (int s = 0; s < n; ++s)
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        double acc = 0.0;
-#pragma unroll
-        for (int q = 0; q < n; ++q) acc += op.s2c[q][s] * Fa[q][v];
-        Fl[s][v] = acc;
-      }
-  } else {
-    double G[NLP][V];
-#pragma unroll
-    for (int s2 = 0; s2 < n; ++s2)
-#pragma unroll
-      for (int q1 = 0; q1 < n; ++q1)
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          double acc = 0.0;
-#pragma unroll
-          for (int q2 = 0; q2 < n; ++q2) acc += op.s2c[q2][s2] * Fa[q1 + n * q2][v];
-          G[q1 + n * s2][v] = acc;
-        }
-#pragma unroll
-    for (int c2 = 0; c2 < n; ++c2)
-#pragma unroll
-      for (int s = 0; s < n; ++s)
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          double acc = 0.0;
-#pragma unroll
-          for (int q1 = 0; q1 < n; ++q1) acc += op.s2c[q1][s] * G[q1 + n * c2][v];
-          Fl[s + n * c2][v] = acc;
-        }
-  }
-  const int sa = stride_of<n>(a);
+  double* slab = reinterpret_cast<double*>(spd_adj_lds);  // [EPB][V][NL]
+  const long long e0 = (long long)blockIdx.x * EPB;
+  const int ne = (int)((g.NE - e0 < EPB) ? (g.NE - e0) : EPB);
+  for (int k = threadIdx.x; k < ne * V * NL; k += blockDim.x) slab[k] = 0.0;
+  __syncthreads();
+  const double* Fhob[3] = {Fhob0, Fhob1, Fhob2};
+  double* Ffb[3] = {Ffb0, Ffb1, Ffb2};
   const Dual gam(prm.gamma);
 #pragma unroll 1
-  for (int lp = 0; lp < NLP; ++lp) {
-    if (f == 0 || f == nf - 1) {
-      const int fa = c[a] + (f == 0 ? 0 : 1);
+  for (int a = 0; a < NDIM; ++a) {
+    long long ext[3] = {1, 1, 1}, nface = 1;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      ext[b] = g.Ns[b] + (b == a ? 1 : 0);
+      nface *= ext[b];
+    }
+    const int sa = stride_of<n>(a);
+    for (int item = threadIdx.x; item < ne * NLP; item += blockDim.x) {
+      const int el = item / NLP, lp = item % NLP;
+      const long long e = e0 + el;
+      int c[3] = {0, 0, 0};
+      elem_coords<NDIM>(g, e, c);
       int cl[3] = {0, 0, 0};
       {
         int r = lp;
@@ -535,33 +517,80 @@ k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Ua
           r /= n;
         }
       }
-      long long plane;
-      const long long i0 = face_grid_base<NDIM, n>(g, a, c, cl, fa, &plane);
-#pragma unroll
-      for (int v = 0; v < V; ++v) Ffb[(long long)(t * V + v) * plane + i0] = Fl[lp][v];
-    } else {
       const int lb = line_base<NDIM, n>(a, lp);
-      double M[V], Mb[V];
+      double u[V][n], ub[V][n];
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        const double* u = Uad + ((long long)(t * V + v) * g.NE + e) * NL + lb;
-        double acc = 0.0;
+      for (int v = 0; v < V; ++v)
 #pragma unroll
-        for (int s = 0; s < n; ++s) acc += op.s2f[f][s] * u[s * sa];
-        M[v] = acc;
+        for (int s2 = 0; s2 < n; ++s2) {
+          u[v][s2] = Uad[((long long)(t * V + v) * g.NE + e) * NL + lb + s2 * sa];
+          ub[v][s2] = 0.0;
+        }
+#pragma unroll 1
+      for (int f = 0; f < nf; ++f) {
+        if (f == nf - 1 && c[a] != g.N[a] - 1) continue;
+        // cotangent of the flux at (f, line lp): transverse-average transpose,
+        // sum over the transverse subcells q of prod_b s2c[q_b][lp_b] Fhob[q]
+        double Fl[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) Fl[v] = 0.0;
+        for (int q = 0; q < NLP; ++q) {
+          int qq[3] = {0, 0, 0};
+          double w = 1.0;
+          int r = q;
+#pragma unroll
+          for (int b = 0; b < NDIM; ++b) {
+            if (b == a) continue;
+            qq[b] = r % n;
+            r /= n;
+            w *= op.s2c[qq[b]][cl[b]];
+          }
+          long long idx = 0;
+#pragma unroll
+          for (int b = NDIM - 1; b >= 0; --b) {
+            const long long cb = (b == a) ? (long long)c[a] * n + f : (long long)c[b] * n + qq[b];
+            idx = idx * ext[b] + cb;
+          }
+#pragma unroll
+          for (int v = 0; v < V; ++v) Fl[v] += w * Fhob[a][v * nface + idx];
+        }
+        if (f == 0 || f == nf - 1) {
+          long long plane;
+          const long long i0 = face_grid_base<NDIM, n>(g, a, c, cl, c[a] + (f == 0 ? 0 : 1),
+                                                       &plane);
+#pragma unroll
+          for (int v = 0; v < V; ++v) Ffb[a][(long long)(t * V + v) * plane + i0] = Fl[v];
+        } else {
+          double M[V], Mb[V];
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            double acc = 0.0;
+#pragma unroll
+            for (int s2 = 0; s2 < n; ++s2) acc += op.s2f[f][s2] * u[v][s2];
+            M[v] = acc;
+          }
+          ad::vjp<V, V>(M, Fl, Mb, [&](const Dual* xd, Dual* yd) {
+            Dual W[V];
+            prims<NDIM, Dual>(xd, W, a, gam);
+            flux<NDIM, Dual>(W, yd, a, gam);
+          });
+#pragma unroll
+          for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int s2 = 0; s2 < n; ++s2) ub[v][s2] += op.s2f[f][s2] * Mb[v];
+        }
       }
-      ad::vjp<V, V>(M, Fl[lp], Mb, [&](const Dual* xd, Dual* yd) {
-        Dual W[V];
-        prims<NDIM, Dual>(xd, W, a, gam);
-        flux<NDIM, Dual>(W, yd, a, gam);
-      });
+      double* sl = slab + (el * V) * NL + lb;
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        double* ub = Uadb + ((long long)(t * V + v) * g.NE + e) * NL + lb;
+      for (int v = 0; v < V; ++v)
 #pragma unroll
-        for (int s = 0; s < n; ++s) ad::acc_add(&ub[s * sa], op.s2f[f][s] * Mb[v]);
-      }
+        for (int s2 = 0; s2 < n; ++s2) sl[v * NL + s2 * sa] += ub[v][s2];
     }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < ne * V * NL; k += blockDim.x) {
+    const int el = k / (V * NL), v = (k / NL) % V, l = k % NL;
+    Uadb[((long long)(t * V + v) * g.NE + e0 + el) * NL + l] = slab[k];
   }
 }
 
@@ -994,10 +1023,18 @@ template <int NDIM, int P> struct Adjoint {
     if constexpr (NDIM >= 2) fvface<RS, 1>(c, Ub1, th, dt, w, s);
     if constexpr (NDIM >= 3) fvface<RS, 2>(c, Ub1, th, dt, w, s);
   }
-  template <int A>
+  // elements per block of k_adj_hoflux: 256 lines per dimension, a slab of at
+  // most 40 KiB
+  static int hof_epb() {
+    return std::max(1, std::min(256 / NLP, 40960 / (V * NL * (int)sizeof(double))));
+  }
   static void hoflux(spd_ctx* c, const Ops<P>& op, int i, const double* Uad, hipStream_t s) {
-    hipLaunchKernelGGL((k_adj_hoflux<NDIM, P, A>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s,
-                       op, c->g, c->prm, i, Uad, c->adj->Fhob[A], c->adj->Uadb, c->adj->Ffb[A]);
+    AdjScratch* Ad = c->adj;
+    const int E = hof_epb();
+    hipLaunchKernelGGL((k_adj_hoflux<NDIM, P>), dim3((unsigned)((c->g.NE + E - 1) / E)),
+                       dim3(256), (size_t)E * V * NL * sizeof(double), s, op, c->g, c->prm, i, Uad,
+                       Ad->Fhob[0], Ad->Fhob[1], Ad->Fhob[2], Ad->Uadb, Ad->Ffb[0], Ad->Ffb[1],
+                       Ad->Ffb[2], E);
   }
 
   static int backward(spd_ctx* c, const double* Usp, const double* dt, double* Ub,
@@ -1035,9 +1072,7 @@ template <int NDIM, int P> struct Adjoint {
           hipLaunchKernelGGL((k_adj_hancock<NDIM>), dim3(grid_for(n1_of(c))), dim3(256), 0, s,
                              c->g, c->prm, Ui, th, dt, w, Ad->WbL[0], Ad->WbL[1], Ad->WbL[2],
                              Ad->WbR[0], Ad->WbR[1], Ad->WbR[2], Ub0);
-        hoflux<0>(c, op, i, iterate(m), s);
-        if constexpr (NDIM >= 2) hoflux<1>(c, op, i, iterate(m), s);
-        if constexpr (NDIM >= 3) hoflux<2>(c, op, i, iterate(m), s);
+        hoflux(c, op, i, iterate(m), s);
         cur = 1 - cur;
       }
       // U^0 = sp_to_cv(U_sp)

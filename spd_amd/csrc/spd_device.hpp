@@ -341,34 +341,27 @@ __device__ __forceinline__ void load_W(const double* __restrict__ U, const Geom&
     if (flip[a]) W[a + 1] = -W[a + 1];
 }
 
-// MUSCL-Hancock state of one cell of the one-layer-ghosted grid
-// (src/muscl.py:61-75,129-221; Q1, Q7, Q19): limited slopes of the primitives
-// W from the neighbours Wm[b] (c - e_b) and Wp[b] (c + e_b), the half slopes
-// S[b] = slope * width / 2 and the half-step predicted state Mp with the
-// pressure row of the time derivative zeroed.  `c` is the cell's coordinate
-// (active range [0, Ns)).
+// Limited slope of one variable along dimension b of the cell at coordinate
+// c (src/muscl.py:61-75; Q7): from the cell's value w and its neighbours'
+// wm (c - e_b) and wp (c + e_b).
+template <typename R>
+__device__ __forceinline__ R hancock_slope(const Geom& g, const Params& prm, int b, int cb, R w,
+                                           R wm, R wp) {
+  const int G = cb + NGH;
+  const R L = (w - wm) * (R)g.rhcv[b][G - 1];
+  const R Rr = (wp - w) * (R)g.rhcv[b][G];
+  return prm.moncen ? moncen<R>(L, Rr, (R)g.hcv[b][G - 1], (R)g.hcv[b][G], (R)g.hfp[b][G])
+                    : minmod<R>(L, Rr);
+}
+
+// Half-step predicted state from the primitives W and the limited slopes
+// dMh[b] (src/muscl.py:129-173; Q1: pressure row zeroed, Q19: the transverse
+// loop over the unrolled velocity list).
 template <int NDIM, typename R>
-__device__ __forceinline__ void hancock_state(const Geom& g, const Params& prm, const int* c,
-                                              const R* W, const R (*Wm)[Dims<NDIM>::V],
-                                              const R (*Wp)[Dims<NDIM>::V], R dti, R* Mp,
-                                              R (*S)[Dims<NDIM>::V]) {
+__device__ __forceinline__ void hancock_predict(const Params& prm, const R* W,
+                                                const R (*dMh)[Dims<NDIM>::V], R dti, R* Mp) {
   constexpr int V = Dims<NDIM>::V, P = Dims<NDIM>::P;
   const R gam = (R)prm.gamma;
-  R dMh[NDIM][V];
-#pragma unroll
-  for (int b = 0; b < NDIM; ++b) {
-    const int G = c[b] + NGH;
-    const R rhl = (R)g.rhcv[b][G - 1], rhr = (R)g.rhcv[b][G];
-    const R hf = (R)g.hfp[b][G];
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const R L = (W[v] - Wm[b][v]) * rhl;
-      const R Rr = (Wp[b][v] - W[v]) * rhr;
-      dMh[b][v] = prm.moncen ? moncen<R>(L, Rr, (R)g.hcv[b][G - 1], (R)g.hcv[b][G], hf)
-                             : minmod<R>(L, Rr);
-      S[b][v] = R(0.5) * dMh[b][v] * hf;
-    }
-  }
   R dtW[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) dtW[v] = R(0);
@@ -385,6 +378,30 @@ __device__ __forceinline__ void hancock_state(const Geom& g, const Params& prm, 
   dtW[P] = dtW[P] * R(0);
 #pragma unroll
   for (int v = 0; v < V; ++v) Mp[v] = W[v] + R(0.5) * dtW[v] * dti;
+}
+
+// MUSCL-Hancock state of one cell of the one-layer-ghosted grid
+// (src/muscl.py:61-75,129-221): limited slopes of the primitives W from the
+// neighbours Wm[b] (c - e_b) and Wp[b] (c + e_b), the half slopes
+// S[b] = slope * width / 2 and the half-step predicted state Mp.  `c` is the
+// cell's coordinate (active range [0, Ns)).
+template <int NDIM, typename R>
+__device__ __forceinline__ void hancock_state(const Geom& g, const Params& prm, const int* c,
+                                              const R* W, const R (*Wm)[Dims<NDIM>::V],
+                                              const R (*Wp)[Dims<NDIM>::V], R dti, R* Mp,
+                                              R (*S)[Dims<NDIM>::V]) {
+  constexpr int V = Dims<NDIM>::V;
+  R dMh[NDIM][V];
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {
+    const R hf = (R)g.hfp[b][c[b] + NGH];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      dMh[b][v] = hancock_slope<R>(g, prm, b, c[b], W[v], Wm[b][v], Wp[b][v]);
+      S[b][v] = R(0.5) * dMh[b][v] * hf;
+    }
+  }
+  hancock_predict<NDIM, R>(prm, W, dMh, dti, Mp);
 }
 
 }  // namespace spd
