@@ -341,6 +341,23 @@ __device__ __forceinline__ void load_W(const double* __restrict__ U, const Geom&
     if (flip[a]) W[a + 1] = -W[a + 1];
 }
 
+// The same value from a precomputed primitive array Wp (V, Nc) of the state.
+template <int NDIM>
+__device__ __forceinline__ void load_prim(const double* __restrict__ Wp, const Geom& g,
+                                          const int* c, double* W) {
+  constexpr int V = Dims<NDIM>::V;
+  int m[3];
+  bool flip[3] = {false, false, false};
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a) m[a] = ghost_map(c[a], g.Ns[a], g.bc[a], &flip[a]);
+  const long long idx = cell_index<NDIM>(g, m);
+#pragma unroll
+  for (int v = 0; v < V; ++v) W[v] = Wp[v * g.Nc + idx];
+#pragma unroll
+  for (int a = 0; a < NDIM; ++a)
+    if (flip[a]) W[a + 1] = -W[a + 1];
+}
+
 // Limited slope of one variable along dimension b of the cell at coordinate
 // c (src/muscl.py:61-75; Q7): from the cell's value w and its neighbours'
 // wm (c - e_b) and wp (c + e_b).

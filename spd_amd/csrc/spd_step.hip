@@ -742,26 +742,62 @@ k_apply(Geom g, const double* Uin, const B* __restrict__ F0, const B* __restrict
 // smooth-extrema relaxation (p > 1), PAD bounds.  Sets *any when any variable
 // of any cell is a NAD candidate (the reference's global gate, Q3).
 // ---------------------------------------------------------------------------
+// Primitives of the FV state entering a substep (V x Nc), one thread per
+// active cell: detection and the Hancock states read them on their stencils.
 template <int NDIM>
-__device__ __forceinline__ double sed_alpha(const double* __restrict__ Unew, const Geom& g,
-                                            const int* c, int a, int var, double gamma) {
+__global__ void __launch_bounds__(256)
+k_prims_cells(Geom g, Params prm, const double* __restrict__ U, double* __restrict__ W) {
+  SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V;
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= g.Nc) return;
+  double u[V], w[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) u[v] = U[v * g.Nc + cell];
+  prims<NDIM>(u, w, 0, prm.gamma);
+#pragma unroll
+  for (int v = 0; v < V; ++v) W[v * g.Nc + cell] = w[v];
+}
+
+// Density and pressure of the candidate state, one thread per active cell:
+// the smooth-extrema test reads them up to three cells away along every
+// dimension (rp: 2 x Nc).
+template <int NDIM>
+__global__ void __launch_bounds__(256)
+k_rho_p(Geom g, Params prm, const double* __restrict__ Unew, double* __restrict__ rp) {
+  SPD_SKIP(g);
+  constexpr int V = Dims<NDIM>::V, P = Dims<NDIM>::P;
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= g.Nc) return;
+  double u[V], W[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) u[v] = Unew[v * g.Nc + cell];
+  prims<NDIM>(u, W, 0, prm.gamma);
+  rp[cell] = W[0];
+  rp[g.Nc + cell] = W[P];
+}
+
+// alpha of the smooth-extrema relaxation of variable k (0: density, 1:
+// pressure, read from rp) at the active cell c along dimension a.
+template <int NDIM>
+__device__ __forceinline__ double sed_alpha(const double* __restrict__ rp, const Geom& g,
+                                            const int* c, int a, int k) {
   const int Ns = g.Ns[a];
   const int ca = c[a];
   const double* rdc = g.rdc[a] + NGH;  // 1 / (centre spacing over two cells), active-relative
   const double* hf = g.hfp[a] + NGH;
+  const double* r = rp + k * g.Nc;
   double w[7];
 #pragma unroll
-  for (int k = -3; k <= 3; ++k) {
-    const int x = ca + k;
+  for (int k2 = -3; k2 <= 3; ++k2) {
+    const int x = ca + k2;
     if (x >= -NGH && x < Ns + NGH) {
       int cc[3] = {c[0], c[1], c[2]};
-      cc[a] = x;
-      double W[V];
-      load_W<NDIM>(Unew, g, cc, W, gamma);
-      w[k + 3] = W[var];
+      bool fl = false;
+      cc[a] = ghost_map(x, Ns, g.bc[a], &fl);  // rho and P are even under the flips
+      w[k2 + 3] = r[cell_index<NDIM>(g, cc)];
     } else {
-      w[k + 3] = 0.0;
+      w[k2 + 3] = 0.0;
     }
   }
   double dU[5];  // relative -2..2
@@ -803,8 +839,8 @@ __device__ __forceinline__ double sed_alpha(const double* __restrict__ Unew, con
 
 template <int NDIM>
 __global__ void __launch_bounds__(256)
-k_detect(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ Unew,
-         uint8_t* __restrict__ flags, int* __restrict__ any_flag) {
+k_detect(Geom g, Params prm, const double* __restrict__ Wold, const double* __restrict__ Unew,
+         const double* __restrict__ rp, uint8_t* __restrict__ flags, int* __restrict__ any_flag) {
   SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V, P = Dims<NDIM>::P;
   const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -821,7 +857,7 @@ k_detect(Geom g, Params prm, const double* __restrict__ Ucv, const double* __res
     }
     double Wn[V], Wo[V], mx[V], mn[V];
     load_W<NDIM>(Unew, g, c, Wn, prm.gamma);
-    load_W<NDIM>(Ucv, g, c, Wo, prm.gamma);
+    load_prim<NDIM>(Wold, g, c, Wo);
 #pragma unroll
     for (int v = 0; v < V; ++v) mx[v] = mn[v] = Wo[v];
 #pragma unroll
@@ -831,7 +867,7 @@ k_detect(Geom g, Params prm, const double* __restrict__ Ucv, const double* __res
         int cc[3] = {c[0], c[1], c[2]};
         cc[a] += s;
         double Wb[V];
-        load_W<NDIM>(Ucv, g, cc, Wb, prm.gamma);
+        load_prim<NDIM>(Wold, g, cc, Wb);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           mx[v] = nmax(mx[v], Wb[v]);
@@ -864,7 +900,7 @@ k_detect(Geom g, Params prm, const double* __restrict__ Ucv, const double* __res
         double alpha = 1.0;
 #pragma unroll
         for (int a = 0; a < NDIM; ++a) {
-          const double an = sed_alpha<NDIM>(Unew, g, c, a, var, prm.gamma);
+          const double an = sed_alpha<NDIM>(rp, g, c, a, k);
           alpha = (an < alpha) ? an : alpha;
         }
         pt[var] = pt[var] && (alpha < 1.0);
@@ -1086,7 +1122,7 @@ namespace spd {
 // [Mp (V), S_0 (V), ..., S_{NDIM-1} (V)] on the grid (Ns+2)^NDIM.
 template <int NDIM, typename B>
 __global__ void __launch_bounds__(256)
-k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __restrict__ theta,
+k_hancock(Geom g, Params prm, const double* __restrict__ Wold, const double* __restrict__ theta,
           const double* __restrict__ dt_dev, double w_i, B* __restrict__ HS) {
   SPD_SKIP(g);
   constexpr int V = Dims<NDIM>::V;
@@ -1140,7 +1176,7 @@ k_hancock(Geom g, Params prm, const double* __restrict__ Ucv, const double* __re
     // primitives from the FP64 state (FP64 pressure recovery)
     auto load = [&](const int* cc, B* Wb) {
       double Wd[V];
-      load_W<NDIM>(Ucv, g, cc, Wd, prm.gamma);
+      load_prim<NDIM>(Wold, g, cc, Wd);
 #pragma unroll
       for (int v = 0; v < V; ++v) Wb[v] = (B)Wd[v];
     };
@@ -1640,7 +1676,7 @@ template <int NDIM, int P, typename B> struct Stepper {
     for (int b = 0; b < NDIM; ++b) n1 *= c->g.Ns[b] + 2;
     c->timed(KC_HANCOCK, s, [&] {
       hipLaunchKernelGGL((k_hancock<NDIM, B>), dim3(grid_for(n1)), dim3(256), 0, s, c->g, c->prm,
-                         c->Ucv, c->theta, dt, w, (B*)c->HS);
+                         c->Wold, c->theta, dt, w, (B*)c->HS);
     });
     fbface_one<RS, 0>(c, s);
     if constexpr (NDIM >= 2) fbface_one<RS, 1>(c, s);
@@ -1847,12 +1883,19 @@ template <int NDIM, int P, typename B> struct Stepper {
       tape_state(c, i, s);
       if (c->cfg.FB) {
         apply(c, dt, w, c->Unew, s);
+        c->timed(KC_DETECT, s, [&] {
+          hipLaunchKernelGGL((k_prims_cells<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s,
+                             c->g, c->prm, c->Ucv, c->Wold);
+        });
         if (!c->tape_replay) {
           if (c->need_gate)
             hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
           c->timed(KC_DETECT, s, [&] {
+            if (c->prm.p > 1 && c->prm.sed)
+              hipLaunchKernelGGL((k_rho_p<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s,
+                                 c->g, c->prm, c->Unew, c->rp);
             hipLaunchKernelGGL((k_detect<NDIM>), dim3(grid_for(c->g.Nc)), dim3(256), 0, s, c->g,
-                               c->prm, c->Ucv, c->Unew, c->flags,
+                               c->prm, c->Wold, c->Unew, c->rp, c->flags,
                                c->need_gate ? c->any_flag : nullptr);
           });
           double* th_out = c->prm.blending ? c->theta1 : c->theta;
@@ -2145,6 +2188,8 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
       long long n1 = 1;
       for (int a = 0; a < nd; ++a) n1 *= g.Ns[a] + 2;
       if (nd != 2 && (rc = alloc_bulk(&c->HS, (1 + nd) * V * n1)) != SPD_OK) return rc;
+      if (nd != 2 && (rc = c->alloc(&c->rp, (size_t)(2 * g.Nc))) != SPD_OK) return rc;
+      if (nd != 2 && (rc = c->alloc(&c->Wold, (size_t)(V * g.Nc))) != SPD_OK) return rc;
     }
   }
   if ((rc = c->alloc(&c->any_flag, 1)) != SPD_OK) return rc;
