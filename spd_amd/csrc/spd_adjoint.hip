@@ -595,15 +595,17 @@ k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Ua
 }
 
 // ---------------------------------------------------------------------------
-// Element faces of dimension A (k_face_sd): the flux cotangent Ffb through the
-// Riemann solver and prims to the two traces, then through the boundary
-// traces (periodic wrap, own trace, reflective flip) and the sp -> fp
-// interpolation into the source cotangent Sb (Tsrc time nodes).
+// Element faces of dimension A (k_face_sd), first half: the flux cotangent
+// through the Riemann solver and prims to the face's two traces; a boundary
+// trace copied from the element's own trace (outflow, reflective) hands its
+// cotangent (flipped back) to that trace.  The left trace's cotangent
+// replaces Ffb in place (each thread reads and writes its own face), the
+// right trace's goes to Qr; k_adj_trace gathers them into the lines.
 // ---------------------------------------------------------------------------
 template <int NDIM, int P, int RS, int A>
 __global__ void __launch_bounds__(256)
 k_adj_face(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
-           const double* __restrict__ Ffb, double* __restrict__ Sb) {
+           double* __restrict__ Ffb, double* __restrict__ Qr) {
   constexpr int a = A;
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
   constexpr int NL = IPow<n, NDIM>::v;
@@ -665,15 +667,6 @@ k_adj_face(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Ts
       q[v] = acc;
     }
   };
-  auto scatter = [&](int ea, int fp, const double* qb) {
-    const long long e = elem(ea);
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      double* ub = Sb + tbase + ((long long)v * g.NE + e) * NL + lb;
-#pragma unroll
-      for (int s = 0; s < n; ++s) ad::acc_add(&ub[s * sa], op.s2f[fp][s] * qb[v]);
-    }
-  };
   const int bl = g.bc[a][0], br = g.bc[a][1];
   double x[2 * V];
   double* qL = x;
@@ -728,14 +721,70 @@ k_adj_face(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Ts
       qRb[v] = 0.0;
     }
   }
-  if (f > 0)
-    scatter(f - 1, nf - 1, qLb);
-  else if (bl == BC_PERIODIC)
-    scatter(Na - 1, nf - 1, qLb);
-  if (f < Na)
-    scatter(f, 0, qRb);
-  else if (br == BC_PERIODIC)
-    scatter(0, 0, qRb);
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const long long k = (long long)(t * V + v) * plane + pidx;
+    Ffb[k] = qLb[v];
+    Qr[k] = qRb[v];
+  }
+}
+
+// Second half: per (node, element, line along A), the cotangents of the
+// line's two traces -- the left state of the face above it and the right
+// state of the face below it, plus on periodic sides the wrapped copies at
+// the domain's first / last face -- through the sp -> fp interpolation into
+// the source cotangent Sb.  Each work item owns its line's points; launches
+// on one stream are ordered, so the update needs no atomics.
+template <int NDIM, int P, int A>
+__global__ void __launch_bounds__(256)
+k_adj_trace(Ops<P> op, Geom g, int Tsrc, const double* __restrict__ QlF,
+            const double* __restrict__ Qr, double* __restrict__ Sb) {
+  constexpr int a = A;
+  constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
+  constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long long)Tsrc * g.NE * NLP) return;
+  const int lp = (int)(tid % NLP);
+  const long long e = (tid / NLP) % g.NE;
+  const int t = (int)(tid / ((long long)NLP * g.NE));
+  int c[3] = {0, 0, 0};
+  elem_coords<NDIM>(g, e, c);
+  int cl[3] = {0, 0, 0};
+  {
+    int r = lp;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      if (b == a) continue;
+      cl[b] = r % n;
+      r /= n;
+    }
+  }
+  const int Na = g.N[a];
+  long long plane;
+  const long long i0 = face_grid_base<NDIM, n>(g, a, c, cl, c[a], &plane);
+  const long long sfa = stride_face<NDIM, n>(g, a);
+  // a periodic first face takes its left trace from the last element, a
+  // periodic last face its right trace from the first
+  const bool wrapL = g.bc[a][0] == BC_PERIODIC && c[a] == Na - 1;
+  const bool wrapR = g.bc[a][1] == BC_PERIODIC && c[a] == 0;
+  double rb[V], lb_[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const long long k = (long long)(t * V + v) * plane + i0;
+    rb[v] = QlF[k + sfa];   // face c+1: this element is its left state
+    lb_[v] = Qr[k];         // face c: this element is its right state
+    if (wrapL) rb[v] += QlF[k + sfa - (long long)Na * sfa];  // face 0
+    if (wrapR) lb_[v] += Qr[k + (long long)Na * sfa];         // face Na
+  }
+  const int lbase = line_base<NDIM, n>(a, lp);
+  const int sa = stride_of<n>(a);
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    double* ub = Sb + (long long)t * V * g.NE * NL + ((long long)v * g.NE + e) * NL + lbase;
+#pragma unroll
+    for (int s2 = 0; s2 < n; ++s2)
+      ub[s2 * sa] += op.s2f[nf - 1][s2] * rb[v] + op.s2f[0][s2] * lb_[v];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -888,6 +937,7 @@ struct AdjScratch {
   double* Uadb = nullptr;       // (T, V, E, L) cotangent of a predictor iterate
   double* Db = nullptr;         // (T, V, E, L) cotangent of dudt
   double* Ffb[3] = {nullptr, nullptr, nullptr};  // (T, V, plane_a) element-face cotangents
+  double* Qr[3] = {nullptr, nullptr, nullptr};   // (T, V, plane_a) right-trace cotangents
   double* Ucvb[2] = {nullptr, nullptr};          // (V, Nc) FV-state cotangents
   double* Fhob[3] = {nullptr, nullptr, nullptr};  // (V, nface_a) HO-flux cotangents
   double* WbL[3] = {nullptr, nullptr, nullptr};   // (V, nface_a) fallback left-state cotangents
@@ -942,8 +992,10 @@ int ensure_scratch(spd_ctx* c) {
   int rc;
   if ((rc = c->alloc(&A->Uadb, (size_t)it)) != SPD_OK) return rc;
   if ((rc = c->alloc(&A->Db, (size_t)it)) != SPD_OK) return rc;
-  for (int a = 0; a < c->ndim; ++a)
+  for (int a = 0; a < c->ndim; ++a) {
     if ((rc = c->alloc(&A->Ffb[a], (size_t)(c->T * c->V * plane_of(c, a)))) != SPD_OK) return rc;
+    if ((rc = c->alloc(&A->Qr[a], (size_t)(c->T * c->V * plane_of(c, a)))) != SPD_OK) return rc;
+  }
   if (c->cfg.update == SPD_UPDATE_FV) {
     for (int k = 0; k < 2; ++k)
       if ((rc = c->alloc(&A->Ucvb[k], (size_t)(c->V * c->g.Nc))) != SPD_OK) return rc;
@@ -961,6 +1013,7 @@ int ensure_scratch(spd_ctx* c) {
   }
   for (int a = c->ndim; a < 3; ++a) {
     A->Ffb[a] = A->Ffb[0];
+    A->Qr[a] = A->Qr[0];
     A->Fhob[a] = A->Fhob[0];
     A->WbL[a] = A->WbL[0];
     A->WbR[a] = A->WbR[0];
@@ -976,13 +1029,17 @@ template <int NDIM, int P> struct Adjoint {
   template <int A>
   static void face(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, double* Sb,
                    hipStream_t s) {
+    AdjScratch* Ad = c->adj;
     const long long tot = (long long)Tsrc * plane_of(c, A);
     if (c->cfg.riemann_sd != SPD_RS_LLF)
       hipLaunchKernelGGL((k_adj_face<NDIM, P, RS_HLLC, A>), dim3(grid_for(tot)), dim3(256), 0, s,
-                         op, c->g, c->prm, src, Tsrc, c->adj->Ffb[A], Sb);
+                         op, c->g, c->prm, src, Tsrc, Ad->Ffb[A], Ad->Qr[A]);
     else
       hipLaunchKernelGGL((k_adj_face<NDIM, P, RS_LLF, A>), dim3(grid_for(tot)), dim3(256), 0, s,
-                         op, c->g, c->prm, src, Tsrc, c->adj->Ffb[A], Sb);
+                         op, c->g, c->prm, src, Tsrc, Ad->Ffb[A], Ad->Qr[A]);
+    const long long lines = (long long)Tsrc * c->g.NE * NLP;
+    hipLaunchKernelGGL((k_adj_trace<NDIM, P, A>), dim3(grid_for(lines)), dim3(256), 0, s, op,
+                       c->g, Tsrc, (const double*)Ad->Ffb[A], (const double*)Ad->Qr[A], Sb);
   }
   static void faces(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, double* Sb,
                     hipStream_t s) {
