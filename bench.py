@@ -125,6 +125,7 @@ CLASS_KERNELS = {
     "k_tensor": [("k_tensor_el<2, 3>", 1)],
 }
 TRAFFIC_SUMMARY = os.path.join(HERE, "profiles", "kh1024_p3_hbm_traffic.md")
+FLOPS_SUMMARY = os.path.join(HERE, "profiles", "kh1024_p3_fp64_flops.md")
 
 
 def measured_traffic(cls):
@@ -143,6 +144,29 @@ def measured_traffic(cls):
             if len(cells) >= 6 and cells[0].strip("`") in names:
                 try:
                     vals[cells[0].strip("`")] = (float(cells[4]) + float(cells[5])) * 1024.0 * 1024.0
+                except ValueError:
+                    pass
+    if len(vals) != len(names):
+        return None
+    return sum(vals[k] * w for k, w in names.items()) / sum(names.values())
+
+
+def measured_fp64_flops(cls):
+    """FP64 flop per launch of a kernel class from the committed counter
+    summary (64 x (ADD + MUL + 2 FMA + TRANS) FP64 instructions per dispatch),
+    averaged over the class's kernels weighted by their launches per step;
+    None when not recorded."""
+    entries = CLASS_KERNELS.get(cls)
+    if not entries or not os.path.exists(FLOPS_SUMMARY):
+        return None
+    names = dict(entries)
+    vals = {}
+    with open(FLOPS_SUMMARY) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 8 and cells[0].strip("`") in names:
+                try:
+                    vals[cells[0].strip("`")] = float(cells[6]) * 1e9
                 except ValueError:
                     pass
     if len(vals) != len(names):
@@ -450,9 +474,17 @@ def main():
     avg_ms = dom_ms / dom_cnt
     achieved = bpl[dom] / (avg_ms * 1e-3) / 1e9
     copy_gbps = copy_bandwidth(torch)
-    kernels = {k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps,
-                   "GBps_alg": (bpl[k] / (v[0] / v[1] * 1e-3) / 1e9) if k in bpl else None}
-               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    kernels = {}
+    step_flops = 0.0
+    for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        fl = measured_fp64_flops(k)
+        avg_s = v[0] / v[1] * 1e-3
+        kernels[k] = {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps,
+                      "GBps_alg": (bpl[k] / avg_s / 1e9) if k in bpl else None,
+                      "TFLOPs_fp64": (fl / avg_s / 1e12) if fl is not None else None}
+        if fl is not None:
+            step_flops += fl * v[1] / args.steps
+    dom_fl = measured_fp64_flops(dom)
 
     value32 = dof * args.steps / elapsed32
     out = {
@@ -477,6 +509,15 @@ def main():
                      "alg_bytes_per_launch": bpl[dom],
                      "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, HERE)},
         "step_hbm_roofline_frac": value * 64.0 / (HBM_PEAK_GBPS * 1e9),
+        "compute_roofline": {
+            "bound": "fp64 valu", "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "kernel": dom,
+            "achieved": (dom_fl / (avg_ms * 1e-3) / 1e12) if dom_fl is not None else None,
+            "frac": (dom_fl / (avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS)
+            if dom_fl is not None else None,
+            "step_achieved": step_flops / (ms_step * 1e-3) / 1e12 if step_flops else None,
+            "step_flop_per_dof": step_flops / dof if step_flops else None,
+            "flops_source": os.path.relpath(FLOPS_SUMMARY, HERE)},
         "copy_GBps": copy_gbps,
         "troubled_fraction": frac,
         "kernels": kernels,
