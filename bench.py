@@ -27,7 +27,8 @@ before the timed region.  The JSON line carries:
     amplitude 1e-2): gradient of the final kinetic energy with respect to the
     initial density field through the checkpointed torch.autograd.Function,
     its time against the forward run, and central differences of the
-    frozen-flag map on random directions;
+    frozen-flag map on random directions; adjoint_fp32_bulk: the same in the
+    FP32-bulk form, its gradient compared with the FP64 form's;
   * cpu_baseline: the NumPy oracle on a bounded sample, one host core.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
@@ -298,14 +299,17 @@ def smooth_direction(torch, s, rng, modes=6, kmax=2):
     return out.contiguous()
 
 
-def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2):
+def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=None):
     """Gradient of the final kinetic energy of the 3-D Taylor-Green flow with
     respect to the initial density field (``steps`` steps, a checkpoint every
     ``k``), timed against the forward run; central differences (steps 1e-7, 1e-8
     and 1e-9 of the density scale, the closest one kept) of the same frozen-flag
     map (the forward's dt and blending coefficients replayed) on ``n_fd``
     random smooth directions (white-noise directions put many limiter
-    arguments within the difference step of a kink at this size)."""
+    arguments within the difference step of a kink at this size).  With
+    ``precision="fp32"`` the FP32-bulk form is differentiated and its gradient
+    compared with ``ref_grad`` (the FP64 form's) instead.  Returns (summary,
+    gradient)."""
     import math
     import numpy as np
     from spd_amd import initial_conditions as ic
@@ -314,7 +318,8 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2):
     L = 2 * math.pi
     s = SDADER_Simulator(init_fct=ic.taylor_green, p=3, N=(N, N, N), xlim=(0, L), ylim=(0, L),
                          zlim=(0, L), update="FV", FB=True, riemann_solver_sd="hllc",
-                         riemann_solver_fv="hllc", device="cuda:0", verbose=False)
+                         riemann_solver_fv="hllc", device="cuda:0", verbose=False,
+                         precision=precision)
     adj = Adjoint(s)
     U0 = s.dm.device_value("U_sp").clone().contiguous()
     gen = torch.Generator(device="cuda").manual_seed(0)
@@ -350,10 +355,11 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2):
     t_grad = time.perf_counter() - t0
     peak = torch.cuda.max_memory_allocated()
     g = rho.grad
+    errs = []
     with torch.no_grad():
         U = cons(W0[0])
         thetas = []
-        for st in range(steps):
+        for st in range(steps if n_fd > 0 else 0):
             U, th = adj.step(U, dts[st:st + 1], record_theta=True)
             thetas.append(th)
 
@@ -369,22 +375,25 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2):
             return float((ke_pts(final(W0[0] + eps * d)) - ke_pts(final(W0[0] - eps * d))).sum())
 
         rng = np.random.default_rng(5)
-        errs = []
         for _ in range(n_fd):
             d = smooth_direction(torch, s, rng)
             ad = float((g * d).sum())
             e = [abs(dJ(d, eps) / (2 * eps) - ad) / abs(ad) for eps in (1e-7, 1e-8, 1e-9)]
             errs.append(min(e))
-    out = {"workload": f"Taylor-Green 3-D {N}^3 p=3 FV+FB HLLC periodic, FP64, density "
-                       f"perturbation 1e-2, d(final kinetic energy)/d(initial density)",
+    out = {"workload": f"Taylor-Green 3-D {N}^3 p=3 FV+FB HLLC periodic, "
+                       f"{precision.upper()}, density perturbation 1e-2, "
+                       f"d(final kinetic energy)/d(initial density)",
            "steps": steps, "checkpoint_every": k, "dof": (N * 4) ** 3,
            "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
            "peak_torch_GiB": peak / 2 ** 30,
            "scratch_GiB": float(s._lib.spd_ctx_scratch_bytes(s._ctx)) / 2 ** 30,
            "fd_directions": n_fd, "fd_rel_err_max": max(errs) if errs else None}
+    if ref_grad is not None:
+        out["grad_rel_l2_vs_fp64"] = float((g - ref_grad).norm() / ref_grad.norm())
+    g = g.detach().clone()
     del adj, s
     torch.cuda.empty_cache()
-    return out
+    return out, g
 
 
 def main():
@@ -430,7 +439,7 @@ def main():
     torch.cuda.set_device(dev)
     if args.adjoint_only:
         print(json.dumps({"adjoint": run_adjoint(torch, args.adjoint_N, args.adjoint_steps,
-                                                 args.adjoint_k, args.adjoint_fd)}))
+                                                 args.adjoint_k, args.adjoint_fd)[0]}))
         return
     dof = (N * (p + 1)) ** 2
     noise = ic.subcell_noise((N, N), p, [1, 2], 1e-3, seed=0)
@@ -541,8 +550,12 @@ def main():
         if "ensemble" not in out:
             del s
             torch.cuda.empty_cache()
-        out["adjoint"] = run_adjoint(torch, args.adjoint_N, args.adjoint_steps, args.adjoint_k,
-                                     args.adjoint_fd)
+        out["adjoint"], g64 = run_adjoint(torch, args.adjoint_N, args.adjoint_steps,
+                                          args.adjoint_k, args.adjoint_fd)
+        out["adjoint_fp32_bulk"], _ = run_adjoint(torch, args.adjoint_N, args.adjoint_steps,
+                                                  args.adjoint_k, 0, precision="fp32",
+                                                  ref_grad=g64)
+        del g64
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline(args.cpu_N, args.cpu_steps, args.cpu_warmup)
     print(json.dumps(out))

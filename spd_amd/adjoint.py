@@ -17,7 +17,9 @@ then walks the interval backwards, recording one step's tape (Picard
 iterates, FV substep states, blending coefficients) and running its reverse
 sweep.  Memory: ``nsteps / k + k`` states plus one step's tape.
 
-Only the FP64 form is differentiated.
+Both forms are differentiated; the reverse sweep is FP64 in both.  In the
+FP32-bulk form the tape holds the form's FP32 predictor iterates, so the sweep
+linearises the FP64 formulas at the FP32 form's states.
 """
 from __future__ import annotations
 
@@ -33,8 +35,6 @@ class Adjoint:
     and mesh).  The simulator's own state is not touched."""
 
     def __init__(self, sim):
-        if sim.precision != "fp64":
-            raise NotImplementedError("the adjoint is built for precision='fp64' only")
         self.sim = sim
         self.lib = _lib.load()
         self.device = sim.device

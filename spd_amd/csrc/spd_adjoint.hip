@@ -1,4 +1,6 @@
-// Reverse-mode derivative (adjoint) of the SD-ADER step, FP64.
+// Reverse-mode derivative (adjoint) of the SD-ADER step.  The reverse sweep
+// is FP64; in the FP32-bulk form the tape's predictor iterates are the
+// form's FP32 values (the sweep linearises the FP64 formulas there).
 //
 // The step is differentiated with its trouble flags, blending coefficients
 // and dt frozen (taken from a recorded forward pass): with those fixed the
@@ -476,9 +478,9 @@ k_adj_hancock(Geom g, Params prm, const double* __restrict__ U, const double* __
 // Uadb is written once, at the end (nothing else writes it in the substep
 // sweep).
 // ---------------------------------------------------------------------------
-template <int NDIM, int P>
+template <int NDIM, int P, typename S>
 __global__ void __launch_bounds__(256)
-k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Uad,
+k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const S* __restrict__ Uad,
              const double* __restrict__ Fhob0, const double* __restrict__ Fhob1,
              const double* __restrict__ Fhob2, double* __restrict__ Uadb,
              double* __restrict__ Ffb0, double* __restrict__ Ffb1, double* __restrict__ Ffb2,
@@ -523,7 +525,7 @@ k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Ua
       for (int v = 0; v < V; ++v)
 #pragma unroll
         for (int s2 = 0; s2 < n; ++s2) {
-          u[v][s2] = Uad[((long long)(t * V + v) * g.NE + e) * NL + lb + s2 * sa];
+          u[v][s2] = (double)Uad[((long long)(t * V + v) * g.NE + e) * NL + lb + s2 * sa];
           ub[v][s2] = 0.0;
         }
 #pragma unroll 1
@@ -602,9 +604,9 @@ k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const double* __restrict__ Ua
 // replaces Ffb in place (each thread reads and writes its own face), the
 // right trace's goes to Qr; k_adj_trace gathers them into the lines.
 // ---------------------------------------------------------------------------
-template <int NDIM, int P, int RS, int A>
+template <int NDIM, int P, int RS, int A, typename S>
 __global__ void __launch_bounds__(256)
-k_adj_face(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
+k_adj_face(Ops<P> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
            double* __restrict__ Ffb, double* __restrict__ Qr) {
   constexpr int a = A;
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
@@ -660,10 +662,10 @@ k_adj_face(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Ts
     const long long e = elem(ea);
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-      const double* u = src + tbase + ((long long)v * g.NE + e) * NL + lb;
+      const S* u = src + tbase + ((long long)v * g.NE + e) * NL + lb;
       double acc = 0.0;
 #pragma unroll
-      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[s * sa];
+      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * (double)u[s * sa];
       q[v] = acc;
     }
   };
@@ -793,9 +795,9 @@ k_adj_trace(Ops<P> op, Geom g, int Tsrc, const double* __restrict__ QlF,
 // the Euler flux into Sb, end points into the element-face cotangent Ffb.
 // One thread per (node, element, line along A).
 // ---------------------------------------------------------------------------
-template <int NDIM, int P, int A>
+template <int NDIM, int P, int A, typename S>
 __global__ void __launch_bounds__(256)
-k_adj_pred(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Tsrc,
+k_adj_pred(Ops<P> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
            const double* __restrict__ Db, double* __restrict__ Sb, double* __restrict__ Ffb) {
   constexpr int a = A;
   constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
@@ -850,7 +852,7 @@ k_adj_pred(Ops<P> op, Geom g, Params prm, const double* __restrict__ src, int Ts
   for (int v = 0; v < V; ++v)
 #pragma unroll
     for (int s = 0; s < n; ++s) {
-      u[v][s] = src[tbase + ((long long)v * g.NE + e) * NL + lb + s * sa];
+      u[v][s] = (double)src[tbase + ((long long)v * g.NE + e) * NL + lb + s * sa];
       ub[v][s] = 0.0;
     }
   const Dual gam(prm.gamma);
@@ -973,8 +975,9 @@ int ensure_tape(spd_ctx* c) {
   if (c->tape_uad) return SPD_OK;
   const long long it = (long long)c->T * c->V * c->g.NE * c->NL;
   int rc;
-  double* uad = nullptr;
-  if ((rc = c->alloc(&uad, (size_t)(c->cfg.m * it))) != SPD_OK) return rc;
+  unsigned char* uad = nullptr;  // predictor iterates in the bulk type of the form
+  const size_t bsz = c->cfg.precision == SPD_PRECISION_FP32 ? sizeof(float) : sizeof(double);
+  if ((rc = c->alloc(&uad, (size_t)(c->cfg.m * it) * bsz)) != SPD_OK) return rc;
   c->tape_uad = uad;
   if (c->cfg.update == SPD_UPDATE_FV) {
     if ((rc = c->alloc(&c->tape_ucv, (size_t)(c->T * c->V * c->g.Nc))) != SPD_OK) return rc;
@@ -1022,40 +1025,44 @@ int ensure_scratch(spd_ctx* c) {
   return SPD_OK;
 }
 
-template <int NDIM, int P> struct Adjoint {
+// B: element type of the tape's predictor iterates (the bulk type of the
+// form: FP64, or FP32 in the FP32-bulk form); the reverse sweep itself is FP64.
+template <int NDIM, int P, typename B> struct Adjoint {
   static constexpr int V = NDIM + 2, n = P + 1, nf = P + 2, T = P + 1;
   static constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
 
-  template <int A>
-  static void face(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, double* Sb,
+  template <int A, typename S>
+  static void face(spd_ctx* c, const Ops<P>& op, const S* src, int Tsrc, double* Sb,
                    hipStream_t s) {
     AdjScratch* Ad = c->adj;
     const long long tot = (long long)Tsrc * plane_of(c, A);
     if (c->cfg.riemann_sd != SPD_RS_LLF)
-      hipLaunchKernelGGL((k_adj_face<NDIM, P, RS_HLLC, A>), dim3(grid_for(tot)), dim3(256), 0, s,
-                         op, c->g, c->prm, src, Tsrc, Ad->Ffb[A], Ad->Qr[A]);
+      hipLaunchKernelGGL((k_adj_face<NDIM, P, RS_HLLC, A, S>), dim3(grid_for(tot)), dim3(256), 0,
+                         s, op, c->g, c->prm, src, Tsrc, Ad->Ffb[A], Ad->Qr[A]);
     else
-      hipLaunchKernelGGL((k_adj_face<NDIM, P, RS_LLF, A>), dim3(grid_for(tot)), dim3(256), 0, s,
-                         op, c->g, c->prm, src, Tsrc, Ad->Ffb[A], Ad->Qr[A]);
+      hipLaunchKernelGGL((k_adj_face<NDIM, P, RS_LLF, A, S>), dim3(grid_for(tot)), dim3(256), 0,
+                         s, op, c->g, c->prm, src, Tsrc, Ad->Ffb[A], Ad->Qr[A]);
     const long long lines = (long long)Tsrc * c->g.NE * NLP;
     hipLaunchKernelGGL((k_adj_trace<NDIM, P, A>), dim3(grid_for(lines)), dim3(256), 0, s, op,
                        c->g, Tsrc, (const double*)Ad->Ffb[A], (const double*)Ad->Qr[A], Sb);
   }
-  static void faces(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, double* Sb,
+  template <typename S>
+  static void faces(spd_ctx* c, const Ops<P>& op, const S* src, int Tsrc, double* Sb,
                     hipStream_t s) {
     face<0>(c, op, src, Tsrc, Sb, s);
     if constexpr (NDIM >= 2) face<1>(c, op, src, Tsrc, Sb, s);
     if constexpr (NDIM >= 3) face<2>(c, op, src, Tsrc, Sb, s);
   }
-  template <int A>
-  static void pred_one(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, double* Sb,
+  template <int A, typename S>
+  static void pred_one(spd_ctx* c, const Ops<P>& op, const S* src, int Tsrc, double* Sb,
                        hipStream_t s) {
     const long long tot = (long long)Tsrc * c->g.NE * NLP;
-    hipLaunchKernelGGL((k_adj_pred<NDIM, P, A>), dim3(grid_for(tot)), dim3(256), 0, s, op, c->g,
+    hipLaunchKernelGGL((k_adj_pred<NDIM, P, A, S>), dim3(grid_for(tot)), dim3(256), 0, s, op, c->g,
                        c->prm, src, Tsrc, c->adj->Db, Sb, c->adj->Ffb[A]);
   }
   // cotangent of D(src) (held in Db) -> Sb
-  static void spatial(spd_ctx* c, const Ops<P>& op, const double* src, int Tsrc, double* Sb,
+  template <typename S>
+  static void spatial(spd_ctx* c, const Ops<P>& op, const S* src, int Tsrc, double* Sb,
                       hipStream_t s) {
     for (int a = 0; a < NDIM; ++a)
       (void)hipMemsetAsync(c->adj->Ffb[a], 0, (size_t)Tsrc * V * plane_of(c, a) * sizeof(double),
@@ -1085,10 +1092,10 @@ template <int NDIM, int P> struct Adjoint {
   static int hof_epb() {
     return std::max(1, std::min(256 / NLP, 40960 / (V * NL * (int)sizeof(double))));
   }
-  static void hoflux(spd_ctx* c, const Ops<P>& op, int i, const double* Uad, hipStream_t s) {
+  static void hoflux(spd_ctx* c, const Ops<P>& op, int i, const B* Uad, hipStream_t s) {
     AdjScratch* Ad = c->adj;
     const int E = hof_epb();
-    hipLaunchKernelGGL((k_adj_hoflux<NDIM, P>), dim3((unsigned)((c->g.NE + E - 1) / E)),
+    hipLaunchKernelGGL((k_adj_hoflux<NDIM, P, B>), dim3((unsigned)((c->g.NE + E - 1) / E)),
                        dim3(256), (size_t)E * V * NL * sizeof(double), s, op, c->g, c->prm, i, Uad,
                        Ad->Fhob[0], Ad->Fhob[1], Ad->Fhob[2], Ad->Uadb, Ad->Ffb[0], Ad->Ffb[1],
                        Ad->Ffb[2], E);
@@ -1101,7 +1108,7 @@ template <int NDIM, int P> struct Adjoint {
     const int m = c->cfg.m;
     const long long it_size = (long long)T * V * c->g.NE * NL;
     const long long npt = (long long)V * c->g.NE * NL;
-    const double* tape = (const double*)c->tape_uad;
+    const B* tape = (const B*)c->tape_uad;
     auto iterate = [&](int k) { return tape + (long long)(k - 1) * it_size; };  // U_ad^(k)
     (void)hipMemsetAsync(Ad->Uadb, 0, (size_t)it_size * sizeof(double), s);
     if (c->cfg.update == SPD_UPDATE_FV) {
@@ -1162,23 +1169,21 @@ template <int NDIM, int P> struct Adjoint {
 
 using BackFn = int (*)(spd_ctx*, const double*, const double*, double*, hipStream_t);
 struct AdjEntry {
-  int ndim, p;
+  int ndim, p, precision;
   BackFn back;
 };
+#define SPD_ADJ(D, P)                                                        \
+  {D, P, SPD_PRECISION_FP64, &Adjoint<D, P, double>::backward},              \
+      {D, P, SPD_PRECISION_FP32, &Adjoint<D, P, float>::backward}
 const AdjEntry kAdj[] = {
-    {1, 1, &Adjoint<1, 1>::backward}, {1, 2, &Adjoint<1, 2>::backward},
-    {1, 3, &Adjoint<1, 3>::backward}, {1, 4, &Adjoint<1, 4>::backward},
-    {1, 5, &Adjoint<1, 5>::backward}, {2, 1, &Adjoint<2, 1>::backward},
-    {2, 2, &Adjoint<2, 2>::backward}, {2, 3, &Adjoint<2, 3>::backward},
-    {2, 4, &Adjoint<2, 4>::backward}, {2, 5, &Adjoint<2, 5>::backward},
-    {3, 1, &Adjoint<3, 1>::backward}, {3, 2, &Adjoint<3, 2>::backward},
-    {3, 3, &Adjoint<3, 3>::backward},
+    SPD_ADJ(1, 1), SPD_ADJ(1, 2), SPD_ADJ(1, 3), SPD_ADJ(1, 4), SPD_ADJ(1, 5),
+    SPD_ADJ(2, 1), SPD_ADJ(2, 2), SPD_ADJ(2, 3), SPD_ADJ(2, 4), SPD_ADJ(2, 5),
+    SPD_ADJ(3, 1), SPD_ADJ(3, 2), SPD_ADJ(3, 3),
 };
+#undef SPD_ADJ
 
 int adj_check(const spd_ctx* c) {
   if (!c) return fail(SPD_E_INVALID, "null context");
-  if (c->cfg.precision != SPD_PRECISION_FP64)
-    return fail(SPD_E_UNSUPPORTED, "the adjoint is built for the FP64 form only");
   return SPD_OK;
 }
 
@@ -1221,7 +1226,7 @@ int spd_adj_backward(spd_ctx* ctx, const double* U_in, const double* dt_dev, dou
   if (!ctx->tape_uad) return fail(SPD_E_INVALID, "no tape: call spd_adj_step first");
   if ((rc = ensure_scratch(ctx)) != SPD_OK) return rc;
   for (const AdjEntry& e : kAdj)
-    if (e.ndim == ctx->ndim && e.p == ctx->p)
+    if (e.ndim == ctx->ndim && e.p == ctx->p && e.precision == ctx->cfg.precision)
       return e.back(ctx, U_in, dt_dev, U_bar, (hipStream_t)stream);
   return fail(SPD_E_UNSUPPORTED, "no adjoint kernels for this dimension and order");
 }
