@@ -10,7 +10,9 @@ rho = 1, vx = -0.5, P = 2.5, vy = 0.01 sin(4 pi x), plus seeded uniform noise
 A step is one pass of the hot path over the whole state: compute_dt on the
 device (its scalar read back to the host, as in the reference's loop) and
 one spd_step (predictor + FV/FB corrector).  The state is resident in HBM
-before the timed region.  The JSON line carries:
+before the timed region (10 warm-up steps, then 200 timed steps from t = 0),
+and the timed steps run without instrumentation; the per-kernel durations
+come from 20 further steps with HIP events around every launch.  The JSON line carries:
   * value: DOF-updates/s of the FP64 form (DOF = solution points; one
     DOF-update = one point advanced one full step), with the roofline of the
     dominant kernel (HIP-event durations over the timed region on the
@@ -31,7 +33,7 @@ before the timed region.  The JSON line carries:
     FP32-bulk form, its gradient compared with the FP64 form's;
   * cpu_baseline: the NumPy oracle on a bounded sample, one host core.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10]
 """
 from __future__ import annotations
 
@@ -399,8 +401,10 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=N
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=20,
+                    help="steps of the HIP-event window that times each kernel")
     ap.add_argument("--N", type=int, default=1024)
     ap.add_argument("--p", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -445,38 +449,44 @@ def main():
     noise = ic.subcell_noise((N, N), p, [1, 2], 1e-3, seed=0)
 
     def run_form(precision):
-        """Warm-up, then the timed steps (HIP events around every kernel), then
-        one more step that measures the troubled fraction."""
+        """Warm-up, then the timed steps (no instrumentation), then as many
+        steps with HIP events around every kernel (the per-kernel durations of
+        the roofline), then one more step that measures the troubled
+        fraction."""
         s = SDADER_Simulator(init_fct=ic.kelvin_helmholtz, device=dev, verbose=False,
                              precision=precision, **kh_kwargs(N, p))
         s.perturb_initial_condition(noise)
         s.advance(args.warmup)
-        torch.cuda.synchronize()
-        s.profile_begin(max_launches=200 * max(args.steps, 1))
+        early = s.dm.device_value("U_sp").clone()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         s.advance(args.steps)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+        s.profile_begin(max_launches=200 * max(args.profile_steps, 1))
+        s.advance(args.profile_steps)
         prof = s.profile_end()
         frac = s.troubled_fraction_next_step()
-        return s, elapsed, prof, frac
+        return s, elapsed, prof, frac, early
 
-    s, elapsed, prof, frac = run_form("fp64")
-    s32, elapsed32, prof32, frac32 = run_form("fp32")
+    s, elapsed, prof, frac, early64 = run_form("fp64")
+    s32, elapsed32, prof32, frac32, early32 = run_form("fp32")
+
     # the two forms after the same number of steps from the same state
+    def l1_rel(r32, r64):
+        out = {name: float((r32[v] - r64[v]).abs().sum() / r64[v].abs().sum())
+               for name, v in (("rho", 0), ("E", 3))}
+        out["momenta"] = float((r32[1:3] - r64[1:3]).abs().sum() / r64[1:3].abs().sum())
+        return out
     torch.cuda.synchronize()
-    r64 = s.dm.device_value("U_sp")
-    r32 = s32.dm.device_value("U_sp")
-    l1 = {name: float((r32[v] - r64[v]).abs().sum() / r64[v].abs().sum())
-          for name, v in (("rho", 0), ("E", 3))}
-    l1["momenta"] = float((r32[1:3] - r64[1:3]).abs().sum() / r64[1:3].abs().sum())
-    del s32, r32
+    l1 = l1_rel(s32.dm.device_value("U_sp"), s.dm.device_value("U_sp"))
+    l1_early = l1_rel(early32, early64)
+    del s32, early32, early64
 
     ms_step = elapsed / args.steps * 1e3
     value = dof * args.steps / elapsed
     launches = dict(prof)
-    launches["_steps"] = args.steps
+    launches["_steps"] = args.profile_steps
     bpl = algorithmic_bytes(N, p, launches)
     dom = max(prof, key=lambda k: prof[k][0])
     dom_ms, dom_cnt = prof[dom]
@@ -488,11 +498,12 @@ def main():
     for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0]):
         fl = measured_fp64_flops(k)
         avg_s = v[0] / v[1] * 1e-3
-        kernels[k] = {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps,
+        kernels[k] = {"ms_per_step": v[0] / args.profile_steps,
+                      "launches_per_step": v[1] / args.profile_steps,
                       "GBps_alg": (bpl[k] / avg_s / 1e9) if k in bpl else None,
                       "TFLOPs_fp64": (fl / avg_s / 1e12) if fl is not None else None}
         if fl is not None:
-            step_flops += fl * v[1] / args.steps
+            step_flops += fl * v[1] / args.profile_steps
     dom_fl = measured_fp64_flops(dom)
 
     value32 = dof * args.steps / elapsed32
@@ -534,9 +545,11 @@ def main():
             "value": value32, "unit": "DOF-updates/s", "ms_per_step": elapsed32 / args.steps * 1e3,
             "speedup_vs_fp64": value32 / value,
             "step_hbm_roofline_frac": value32 * 64.0 / (HBM_PEAK_GBPS * 1e9),
-            "l1_rel_vs_fp64": l1, "steps_compared": args.warmup + args.steps + 1,
+            "l1_rel_vs_fp64": l1,
+            "steps_compared": args.warmup + args.steps + args.profile_steps + 1,
+            "l1_rel_vs_fp64_after_warmup": l1_early, "steps_compared_after_warmup": args.warmup,
             "troubled_fraction": frac32,
-            "kernels_ms_per_step": {k: v[0] / args.steps
+            "kernels_ms_per_step": {k: v[0] / args.profile_steps
                                     for k, v in sorted(prof32.items(), key=lambda kv: -kv[1][0])},
         },
     }
