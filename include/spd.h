@@ -205,8 +205,10 @@ int spd_prims_to_cons(spd_ctx* ctx, const double* W, double* U, long long count,
 int spd_set_debug_tap(spd_ctx* ctx, int substep, double* troubles, double* theta,
                       double* aff_x, double* aff_y, double* aff_z);
 
-/* Reverse-mode derivative (adjoint) of one step, FP64 form only.  No entry
- * of the reference corresponds: the reference has no derivative.  These
+/* Reverse-mode derivative (adjoint) of one step, both precision forms (the
+ * reverse sweep is FP64; in the FP32-bulk form the tape holds the form's FP32
+ * predictor iterates).  No entry of the reference corresponds: the reference
+ * has no derivative.  These
  * differentiate perform_update (src/sdader_simulator.py:361-378: the predictor
  * src/sdader_simulator.py:160-200,213-240, the FV corrector :288-349 with
  * src/fv_simulator.py:167-205, src/muscl.py:129-221 and the blend :325-331)
@@ -223,8 +225,11 @@ int spd_set_debug_tap(spd_ctx* ctx, int substep, double* troubles, double* theta
  * spd_adj_backward takes the cotangent of the step's output in U_bar and
  * replaces it with the cotangent of its input, U_bar^T d(step)/d(U_in), by the
  * reverse sweep over the tape of the last spd_adj_step, which must have been
- * run from the same U_in and dt.  The tape (m (m+1) V E L + (m+1)(V Nc + Ngc)
- * doubles) and the adjoint scratch are allocated on first use. */
+ * run from the same U_in and dt.  U_bar must be ordinary (coarse-grained)
+ * device memory, as hipMalloc returns: the sweep accumulates into it with
+ * hardware FP64 atomics.  The tape (m (m+1) V E L values of the bulk
+ * type + (m+1)(V Nc + Ngc) doubles) and the adjoint scratch are allocated on
+ * first use. */
 int spd_adj_step(spd_ctx* ctx, const double* U_in, double* U_out, const double* dt_dev,
                  const double* theta_in, double* theta_out, void* stream);
 int spd_adj_backward(spd_ctx* ctx, const double* U_in, const double* dt_dev, double* U_bar,
