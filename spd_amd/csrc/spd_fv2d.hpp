@@ -31,6 +31,21 @@ constexpr int DH = 3;
 constexpr int DNX = DTX + 2 * DH, DNY = DTY + 2 * DH;
 constexpr int ONX = DTX + 2, ONY = DTY + 2;
 
+// Conserved state of the ghost-filled array at ghosted cell (x, y) (the
+// mapping of load_W); false outside the ghosted grid.  fl: reflective flips of
+// the x and y velocities.
+__device__ __forceinline__ bool stage_cons(const double* __restrict__ U, const Geom& g, int x, int y,
+                                           double* u, bool* fl) {
+  if (!in_ghost_range(g, x, y)) return false;
+  fl[0] = fl[1] = false;
+  const int mx = ghost_map(x, g.Ns[0], g.bc[0], &fl[0]);
+  const int my = ghost_map(y, g.Ns[1], g.bc[1], &fl[1]);
+  const long long idx = (long long)my * g.Ns[0] + mx;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) u[v] = U[v * g.Nc + idx];
+  return true;
+}
+
 // Smooth-extrema test along one axis from LDS (src/trouble_detection.py:
 // 104-137).  w points at the cell of coordinate x; w[k*stride] is its k-th
 // neighbour along the axis.  dU is zero outside [-1, Ns].
@@ -94,30 +109,54 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
   double(*wo)[ONY][ONX] = reinterpret_cast<double(*)[ONY][ONX]>(wbuf);
   const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
   const int Nx = g.Ns[0], Ny = g.Ns[1];
-  for (int k = threadIdx.x; k < DNX * DNY; k += DNT) {
-    const int lx = k % DNX, ly = k / DNX;
-    const int x = x0 + lx - DH, y = y0 + ly - DH;
-    double r = qnan(), p = qnan();
-    if (in_ghost_range(g, x, y)) {
-      int c[3] = {x, y, 0};
-      double W[4];
-      load_W<2>(Unew, g, c, W, prm.gamma);
-      r = W[0];
-      p = W[3];
-    }
-    wn[0][ly][lx] = r;
-    wn[1][ly][lx] = p;
-  }
-  for (int k = threadIdx.x; k < ONX * ONY; k += DNT) {
-    const int lx = k % ONX, ly = k / ONX;
-    const int x = x0 + lx - 1, y = y0 + ly - 1;
-    double W[4] = {qnan(), qnan(), qnan(), qnan()};
-    if (in_ghost_range(g, x, y)) {
-      int c[3] = {x, y, 0};
-      load_W<2>(Ucv, g, c, W, prm.gamma);
+  // staging: each thread issues the loads of all its cells of a tile before
+  // it uses the first (new rho, P on tile + 3, then old primitives on tile + 1)
+  {
+    constexpr int NI = (DNX * DNY + DNT - 1) / DNT;
+    double u[NI][4];
+    bool ok[NI], fl[NI][2];
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+      const int k = threadIdx.x + j * DNT;
+      ok[j] = k < DNX * DNY && stage_cons(Unew, g, x0 + k % DNX - DH, y0 + k / DNX - DH, u[j], fl[j]);
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) wo[v][ly][lx] = W[v];
+    for (int j = 0; j < NI; ++j) {
+      const int k = threadIdx.x + j * DNT;
+      if (k >= DNX * DNY) continue;
+      double r = qnan(), p = qnan();
+      if (ok[j]) {
+        double W[4];
+        prims<2>(u[j], W, 0, prm.gamma);
+        r = W[0];
+        p = W[3];
+      }
+      wn[0][k / DNX][k % DNX] = r;
+      wn[1][k / DNX][k % DNX] = p;
+    }
+  }
+  {
+    constexpr int NI = (ONX * ONY + DNT - 1) / DNT;
+    double u[NI][4];
+    bool ok[NI], fl[NI][2];
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+      const int k = threadIdx.x + j * DNT;
+      ok[j] = k < ONX * ONY && stage_cons(Ucv, g, x0 + k % ONX - 1, y0 + k / ONX - 1, u[j], fl[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+      const int k = threadIdx.x + j * DNT;
+      if (k >= ONX * ONY) continue;
+      double W[4] = {qnan(), qnan(), qnan(), qnan()};
+      if (ok[j]) {
+        prims<2>(u[j], W, 0, prm.gamma);
+        if (fl[j][0]) W[1] = -W[1];
+        if (fl[j][1]) W[2] = -W[2];
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) wo[v][k / ONX][k % ONX] = W[v];
+    }
   }
   __syncthreads();
   const int tx = threadIdx.x % DTX, ty = threadIdx.x / DTX;
@@ -252,17 +291,77 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
   __shared__ double t1[HTY + 2][HTX + 2];
   const int gx0 = blockIdx.x * HTX, gy0 = blockIdx.y * HTY;
   const int NGx = g.Ng[0], NGy = g.Ng[1];
-  const bool gate = (*any_flag != 0) && prm.p > 1 && prm.sed;
-  const double* Ulast = gate ? Unew : Ucv;
-  for (int k = threadIdx.x; k < TNX * TNY; k += HNT) {
-    const int lx = k % TNX, ly = k / TNX;
-    const int gx = gx0 + lx - TH, gy = gy0 + ly - TH;
-    double v = 0.0;
-    if (gx >= 0 && gx < NGx && gy >= 0 && gy < NGy) {
-      int c[3] = {gx - NGH, gy - NGH, 0};
-      v = trouble_at<2>(g, flags, Ulast, c);
+  // A tile whose staged cells are all active cells (most tiles) holds trouble
+  // values 0 / 1 only, and with blending theta is then a function of the
+  // distance to the nearest troubled cell (the 3 x 3 maximum of theta1 with
+  // weights 1, 0.75 (edge neighbour), 0.5 (corner), followed by the two 0.25
+  // rings, which set 0.25 wherever theta1 = 0 has a non-zero theta1 in its
+  // 3 x 3 neighbourhood):
+  //   1 troubled, 0.75 edge neighbour of one, 0.5 corner neighbour of one,
+  //   0.25 a troubled cell within the 5 x 5 neighbourhood, else 0.
+  // That case runs on bytes: flags, then per staged row a code of the 1-, 3-
+  // and 5-wide horizontal ORs, then the rows' codes are combined per output.
+  const int cx0 = gx0 - TH - NGH, cy0 = gy0 - TH - NGH;  // cell coordinates of tr[0][0]
+  if (prm.blending && cx0 >= 0 && cy0 >= 0 && cx0 + TNX <= g.Ns[0] && cy0 + TNY <= g.Ns[1]) {
+    uint8_t* fb = reinterpret_cast<uint8_t*>(&tr[0][0]);  // [TNY][TNX] flags
+    uint8_t* hc = reinterpret_cast<uint8_t*>(&t1[0][0]);  // [TNY][HTX] row codes
+    constexpr int NIT = (TNX * TNY + HNT - 1) / HNT;
+    int f[NIT];
+    int any = 0;
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {  // every load issued before the first is used
+      const int k = threadIdx.x + j * HNT;
+      f[j] = (k < TNX * TNY)
+                 ? (int)flags[(long long)(cy0 + k / TNX) * g.Ns[0] + cx0 + k % TNX]
+                 : 0;
     }
-    tr[ly][lx] = v;
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+      const int k = threadIdx.x + j * HNT;
+      if (k < TNX * TNY) fb[k] = (uint8_t)f[j];
+      any |= f[j];
+    }
+    if (!__syncthreads_or(any)) {
+      for (int k = threadIdx.x; k < HTX * HTY; k += HNT)
+        theta[(long long)(gy0 + k / HTX) * NGx + gx0 + k % HTX] = 0.0;
+      return;
+    }
+    for (int k = threadIdx.x; k < TNY * HTX; k += HNT) {
+      const uint8_t* r = fb + (k / HTX) * TNX + k % HTX + TH;
+      const int c = r[0], h1 = r[-1] | c | r[1], h2 = r[-2] | h1 | r[2];
+      hc[k] = (uint8_t)(c | (h1 << 1) | (h2 << 2));
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < HTX * HTY; k += HNT) {
+      const uint8_t* col = hc + (k / HTX + TH) * HTX + k % HTX;
+      const int a = col[0], u1 = col[-HTX], d1 = col[HTX];
+      const int r2 = (a | u1 | d1 | col[-2 * HTX] | col[2 * HTX]) >> 2;
+      double th = 0.0;
+      if (a & 1)
+        th = 1.0;
+      else if (((a >> 1) | u1 | d1) & 1)
+        th = 0.75;
+      else if (((u1 | d1) >> 1) & 1)
+        th = 0.5;
+      else if (r2 & 1)
+        th = 0.25;
+      theta[(long long)(gy0 + k / HTX) * NGx + gx0 + k % HTX] = th;
+    }
+    return;
+  }
+  {
+    const bool gate = (*any_flag != 0) && prm.p > 1 && prm.sed;
+    const double* Ulast = gate ? Unew : Ucv;
+    for (int k = threadIdx.x; k < TNX * TNY; k += HNT) {
+      const int lx = k % TNX, ly = k / TNX;
+      const int gx = gx0 + lx - TH, gy = gy0 + ly - TH;
+      double v = 0.0;
+      if (gx >= 0 && gx < NGx && gy >= 0 && gy < NGy) {
+        int c[3] = {gx - NGH, gy - NGH, 0};
+        v = trouble_at<2>(g, flags, Ulast, c);
+      }
+      tr[ly][lx] = v;
+    }
   }
   __syncthreads();
   if (!prm.blending) {

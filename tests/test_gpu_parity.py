@@ -128,3 +128,43 @@ def test_reference_preserve_invariant():
                 s.perform_time_evolution(1)
                 assert np.mean(np.abs(s.dm.W_cv[0] - s.W_init_cv[0])) > 1e-14
                 assert np.mean(np.abs(s.dm.W_cv[1:] - s.W_init_cv[1:])) < 1e-14
+
+
+@pytest.mark.parametrize("bc", ["periodic", "gradfree"])
+def test_theta_of_flags_on_large_grid(bc):
+    """The 2-D blending kernel evaluates tiles whose staged cells (64 x 16
+    outputs + 2 on every side) are all active cells on bytes, as the distance
+    to the nearest troubled cell; that form needs grids of at least 68 cells
+    per side, so the fixtures above never reach it.  On a Kelvin-Helmholtz grid
+    of 256^2 cells, theta of every substep equals the oracle's blending rings
+    (src/trouble_detection.py:139-169) applied to the step's own flags, bit for
+    bit: everywhere when periodic, and on the cells whose 5 x 5 neighbourhood
+    is active with outflow sides (their ghosts carry stale state values)."""
+    from oracle.spd_oracle import OracleSimulator
+    from spd_amd import initial_conditions as ic
+    print(f"[gpu theta 256^2] {bc}", flush=True)
+    kw = dict(p=3, N=(64, 64), BC=((bc, bc),) * 2, update="FV", FB=True,
+              riemann_solver_sd="hllc", riemann_solver_fv="hllc")
+    from spd_amd.simulator import SDADER_Simulator
+    s = SDADER_Simulator(init_fct=ic.kelvin_helmholtz, device="cuda:0", verbose=False, **kw)
+    # no velocity noise: the troubled cells lie along the two density jumps,
+    # which leaves tiles with and without them
+    blend = OracleSimulator(ic.kelvin_helmholtz, p=1, N=(2, 2))._blend
+    n_trouble, n_empty_tiles = 0, 0
+    for k in range(3):
+        s.compute_dt()
+        for rec in s.perform_update_with_flags():
+            tr = rec["troubles"][0]
+            n_trouble += int(tr.sum())
+            pad = np.pad(tr, 2, mode="wrap" if bc == "periodic" else "constant")
+            want = pad.copy()
+            blend(pad, want)
+            got = rec["theta"][0]
+            assert got.shape == want.shape
+            if bc == "periodic":
+                assert np.array_equal(got, want), k
+            else:
+                assert np.array_equal(got[4:-4, 4:-4], want[4:-4, 4:-4]), k
+            blocks = tr[:240, :192].reshape(15, 16, 3, 64)
+            n_empty_tiles += int((blocks.sum(axis=(1, 3)) == 0).sum())
+    assert n_trouble > 0 and n_empty_tiles > 0, (n_trouble, n_empty_tiles)
