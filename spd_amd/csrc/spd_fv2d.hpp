@@ -697,6 +697,28 @@ k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
     return (xth && y >= -1 && y <= Ny) ? theta[(long long)(y + NGH) * NGx + (x + NGH)] : 0.0;
   };
 
+  // Rows with a non-zero theta anywhere across the wave's columns: bit r for
+  // row y0 - 3 + r.  Row k of the march is quiet when rows k - 2 .. k + 2 all
+  // have theta = 0: then no face of row k or of its neighbours k - 1, k + 1 is
+  // blended, so row k's Hancock state is never used, cell row k - 1 keeps the
+  // candidate and the fluxes of row k are not needed.  Quiet rows are
+  // skipped; the row window is reloaded at the next row that is not.
+  unsigned long long busy = ~0ull;
+  if (!godunov) {
+    busy = 0;
+    const int nr = y1 - y0 + 6;  // MR <= 58
+    for (int r0 = 0; r0 < nr; r0 += 8) {
+      double tv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) tv[j] = (r0 + j < nr) ? theta_at(y0 - 3 + r0 + j) : 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (__ballot(!(tv[j] == 0.0))) busy |= 1ull << (r0 + j);
+    }
+    if (busy == 0) return;  // the candidate stands everywhere in the chunk
+  }
+  auto quiet = [&](int k) { return ((busy >> (k - y0 + 1)) & 31ull) == 0; };
+
   B Wm[4], W0[4], Wp[4];
   fetch(y0 - 2);
   to_W(Wm);
@@ -709,7 +731,27 @@ k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
   B Qm[4] = {B(0), B(0), B(0), B(0)};  // Hancock state + half y-slope of row k - 1
   B FxR[4] = {B(0), B(0), B(0), B(0)};  // blended flux at (k - 1, x + 1/2)
   B FyB[4] = {B(0), B(0), B(0), B(0)};  // blended flux at y-face k - 1
+  bool primed = true;  // Wm, W0, un, th, thn hold rows k - 1, k, k + 1
   for (int k = y0 - 1; k <= y1; ++k) {
+    if (quiet(k)) {
+      // theta = 0 on rows k - 2 .. k + 2: nothing of rows k - 1 .. k is
+      // blended, and the state of the next busy row is reloaded
+      primed = false;
+      thm = 0.0;
+      tfxR = 0.0;
+      tfyB = 0.0;
+      continue;
+    }
+    if (!primed) {
+      fetch(k - 1);
+      to_W(Wm);
+      fetch(k);
+      to_W(W0);
+      fetch(k + 1);
+      th = theta_at(k);
+      thn = theta_at(k + 1);
+      primed = true;
+    }
     to_W(Wp);  // row k + 1
     fetch(k + 2);
     const double thk = th;
