@@ -885,6 +885,11 @@ k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
 // point) evaluates the flux on the element's lines, averages it over the
 // transverse subcells and writes the face values.
 // ---------------------------------------------------------------------------
+// Element stride of the LDS slab: NL + 1 values, odd, so that the lanes of a
+// wave, which read up to 13 different elements at the same point, fall into
+// different LDS banks (NL is a multiple of 16 at p = 3).
+__host__ __device__ constexpr int ho_nlp(int P) { return (P + 1) * (P + 1) + 1; }
+
 template <int P, int A, typename B>
 __device__ __forceinline__ void ho2d_items(const Ops<P, B>& op, const Geom& g, const Params& prm,
                                            int t, const B* __restrict__ Ff, B* __restrict__ Fh,
@@ -897,12 +902,13 @@ k_ho2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
        B* __restrict__ Fhy, int EPB) {
   SPD_SKIP(g);
   constexpr int V = 4, n = P + 1, nf = P + 2, NL = n * n;
-  B* slab = reinterpret_cast<B*>(spd_dyn_lds);  // [V][EPB][NL]
+  B* slab = reinterpret_cast<B*>(spd_dyn_lds);  // [V][EPB][ho_nlp(P)]
   const long long e0 = (long long)blockIdx.x * EPB;
   const int ne = (int)((g.NE - e0 < EPB) ? (g.NE - e0) : EPB);
   for (int k = threadIdx.x; k < V * ne * NL; k += blockDim.x) {
     const int v = k / (ne * NL), r = k % (ne * NL);
-    slab[v * EPB * NL + r] = Uad[((long long)(t * V + v) * g.NE + e0) * NL + r];
+    slab[(v * EPB + r / NL) * ho_nlp(P) + r % NL] =
+        Uad[((long long)(t * V + v) * g.NE + e0) * NL + r];
   }
   __syncthreads();
   ho2d_items<P, 0, B>(op, g, prm, t, Ffx, Fhx, slab, e0, ne, EPB);
@@ -942,7 +948,7 @@ __device__ __forceinline__ void ho2d_items(const Ops<P, B>& op, const Geom& g, c
         B M[V], W[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          const B* u = slab + (v * EPB + el) * NL;
+          const B* u = slab + (v * EPB + el) * ho_nlp(P);
           B acc = B(0);
 #pragma unroll
           for (int s = 0; s < n; ++s) {

@@ -44,13 +44,20 @@ template <int P, bool FIRST> struct Layout {
   static constexpr int TE = TX * TY;
   static constexpr int NTH = TE * TA * n;            // threads per block
   static constexpr int TM = TX > TY ? TX : TY;       // rows of a halo side
-  static constexpr int SLAB = V * NL + 1;            // dudt slab (padded)
+  // dudt slab of one (element, time node), padded so that the 32 lanes of a
+  // half-wave (p = 3: two elements x 4 nodes x 4 lines) that write or
+  // accumulate one point of their lines fall into different LDS banks (at
+  // most two lanes per bank); other degrees keep one value of padding
+  static constexpr int SLAB = V * NL + (P == 3 ? 5 : 1);
   static constexpr int LINE = TA * n * V;            // traces of one element / halo row
-  // LDS offsets, in values of the bulk type
+  static constexpr int TRV = TE * TA * n;            // one variable's plane of the traces
+  static constexpr int HV = TM * TA * n;             // one variable's plane of a halo side
+  // LDS offsets, in values of the bulk type; the traces are stored variable
+  // by variable, so that consecutive lanes touch consecutive values
   static constexpr int O_DUDT = 0;                   // [TE][TA][SLAB]
-  static constexpr int O_TR = O_DUDT + TE * TA * SLAB;  // [TE][TA][n][V] upper traces, then lower-face fluxes
-  static constexpr int O_HL = O_TR + TE * LINE;      // [TM][TA][n][V] upper traces of the lower halo
-  static constexpr int O_HU = O_HL + TM * LINE;      // [TM][TA][n][V] lower traces of the upper halo
+  static constexpr int O_TR = O_DUDT + TE * TA * SLAB;  // [V][TE][TA][n] upper traces, then lower-face fluxes
+  static constexpr int O_HL = O_TR + TE * LINE;      // [V][TM][TA][n] upper traces of the lower halo
+  static constexpr int O_HU = O_HL + TM * LINE;      // [V][TM][TA][n] lower traces of the upper halo
   static constexpr int TOTAL = O_HU + TM * LINE;
   static_assert(NTH <= 256, "one block per tile");
   static_assert(2 * TM * TA * n <= NTH, "one halo line per thread");
@@ -141,9 +148,9 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
       if (!PREFETCH && a == 1) load_line(gx, gy, t, 1, lp, uy);
       trace(u, 0, qL);
       trace(u, nf - 1, qR);
-      B* d = sTR + ((el * T + t) * n + lp) * V;
+      B* d = sTR + (el * T + t) * n + lp;
 #pragma unroll
-      for (int v = 0; v < V; ++v) d[v] = qR[v];
+      for (int v = 0; v < V; ++v) d[v * L::TRV] = qR[v];
     }
     // halo traces: side 0 = the element below the tile's first one along a
     // (its upper trace), side 1 = the element above the tile's last element
@@ -173,9 +180,9 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
           else
             load_line(across, along, th, a, lh, uh);
           trace(uh, side == 0 ? nf - 1 : 0, q);
-          B* d = ((side == 0) ? sHL : sHU) + ((k * T + th) * n + lh) * V;
+          B* d = ((side == 0) ? sHL : sHU) + (k * T + th) * n + lh;
 #pragma unroll
-          for (int v = 0; v < V; ++v) d[v] = q[v];
+          for (int v = 0; v < V; ++v) d[v * L::HV] = q[v];
         }
       }
     }
@@ -189,10 +196,10 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
         for (int v = 0; v < V; ++v) qa[v] = qL[v];
         if (g.bc[a][0] == BC_REFLECTIVE) qa[a + 1] = -qa[a + 1];
       } else {
-        const B* s = (la > 0) ? sTR + (((el - nbr) * T + t) * n + lp) * V
-                              : sHL + ((lt * T + t) * n + lp) * V;
+        const B* s = (la > 0) ? sTR + ((el - nbr) * T + t) * n + lp : sHL + (lt * T + t) * n + lp;
+        const int sv = (la > 0) ? L::TRV : L::HV;
 #pragma unroll
-        for (int v = 0; v < V; ++v) qa[v] = s[v];
+        for (int v = 0; v < V; ++v) qa[v] = s[v * sv];
       }
       solve(qa, qL, a, F[0]);
       if (ca == Na - 1 || la == TA - 1) {
@@ -202,25 +209,25 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
           for (int v = 0; v < V; ++v) qb[v] = qR[v];
           if (g.bc[a][1] == BC_REFLECTIVE) qb[a + 1] = -qb[a + 1];
         } else {
-          const B* s = sHU + ((lt * T + t) * n + lp) * V;
+          const B* s = sHU + (lt * T + t) * n + lp;
 #pragma unroll
-          for (int v = 0; v < V; ++v) qb[v] = s[v];
+          for (int v = 0; v < V; ++v) qb[v] = s[v * L::HV];
         }
         solve(qR, qb, a, F[nf - 1]);
       }
     }
     __syncthreads();  // every upper trace has been read
     if (active) {
-      B* d = sTR + ((el * T + t) * n + lp) * V;
+      B* d = sTR + (el * T + t) * n + lp;
 #pragma unroll
-      for (int v = 0; v < V; ++v) d[v] = F[0][v];
+      for (int v = 0; v < V; ++v) d[v * L::TRV] = F[0][v];
     }
     __syncthreads();
     if (active) {
       if (!(ca == Na - 1 || la == TA - 1)) {
-        const B* s = sTR + (((el + nbr) * T + t) * n + lp) * V;
+        const B* s = sTR + ((el + nbr) * T + t) * n + lp;
 #pragma unroll
-        for (int v = 0; v < V; ++v) F[nf - 1][v] = s[v];
+        for (int v = 0; v < V; ++v) F[nf - 1][v] = s[v * L::TRV];
       }
       // interior flux points, derivative, accumulation (k_pred)
 #pragma unroll

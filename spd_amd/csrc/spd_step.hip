@@ -1707,7 +1707,7 @@ template <int NDIM, int P, typename B> struct Stepper {
 
   // Elements per block of the 2-D high-order flux kernel (LDS slab of the
   // block's elements, at most 48 KiB).
-  static int ho_epb_max() { return std::max(1, 49152 / (V * NL * (int)sizeof(B))); }
+  static int ho_epb_max() { return std::max(1, 49152 / (V * fv2d::ho_nlp(P) * (int)sizeof(B))); }
   static int ho_epb(const spd_ctx* c) {
     const int mx = ho_epb_max();
     // default: the element count of a 24 KiB FP64 slab in both forms (the
@@ -1735,8 +1735,8 @@ template <int NDIM, int P, typename B> struct Stepper {
     const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
     c->timed(KC_HOFLUX, s, [&] {
       hipLaunchKernelGGL((fv2d::k_ho2d<P, B>), dim3(blocks), dim3(256),
-                         (size_t)V * E * NL * sizeof(B), s, op, c->g, c->prm, i, Uad(c), Ff(c, 0),
-                         Ff(c, 1), Fh(c, 0), Fh(c, 1), E);
+                         (size_t)V * E * fv2d::ho_nlp(P) * sizeof(B), s, op, c->g, c->prm, i,
+                         Uad(c), Ff(c, 0), Ff(c, 1), Fh(c, 0), Fh(c, 1), E);
     });
     if (!c->cfg.FB) {
       apply(c, dt, w, c->Ucv, s);
