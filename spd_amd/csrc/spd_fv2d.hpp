@@ -923,8 +923,11 @@ __device__ __forceinline__ void ho2d_items(const Ops<P, B>& op, const Geom& g, c
   constexpr int a = A;
   const int Nx = g.N[0], Ny = g.N[1];
   for (int item = threadIdx.x; item < ne * nf; item += blockDim.x) {
-    const int rem = item;
-    const int el = rem / nf, f = rem % nf;
+    // x: consecutive items walk the flux points of an element (contiguous x
+    // faces); y: the elements at one flux point (the lanes of a wave share
+    // the face-or-interior branch and write along one row of y faces)
+    const int el = (a == 0) ? item / nf : item % ne;
+    const int f = (a == 0) ? item % nf : item / ne;
     const long long e = e0 + el;
     const int cx = (int)(e % Nx), cy = (int)(e / Nx);
     const int ca = (a == 0) ? cx : cy;

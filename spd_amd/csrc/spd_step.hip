@@ -683,6 +683,157 @@ k_wave_cells(Geom g, Params prm, int n, const double* __restrict__ Ucv,
 }
 
 // ---------------------------------------------------------------------------
+// 2-D sp <-> cv transforms of the time step through LDS: a block owns a row
+// segment of T2D_EB(P) elements (about 256 cells), loads all variables of its
+// cells with coalesced reads in the input layout, transforms them with the
+// separable passes of k_tensor_el (same arithmetic, same order) and writes
+// the output layout with coalesced stores.  cv -> sp (mode 0) also evaluates
+// the next dt's wave speed of every cell from the loaded FV state (as
+// k_wave_cells: block maxima in `partial`) and the optional snapshots (the
+// input in element layout, its primitives in element layout).
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int t2d_eb(int P) {
+  return 256 / ((P + 1) * (P + 1)) > 0 ? 256 / ((P + 1) * (P + 1)) : 1;
+}
+inline long long t2d_blocks(const Geom& g, int P) {
+  const int eb = t2d_eb(P);
+  return (long long)g.N[1] * ((g.N[0] + eb - 1) / eb);
+}
+
+template <int P> struct T2D {
+  static constexpr int V = 4, n = P + 1, NL = n * n, EB = t2d_eb(P), WP = EB * n + 1;
+  using Tile = double[V][n][WP];  // [variable][row j][x], x = element * n + i
+};
+
+// One row segment; FULL: ne == EB (constant index arithmetic).
+template <int P, int MODE, bool FULL>
+__device__ __forceinline__ void tensor2d_segment(const Ops<P>& op, const Geom& g, const Params& prm,
+                                                 const double* __restrict__ in,
+                                                 double* __restrict__ out,
+                                                 double* __restrict__ in_copy,
+                                                 double* __restrict__ Wcv_sd,
+                                                 double* __restrict__ partial, int ey, int ex0,
+                                                 int ne_, typename T2D<P>::Tile& su,
+                                                 typename T2D<P>::Tile& st) {
+  using T = T2D<P>;
+  constexpr int V = T::V, n = T::n, NL = T::NL, EB = T::EB;
+  const int ne = FULL ? EB : ne_;
+  const int wc = ne * n;  // cells of the segment along x
+  const long long e0 = (long long)ey * g.N[0] + ex0;
+  const long long row0 = (long long)ey * n * g.Ns[0] + (long long)ex0 * n;  // FV index of (j=0, x=0)
+  // every load of the thread is issued before the first is used (256 threads)
+  constexpr int NIT = (V * n * EB * n + 255) / 256;
+  double ld[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int k = threadIdx.x + it * 256;
+    ld[it] = 0.0;
+    if (k < V * n * wc) {
+      const int v = k / (n * wc), r = k % (n * wc);
+      if (MODE == 0)  // FV layout: rows of wc cells
+        ld[it] = in[(long long)v * g.Nc + row0 + (long long)(r / wc) * g.Ns[0] + r % wc];
+      else  // element layout: ne * NL contiguous values per variable
+        ld[it] = in[((long long)v * g.NE + e0) * NL + r];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int k = threadIdx.x + it * 256;
+    if (k < V * n * wc) {
+      const int v = k / (n * wc), r = k % (n * wc);
+      if (MODE == 0)
+        su[v][r / wc][r % wc] = ld[it];
+      else
+        su[v][(r % NL) / n][(r / NL) * n + (r % NL) % n] = ld[it];
+    }
+  }
+  __syncthreads();
+  const double(*M)[n] = (MODE == 1) ? op.s2c : op.c2s;
+  for (int k = threadIdx.x; k < V * n * ne; k += 256) {  // x pass
+    const int v = k / (n * ne), r = k % (n * ne), j = r / ne, el = r % ne;
+    double u[n];
+#pragma unroll
+    for (int i = 0; i < n; ++i) u[i] = su[v][j][el * n + i];
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < n; ++q) acc += M[i][q] * u[q];
+      st[v][j][el * n + i] = acc;
+    }
+  }
+  if (MODE == 0) {
+    double cw = 0.0;
+    for (int k = threadIdx.x; k < n * wc; k += 256) {
+      const int j = k / wc, x = k % wc;
+      double u[V], W[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) u[v] = su[v][j][x];
+      prims<2>(u, W, 0, prm.gamma);
+      if (Wcv_sd) {
+        const long long e = e0 + x / n;
+        const int l = j * n + x % n;
+#pragma unroll
+        for (int v = 0; v < V; ++v) Wcv_sd[((long long)v * g.NE + e) * NL + l] = W[v];
+      }
+      double c = sound(W[3], W[0], prm.gamma, prm.min_c2) * 2;
+      c = c + fabs(W[1]);
+      c = c + fabs(W[2]);
+      cw = nmax(cw, c);
+    }
+    cw = block_max(cw);
+    if (threadIdx.x == 0) partial[blockIdx.x] = cw;
+    if (in_copy) {
+      for (int k = threadIdx.x; k < V * n * wc; k += 256) {
+        const int v = k / (n * wc), r = k % (n * wc), el = r / NL, l = r % NL;
+        in_copy[((long long)v * g.NE + e0) * NL + r] = su[v][l / n][el * n + l % n];
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < V * wc; k += 256) {  // y pass, one column per item
+    const int v = k / wc, x = k % wc;
+    double t[n];
+#pragma unroll
+    for (int q = 0; q < n; ++q) t[q] = st[v][q][x];
+#pragma unroll
+    for (int jo = 0; jo < n; ++jo) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < n; ++q) acc += M[jo][q] * t[q];
+      st[v][jo][x] = acc;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < V * n * wc; k += 256) {
+    const int v = k / (n * wc), r = k % (n * wc);
+    if (MODE == 1)
+      out[(long long)v * g.Nc + row0 + (long long)(r / wc) * g.Ns[0] + r % wc] = st[v][r / wc][r % wc];
+    else
+      out[((long long)v * g.NE + e0) * NL + r] = st[v][(r % NL) / n][(r / NL) * n + (r % NL) % n];
+  }
+}
+
+// MODE: 1 sp -> cv, 0 cv -> sp (with the wave speed).  Blocks of 256 threads.
+template <int P, int MODE>
+__global__ void __launch_bounds__(256)
+k_tensor2d(Ops<P> op, Geom g, Params prm, const double* __restrict__ in, double* __restrict__ out,
+           double* __restrict__ in_copy, double* __restrict__ Wcv_sd, double* __restrict__ partial) {
+  SPD_SKIP(g);
+  constexpr int EB = T2D<P>::EB;
+  __shared__ typename T2D<P>::Tile su, st;  // input; after the x pass, then the output
+  const int nbx = (g.N[0] + EB - 1) / EB;
+  const int ey = (int)(blockIdx.x / nbx), ex0 = (int)(blockIdx.x % nbx) * EB;
+  const int ne = (g.N[0] - ex0 < EB) ? g.N[0] - ex0 : EB;
+  if (ne == EB)
+    tensor2d_segment<P, MODE, true>(op, g, prm, in, out, in_copy, Wcv_sd, partial, ey, ex0, ne, su,
+                                    st);
+  else
+    tensor2d_segment<P, MODE, false>(op, g, prm, in, out, in_copy, Wcv_sd, partial, ey, ex0, ne,
+                                     su, st);
+}
+
+// ---------------------------------------------------------------------------
 // FV update U_out = U_in - dt_i * sum_a (F_a[f+1] - F_a[f]) / dx_a
 // (src/fv_simulator.py:189-205); in place when U_out == U_in.
 // ---------------------------------------------------------------------------
@@ -1864,7 +2015,11 @@ template <int NDIM, int P, typename B> struct Stepper {
     }
     // --- FV corrector (src/sdader_simulator.py:288-349)
     c->timed(KC_TENSOR, s, [&] {
-      if constexpr (NDIM <= 2)
+      if constexpr (NDIM == 2)
+        hipLaunchKernelGGL((k_tensor2d<P, 1>), dim3((unsigned)t2d_blocks(c->g, P)), dim3(256), 0,
+                           s, op, c->g, c->prm, Usp, c->Ucv, (double*)nullptr, (double*)nullptr,
+                           (double*)nullptr);
+      else if constexpr (NDIM == 1)
         hipLaunchKernelGGL((k_tensor_el<NDIM, P>), dim3(grid_for(c->g.NE * V)), dim3(256), 0, s,
                            op, c->g, 1, Usp, 0, c->Ucv, 1, (double*)nullptr);
       else
@@ -1918,8 +2073,15 @@ template <int NDIM, int P, typename B> struct Stepper {
       }
       apply(c, dt, w, c->Ucv, s);
     }
-    const unsigned nb = grid_for(c->g.NE * NL);
-    if constexpr (NDIM <= 2) {
+    unsigned nb = grid_for(c->g.NE * NL);
+    if constexpr (NDIM == 2) {
+      nb = (unsigned)t2d_blocks(c->g, P);
+      c->timed(KC_CV2SP, s, [&] {
+        hipLaunchKernelGGL((k_tensor2d<P, 0>), dim3(nb), dim3(256), 0, s, op, c->g, c->prm,
+                           c->Ucv, Usp, snapshot ? Ucv_sd : nullptr, snapshot ? Wcv_sd : nullptr,
+                           c->partial);
+      });
+    } else if constexpr (NDIM == 1) {
       c->timed(KC_CV2SP, s, [&] {
         hipLaunchKernelGGL((k_tensor_el<NDIM, P>), dim3(grid_for(c->g.NE * V)), dim3(256), 0, s,
                            op, c->g, 0, c->Ucv, 1, Usp, 0, snapshot ? Ucv_sd : nullptr);
@@ -2204,6 +2366,7 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
   c->g.skip = c->zero_flag;
   if ((rc = c->alloc(&c->cmax, 1)) != SPD_OK) return rc;
   c->npartial = (g.NE * c->NL + 255) / 256;
+  if (nd == 2) c->npartial = std::max(c->npartial, t2d_blocks(g, c->cfg.p));
   if ((rc = c->alloc(&c->partial, (size_t)c->npartial)) != SPD_OK) return rc;
   SPD_HIP_CHECK(hipMemset(c->any_flag, 0, sizeof(int)));
   SPD_HIP_CHECK(hipMemset(c->cmax, 0, sizeof(double)));
