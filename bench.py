@@ -78,9 +78,10 @@ def algorithmic_bytes(N, p, launches):
       k_theta1   T launches (k_theta2d): read C flags, write G
       k_fbface   T launches (k_fbmarch2d): read U_cv, theta, both face arrays,
                  write V C (at most: cells with no blended face keep the
-                 candidate)
-      k_cv2sp    twice (cv->sp transform; wave-speed maximum of the cells):
-                 read V C and write V C, then read V C
+                 candidate; rows without a blended face are skipped, so the
+                 launch reads less where few cells are troubled)
+      k_cv2sp    once (k_tensor2d cv->sp with the wave-speed maximum of the
+                 cells fused): read V C and write V C
     """
     V, n = 4, p + 1
     T = n
@@ -99,7 +100,7 @@ def algorithmic_bytes(N, p, launches):
         "k_detect": T * (2 * V * C),
         "k_theta1": T * G,
         "k_fbface": T * (2 * V * C + G + 2 * Fd),
-        "k_cv2sp": 3 * V * C,
+        "k_cv2sp": 2 * V * C,
     }
     extra_bytes = {"k_detect": T * C * 1.0, "k_theta1": T * C * 1.0}
     out = {}
@@ -124,8 +125,8 @@ CLASS_KERNELS = {
     "k_hoflux": [("k_ho2d<3, double>", 4)],
     "k_apply": [("k_apply<2, double>", 4)],
     "k_theta1": [("k_theta2d", 4)],
-    "k_cv2sp": [("k_tensor_el<2, 3>", 1), ("k_wave_cells<2>", 1)],
-    "k_tensor": [("k_tensor_el<2, 3>", 1)],
+    "k_cv2sp": [("k_tensor2d<3, 0>", 1)],
+    "k_tensor": [("k_tensor2d<3, 1>", 1)],
 }
 TRAFFIC_SUMMARY = os.path.join(HERE, "profiles", "kh1024_p3_hbm_traffic.md")
 FLOPS_SUMMARY = os.path.join(HERE, "profiles", "kh1024_p3_fp64_flops.md")

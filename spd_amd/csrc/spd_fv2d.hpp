@@ -703,18 +703,23 @@ k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
   // blended, so row k's Hancock state is never used, cell row k - 1 keeps the
   // candidate and the fluxes of row k are not needed.  Quiet rows are
   // skipped; the row window is reloaded at the next row that is not.
+  // All rows' theta are loaded at once (the march's registers are not live
+  // yet); chunks of more than 34 rows are never skipped.
+  constexpr int NBR = 40;
   unsigned long long busy = ~0ull;
-  if (!godunov) {
-    busy = 0;
-    const int nr = y1 - y0 + 6;  // MR <= 58
-    for (int r0 = 0; r0 < nr; r0 += 8) {
-      double tv[8];
+  const int nr = y1 - y0 + 6;
+  if (!godunov && nr <= NBR) {
+    double tv[NBR];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) tv[j] = (r0 + j < nr) ? theta_at(y0 - 3 + r0 + j) : 0.0;
+    for (int r = 0; r < NBR; ++r) tv[r] = (r < nr) ? theta_at(y0 - 3 + r) : 0.0;
+    unsigned long long lb = 0;  // the lane's rows, then OR over the wave
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (__ballot(!(tv[j] == 0.0))) busy |= 1ull << (r0 + j);
-    }
+    for (int r = 0; r < NBR; ++r)
+      if (!(tv[r] == 0.0)) lb |= 1ull << r;
+#pragma unroll
+    for (int off = MW / 2; off > 0; off >>= 1) lb |= __shfl_xor(lb, off, MW);
+    busy = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(lb >> 32)) << 32) |
+           (unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)lb);
     if (busy == 0) return;  // the candidate stands everywhere in the chunk
   }
   auto quiet = [&](int k) { return ((busy >> (k - y0 + 1)) & 31ull) == 0; };
