@@ -7,6 +7,8 @@
 // is included into (it uses trouble_at, ring1, kRules2 and Ops from there).
 #pragma once
 
+#include <type_traits>
+
 namespace spd {
 namespace fv2d {
 
@@ -724,162 +726,175 @@ k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
   }
   auto quiet = [&](int k) { return ((busy >> (k - y0 + 1)) & 31ull) == 0; };
 
-  B Wm[4], W0[4], Wp[4];
-  fetch(y0 - 2);
-  to_W(Wm);
-  fetch(y0 - 1);
-  to_W(W0);
-  fetch(y0);
-  double th = theta_at(y0 - 1), thn = theta_at(y0);
-  double thm = 0.0;                  // theta of row k - 1
-  double tfxR = 0.0, tfyB = 0.0;     // coefficients of FxR and FyB
-  B Qm[4] = {B(0), B(0), B(0), B(0)};  // Hancock state + half y-slope of row k - 1
-  B FxR[4] = {B(0), B(0), B(0), B(0)};  // blended flux at (k - 1, x + 1/2)
-  B FyB[4] = {B(0), B(0), B(0), B(0)};  // blended flux at y-face k - 1
-  bool primed = true;  // Wm, W0, un, th, thn hold rows k - 1, k, k + 1
-  for (int k = y0 - 1; k <= y1; ++k) {
-    if (quiet(k)) {
-      // theta = 0 on rows k - 2 .. k + 2: nothing of rows k - 1 .. k is
-      // blended, and the state of the next busy row is reloaded
-      primed = false;
-      thm = 0.0;
-      tfxR = 0.0;
-      tfyB = 0.0;
-      continue;
-    }
-    if (!primed) {
-      fetch(k - 1);
-      to_W(Wm);
-      fetch(k);
-      to_W(W0);
-      fetch(k + 1);
-      th = theta_at(k);
-      thn = theta_at(k + 1);
-      primed = true;
-    }
-    to_W(Wp);  // row k + 1
-    fetch(k + 2);
-    const double thk = th;
-    th = thn;
-    thn = theta_at(k + 2);
-    // high-order fluxes this row needs: y-face k, x-face (k, x + 1/2)
-    B Fyk[4], Fxk[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      Fyk[v] = (k >= y0 && fyin) ? Fy[v * nfy + (long long)k * Nx + x] : B(0);
-      Fxk[v] = (k >= y0 && k < y1 && fxin) ? Fx[v * nfx + (long long)k * (Nx + 1) + x + 1] : B(0);
-    }
-
-    // Hancock state and half slopes of row k (as k_fbcorrect2d step 2)
-    const int Gy = k + NGH;  // k in [-1, Ny]
-    const B rhly = (B)g.rhcv[1][Gy - 1], rhry = (B)g.rhcv[1][Gy], hfy = (B)g.hfp[1][Gy];
-    B Mp[4], Sx[4], Sy[4], dMx[4], dMy[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const B Wl = from_left(W0[v]), Wr = from_right(W0[v]);
-      const B Lx = (W0[v] - Wl) * rhlx, Rx = (Wr - W0[v]) * rhrx;
-      const B Ly = (W0[v] - Wm[v]) * rhly, Ry = (Wp[v] - W0[v]) * rhry;
-      if (prm.moncen) {
-        dMx[v] = moncen<B>(Lx, Rx, hclx, hcrx, hfx);
-        dMy[v] = moncen<B>(Ly, Ry, (B)g.hcv[1][Gy - 1], (B)g.hcv[1][Gy], hfy);
-      } else {
-        dMx[v] = minmod<B>(Lx, Rx);
-        dMy[v] = minmod<B>(Ly, Ry);
+  // SKIP: the loop with the quiet-row tests, for chunks that have quiet rows;
+  // chunks without one run the plain loop
+  auto march = [&](auto skip) {
+    constexpr bool SKIP = decltype(skip)::value;
+    B Wm[4], W0[4], Wp[4];
+    fetch(y0 - 2);
+    to_W(Wm);
+    fetch(y0 - 1);
+    to_W(W0);
+    fetch(y0);
+    double th = theta_at(y0 - 1), thn = theta_at(y0);
+    double thm = 0.0;                  // theta of row k - 1
+    double tfxR = 0.0, tfyB = 0.0;     // coefficients of FxR and FyB
+    B Qm[4] = {B(0), B(0), B(0), B(0)};  // Hancock state + half y-slope of row k - 1
+    B FxR[4] = {B(0), B(0), B(0), B(0)};  // blended flux at (k - 1, x + 1/2)
+    B FyB[4] = {B(0), B(0), B(0), B(0)};  // blended flux at y-face k - 1
+    bool primed = true;  // Wm, W0, un, th, thn hold rows k - 1, k, k + 1
+    for (int k = y0 - 1; k <= y1; ++k) {
+      if (SKIP && quiet(k)) {
+        // theta = 0 on rows k - 2 .. k + 2: nothing of rows k - 1 .. k is
+        // blended, and the state of the next busy row is reloaded
+        primed = false;
+        thm = 0.0;
+        tfxR = 0.0;
+        tfyB = 0.0;
+        continue;
       }
-      Sx[v] = B(0.5) * dMx[v] * hfx;
-      Sy[v] = B(0.5) * dMy[v] * hfy;
-    }
-    {
-      B dtW[4] = {B(0), B(0), B(0), B(0)};
-      const B rrho = qrcp<B>(W0[0]);
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int vel = b + 1;
-        const B* dW = (b == 0) ? dMx : dMy;
-        dtW[0] = dtW[0] - (W0[vel] * dW[0] + W0[0] * dW[vel]);
-        dtW[3] = dtW[3] - (W0[vel] * dW[3] + gamma * W0[3] * dW[vel]);
-        dtW[vel] = dtW[vel] - (W0[vel] * dW[vel] + dW[3] * rrho);
-        dtW[2] = dtW[2] - W0[vel] * dW[2];
+      if (SKIP && !primed) {
+        fetch(k - 1);
+        to_W(Wm);
+        fetch(k);
+        to_W(W0);
+        fetch(k + 1);
+        th = theta_at(k);
+        thn = theta_at(k + 1);
+        primed = true;
       }
-      dtW[3] = dtW[3] * B(0);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Mp[v] = W0[v] + B(0.5) * dtW[v] * dtib;
-    }
-    // lane + 1's lower x-face state, its theta
-    B WRx[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) WRx[v] = from_right(Mp[v] - Sx[v]);
-    const double thr = from_right(thk);
+      to_W(Wp);  // row k + 1
+      fetch(k + 2);
+      const double thk = th;
+      th = thn;
+      thn = theta_at(k + 2);
+      // high-order fluxes this row needs: y-face k, x-face (k, x + 1/2)
+      B Fyk[4], Fxk[4];
+  #pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        Fyk[v] = (k >= y0 && fyin) ? Fy[v * nfy + (long long)k * Nx + x] : B(0);
+        Fxk[v] = (k >= y0 && k < y1 && fxin) ? Fx[v * nfx + (long long)k * (Nx + 1) + x + 1] : B(0);
+      }
 
-    if (k >= y0) {
-      // y-face k, between rows k - 1 and k
-      double tf = godunov ? 1.0 : nmax(thm, thk);
-      if (fyin && !(tf == 0.0)) {
-        B WL[4], WR[4], UL[4], UR[4], Ff[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          WL[v] = Qm[v];
-          WR[v] = Mp[v] - Sy[v];
+      // Hancock state and half slopes of row k (as k_fbcorrect2d step 2)
+      const int Gy = k + NGH;  // k in [-1, Ny]
+      const B rhly = (B)g.rhcv[1][Gy - 1], rhry = (B)g.rhcv[1][Gy], hfy = (B)g.hfp[1][Gy];
+      B Mp[4], Sx[4], Sy[4], dMx[4], dMy[4];
+  #pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const B Wl = from_left(W0[v]), Wr = from_right(W0[v]);
+        const B Lx = (W0[v] - Wl) * rhlx, Rx = (Wr - W0[v]) * rhrx;
+        const B Ly = (W0[v] - Wm[v]) * rhly, Ry = (Wp[v] - W0[v]) * rhry;
+        if (prm.moncen) {
+          dMx[v] = moncen<B>(Lx, Rx, hclx, hcrx, hfx);
+          dMy[v] = moncen<B>(Ly, Ry, (B)g.hcv[1][Gy - 1], (B)g.hcv[1][Gy], hfy);
+        } else {
+          dMx[v] = minmod<B>(Lx, Rx);
+          dMy[v] = minmod<B>(Ly, Ry);
         }
-        cons<2>(WL, UL, 1, gamma);
-        cons<2>(WR, UR, 1, gamma);
-        riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, mc2, prm.lowmach_fv);
-        const B tb = (B)tf;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Fyk[v] = tb * Ff[v] + (B(1) - tb) * Fyk[v];
+        Sx[v] = B(0.5) * dMx[v] * hfx;
+        Sy[v] = B(0.5) * dMy[v] * hfy;
       }
-      // update of row k - 1
-      B FxL[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) FxL[v] = from_left(FxR[v]);
-      const double tfxL = from_left(tfxR);
-      if (k - 1 >= y0 && out) {
-        const bool touched =
-            godunov || !(tfxL == 0.0) || !(tfxR == 0.0) || !(tfyB == 0.0) || !(tf == 0.0);
-        if (touched) {
-          const int y = k - 1;
-          const double rdy = g.rhfp[1][y + NGH];
-          const long long ci = (long long)y * Nx + x;
-#pragma unroll
+      {
+        B dtW[4] = {B(0), B(0), B(0), B(0)};
+        const B rrho = qrcp<B>(W0[0]);
+  #pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const int vel = b + 1;
+          const B* dW = (b == 0) ? dMx : dMy;
+          dtW[0] = dtW[0] - (W0[vel] * dW[0] + W0[0] * dW[vel]);
+          dtW[3] = dtW[3] - (W0[vel] * dW[3] + gamma * W0[3] * dW[vel]);
+          dtW[vel] = dtW[vel] - (W0[vel] * dW[vel] + dW[3] * rrho);
+          dtW[2] = dtW[2] - W0[vel] * dW[2];
+        }
+        dtW[3] = dtW[3] * B(0);
+  #pragma unroll
+        for (int v = 0; v < 4; ++v) Mp[v] = W0[v] + B(0.5) * dtW[v] * dtib;
+      }
+      // lane + 1's lower x-face state, its theta
+      B WRx[4];
+  #pragma unroll
+      for (int v = 0; v < 4; ++v) WRx[v] = from_right(Mp[v] - Sx[v]);
+      const double thr = from_right(thk);
+
+      if (k >= y0) {
+        // y-face k, between rows k - 1 and k
+        double tf = godunov ? 1.0 : nmax(thm, thk);
+        if (fyin && !(tf == 0.0)) {
+          B WL[4], WR[4], UL[4], UR[4], Ff[4];
+  #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const double u = Ucv[v * g.Nc + ci];
-            double d = u * 0.0;
-            d = d + ((double)FxR[v] - (double)FxL[v]) * rdx;
-            d = d + ((double)Fyk[v] - (double)FyB[v]) * rdy;
-            Uout[v * g.Nc + ci] = u - d * dti;
+            WL[v] = Qm[v];
+            WR[v] = Mp[v] - Sy[v];
+          }
+          cons<2>(WL, UL, 1, gamma);
+          cons<2>(WR, UR, 1, gamma);
+          riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, mc2, prm.lowmach_fv);
+          const B tb = (B)tf;
+  #pragma unroll
+          for (int v = 0; v < 4; ++v) Fyk[v] = tb * Ff[v] + (B(1) - tb) * Fyk[v];
+        }
+        // update of row k - 1
+        B FxL[4];
+  #pragma unroll
+        for (int v = 0; v < 4; ++v) FxL[v] = from_left(FxR[v]);
+        const double tfxL = from_left(tfxR);
+        if (k - 1 >= y0 && out) {
+          const bool touched =
+              godunov || !(tfxL == 0.0) || !(tfxR == 0.0) || !(tfyB == 0.0) || !(tf == 0.0);
+          if (touched) {
+            const int y = k - 1;
+            const double rdy = g.rhfp[1][y + NGH];
+            const long long ci = (long long)y * Nx + x;
+  #pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const double u = Ucv[v * g.Nc + ci];
+              double d = u * 0.0;
+              d = d + ((double)FxR[v] - (double)FxL[v]) * rdx;
+              d = d + ((double)Fyk[v] - (double)FyB[v]) * rdy;
+              Uout[v * g.Nc + ci] = u - d * dti;
+            }
           }
         }
+  #pragma unroll
+        for (int v = 0; v < 4; ++v) FyB[v] = Fyk[v];
+        tfyB = tf;
       }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) FyB[v] = Fyk[v];
-      tfyB = tf;
-    }
-    if (k >= y0 && k < y1) {
-      // x-face (k, x + 1/2)
-      double tf = godunov ? 1.0 : nmax(thk, thr);
-      if (fxin && !(tf == 0.0)) {
-        B WL[4], UL[4], UR[4], Ff[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) WL[v] = Mp[v] + Sx[v];
-        cons<2>(WL, UL, 0, gamma);
-        cons<2>(WRx, UR, 0, gamma);
-        riemann<2, RS>(WL, WRx, UL, UR, Ff, 0, gamma, mc2, prm.lowmach_fv);
-        const B tb = (B)tf;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Fxk[v] = tb * Ff[v] + (B(1) - tb) * Fxk[v];
+      if (k >= y0 && k < y1) {
+        // x-face (k, x + 1/2)
+        double tf = godunov ? 1.0 : nmax(thk, thr);
+        if (fxin && !(tf == 0.0)) {
+          B WL[4], UL[4], UR[4], Ff[4];
+  #pragma unroll
+          for (int v = 0; v < 4; ++v) WL[v] = Mp[v] + Sx[v];
+          cons<2>(WL, UL, 0, gamma);
+          cons<2>(WRx, UR, 0, gamma);
+          riemann<2, RS>(WL, WRx, UL, UR, Ff, 0, gamma, mc2, prm.lowmach_fv);
+          const B tb = (B)tf;
+  #pragma unroll
+          for (int v = 0; v < 4; ++v) Fxk[v] = tb * Ff[v] + (B(1) - tb) * Fxk[v];
+        }
+  #pragma unroll
+        for (int v = 0; v < 4; ++v) FxR[v] = Fxk[v];
+        tfxR = tf;
       }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) FxR[v] = Fxk[v];
-      tfxR = tf;
+  #pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        Qm[v] = Mp[v] + Sy[v];
+        Wm[v] = W0[v];
+        W0[v] = Wp[v];
+      }
+      thm = thk;
     }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      Qm[v] = Mp[v] + Sy[v];
-      Wm[v] = W0[v];
-      W0[v] = Wp[v];
-    }
-    thm = thk;
-  }
+  };
+  // bit i of wnd: a busy row among bits i .. i + 4 of busy (row k: i = k - y0 + 1)
+  const unsigned long long wnd = busy | (busy >> 1) | (busy >> 2) | (busy >> 3) | (busy >> 4);
+  const int nk = y1 - y0 + 2;  // rows k = y0 - 1 .. y1
+  const unsigned long long all = (nk >= 64) ? ~0ull : ((1ull << nk) - 1);
+  if ((wnd & all) == all)
+    march(std::false_type{});
+  else
+    march(std::true_type{});
 }
 
 // ---------------------------------------------------------------------------
