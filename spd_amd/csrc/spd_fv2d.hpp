@@ -276,10 +276,11 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
 // Blending coefficients theta on the ghosted grid (src/trouble_detection.py:
 // 73-87,139-169): trouble values on the tile + 2 halo, theta1 on tile + 1
 // halo, then the x and y 0.25 rings.  Tiles cover the ghosted grid; (gx, gy)
-// are ghosted indices.  Without blending theta = trouble.
+// are ghosted indices.  Without blending theta = trouble.  Tiles of active
+// cells take the byte form below (flags only, no FP64 work); tiles that reach
+// the ghost layers take the general form.
 // ---------------------------------------------------------------------------
-// 64 x 16 outputs per block of 256 threads (four per thread): the kernel is
-// latency-bound, so fewer, larger tiles.
+// 64 x 16 outputs per block of 256 threads (four per thread).
 constexpr int HTX = 64, HTY = 16, HNT = 256;
 constexpr int TH = 2;
 constexpr int TNX = HTX + 2 * TH, TNY = HTY + 2 * TH;
