@@ -165,12 +165,15 @@ def measured_fp64_flops(cls):
         return None
     names = dict(entries)
     vals = {}
+    col = None  # the column headed "FP64 GFLOP per dispatch"
     with open(FLOPS_SUMMARY) as f:
         for line in f:
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) >= 8 and cells[0].strip("`") in names:
-                try:
-                    vals[cells[0].strip("`")] = float(cells[6]) * 1e9
+            if "FP64 GFLOP per dispatch" in cells:
+                col = cells.index("FP64 GFLOP per dispatch")
+            elif col is not None and len(cells) > col and cells[0].strip("`") in names:
+                try:  # the first table (the bench window) holds
+                    vals.setdefault(cells[0].strip("`"), float(cells[col]) * 1e9)
                 except ValueError:
                     pass
     if len(vals) != len(names):
