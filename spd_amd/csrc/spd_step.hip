@@ -1861,10 +1861,13 @@ template <int NDIM, int P, typename B> struct Stepper {
   static int ho_epb_max() { return std::max(1, 49152 / (V * fv2d::ho_nlp(P) * (int)sizeof(B))); }
   static int ho_epb(const spd_ctx* c) {
     const int mx = ho_epb_max();
-    // default: the element count of a 24 KiB FP64 slab in both forms (the
-    // best of a sweep at Kelvin-Helmholtz 1024^2 p=3 in both precisions)
+    // default: the element count of a 24 KiB FP64 slab in both forms; p = 3:
+    // 64 elements (a sweep at Kelvin-Helmholtz 1024^2 p=3 over 16..80 in both
+    // precisions: 64 is the fastest, its blocks cover whole 64-element row
+    // segments)
+    const int dflt = (P == 3) ? 64 : std::max(1, 24576 / (V * NL * 8));
     return c->launch[SPD_LAUNCH_HO_EPB] > 0 ? std::min(mx, c->launch[SPD_LAUNCH_HO_EPB])
-                                            : std::min(mx, std::max(1, 24576 / (V * NL * 8)));
+                                            : std::min(mx, dflt);
   }
   // Form of the 2-D fallback correction: 1 cell tiles, 2 column strips (default).
   static int fb_form(const spd_ctx* c) {
