@@ -68,8 +68,10 @@ def algorithmic_bytes(N, p, launches):
       k_face_sd  2 launches (the final predictor state, for the corrector):
                  read T V C, write the element-face fluxes Fe
       k_pred     m launches (k_pred2d, element faces solved inside): read the
-                 state (T V C; the first iteration V C of U_sp) and U_sp (V C),
-                 write T V C (the one-element halo reads are not counted)
+                 state (T V C) and U_sp (V C), write T V C; the first iteration
+                 reads U_sp and, when another follows, writes its increment
+                 (V C), which the second reads with U_sp (2 V C) in place of
+                 the state (the one-element halo reads are not counted)
       k_hoflux   T launches (k_ho2d): read the node's state slice (V C) and its
                  element-face fluxes (2 V (N+1) N n), write both FV-face arrays
       k_tensor   once: read + write V C
@@ -93,7 +95,8 @@ def algorithmic_bytes(N, p, launches):
     G = (N * n + 4) ** 2
     per_step = {
         "k_face_sd": 2 * (T * V * C + Fe),
-        "k_pred": (V * C + V * C + T * V * C) + (m - 1) * (T * V * C + V * C + T * V * C),
+        "k_pred": ((2 * V * C) + (2 * V * C + T * V * C) + (m - 2) * (2 * T * V * C + V * C)
+                   if m >= 2 else V * C + T * V * C),
         "k_hoflux": T * (V * C + 2 * V * (N + 1) * N * n + 2 * Fd),
         "k_tensor": 2 * V * C,
         "k_apply": T * (2 * V * C + 2 * Fd),

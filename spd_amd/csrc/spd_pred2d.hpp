@@ -65,10 +65,18 @@ template <int P, bool FIRST> struct Layout {
 };
 
 // FIRST: the first iteration, src = U_sp (Tsrc = 1); otherwise Tsrc = T.
-template <int P, int RS, typename S, typename B, bool FIRST>
+// Compact first iterate: the first iteration's Picard update gives every node
+// the same increment, out[tn] = U_sp - dt (sum_p inv[tn][p]) dudt, so with
+// out_cmp it writes dudt alone (one node, layout [v][e][l]) and the next
+// iteration (SRC_CMP) forms each node's value from U_sp and dudt as it loads
+// it, with the same operations in the same order (the same values): one node
+// of writes and one of reads instead of T each.
+template <int P, int RS, typename S, typename B, bool FIRST, bool SRC_CMP>
 __global__ void __launch_bounds__(256)
 k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
-         const double* __restrict__ Usp, const double* __restrict__ dt_dev, B* __restrict__ out) {
+         const double* __restrict__ Usp, const double* __restrict__ dt_dev, B* __restrict__ out,
+         int out_cmp) {
+  static_assert(!(FIRST && SRC_CMP), "the first iteration reads U_sp");
   SPD_SKIP(g);
   using L = Layout<P, FIRST>;
   constexpr int V = 4, n = L::n, nf = L::nf, T = L::TA, NL = L::NL;
@@ -92,15 +100,36 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
   const B gam = (B)prm.gamma, mc2 = (B)prm.min_c2;
 
   // line `l` of element (xe, ye) along dimension a at time node tt
+  const double dt_src = SRC_CMP ? *dt_dev : 0.0;
   auto load_line = [&](int xe, int ye, int tt, int a, int l, B (*u)[n]) {
     const long long e = (long long)ye * Nx + xe;
     const int lb = (a == 0) ? l * n : l;
     const int sa = (a == 0) ? 1 : n;
+    if constexpr (SRC_CMP) {
+      // node tt of the first iterate from U_sp and its increment (the first
+      // iteration's Picard update, below)
+      B cn[L::T];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const S* s0 = src + ((long long)(tt * V + v) * NE + e) * NL + lb;
+      for (int p = 0; p < L::T; ++p) cn[p] = op.inv[tt][p];
 #pragma unroll
-      for (int s = 0; s < n; ++s) u[v][s] = (B)s0[s * sa];
+      for (int v = 0; v < V; ++v) {
+        const long long o = ((long long)v * NE + e) * NL + lb;
+#pragma unroll
+        for (int s = 0; s < n; ++s) {
+          const B d0 = (B)src[o + s * sa];
+          B acc = B(0);
+#pragma unroll
+          for (int p = 0; p < L::T; ++p) acc += cn[p] * d0;
+          u[v][s] = (B)(Usp[o + s * sa] - (double)acc * dt_src);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const S* s0 = src + ((long long)(tt * V + v) * NE + e) * NL + lb;
+#pragma unroll
+        for (int s = 0; s < n; ++s) u[v][s] = (B)s0[s * sa];
+      }
     }
   };
   auto trace = [&](B (*u)[n], int fp, B* q) {
@@ -276,8 +305,12 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
     if (xe >= Nx || ye >= Ny) continue;
     const long long e = (long long)ye * Nx + xe;
     const long long off = ((long long)v * NE + e) * NL + l;
-    const double u0 = Usp[off];
     const B* dd = dudt + (long long)le * T * SLAB + v * NL + l;
+    if (FIRST && out_cmp) {  // the increment alone (compact first iterate)
+      out[off] = dd[0];
+      continue;
+    }
+    const double u0 = Usp[off];
 #pragma unroll
     for (int tn = 0; tn < L::T; ++tn) {
       B acc = B(0);

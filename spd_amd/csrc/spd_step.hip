@@ -1780,33 +1780,37 @@ template <int NDIM, int P, typename B> struct Stepper {
     else
       faces_rs<RS_LLF>(c, op, src, Tsrc, s);
   }
-  // fused 2-D predictor iteration (element faces solved in the kernel)
-  template <int RS, bool FIRST, typename S>
+  // fused 2-D predictor iteration (element faces solved in the kernel).
+  // cmp: 1 the first iteration writes its compact form (the increment of
+  // one node), 2 the iteration reads the first iterate in that form.
+  template <int RS, bool FIRST, bool SRC_CMP, typename S>
   static void pred2d_launch(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
-                            const double* dt, B* out, hipStream_t s) {
+                            const double* dt, B* out, int out_cmp, hipStream_t s) {
     using L = pred2d::Layout<P, FIRST>;
     const int ntx = (c->g.N[0] + L::TX - 1) / L::TX, nty = (c->g.N[1] + L::TY - 1) / L::TY;
     c->timed(KC_PRED, s, [&] {
-      hipLaunchKernelGGL((pred2d::k_pred2d<P, RS, S, B, FIRST>), dim3((unsigned)(ntx * nty)),
-                         dim3(L::NTH), (size_t)L::TOTAL * sizeof(B), s, op, c->g, c->prm, src,
-                         Tsrc, Usp, dt, out);
+      hipLaunchKernelGGL((pred2d::k_pred2d<P, RS, S, B, FIRST, SRC_CMP>),
+                         dim3((unsigned)(ntx * nty)), dim3(L::NTH), (size_t)L::TOTAL * sizeof(B), s,
+                         op, c->g, c->prm, src, Tsrc, Usp, dt, out, out_cmp);
     });
   }
   template <int RS, typename S>
   static void pred2d_rs(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
-                        const double* dt, B* out, hipStream_t s) {
+                        const double* dt, B* out, int cmp, hipStream_t s) {
     if (Tsrc == 1)
-      pred2d_launch<RS, true>(c, op, src, Tsrc, Usp, dt, out, s);
+      pred2d_launch<RS, true, false>(c, op, src, Tsrc, Usp, dt, out, cmp == 1 ? 1 : 0, s);
+    else if (cmp == 2)
+      pred2d_launch<RS, false, true>(c, op, src, Tsrc, Usp, dt, out, 0, s);
     else
-      pred2d_launch<RS, false>(c, op, src, Tsrc, Usp, dt, out, s);
+      pred2d_launch<RS, false, false>(c, op, src, Tsrc, Usp, dt, out, 0, s);
   }
   template <typename S>
   static void pred2d(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, const double* Usp,
-                     const double* dt, B* out, hipStream_t s) {
+                     const double* dt, B* out, int cmp, hipStream_t s) {
     if (c->cfg.riemann_sd != SPD_RS_LLF)
-      pred2d_rs<RS_HLLC>(c, op, src, Tsrc, Usp, dt, out, s);
+      pred2d_rs<RS_HLLC>(c, op, src, Tsrc, Usp, dt, out, cmp, s);
     else
-      pred2d_rs<RS_LLF>(c, op, src, Tsrc, Usp, dt, out, s);
+      pred2d_rs<RS_LLF>(c, op, src, Tsrc, Usp, dt, out, cmp, s);
   }
 
   template <typename S>
@@ -2000,8 +2004,12 @@ template <int NDIM, int P, typename B> struct Stepper {
     };
     if constexpr (NDIM == 2) {
       // the face kernels run once, on the final state, for the corrector
-      pred2d(c, opb, (const double*)Usp, 1, Usp, dt, dst(0), s);
-      for (int it = 1; it < m; ++it) pred2d(c, opb, (const B*)dst(it - 1), T, Usp, dt, dst(it), s);
+      // the first iterate in compact form when another iteration follows and
+      // no tape records the iterates
+      const bool cmp = m >= 2 && !c->tape_on;
+      pred2d(c, opb, (const double*)Usp, 1, Usp, dt, dst(0), cmp ? 1 : 0, s);
+      for (int it = 1; it < m; ++it)
+        pred2d(c, opb, (const B*)dst(it - 1), T, Usp, dt, dst(it), (cmp && it == 1) ? 2 : 0, s);
     } else {
       faces(c, opb, (const double*)Usp, 1, s);
       pred(c, opb, (const double*)Usp, 1, Usp, dt, dst(0), nullptr, 0, s);
