@@ -122,8 +122,9 @@ CLASS_KERNELS = {
     "k_fbface": [("k_fbmarch2d<1, double>", 4)],
     "k_face_sd": [("k_face_sd_x<2, 3, 1, double, double>", 1),
                   ("k_face_sd<2, 3, 1, 1, double, double>", 1)],
-    "k_pred": [("k_pred2d<3, 1, double, double, true>", 1),
-               ("k_pred2d<3, 1, double, double, false>", 2)],
+    "k_pred": [("k_pred2d<3, 1, double, double, true, false>", 1),
+               ("k_pred2d<3, 1, double, double, false, true>", 1),
+               ("k_pred2d<3, 1, double, double, false, false>", 1)],
     "k_detect": [("k_detect2d", 4)],
     "k_hoflux": [("k_ho2d<3, double>", 4)],
     "k_apply": [("k_apply<2, double>", 4)],
