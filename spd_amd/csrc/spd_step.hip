@@ -189,7 +189,23 @@ k_face_sd(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
   const long long pidx = tid % plane;
   int ce[3] = {0, 0, 0}, cl[3] = {0, 0, 0};
   int f = 0;
-  {
+  long long widx = pidx;  // face-grid index of the output
+  if constexpr (NDIM == 3 && A == 2) {
+    // z faces: the threads walk the points of an element's xy plane first, so
+    // that the lanes of a wave read whole planes of the elements (its lines
+    // along z are n*n values apart) instead of 4-value pieces of rows
+    long long x = pidx;
+    cl[0] = (int)(x % n);
+    x /= n;
+    cl[1] = (int)(x % n);
+    x /= n;
+    ce[0] = (int)(x % g.N[0]);
+    x /= g.N[0];
+    ce[1] = (int)(x % g.N[1]);
+    f = (int)(x / g.N[1]);
+    widx = ((long long)f * ext[1] + (long long)ce[1] * n + cl[1]) * ext[0] + (long long)ce[0] * n +
+           cl[0];
+  } else {
     long long x = pidx;
 #pragma unroll
     for (int b = 0; b < NDIM; ++b) {
@@ -250,7 +266,7 @@ k_face_sd(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
   prims<NDIM>(qR, WR, a, (B)prm.gamma);
   riemann<NDIM, RS>(WL, WR, qL, qR, F, a, (B)prm.gamma, (B)prm.min_c2, prm.lowmach_sd);
 #pragma unroll
-  for (int v = 0; v < V; ++v) Fface[(long long)(t * V + v) * plane + pidx] = F[v];
+  for (int v = 0; v < V; ++v) Fface[(long long)(t * V + v) * plane + widx] = F[v];
 }
 
 // ---------------------------------------------------------------------------
