@@ -699,6 +699,126 @@ k_wave_cells(Geom g, Params prm, int n, const double* __restrict__ Ucv,
 }
 
 // ---------------------------------------------------------------------------
+// 3-D high-order fluxes of one dimension (the arithmetic of k_hoflux, same
+// order): n adjacent lanes per (element, flux point), lane c2 takes the n lines
+// whose second transverse coordinate is c2, averages them over the first,
+// and the n threads of a group exchange those partial averages (shuffles
+// among adjacent lanes) for the average over the second; thread c2 writes the
+// faces of second coordinate c2.  A wave holds ~3 elements instead of ~13:
+// their slabs stay in L1 for lines across the element (z), and a thread holds
+// n lines instead of n^2.
+// ---------------------------------------------------------------------------
+template <int P, int A, typename B>
+__global__ void __launch_bounds__(256)
+k_hoflux3(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
+          const B* __restrict__ Fface, B* __restrict__ Fho) {
+  SPD_SKIP(g);
+  constexpr int NDIM = 3, a = A;
+  constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
+  constexpr int NL = IPow<n, NDIM>::v;
+  // groups of n adjacent lanes, GPW per wave (a group never straddles waves)
+  constexpr int GPW = 64 / n;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)(tid % 64);
+  if (lane >= GPW * n) return;
+  const long long grp = (tid / 64) * GPW + lane / n;
+  if (grp >= g.NE * nf) return;  // the whole group
+  const int c2 = lane % n;
+  const int f = (int)(grp % nf);
+  const long long e = grp / nf;
+  int c[3] = {0, 0, 0};
+  elem_coords<NDIM>(g, e, c);
+  const int Na = g.N[a];
+  if (f == nf - 1 && c[a] != Na - 1) return;  // the whole group
+  const int sa = stride_of<n>(a);
+  B Fl[n][V];
+#pragma unroll
+  for (int s1 = 0; s1 < n; ++s1) {
+    const int lp = s1 + n * c2;
+    if (f == 0 || f == nf - 1) {
+      const int fa = c[a] + (f == 0 ? 0 : 1);
+      int cl[3] = {0, 0, 0};
+      {
+        int rr = lp;
+#pragma unroll
+        for (int b = 0; b < NDIM; ++b) {
+          if (b == a) continue;
+          cl[b] = rr % n;
+          rr /= n;
+        }
+      }
+      long long plane;
+      const long long i0 = face_grid_base<NDIM, n>(g, a, c, cl, fa, &plane);
+#pragma unroll
+      for (int v = 0; v < V; ++v) Fl[s1][v] = Fface[(long long)(t * V + v) * plane + i0];
+    } else {
+      const int lb = line_base<NDIM, n>(a, lp);
+      B M[V], W[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const B* u = Uad + ((long long)(t * V + v) * g.NE + e) * NL + lb;
+        B acc = B(0);
+#pragma unroll
+        for (int s = 0; s < n; ++s) acc += op.s2f[f][s] * u[s * sa];
+        M[v] = acc;
+      }
+      prims<NDIM>(M, W, a, (B)prm.gamma);
+      flux<NDIM>(W, Fl[s1], a, (B)prm.gamma);
+    }
+  }
+  // average over the first transverse coordinate (G of k_hoflux, column c2)
+  B G[n][V];
+#pragma unroll
+  for (int q1 = 0; q1 < n; ++q1)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      B acc = B(0);
+#pragma unroll
+      for (int s = 0; s < n; ++s) acc += op.s2c[q1][s] * Fl[s][v];
+      G[q1][v] = acc;
+    }
+  // over the second: Fa[q1 + n q2] with q2 = c2, from the group's columns
+  const int lane0 = lane - c2;  // the group's first lane
+  B Fa[n][V];
+#pragma unroll
+  for (int q1 = 0; q1 < n; ++q1)
+#pragma unroll
+    for (int v = 0; v < V; ++v) Fa[q1][v] = B(0);
+#pragma unroll
+  for (int s = 0; s < n; ++s) {
+    const B w = op.s2c[c2][s];
+#pragma unroll
+    for (int q1 = 0; q1 < n; ++q1)
+#pragma unroll
+      for (int v = 0; v < V; ++v) Fa[q1][v] += w * __shfl(G[q1][v], lane0 + s);
+  }
+  long long ext[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) ext[b] = g.Ns[b] + (b == a ? 1 : 0);
+  const long long nface = ext[0] * ext[1] * ext[2];
+#pragma unroll
+  for (int q1 = 0; q1 < n; ++q1) {
+    const int lp = q1 + n * c2;
+    int q[3] = {0, 0, 0};
+    int rr = lp;
+#pragma unroll
+    for (int b = 0; b < NDIM; ++b) {
+      if (b == a) continue;
+      q[b] = rr % n;
+      rr /= n;
+    }
+    long long idx = 0;
+#pragma unroll
+    for (int b = NDIM - 1; b >= 0; --b) {
+      const long long cb = (b == a) ? (long long)c[a] * n + f : (long long)c[b] * n + q[b];
+      idx = idx * ext[b] + cb;
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) Fho[v * nface + idx] = Fa[q1][v];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 2-D sp <-> cv transforms of the time step through LDS: a block owns a row
 // segment of T2D_EB(P) elements (about 256 cells), loads all variables of its
 // cells with coalesced reads in the input layout, transforms them with the
@@ -1865,8 +1985,13 @@ template <int NDIM, int P, typename B> struct Stepper {
   template <int A>
   static void hoflux_generic(spd_ctx* c, const OpsB& op, int i, hipStream_t s) {
     c->timed(KC_HOFLUX, s, [&] {
-      hipLaunchKernelGGL((k_hoflux<NDIM, P, A, B>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0, s,
-                         op, c->g, c->prm, i, Uad(c), Ff(c, A), Fh(c, A));
+      if constexpr (NDIM == 3)
+        hipLaunchKernelGGL((k_hoflux3<P, A, B>),
+                           dim3(grid_for((c->g.NE * nf + 64 / n - 1) / (64 / n) * 64)), dim3(256), 0,
+                           s, op, c->g, c->prm, i, Uad(c), Ff(c, A), Fh(c, A));
+      else
+        hipLaunchKernelGGL((k_hoflux<NDIM, P, A, B>), dim3(grid_for(c->g.NE * nf)), dim3(256), 0,
+                           s, op, c->g, c->prm, i, Uad(c), Ff(c, A), Fh(c, A));
     });
   }
   static void apply(spd_ctx* c, const double* dt, double w, double* out, hipStream_t s) {
