@@ -970,6 +970,105 @@ k_tensor2d(Ops<P> op, Geom g, Params prm, const double* __restrict__ in, double*
 }
 
 // ---------------------------------------------------------------------------
+// 3-D sp <-> cv transforms of the time step: a block owns t3d_eb(P)
+// consecutive elements (about 256 points), stages all their variables in LDS
+// and applies the 1-D operator along x, then y, then z (one thread per point,
+// each pass through LDS; the separable form of k_tensor's single sum, equal
+// to it up to round-off).  MODE 1 sp -> cv (element layout in, FV layout out);
+// MODE 0 cv -> sp with the next dt's wave speed of every cell (block maxima in
+// `partial`, as k_cv2sp) and the optional snapshots (the input and its
+// primitives in element layout).
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int t3d_eb(int P) {
+  return 256 / ((P + 1) * (P + 1) * (P + 1)) > 0 ? 256 / ((P + 1) * (P + 1) * (P + 1)) : 1;
+}
+
+template <int P, int MODE>
+__global__ void __launch_bounds__(256)
+k_tensor3d(Ops<P> op, Geom g, Params prm, const double* __restrict__ in, double* __restrict__ out,
+           double* __restrict__ in_copy, double* __restrict__ Wcv_sd, double* __restrict__ partial) {
+  SPD_SKIP(g);
+  constexpr int V = 5, n = P + 1, NL = n * n * n, EB = t3d_eb(P), NP = EB * NL;
+  __shared__ double sa[V][NP], sb[V][NP];
+  const long long e0 = (long long)blockIdx.x * EB;
+  const int ne = (g.NE - e0 < EB) ? (int)(g.NE - e0) : EB;
+  // cell index of point l of block element el (FV layout)
+  auto cell_of = [&](int el, int l) -> long long {
+    int ce[3];
+    elem_coords<3>(g, e0 + el, ce);
+    const int cc[3] = {ce[0] * n + l % n, ce[1] * n + (l / n) % n, ce[2] * n + l / (n * n)};
+    return cell_index<3>(g, cc);
+  };
+  for (int k = threadIdx.x; k < V * ne * NL; k += blockDim.x) {
+    const int v = k / (ne * NL), r = k % (ne * NL);
+    sa[v][r] = (MODE == 1) ? in[((long long)v * g.NE + e0) * NL + r]
+                           : in[(long long)v * g.Nc + cell_of(r / NL, r % NL)];
+  }
+  __syncthreads();
+  const double(*M)[n] = (MODE == 1) ? op.s2c : op.c2s;
+  const int k = threadIdx.x;
+  const bool act = k < ne * NL;
+  const int el = k / NL, l = k % NL;
+  const int i = l % n, j = (l / n) % n, z = l / (n * n);
+  const int b0 = el * NL;
+  double cw = 0.0;
+  if (MODE == 0 && act) {
+    double u[V], W[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) u[v] = sa[v][k];
+    prims<3>(u, W, 0, prm.gamma);
+    const long long o = (e0 + el) * NL + l;
+    if (in_copy) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        in_copy[(long long)v * g.NE * NL + o] = u[v];
+        Wcv_sd[(long long)v * g.NE * NL + o] = W[v];
+      }
+    }
+    cw = sound(W[4], W[0], prm.gamma, prm.min_c2) * 3;
+#pragma unroll
+    for (int q = 1; q <= 3; ++q) cw = cw + fabs(W[q]);
+  }
+  if (act) {  // x
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < n; ++q) acc += M[i][q] * sa[v][b0 + q + n * (j + n * z)];
+      sb[v][k] = acc;
+    }
+  }
+  __syncthreads();
+  if (act) {  // y
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < n; ++q) acc += M[j][q] * sb[v][b0 + i + n * (q + n * z)];
+      sa[v][k] = acc;
+    }
+  }
+  __syncthreads();
+  if (act) {  // z, written out
+    const long long ci = (MODE == 1) ? cell_of(el, l) : 0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < n; ++q) acc += M[z][q] * sa[v][b0 + i + n * (j + n * q)];
+      if (MODE == 1)
+        out[(long long)v * g.Nc + ci] = acc;
+      else
+        out[((long long)v * g.NE + e0) * NL + k] = acc;
+    }
+  }
+  if (MODE == 0) {
+    cw = block_max(cw);
+    if (threadIdx.x == 0) partial[blockIdx.x] = cw;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FV update U_out = U_in - dt_i * sum_a (F_a[f+1] - F_a[f]) / dx_a
 // (src/fv_simulator.py:189-205); in place when U_out == U_in.
 // ---------------------------------------------------------------------------
@@ -2175,8 +2274,10 @@ template <int NDIM, int P, typename B> struct Stepper {
         hipLaunchKernelGGL((k_tensor_el<NDIM, P>), dim3(grid_for(c->g.NE * V)), dim3(256), 0, s,
                            op, c->g, 1, Usp, 0, c->Ucv, 1, (double*)nullptr);
       else
-        hipLaunchKernelGGL((k_tensor<NDIM, P>), dim3(grid_for(c->g.NE * V * NL)), dim3(256), 0, s,
-                           op, c->g, 1, Usp, c->Ucv, 1);
+        hipLaunchKernelGGL((k_tensor3d<P, 1>),
+                           dim3((unsigned)((c->g.NE + t3d_eb(P) - 1) / t3d_eb(P))), dim3(256), 0, s,
+                           op, c->g, c->prm, Usp, c->Ucv, (double*)nullptr, (double*)nullptr,
+                           (double*)nullptr);
     });
     for (int i = 0; i < T; ++i) {
       const double w = c->ops->w_tp[i];
@@ -2243,8 +2344,9 @@ template <int NDIM, int P, typename B> struct Stepper {
                            c->Ucv, snapshot ? Wcv_sd : nullptr, c->partial);
       });
     } else {
+      nb = (unsigned)((c->g.NE + t3d_eb(P) - 1) / t3d_eb(P));
       c->timed(KC_CV2SP, s, [&] {
-        hipLaunchKernelGGL((k_cv2sp<NDIM, P>), dim3(nb), dim3(256), 0, s, op, c->g, c->prm,
+        hipLaunchKernelGGL((k_tensor3d<P, 0>), dim3(nb), dim3(256), 0, s, op, c->g, c->prm,
                            c->Ucv, Usp, snapshot ? Ucv_sd : nullptr, snapshot ? Wcv_sd : nullptr,
                            c->partial);
       });
@@ -2519,6 +2621,7 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
   if ((rc = c->alloc(&c->cmax, 1)) != SPD_OK) return rc;
   c->npartial = (g.NE * c->NL + 255) / 256;
   if (nd == 2) c->npartial = std::max(c->npartial, t2d_blocks(g, c->cfg.p));
+  if (nd == 3) c->npartial = std::max(c->npartial, (g.NE + t3d_eb(c->cfg.p) - 1) / t3d_eb(c->cfg.p));
   if ((rc = c->alloc(&c->partial, (size_t)c->npartial)) != SPD_OK) return rc;
   SPD_HIP_CHECK(hipMemset(c->any_flag, 0, sizeof(int)));
   SPD_HIP_CHECK(hipMemset(c->cmax, 0, sizeof(double)));
