@@ -1335,6 +1335,26 @@ __device__ __forceinline__ double trouble_at(const Geom& g, const uint8_t* __res
 
 // Blending stencil: (target slice per dim: 0 full, 1 = [:-1], 2 = [1:]),
 // source offset per dim, weight (src/trouble_detection.py:139-169, Q6).
+// Coordinates of linear index x on a grid of extents ext[0..NDIM) (x
+// fastest); 32-bit arithmetic when the grid has fewer than 2^32 points.
+template <int NDIM>
+__device__ __forceinline__ void grid_coords(long long x, const int* ext, long long total, int* c) {
+  if (total <= 0xffffffffll) {
+    unsigned u = (unsigned)x;
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+      c[a] = (int)(u % (unsigned)ext[a]);
+      u /= (unsigned)ext[a];
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+      c[a] = (int)(x % ext[a]);
+      x /= ext[a];
+    }
+  }
+}
+
 struct BlendRule {
   signed char tgt[3];
   signed char off[3];
@@ -1387,19 +1407,53 @@ k_theta1(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* __
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= g.Ngc) return;
   int gx[3] = {0, 0, 0};
-  {
-    long long x = tid;
+  grid_coords<NDIM>(tid, g.Ng, g.Ngc, gx);
+  int c[3] = {0, 0, 0};
 #pragma unroll
-    for (int a = 0; a < NDIM; ++a) {
-      gx[a] = (int)(x % g.Ng[a]);
-      x /= g.Ng[a];
+  for (int a = 0; a < NDIM; ++a) c[a] = gx[a] - NGH;
+  if (prm.blending) {
+    // a cell whose 3^NDIM neighbourhood is all active cells reads the flags
+    // directly: every neighbour and rule applies (the same values and maximum
+    // as below)
+    bool inner = true;
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) inner = inner && c[a] >= 1 && c[a] <= g.Ns[a] - 2;
+    if (inner) {
+      const long long i0 = cell_index<NDIM>(g, c);
+      long long st[3] = {1, 1, 1};
+#pragma unroll
+      for (int a = 1; a < NDIM; ++a) st[a] = st[a - 1] * g.Ns[a - 1];
+      auto F = [&](const int* o) -> double {
+        long long d = 0;
+#pragma unroll
+        for (int a = 0; a < NDIM; ++a) d += o[a] * st[a];
+        return (double)flags[i0 + d];
+      };
+      const int o0[3] = {0, 0, 0};
+      double th = F(o0);
+#pragma unroll
+      for (int a = 0; a < NDIM; ++a)
+#pragma unroll
+        for (int s = -1; s <= 1; s += 2) {
+          int o[3] = {0, 0, 0};
+          o[a] = s;
+          th = nmax(0.75 * F(o), th);
+        }
+      if constexpr (NDIM >= 2) {
+        const BlendRule* rules = (NDIM == 2) ? kRules2 : kRules3;
+        const int nr = (NDIM == 2) ? 4 : 16;
+        for (int k = 0; k < nr; ++k) {
+          const BlendRule R = rules[k];
+          const int o[3] = {R.off[0], R.off[1], R.off[2]};
+          th = nmax(R.w * F(o), th);
+        }
+      }
+      theta1[tid] = th;
+      return;
     }
   }
   const bool gate = (*any_flag != 0) && prm.p > 1 && prm.sed;
   const double* Ulast = gate ? Unew : Ucv;  // density is variable 0 of both sets
-  int c[3] = {0, 0, 0};
-#pragma unroll
-  for (int a = 0; a < NDIM; ++a) c[a] = gx[a] - NGH;
   double th = trouble_at<NDIM>(g, flags, Ulast, c);
   if (prm.blending) {
     // first neighbours
@@ -1448,14 +1502,7 @@ k_ring(Geom g, const double* __restrict__ t1, double* __restrict__ theta) {
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= g.Ngc) return;
   int gx[3] = {0, 0, 0};
-  {
-    long long x = tid;
-#pragma unroll
-    for (int a = 0; a < NDIM; ++a) {
-      gx[a] = (int)(x % g.Ng[a]);
-      x /= g.Ng[a];
-    }
-  }
+  grid_coords<NDIM>(tid, g.Ng, g.Ngc, gx);
   const int nx = g.Ng[0], ny = g.Ng[1], nz = g.Ng[2];
   auto at = [&](int i, int j, int k) -> double {
     return t1[((long long)k * ny + j) * nx + i];
