@@ -12,12 +12,13 @@
 // One step (SDADER_Simulator.perform_update, src/sdader_simulator.py:361-378):
 //   predictor  m+1 x [element-face Riemann (k_face_sd) + update (k_pred)]
 //   corrector  SD : k_pred in SD-update mode
-//              FV : sp->cv (k_sp2cv), then T substeps of
+//              FV : sp->cv (k_tensor2d / k_tensor3d / k_tensor_el), then T substeps of
 //                   [HO face fluxes (k_hoflux), candidate (k_apply),
 //                    detection (k_detect), theta (k_theta1, k_ring),
 //                    fallback states (k_hancock) + flux and blend (k_fbface),
 //                    apply (k_apply)],
-//                   then cv->sp with the dt reduction (k_cv2sp).
+//                   then cv->sp with the wave speed for the next dt
+//                   (k_tensor2d / k_tensor3d; 1-D: k_tensor_el + k_wave_cells).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -976,7 +977,7 @@ k_tensor2d(Ops<P> op, Geom g, Params prm, const double* __restrict__ in, double*
 // each pass through LDS; the separable form of k_tensor's single sum, equal
 // to it up to round-off).  MODE 1 sp -> cv (element layout in, FV layout out);
 // MODE 0 cv -> sp with the next dt's wave speed of every cell (block maxima in
-// `partial`, as k_cv2sp) and the optional snapshots (the input and its
+// `partial`, as k_wave_cells) and the optional snapshots (the input and its
 // primitives in element layout).
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int t3d_eb(int P) {
@@ -1691,85 +1692,6 @@ k_fbface(Geom g, Params prm, const B* __restrict__ HS, const double* __restrict_
   for (int v = 0; v < V; ++v) {
     B* dst = Fho + v * nface + tid;
     *dst = tb * F[v] + (B(1) - tb) * (*dst);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// End of the FV corrector: U_sp = cv_to_sp(U_cv) (src/sdader_simulator.py:300-307),
-// optional snapshot of U_cv / W_cv in element layout, and the maximum wave
-// speed of the new subcell-mean state for the next dt
-// (src/sd_simulator.py:214-227).  One thread per (element, point).
-// ---------------------------------------------------------------------------
-template <int NDIM, int P>
-__global__ void __launch_bounds__(256)
-k_cv2sp(Ops<P> op, Geom g, Params prm, const double* __restrict__ Ucv, double* __restrict__ Usp,
-        double* __restrict__ Ucv_sd, double* __restrict__ Wcv_sd, double* __restrict__ partial) {
-  SPD_SKIP(g);
-  constexpr int V = Dims<NDIM>::V, P_ = Dims<NDIM>::P, n = P + 1;
-  constexpr int NL = IPow<n, NDIM>::v;
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  double cw = 0.0;
-  if (tid < g.NE * NL) {
-    const int lo = (int)(tid % NL);
-    const long long e = tid / NL;
-    int ce[3] = {0, 0, 0};
-    elem_coords<NDIM>(g, e, ce);
-    int qo[3] = {0, 0, 0};
-    {
-      int x = lo;
-#pragma unroll
-      for (int b = 0; b < NDIM; ++b) {
-        qo[b] = x % n;
-        x /= n;
-      }
-    }
-    double acc[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) acc[v] = 0.0;
-    for (int li = 0; li < NL; ++li) {
-      int x = li;
-      int qi[3] = {0, 0, 0};
-#pragma unroll
-      for (int b = 0; b < NDIM; ++b) {
-        qi[b] = x % n;
-        x /= n;
-      }
-      double w = 1.0;
-#pragma unroll
-      for (int b = NDIM - 1; b >= 0; --b)
-        w = (b == NDIM - 1) ? op.c2s[qo[b]][qi[b]] : w * op.c2s[qo[b]][qi[b]];
-      int cc[3];
-#pragma unroll
-      for (int b = 0; b < NDIM; ++b) cc[b] = ce[b] * n + qi[b];
-      const long long ci = cell_index<NDIM>(g, cc);
-#pragma unroll
-      for (int v = 0; v < V; ++v) acc[v] += w * Ucv[v * g.Nc + ci];
-    }
-    int cc[3];
-#pragma unroll
-    for (int b = 0; b < NDIM; ++b) cc[b] = ce[b] * n + qo[b];
-    const long long ci = cell_index<NDIM>(g, cc);
-    double u[V], W[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      Usp[((long long)v * g.NE + e) * NL + lo] = acc[v];
-      u[v] = Ucv[v * g.Nc + ci];
-    }
-    prims<NDIM>(u, W, 0, prm.gamma);
-    if (Ucv_sd) {
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        Ucv_sd[((long long)v * g.NE + e) * NL + lo] = u[v];
-        Wcv_sd[((long long)v * g.NE + e) * NL + lo] = W[v];
-      }
-    }
-    cw = sound(W[P_], W[0], prm.gamma, prm.min_c2) * NDIM;
-#pragma unroll
-    for (int k = 1; k <= NDIM; ++k) cw = cw + fabs(W[k]);
-  }
-  if (partial) {
-    cw = block_max(cw);
-    if (threadIdx.x == 0) partial[blockIdx.x] = cw;
   }
 }
 
