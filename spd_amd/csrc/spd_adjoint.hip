@@ -78,7 +78,11 @@ __device__ __forceinline__ void vjp(const double* x, const double* ybar, double*
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < NO; ++j) acc += ybar[j] * yd[j].d;
-    xbar[k] = acc;
+    // a store at a constant index: xbar (the caller's array) stays in
+    // registers instead of going to scratch memory for a run-time index
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+      if (i == k) xbar[i] = acc;
   }
 }
 
