@@ -444,7 +444,9 @@ k_adj_hancock(Geom g, Params prm, const double* __restrict__ U, const double* __
   }
   // W of each stencil cell = flip(prims(U[map(cell)])): transpose into Ub0
   const Dual gam(prm.gamma);
-#pragma unroll 1
+  // unrolled: xb is read at k * V + v, which a run-time k would index through
+  // scratch memory
+#pragma unroll
   for (int k = 0; k < 1 + 2 * NDIM; ++k) {
     int cc[3] = {c[0], c[1], c[2]};
     if (k >= 1 && k <= NDIM) cc[k - 1] -= 1;
@@ -499,7 +501,9 @@ k_adj_hoflux(Ops<P> op, Geom g, Params prm, int t, const S* __restrict__ Uad,
   const double* Fhob[3] = {Fhob0, Fhob1, Fhob2};
   double* Ffb[3] = {Ffb0, Ffb1, Ffb2};
   const Dual gam(prm.gamma);
-#pragma unroll 1
+  // unrolled: with a known at compile time the dual-number arrays of the
+  // flux VJP stay in registers (a run-time a indexes them through scratch)
+#pragma unroll
   for (int a = 0; a < NDIM; ++a) {
     long long ext[3] = {1, 1, 1}, nface = 1;
 #pragma unroll
@@ -860,7 +864,7 @@ k_adj_pred(Ops<P> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
       ub[v][s] = 0.0;
     }
   const Dual gam(prm.gamma);
-#pragma unroll 1
+#pragma unroll
   for (int fp = 1; fp < nf - 1; ++fp) {
     double M[V], Mb[V];
 #pragma unroll
