@@ -147,6 +147,75 @@ k_adj_tensor(Ops<P> op, Geom g, int which, const double* __restrict__ in, int in
 }
 
 // ---------------------------------------------------------------------------
+// The same transposes in 3-D through LDS: a block owns ~256 points of
+// consecutive elements, loads all variables in the input layout, applies M^T
+// along x, then y, then z (one thread per point; the separable form of the
+// single sum above, equal to it up to round-off) and writes the other layout.
+// IN_FV: input in FV layout, output in element layout (else the reverse).
+// ---------------------------------------------------------------------------
+template <int P, int IN_FV>
+__global__ void __launch_bounds__(256)
+k_adj_tensor3d(Ops<P> op, Geom g, int which, const double* __restrict__ in,
+               double* __restrict__ out) {
+  constexpr int V = 5, n = P + 1, NL = n * n * n;
+  constexpr int EB = 256 / NL > 0 ? 256 / NL : 1, NP = EB * NL;
+  __shared__ double sa[V][NP], sb[V][NP];
+  const long long e0 = (long long)blockIdx.x * EB;
+  const int ne = (g.NE - e0 < EB) ? (int)(g.NE - e0) : EB;
+  auto cell_of = [&](int el, int l) -> long long {
+    int ce[3];
+    elem_coords<3>(g, e0 + el, ce);
+    const int cc[3] = {ce[0] * n + l % n, ce[1] * n + (l / n) % n, ce[2] * n + l / (n * n)};
+    return cell_index<3>(g, cc);
+  };
+  for (int k = threadIdx.x; k < V * ne * NL; k += blockDim.x) {
+    const int v = k / (ne * NL), r = k % (ne * NL);
+    sa[v][r] = IN_FV ? in[(long long)v * g.Nc + cell_of(r / NL, r % NL)]
+                     : in[((long long)v * g.NE + e0) * NL + r];
+  }
+  __syncthreads();
+  const double(*M)[n] = which ? op.s2c : op.c2s;
+  const int k = threadIdx.x;
+  const bool act = k < ne * NL;
+  const int el = k / NL, l = k % NL;
+  const int i = l % n, j = (l / n) % n, z = l / (n * n);
+  const int b0 = el * NL;
+  if (act) {  // x
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < n; ++q) acc += M[q][i] * sa[v][b0 + q + n * (j + n * z)];
+      sb[v][k] = acc;
+    }
+  }
+  __syncthreads();
+  if (act) {  // y
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < n; ++q) acc += M[q][j] * sb[v][b0 + i + n * (q + n * z)];
+      sa[v][k] = acc;
+    }
+  }
+  __syncthreads();
+  if (act) {  // z, written out
+    const long long ci = IN_FV ? 0 : cell_of(el, l);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < n; ++q) acc += M[q][z] * sa[v][b0 + i + n * (j + n * q)];
+      if (IN_FV)
+        out[((long long)v * g.NE + e0) * NL + k] = acc;
+      else
+        out[(long long)v * g.Nc + ci] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Hancock states and half slopes of the cells the fallback needs (the cells
 // next to a face with a non-zero coefficient), as in the step; layout
 // [Mp (V), S_0 (V), ..., S_{d-1} (V)] per cell of the (Ns+2)^d grid.
@@ -1036,6 +1105,11 @@ int ensure_scratch(spd_ctx* c) {
 // B: element type of the tape's predictor iterates (the bulk type of the
 // form: FP64, or FP32 in the FP32-bulk form); the reverse sweep itself is FP64.
 template <int NDIM, int P, typename B> struct Adjoint {
+  // blocks of k_adj_tensor3d (256 / (p+1)^3 elements each)
+  static unsigned adj_t3d_blocks(long long NE) {
+    constexpr int NL3 = (P + 1) * (P + 1) * (P + 1), EB = 256 / NL3 > 0 ? 256 / NL3 : 1;
+    return (unsigned)((NE + EB - 1) / EB);
+  }
   static constexpr int V = NDIM + 2, n = P + 1, nf = P + 2, T = P + 1;
   static constexpr int NL = IPow<n, NDIM>::v, NLP = IPow<n, NDIM - 1>::v;
 
@@ -1122,8 +1196,12 @@ template <int NDIM, int P, typename B> struct Adjoint {
     if (c->cfg.update == SPD_UPDATE_FV) {
       const long long Nc = c->g.Nc;
       // U_sp' = cv_to_sp(U^T)
-      hipLaunchKernelGGL((k_adj_tensor<NDIM, P>), dim3(grid_for(npt)), dim3(256), 0, s, op, c->g,
-                         0, (const double*)Ub, 0, Ad->Ucvb[0], 1);
+      if constexpr (NDIM == 3)
+        hipLaunchKernelGGL((k_adj_tensor3d<P, 0>), dim3(adj_t3d_blocks(c->g.NE)), dim3(256), 0, s,
+                           op, c->g, 0, (const double*)Ub, Ad->Ucvb[0]);
+      else
+        hipLaunchKernelGGL((k_adj_tensor<NDIM, P>), dim3(grid_for(npt)), dim3(256), 0, s, op,
+                           c->g, 0, (const double*)Ub, 0, Ad->Ucvb[0], 1);
       int cur = 0;
       for (int i = T - 1; i >= 0; --i) {
         const double w = c->ops->w_tp[i];
@@ -1148,8 +1226,12 @@ template <int NDIM, int P, typename B> struct Adjoint {
         cur = 1 - cur;
       }
       // U^0 = sp_to_cv(U_sp)
-      hipLaunchKernelGGL((k_adj_tensor<NDIM, P>), dim3(grid_for(npt)), dim3(256), 0, s, op, c->g,
-                         1, (const double*)Ad->Ucvb[cur], 1, Ub, 0);
+      if constexpr (NDIM == 3)
+        hipLaunchKernelGGL((k_adj_tensor3d<P, 1>), dim3(adj_t3d_blocks(c->g.NE)), dim3(256), 0, s,
+                           op, c->g, 1, (const double*)Ad->Ucvb[cur], Ub);
+      else
+        hipLaunchKernelGGL((k_adj_tensor<NDIM, P>), dim3(grid_for(npt)), dim3(256), 0, s, op,
+                           c->g, 1, (const double*)Ad->Ucvb[cur], 1, Ub, 0);
       // element faces of the final iterate (every node, written by k_adj_hoflux)
       faces(c, op, iterate(m), T, Ad->Uadb, s);
     } else {
