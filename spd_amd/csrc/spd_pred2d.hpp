@@ -27,7 +27,7 @@ template <int P> struct Tile;
 template <> struct Tile<1> { static constexpr int X = 8, Y = 8; };
 template <> struct Tile<2> { static constexpr int X = 5, Y = 5; };
 template <> struct Tile<3> { static constexpr int X = 4, Y = 4; };
-template <> struct Tile<4> { static constexpr int X = 3, Y = 2; };
+template <> struct Tile<4> { static constexpr int X = 3, Y = 3; };
 template <> struct Tile<5> { static constexpr int X = 2, Y = 2; };
 template <int P> struct TileFirst;
 template <> struct TileFirst<1> { static constexpr int X = 16, Y = 8; };
