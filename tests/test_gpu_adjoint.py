@@ -196,3 +196,30 @@ def test_fp32_form_vjp_close_to_fp64(name):
     rel = float((g32 - g64).abs().sum() / g64.abs().sum())
     print(f"  relative L1 difference fp32 vs fp64: {rel:.2e}", flush=True)
     assert rel < 1e-5, rel
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["kh16_p3", "quad12_p2", "step12_p2_refl", "vortex12_p5",
+                                  "sine2d_p3_SD", "tg4_p3"])
+def test_taped_step_equals_product_step(name, precision):
+    """The product step of the 2-D path writes the first Picard iterate in
+    compact form (its one-node increment) when another iteration follows, and
+    the next iteration forms every node from U_sp and that increment; the
+    taped step of the derivative keeps the full iterates.  Both form the same
+    values with the same operations in the same order, so one step from the
+    same state and dt agrees bit for bit."""
+    from spd_amd.adjoint import Adjoint
+    from spd_amd.simulator import SDADER_Simulator
+    print(f"[gpu taped step] {name} {precision}", flush=True)
+    cfg = load(name)["cfg"]
+    s = SDADER_Simulator(init_fct=make_ic(cfg["ic"]), device="cuda:0", verbose=False,
+                         precision=precision, **sim_kwargs(cfg))
+    adj = Adjoint(s)
+    U0 = s.dm.device_value("U_sp").clone().contiguous()
+    dt = adj.dt_of(U0)
+    taped = adj.step(U0, dt)
+    U = U0.clone()
+    adj._advance(U, dt)
+    torch.cuda.synchronize()
+    assert torch.isfinite(U).all()
+    assert torch.equal(taped, U), float((taped - U).abs().max())
