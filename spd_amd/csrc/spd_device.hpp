@@ -281,6 +281,26 @@ struct Geom {
 #define SPD_SKIP(g) \
   if (*(g).skip) return
 
+// Tile of this block of a 2-D tile grid, with the tiles dealt to the 8 XCDs in
+// column bands.  The dispatcher hands consecutive workgroups round-robin to the
+// XCDs, each with its own L2, so in launch order the tiles left and right of a
+// tile run on other XCDs and every x halo is fetched again.  With gridDim.x a
+// multiple of 8, XCD k takes the tile columns [k W, (k + 1) W), W = gridDim.x
+// / 8, in row order: a tile's neighbours run on its XCD one and W blocks
+// apart.  Other grids keep the launch order.
+__device__ __forceinline__ void xcd_band_tile(int& bx, int& by) {
+  const int nbx = gridDim.x;
+  if (nbx % 8 != 0) {
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    return;
+  }
+  const int b = blockIdx.y * nbx + blockIdx.x;
+  const int W = nbx >> 3, j = b >> 3;
+  bx = (b & 7) * W + j % W;
+  by = j / W;
+}
+
 struct Params {
   double gamma, min_c2;
   double tol, min_rho, max_rho, min_P;

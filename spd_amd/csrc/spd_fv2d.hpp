@@ -109,7 +109,9 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
   __shared__ double wn[2][DNY][DNX];  // new rho, P
   __shared__ double wbuf[DWO];        // old primitives, then the smooth-extrema bits
   double(*wo)[ONY][ONX] = reinterpret_cast<double(*)[ONY][ONX]>(wbuf);
-  const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
+  int bx, by;
+  xcd_band_tile(bx, by);
+  const int x0 = bx * DTX, y0 = by * DTY;
   const int Nx = g.Ns[0], Ny = g.Ns[1];
   // staging: each thread issues the loads of all its cells of a tile before
   // it uses the first (new rho, P on tile + 3, then old primitives on tile + 1)
