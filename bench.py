@@ -65,8 +65,9 @@ def algorithmic_bytes(N, p, launches):
     Per step (V = 4, T = n = p+1, m = p, E = N^2 elements, C = E n^2 cells,
     Fe = T V (N+1) N n element-face values per dimension, Fd = V (N n + 1) N n
     FV-face values per dimension, G = (N n + 4)^2 ghosted cells):
-      k_face_sd  2 launches (the final predictor state, for the corrector):
-                 read T V C, write the element-face fluxes Fe
+      k_face_sd  1 launch (k_face_sd2d, the final predictor state, for the
+                 corrector): read T V C, write the element-face fluxes of both
+                 dimensions (2 Fe)
       k_pred     m launches (k_pred2d, element faces solved inside): read the
                  state (T V C) and U_sp (V C), write T V C; the first iteration
                  reads U_sp and, when another follows, writes its increment
@@ -94,7 +95,7 @@ def algorithmic_bytes(N, p, launches):
     Fe = T * V * (N + 1) * N * n
     G = (N * n + 4) ** 2
     per_step = {
-        "k_face_sd": 2 * (T * V * C + Fe),
+        "k_face_sd": T * V * C + 2 * Fe,
         "k_pred": ((2 * V * C) + (2 * V * C + T * V * C) + (m - 2) * (2 * T * V * C + V * C)
                    if m >= 2 else V * C + T * V * C),
         "k_hoflux": T * (V * C + 2 * V * (N + 1) * N * n + 2 * Fd),
@@ -120,8 +121,7 @@ def algorithmic_bytes(N, p, launches):
 # with their launches per step
 CLASS_KERNELS = {
     "k_fbface": [("k_fbmarch2d<1, double>", 4)],
-    "k_face_sd": [("k_face_sd_x<2, 3, 1, double, double>", 1),
-                  ("k_face_sd<2, 3, 1, 1, double, double>", 1)],
+    "k_face_sd": [("k_face_sd2d<3, 1, double, double>", 1)],
     "k_pred": [("k_pred2d<3, 1, double, double, true, false>", 1),
                ("k_pred2d<3, 1, double, double, false, true>", 1),
                ("k_pred2d<3, 1, double, double, false, false>", 1)],

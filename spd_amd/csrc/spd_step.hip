@@ -271,6 +271,184 @@ k_face_sd(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
 }
 
 // ---------------------------------------------------------------------------
+// 2-D element faces of both dimensions in one pass over the source (the
+// faces of k_face_sd_x and k_face_sd<2, P, RS, 1> with the same values): a
+// block takes a tile of EX x EY elements (FaceTile2) at one time node.  Phase 1: one
+// thread per (variable, element) walks the element's rows once and forms the
+// two x-end traces of every row and the two y-end traces of every column in
+// LDS (every sum in the order of those kernels).  Phase 2: one thread per
+// face point solves the x faces left of the tile's elements and the y faces
+// below them, and the grid's last x / y face where the tile ends there.  A
+// trace of an element outside the tile (the left / lower neighbour, a
+// periodic wrap) is formed from the source.
+// ---------------------------------------------------------------------------
+template <int P> struct FaceTile2 {
+  static constexpr int EX = 16, EY = (16 / (P + 1) > 0) ? 16 / (P + 1) : 1;
+};
+
+template <int P, int RS, typename S, typename B>
+__global__ void __launch_bounds__(256)
+k_face_sd2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __restrict__ Fx,
+            B* __restrict__ Fy) {
+  SPD_SKIP(g);
+  constexpr int V = 4, n = P + 1, nf = P + 2, NL = n * n;
+  constexpr int EX = FaceTile2<P>::EX, EY = FaceTile2<P>::EY;
+  __shared__ B tr[4][V][EY][EX][n];  // end traces: x left, x right, y lower, y upper
+  const int Nx = g.N[0], Ny = g.N[1];
+  const int ntx = (Nx + EX - 1) / EX, nty = (Ny + EY - 1) / EY;
+  const int t = (int)(blockIdx.x / (unsigned)(ntx * nty));
+  const int rb = (int)(blockIdx.x % (unsigned)(ntx * nty));
+  const int ex0 = (rb % ntx) * EX, ey0 = (rb / ntx) * EY;
+  const S* base = src + (long long)t * V * g.NE * NL;
+  for (int it = threadIdx.x; it < V * EY * EX; it += 256) {
+    const int exl = it % EX, r = it / EX, eyl = r % EY, v = r / EY;
+    if (ex0 + exl >= Nx || ey0 + eyl >= Ny) continue;
+    const S* u = base + ((long long)v * g.NE + (long long)(ey0 + eyl) * Nx + ex0 + exl) * NL;
+    B yl[n], yu[n];
+#pragma unroll
+    for (int q = 0; q < n; ++q) yl[q] = yu[q] = B(0);
+#pragma unroll
+    for (int s = 0; s < n; ++s) {  // row s of the element
+      B w[n];
+#pragma unroll
+      for (int q = 0; q < n; ++q) w[q] = (B)u[s * n + q];
+      B xl = B(0), xr = B(0);
+#pragma unroll
+      for (int q = 0; q < n; ++q) {
+        xl += op.s2f[0][q] * w[q];
+        xr += op.s2f[nf - 1][q] * w[q];
+      }
+      tr[0][v][eyl][exl][s] = xl;
+      tr[1][v][eyl][exl][s] = xr;
+#pragma unroll
+      for (int q = 0; q < n; ++q) {
+        yl[q] += op.s2f[0][s] * w[q];
+        yu[q] += op.s2f[nf - 1][s] * w[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < n; ++q) {
+      tr[2][v][eyl][exl][q] = yl[q];
+      tr[3][v][eyl][exl][q] = yu[q];
+    }
+  }
+  __syncthreads();
+  // trace of element (ex, ey) from the source: along x on row `line`
+  // (axis 0) or along y on column `line` (axis 1), at flux point fp
+  auto gtrace = [&](int ex, int ey, int fp, int axis, int line, B* q) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const S* u = base + ((long long)v * g.NE + (long long)ey * Nx + ex) * NL;
+      B acc = B(0);
+#pragma unroll
+      for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * (B)u[axis == 0 ? line * n + s : s * n + line];
+      q[v] = acc;
+    }
+  };
+  const B gam = (B)prm.gamma, mc2 = (B)prm.min_c2;
+  // x face f = ex0 + fl on row k of element row ey0 + eyl
+  auto xface = [&](int fl, int k, int eyl) {
+    const int f = ex0 + fl, ey = ey0 + eyl;
+    if (ey >= Ny || f > Nx) return;
+    B qL[V], qR[V];
+    if (f > 0) {
+      if (fl > 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) qL[v] = tr[1][v][eyl][fl - 1][k];
+      } else {
+        gtrace(f - 1, ey, nf - 1, 0, k, qL);
+      }
+    }
+    if (f < Nx) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) qR[v] = tr[0][v][eyl][fl][k];
+    }
+    if (f == 0) {
+      const int b = g.bc[0][0];
+      if (b == BC_PERIODIC) {
+        gtrace(Nx - 1, ey, nf - 1, 0, k, qL);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) qL[v] = qR[v];
+        if (b == BC_REFLECTIVE) qL[1] = -qL[1];
+      }
+    }
+    if (f == Nx) {
+      const int b = g.bc[0][1];
+      if (b == BC_PERIODIC) {
+        gtrace(0, ey, 0, 0, k, qR);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) qR[v] = qL[v];
+        if (b == BC_REFLECTIVE) qR[1] = -qR[1];
+      }
+    }
+    B WL[V], WR[V], F[V];
+    prims<2>(qL, WL, 0, gam);
+    prims<2>(qR, WR, 0, gam);
+    riemann<2, RS>(WL, WR, qL, qR, F, 0, gam, mc2, prm.lowmach_sd);
+    const long long plane = (long long)(Nx + 1) * Ny * n;
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      Fx[(long long)(t * V + v) * plane + (long long)(ey * n + k) * (Nx + 1) + f] = F[v];
+  };
+  // y face f = ey0 + fl on column k of element ex0 + exl
+  auto yface = [&](int fl, int exl, int k) {
+    const int f = ey0 + fl, ex = ex0 + exl;
+    if (ex >= Nx || f > Ny) return;
+    B qL[V], qR[V];
+    if (f > 0) {
+      if (fl > 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) qL[v] = tr[3][v][fl - 1][exl][k];
+      } else {
+        gtrace(ex, f - 1, nf - 1, 1, k, qL);
+      }
+    }
+    if (f < Ny) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) qR[v] = tr[2][v][fl][exl][k];
+    }
+    if (f == 0) {
+      const int b = g.bc[1][0];
+      if (b == BC_PERIODIC) {
+        gtrace(ex, Ny - 1, nf - 1, 1, k, qL);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) qL[v] = qR[v];
+        if (b == BC_REFLECTIVE) qL[2] = -qL[2];
+      }
+    }
+    if (f == Ny) {
+      const int b = g.bc[1][1];
+      if (b == BC_PERIODIC) {
+        gtrace(ex, 0, 0, 1, k, qR);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) qR[v] = qL[v];
+        if (b == BC_REFLECTIVE) qR[2] = -qR[2];
+      }
+    }
+    B WL[V], WR[V], F[V];
+    prims<2>(qL, WL, 1, gam);
+    prims<2>(qR, WR, 1, gam);
+    riemann<2, RS>(WL, WR, qL, qR, F, 1, gam, mc2, prm.lowmach_sd);
+    const long long plane = (long long)(Ny + 1) * Nx * n;
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      Fy[(long long)(t * V + v) * plane + (long long)f * Nx * n + ex * n + k] = F[v];
+  };
+  // faces left of / below the tile's elements (face index fastest along x,
+  // point index fastest along y: the lanes write along the face grids' rows)
+  for (int it = threadIdx.x; it < EX * n * EY; it += 256) xface(it % EX, (it / EX) % n, it / (EX * n));
+  if (ex0 + EX == Nx)
+    for (int it = threadIdx.x; it < n * EY; it += 256) xface(EX, it % n, it / n);
+  for (int it = threadIdx.x; it < EX * n * EY; it += 256) yface(it / (EX * n), (it % (EX * n)) / n, it % n);
+  if (ey0 + EY == Ny)
+    for (int it = threadIdx.x; it < EX * n; it += 256) yface(EY, it / n, it % n);
+}
+
+// ---------------------------------------------------------------------------
 // Predictor update.  A block owns EPB elements and all T time nodes.  Phase
 // "dim a": one thread per (element, t, line along a) interpolates the line to
 // the flux points, evaluates the Euler flux at the interior ones, takes the
@@ -1964,6 +2142,16 @@ template <int NDIM, int P, typename B> struct Stepper {
 
   template <int RS, typename S>
   static void faces_rs(spd_ctx* c, const OpsB& op, const S* src, int Tsrc, hipStream_t s) {
+    if constexpr (NDIM == 2) {
+      using FT = FaceTile2<P>;
+      const long long tiles = (long long)((c->g.N[0] + FT::EX - 1) / FT::EX) *
+                              ((c->g.N[1] + FT::EY - 1) / FT::EY);
+      c->timed(KC_FACE_SD, s, [&] {
+        hipLaunchKernelGGL((k_face_sd2d<P, RS, S, B>), dim3((unsigned)(Tsrc * tiles)), dim3(256), 0,
+                           s, op, c->g, c->prm, src, Ff(c, 0), Ff(c, 1));
+      });
+      return;
+    }
     if constexpr (NDIM >= 2) {
       constexpr int FPB = 256 / NLP;
       const long long Na = c->g.N[0];
