@@ -11,6 +11,8 @@
 //
 // One step (SDADER_Simulator.perform_update, src/sdader_simulator.py:361-378):
 //   predictor  m+1 x [element-face Riemann (k_face_sd) + update (k_pred)]
+//              (2-D: m x k_pred2d, element faces solved inside, then the
+//              faces of the final state for the corrector in k_face_sd2d)
 //   corrector  SD : k_pred in SD-update mode
 //              FV : sp->cv (k_tensor2d / k_tensor3d / k_tensor_el), then T substeps of
 //                   [HO face fluxes (k_hoflux), candidate (k_apply),
