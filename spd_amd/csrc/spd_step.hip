@@ -298,9 +298,23 @@ k_face_sd2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __re
   __shared__ B tr[4][V][EY][EX][n];  // end traces: x left, x right, y lower, y upper
   const int Nx = g.N[0], Ny = g.N[1];
   const int ntx = (Nx + EX - 1) / EX, nty = (Ny + EY - 1) / EY;
-  const int t = (int)(blockIdx.x / (unsigned)(ntx * nty));
-  const int rb = (int)(blockIdx.x % (unsigned)(ntx * nty));
-  const int ex0 = (rb % ntx) * EX, ey0 = (rb / ntx) * EY;
+  // tiles in XCD column bands when the tile columns divide by 8 (blocks go
+  // round-robin to the 8 XCDs: XCD k takes the columns [k W, (k + 1) W) of
+  // every time node, row by row, so a tile's left and lower neighbours ran
+  // on its XCD one and W blocks before it); otherwise launch order
+  int t, txb, tyb;
+  if (ntx % 8 == 0) {
+    const int W = ntx >> 3, j = (int)(blockIdx.x >> 3), jj = j % (W * nty);
+    t = j / (W * nty);
+    txb = (int)(blockIdx.x & 7) * W + jj % W;
+    tyb = jj / W;
+  } else {
+    t = (int)(blockIdx.x / (unsigned)(ntx * nty));
+    const int rb = (int)(blockIdx.x % (unsigned)(ntx * nty));
+    txb = rb % ntx;
+    tyb = rb / ntx;
+  }
+  const int ex0 = txb * EX, ey0 = tyb * EY;
   const S* base = src + (long long)t * V * g.NE * NL;
   for (int it = threadIdx.x; it < V * EY * EX; it += 256) {
     const int exl = it % EX, r = it / EX, eyl = r % EY, v = r / EY;
