@@ -205,6 +205,28 @@ def test_fallback_quiet_rows_vs_oracle(limiter):
     assert n_trouble > 0
 
 
+def test_xcd_band_tile_orders_vs_oracle():
+    """At 128^2 elements (512^2 cells, p=3) the detection tiles (16 columns of
+    32 cells) and the tiles of the final element faces (8 columns of 16
+    elements) run in XCD column bands (the smaller grids of the other tests
+    keep launch order): one step's flags and state against the oracle."""
+    print("[gpu xcd band tile orders 512^2]", flush=True)
+    s, mk = _kh_sims(128)
+    o = mk()
+    s.compute_dt()
+    o.compute_dt()
+    o.captures = []
+    o.perform_update()
+    recs = s.perform_update_with_flags()
+    n_trouble = 0
+    for rg, ro in zip(recs, o.captures):
+        bad, _ = flag_mismatch(rg["troubles"], ro["troubles"], ro["margin"])
+        assert bad == 0, bad
+        n_trouble += int(rg["troubles"].sum())
+    assert n_trouble > 0
+    assert relerr_fields(s.dm.U_sp, o.U_sp) < TOL, relerr_fields(s.dm.U_sp, o.U_sp)
+
+
 def test_fallback_forms_agree_on_large_grid():
     """Column strips (with quiet-row skipping) against the cell-tile form on
     a Kelvin-Helmholtz grid of 256^2 cells without noise, one step at a time
