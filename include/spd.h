@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define SPD_ABI_VERSION 4
+#define SPD_ABI_VERSION 5
 
 enum spd_status {
   SPD_OK = 0,
@@ -52,8 +52,11 @@ enum spd_precision { SPD_PRECISION_FP64 = 0, SPD_PRECISION_FP32 = 1 };
 /* Constructor keywords of SDADER_Simulator (src/sdader_simulator.py:15-27,
  * src/sd_simulator.py:20-26, src/fv_simulator.py:11-17, src/simulator.py:10-36)
  * as a plain struct.  Unsupported reference options (viscosity, potential,
- * well-balancing, isothermal, "ic"/"eq"/"pressure" sides, moncen, lhllc,
- * predictor=False) have no field; the Python facade rejects them. */
+ * well-balancing, isothermal, the "ic", "eq" and "pressure" sides,
+ * predictor=False) have no field;
+ * the Python facade rejects them.  The slope limiters (minmod, moncen) are
+ * `slope_limiter`; the Riemann solvers (llf, hllc, lhllc) are `riemann_sd` /
+ * `riemann_fv`. */
 typedef struct spd_config {
   int ndim;               /* 1, 2 or 3 */
   int N[3];               /* elements per dimension (x, y, z) */
@@ -118,8 +121,21 @@ int spd_get_operators(const spd_ctx* ctx, double* host_out, int len);
  * primitives (src/sd_simulator.py:214-227).  With W_cv != NULL the maximum is
  * taken over that array (element layout); with W_cv == NULL the maximum the
  * last spd_step recorded for its new state is used (the value the reference
- * computes from its post-step W_cv).  Writes dt to dt_dev. */
+ * computes from its post-step W_cv).  Writes dt to dt_dev.  The clamp of dt
+ * to reach t_end and the stop when it falls below 1e-14
+ * (src/sdader_simulator.py:404-410) are not applied here: the caller's loop
+ * applies them (the Python facade's perform_time_evolution), and the device
+ * loop applies them on the device (spd_run_begin with clamp = 1). */
 int spd_compute_dt(spd_ctx* ctx, const double* W_cv, double* dt_dev, void* stream);
+
+/* Exchange the maximum wave speed the context holds for its last state (the
+ * value spd_compute_dt(ctx, NULL, ...) and the device loop use; spd_step and
+ * spd_compute_dt with W_cv != NULL overwrite it) with the device scalar
+ * *slot.  Lets a second user of one context keep its own value: the adjoint
+ * swaps its own in around its calls so that the simulator's next dt is
+ * unchanged.  An extension with no reference counterpart (the reference
+ * recomputes dt from dm.W_cv, src/sd_simulator.py:214-227). */
+int spd_swap_wave_max(spd_ctx* ctx, double* slot, void* stream);
 
 /* One step: ADER predictor + SD or FV(/FB) corrector
  * (SDADER_Simulator.perform_update, src/sdader_simulator.py:361-378).

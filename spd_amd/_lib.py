@@ -30,7 +30,7 @@ EXPORTS = [
     "spd_profile_begin", "spd_profile_end", "spd_kernel_class_count", "spd_kernel_class_name",
     "spd_run_begin", "spd_run_steps", "spd_run_status", "spd_set_launch", "spd_get_launch",
     "spd_ic_points", "spd_ic_mean", "spd_sp_to_regular",
-    "spd_adj_step", "spd_adj_backward", "spd_adj_theta_count",
+    "spd_adj_step", "spd_adj_backward", "spd_adj_theta_count", "spd_swap_wave_max",
 ]
 
 
@@ -95,6 +95,8 @@ def load():
     L.spd_get_operators.restype = i
     L.spd_compute_dt.argtypes = [vp, dp, dp, vp]
     L.spd_compute_dt.restype = i
+    L.spd_swap_wave_max.argtypes = [vp, dp, vp]
+    L.spd_swap_wave_max.restype = i
     L.spd_step.argtypes = [vp, dp, dp, dp, dp, i, vp]
     L.spd_step.restype = i
     for name in ("spd_sp_to_cv", "spd_cv_to_sp"):

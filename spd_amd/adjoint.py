@@ -23,16 +23,33 @@ linearises the FP64 formulas at the FP32 form's states.
 """
 from __future__ import annotations
 
-import ctypes
+import contextlib
+import functools
 
 import torch
 
 from . import _lib
 
 
+def _own_wave_max(fn):
+    """Run ``fn`` with the adjoint's own recorded maximum wave speed swapped
+    into the shared context (and the simulator's swapped back afterwards), so
+    that the steps the adjoint runs never change the dt the simulator's next
+    ``compute_dt`` takes from the context.  Re-entrant: only the outermost
+    call swaps."""
+    @functools.wraps(fn)
+    def wrapper(self, *args, **kwargs):
+        with self._wave_max():
+            return fn(self, *args, **kwargs)
+    return wrapper
+
+
 class Adjoint:
-    """Adjoint of the steps of one ``SDADER_Simulator`` (its context, options
-    and mesh).  The simulator's own state is not touched."""
+    """Adjoint of the steps of one ``SDADER_Simulator``: it runs on the
+    simulator's context (options, mesh, operator matrices and scratch; the
+    tape and the reverse sweep's buffers are added to it on first use).  The
+    simulator's state is not touched: neither its arrays nor the maximum wave
+    speed its context records for the simulator's next dt."""
 
     def __init__(self, sim):
         self.sim = sim
@@ -40,6 +57,22 @@ class Adjoint:
         self.device = sim.device
         self.fb = sim.update == "FV" and bool(sim.FB)
         self.n_theta = int(self.lib.spd_adj_theta_count(sim._ctx))
+        self._cmax = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self._depth = 0
+
+    @contextlib.contextmanager
+    def _wave_max(self):
+        if self._depth == 0:
+            _lib.check(self.lib.spd_swap_wave_max(self.sim._ctx, _lib.ptr(self._cmax),
+                                                  self._stream), "spd_swap_wave_max")
+        self._depth += 1
+        try:
+            yield
+        finally:
+            self._depth -= 1
+            if self._depth == 0:
+                _lib.check(self.lib.spd_swap_wave_max(self.sim._ctx, _lib.ptr(self._cmax),
+                                                      self._stream), "spd_swap_wave_max")
 
     # -- one step ------------------------------------------------------------------
     @property
@@ -53,6 +86,7 @@ class Adjoint:
                              f"got {tuple(U.shape)} {U.dtype} {U.device}")
         return U.contiguous()
 
+    @_own_wave_max
     def dt_of(self, U_sp, out=None):
         """CFL dt of a state (``src/sd_simulator.py:214-227``): from the
         subcell-mean primitives of ``U_sp``.  Returns a 1-element device tensor."""
@@ -63,6 +97,7 @@ class Adjoint:
                                            self._stream), "spd_compute_dt")
         return out
 
+    @_own_wave_max
     def step(self, U_sp, dt, theta=None, record_theta=False):
         """One step out of place, recording the tape.  ``dt``: 1-element device
         tensor.  ``theta`` (optional): blending coefficients of every substep
@@ -83,6 +118,7 @@ class Adjoint:
                                          self._stream), "spd_adj_step")
         return (out, th_out) if record_theta else out
 
+    @_own_wave_max
     def step_vjp(self, U_sp, dt, U_bar, theta=None):
         """Vector-Jacobian product of one step at ``U_sp`` (flags frozen as
         detected at ``U_sp``, or ``theta``): returns ``U_bar^T d step / d U_sp``."""
@@ -98,6 +134,7 @@ class Adjoint:
         _lib.check(self.lib.spd_step(self.sim._ctx, _lib.ptr(U), None, None, _lib.ptr(dt), 0,
                                      self._stream), "spd_step")
 
+    @_own_wave_max
     def _forward(self, U0, nsteps, k, dt):
         """Run nsteps from U0; returns (U_N, checkpoints, dts)."""
         U = self._check(U0).clone()
@@ -122,6 +159,7 @@ class Adjoint:
             self._advance(U, d)
         return U, ckpt, dts
 
+    @_own_wave_max
     def _backward(self, U_bar, nsteps, k, ckpt, dts):
         Ub = self._check(U_bar).clone()
         starts = sorted(ckpt)

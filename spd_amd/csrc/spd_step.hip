@@ -1918,6 +1918,14 @@ k_reduce_max(const double* __restrict__ partial, long long n, double* cmax, cons
   if (threadIdx.x == 0) *cmax = v;
 }
 
+__global__ void k_swap_scalar(double* a, double* b) {
+  if (threadIdx.x == 0) {
+    const double t = *a;
+    *a = *b;
+    *b = t;
+  }
+}
+
 __global__ void k_dt_final(const double* cmax, double h, double cfl, double* dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) dt[0] = cfl * (h / *cmax);
 }
@@ -2845,6 +2853,13 @@ int spd_compute_dt(spd_ctx* ctx, const double* W_cv, double* dt_dev, void* strea
   }
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, s, ctx->cmax, ctx->h_dt, ctx->cfg.cfl_coeff,
                      dt_dev);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+}
+
+int spd_swap_wave_max(spd_ctx* ctx, double* slot, void* stream) {
+  if (!ctx || !slot) return fail(SPD_E_INVALID, "null argument");
+  hipLaunchKernelGGL(k_swap_scalar, dim3(1), dim3(64), 0, (hipStream_t)stream, ctx->cmax, slot);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
 }

@@ -18,8 +18,13 @@ if REPO not in sys.path:
 from spd_amd import initial_conditions as ic  # noqa: E402
 
 
-def golden_names():
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+def golden_names(blowup=False):
+    """Names of the golden fixtures.  The fixtures of runs that the reference
+    scheme drives to non-finite values (names ending in ``_blowup``) are
+    listed only with ``blowup=True`` (and then only they): their last step
+    is compared for non-finiteness, not to a tolerance."""
+    names = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    return [n for n in names if n.endswith("_blowup") == blowup]
 
 
 def load(name):

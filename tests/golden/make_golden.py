@@ -155,6 +155,53 @@ def run_case(name, kw, desc, steps, capture_steps=(), record_steps=None,
     print(f"{name}: {os.path.getsize(path)/1024:.0f} KiB", flush=True)
 
 
+def _census_member(args):
+    """First step after which the member's U_sp holds a non-finite value
+    (fixed steps of compute_dt + perform_update, no t_end clamp), or None."""
+    spec, steps = args
+    kw = ensemble_member_kw(spec)
+    kw.update(use_cupy=False, verbose=False)
+    s = SDADER_Simulator(init_fct=ic.four_quadrant(np.asarray(spec["states"])), **kw)
+    s.init_sim()
+    with np.errstate(all="ignore"):
+        for k in range(1, steps + 1):
+            s.compute_dt()
+            s.perform_update()
+            if not np.all(np.isfinite(s.dm.U_sp)):
+                return k
+    return None
+
+
+def ensemble_member_kw(spec):
+    """Keywords of one member of the four-quadrant ensemble (outflow sides,
+    FV + fallback, HLLC, minmod, CFL 0.8)."""
+    N = spec["N"]
+    return dict(p=spec["p"], N=(N, N), BC=GF2, update="FV", FB=True,
+                riemann_solver_sd="hllc", riemann_solver_fv="hllc", slope_limiter="minmod",
+                cfl_coeff=0.8, gamma=1.4)
+
+
+def census(n=64, seed=0, sizes=(16, 32, 64), steps=50, procs=6):
+    """Run every member of the seeded small-member ensemble draw through the
+    reference for ``steps`` steps and write, per member, the first step whose
+    state is non-finite (the reference scheme's own blow-ups on these random
+    four-quadrant states) to ``ensemble_small_nonfinite.json``."""
+    from multiprocessing import Pool
+    from spd_amd.ensemble import riemann_members
+    specs = riemann_members(n, seed=seed, sizes=sizes)
+    with Pool(procs) as pool:
+        first = pool.map(_census_member, [(sp, steps) for sp in specs], chunksize=1)
+    out = {"draw": {"n": n, "seed": seed, "sizes": list(sizes), "orders": [2, 3, 4, 5]},
+           "steps": steps, "members": [{"N": sp["N"], "p": sp["p"]} for sp in specs],
+           "first_nonfinite_step": first}
+    path = os.path.join(HERE, "ensemble_small_nonfinite.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"census: {sum(x is not None for x in first)} of {n} members non-finite "
+          f"within {steps} steps -> {path}", flush=True)
+    return first
+
+
 HLLC = dict(riemann_solver_sd="hllc", riemann_solver_fv="hllc")
 FVFB = dict(update="FV", FB=True, slope_limiter="minmod", cfl_coeff=0.8)
 GF1 = (("gradfree", "gradfree"),)
@@ -167,6 +214,24 @@ def main():
 
     def want(n):
         return not only or n in only
+
+    # Members of the small-member ensemble draw that the reference scheme
+    # drives to non-finite values: the census of all 64, and two members
+    # recorded step by step up to and including their first non-finite step.
+    if "census" in only:
+        census()
+    for idx, name in ((30, "ens30_p4_blowup"), (45, "ens45_p5_blowup")):
+        if name in only:
+            from spd_amd.ensemble import riemann_members
+            spec = riemann_members(64, seed=0, sizes=(16, 32, 64))[idx]
+            k = _census_member((spec, 50))
+            assert k is not None, (idx, "stays finite")
+            with np.errstate(all="ignore"):
+                run_case(name, ensemble_member_kw(spec),
+                         {"kind": "quad", "states": np.asarray(spec["states"]).tolist()},
+                         steps=k)
+    if only and only <= {"census", "ens30_p4_blowup", "ens45_p5_blowup"}:
+        return
 
     # Sod shock tube: the 1-D CPU configuration (outflow = "gradfree").
     if want("sod_p3"):

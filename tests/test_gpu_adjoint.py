@@ -223,3 +223,38 @@ def test_taped_step_equals_product_step(name, precision):
     torch.cuda.synchronize()
     assert torch.isfinite(U).all()
     assert torch.equal(taped, U), float((taped - U).abs().max())
+
+
+@pytest.mark.parametrize("name", ["kh16_p3", "quad12_p2"])
+def test_adjoint_leaves_simulator_dt_alone(name):
+    """The adjoint runs its steps on the simulator's context; the maximum wave
+    speed that context records for the simulator's next dt (after an FV step
+    the simulator takes dt from it, not from dm.W_cv) must survive every
+    adjoint call: perform_update, then adjoint calls from another state, then
+    compute_dt gives the dt of a simulator that made no adjoint call."""
+    from spd_amd.adjoint import Adjoint
+    from spd_amd.simulator import SDADER_Simulator
+    print(f"[adjoint keeps sim dt] {name}", flush=True)
+    fx = load(name)
+    cfg = fx["cfg"]
+    sims = []
+    for _ in range(2):
+        s = SDADER_Simulator(init_fct=make_ic(cfg["ic"]), device="cuda:0", verbose=False,
+                             **sim_kwargs(cfg))
+        nz = noise_of(cfg)
+        if nz is not None:
+            s.perturb_initial_condition(nz)
+        s.compute_dt()
+        s.perform_update()
+        sims.append(s)
+    a, b = sims
+    adj = Adjoint(b)
+    U = torch.as_tensor(noisy(np.asarray(b.dm.U_sp), seed=7), device=DEV)
+    dt = adj.dt_of(U)
+    adj.step_vjp(U, dt, torch.ones_like(U))
+    adj.run(U.clone().requires_grad_(True), 3, checkpoint_every=2).sum().backward()
+    assert a.compute_dt() == b.compute_dt()
+    a.perform_update()
+    b.perform_update()
+    assert a.compute_dt() == b.compute_dt()
+    np.testing.assert_array_equal(a.dm.U_sp, b.dm.U_sp)

@@ -4,7 +4,12 @@ the members run one by one with the host-driven loop.
 
 The same kernels run on the same data in the same order within each member,
 and the tuned launch shapes only regroup independent elements into blocks, so
-the bar is bit-identical states, times and step counts.
+the bar is bit-identical states, times and step counts; a member is
+non-finite only where the CPU oracle is too, and one member's final state is
+held to the oracle at 1e-12.
+The non-finite members of the small-member benchmark draw are those the
+reference scheme itself drives to non-finite values (the census that
+tests/golden/make_golden.py recorded from the reference).
 """
 import numpy as np
 import pytest
@@ -45,6 +50,73 @@ def test_ensemble_matches_solo_runs(run):
         assert s.time == ref.time, i
         np.testing.assert_array_equal(s.dm.U_sp, ref.dm.U_sp)
         np.testing.assert_array_equal(s.dm.W_cv, ref.dm.W_cv)
+    # one member against the oracle: the largest order among the members
+    # that stay physical (one member of this draw collapses: its dt falls
+    # below 1e-14 and the run stops early, as the reference's does; on such
+    # a state round-off is amplified without bound)
+    from fixtures import relerr_fields
+    from oracle.spd_oracle import OracleSimulator
+
+    def oracle_run(j):
+        kw = dict(kws[j])
+        init = kw.pop("init_fct")
+        for key in ("precision", "verbose"):
+            kw.pop(key, None)
+        o = OracleSimulator(init, **kw)
+        with np.errstate(all="ignore"):
+            if "t_end" in run:
+                o.perform_time_evolution(run["t_end"])
+            else:
+                for _ in range(run["steps"]):
+                    o.compute_dt()
+                    o.perform_update()
+        return o
+
+    # a member with a non-finite state is one the reference scheme also
+    # drives to non-finite values
+    for j, s in enumerate(ens.sims):
+        if not np.all(np.isfinite(s.dm.U_sp)):
+            assert not np.all(np.isfinite(oracle_run(j).U_sp)), j
+
+    def physical(j):
+        s = ens.sims[j]
+        if "t_end" in run:
+            done = s.time == run["t_end"]
+        else:
+            done = s.n_step == run["steps"] and s.dm.dt > 1e-10
+        return done and float(np.min(s.dm.U_sp[0])) > 0.0
+    ok = [j for j in range(len(kws)) if physical(j)]
+    assert ok
+    i = max(ok, key=lambda j: (kws[j]["p"], kws[j]["N"]))
+    o = oracle_run(i)
+    assert relerr_fields(ens.sims[i].dm.U_sp, o.U_sp) < 1e-12, relerr_fields(ens.sims[i].dm.U_sp,
+                                                                              o.U_sp)
+
+
+def test_nonfinite_members_are_the_reference_schemes():
+    """The small-member benchmark draw (64 members, N in {16, 32, 64}, p in
+    {2..5}), 12 batched steps: the members with a non-finite state are
+    exactly those whose reference run was non-finite by step 12."""
+    import json
+    import os
+    from fixtures import GOLDEN
+    from spd_amd.ensemble import Ensemble
+    with open(os.path.join(GOLDEN, "ensemble_small_nonfinite.json")) as f:
+        census = json.load(f)
+    d = census["draw"]
+    steps = 12
+    kws = members(d["n"], d["seed"], tuple(d["sizes"]), tuple(d["orders"]))
+    print(f"[ensemble census] {len(kws)} members, {steps} steps", flush=True)
+    ens = Ensemble(kws, device="cuda:0", graph_steps=4, branches=8, tune=False)
+    with np.errstate(all="ignore"):
+        ens.run(steps=steps, finish=False)
+    torch.cuda.synchronize()
+    got = {i for i, s in enumerate(ens.sims)
+           if not bool(torch.isfinite(s.dm.device_value("U_sp")).all())}
+    want = {i for i, k in enumerate(census["first_nonfinite_step"])
+            if k is not None and k <= steps}
+    assert want, "the draw should hold members that blow up within the steps"
+    assert got == want, (sorted(got), sorted(want))
 
 
 def test_autotuner_shapes_do_not_change_results():

@@ -207,9 +207,11 @@ def test_fallback_quiet_rows_vs_oracle(limiter):
 
 def test_xcd_band_tile_orders_vs_oracle():
     """At 128^2 elements (512^2 cells, p=3) the detection tiles (16 columns of
-    32 cells) and the tiles of the final element faces (8 columns of 16
-    elements) run in XCD column bands (the smaller grids of the other tests
-    keep launch order): one step's flags and state against the oracle."""
+    32 cells) run in XCD column bands two columns wide; the final element
+    faces have 8 tile columns of 16 elements, i.e. bands one column wide,
+    which is launch order (bands of two columns, with gradfree and reflective
+    sides too, are covered by test_gpu_parity_large.py): one step's flags and
+    state against the oracle."""
     print("[gpu xcd band tile orders 512^2]", flush=True)
     s, mk = _kh_sims(128)
     o = mk()

@@ -32,7 +32,7 @@ def test_abi_version_and_struct_size():
     import ctypes
     from spd_amd import _lib
     lib = _lib.load()
-    assert lib.spd_abi_version() == 4
+    assert lib.spd_abi_version() == 5
     assert lib.spd_config_size() == ctypes.sizeof(_lib.SpdConfig)
 
 
