@@ -128,6 +128,19 @@ int spd_get_operators(const spd_ctx* ctx, double* host_out, int len);
  * loop applies them on the device (spd_run_begin with clamp = 1). */
 int spd_compute_dt(spd_ctx* ctx, const double* W_cv, double* dt_dev, void* stream);
 
+/* spd_compute_dt with W_cv != NULL for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): no context, and no allocation of any kind.  No
+ * counterpart in the reference.  The kernels are spd_compute_dt's; their
+ * scratch (per-block maxima) is a static device array of 1 MiB + 512 B per
+ * GPU, held until the process ends, which limits W_cv to 2^25 = 33,554,432
+ * solution points (N_x N_y N_z (p+1)^ndim); larger inputs are rejected with
+ * SPD_E_INVALID before anything is launched.  cfg supplies ndim, N, p, lim,
+ * gamma, min_c2 and cfl_coeff.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (on different streams or host threads) race. */
+int spd_compute_dt_static(const spd_config* cfg, const double* W_cv, double* dt_dev,
+                          void* stream);
+
 /* Exchange the maximum wave speed the context holds for its last state (the
  * value spd_compute_dt(ctx, NULL, ...) and the device loop use; spd_step and
  * spd_compute_dt with W_cv != NULL overwrite it) with the device scalar

@@ -31,6 +31,7 @@ EXPORTS = [
     "spd_run_begin", "spd_run_steps", "spd_run_status", "spd_set_launch", "spd_get_launch",
     "spd_ic_points", "spd_ic_mean", "spd_sp_to_regular",
     "spd_adj_step", "spd_adj_backward", "spd_adj_theta_count", "spd_swap_wave_max",
+    "spd_compute_dt_static",
 ]
 
 
@@ -95,6 +96,8 @@ def load():
     L.spd_get_operators.restype = i
     L.spd_compute_dt.argtypes = [vp, dp, dp, vp]
     L.spd_compute_dt.restype = i
+    L.spd_compute_dt_static.argtypes = [ctypes.POINTER(SpdConfig), dp, dp, vp]
+    L.spd_compute_dt_static.restype = i
     L.spd_swap_wave_max.argtypes = [vp, dp, vp]
     L.spd_swap_wave_max.restype = i
     L.spd_step.argtypes = [vp, dp, dp, dp, dp, i, vp]

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2585,6 +2586,42 @@ const Entry* find_entry(int ndim, int p, int precision) {
 }
 size_t bulk_size(int precision) { return precision == SPD_PRECISION_FP32 ? 4 : 8; }
 
+// Scratch of the allocation-free dt (spd_compute_dt_static): a file-scope
+// device array, one per GPU, held until the process ends.  Layout: bytes
+// [0, 256) the skip flag the kernels test (an int, cleared by every call),
+// [256, 512) the maximum wave speed, [512, ...) the per-block maxima of at
+// most kStaticPartials blocks of 256 solution points.
+constexpr long long kStaticPartials = 1ll << 17;
+constexpr size_t kStaticScratchBytes = 512 + (size_t)kStaticPartials * sizeof(double);
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the array's name is only a host-side stand-in for it).
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch(unsigned char** out) {
+  int dev = 0;
+  SPD_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SPD_E_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SPD_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SPD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2853,6 +2890,64 @@ int spd_compute_dt(spd_ctx* ctx, const double* W_cv, double* dt_dev, void* strea
   }
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, s, ctx->cmax, ctx->h_dt, ctx->cfg.cfl_coeff,
                      dt_dev);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+}
+
+int spd_compute_dt_static(const spd_config* cfg, const double* W_cv, double* dt_dev,
+                          void* stream) {
+  if (!cfg || !W_cv || !dt_dev) return fail(SPD_E_INVALID, "null argument");
+  const int nd = cfg->ndim;
+  if (nd < 1 || nd > 3) return fail(SPD_E_INVALID, "ndim must be 1, 2 or 3");
+  if (cfg->p < 0) return fail(SPD_E_INVALID, "p must be >= 0");
+  long long NE = 1, NL = 1;
+  double hmin = 1e10;
+  for (int a = 0; a < nd; ++a) {
+    if (cfg->N[a] < 1) return fail(SPD_E_INVALID, "N must be >= 1");
+    if (!(cfg->lim[a][1] > cfg->lim[a][0])) return fail(SPD_E_INVALID, "empty domain");
+    NE *= cfg->N[a];
+    NL *= cfg->p + 1;
+    hmin = std::min(hmin, (cfg->lim[a][1] - cfg->lim[a][0]) / cfg->N[a]);
+  }
+  const long long npts = NE * NL;
+  const long long nb = (npts + 255) / 256;
+  if (nb > kStaticPartials)
+    return fail(SPD_E_INVALID, "W_cv holds " + std::to_string(npts) +
+                                   " solution points; the allocation-free dt takes at most " +
+                                   std::to_string(kStaticPartials * 256));
+  unsigned char* base = nullptr;
+  int rc = static_scratch(&base);
+  if (rc != SPD_OK) return rc;
+  int* skip = reinterpret_cast<int*>(base);
+  double* cmax = reinterpret_cast<double*>(base + 256);
+  double* partial = reinterpret_cast<double*>(base + 512);
+  Geom g{};
+  g.NE = NE;
+  g.skip = skip;
+  Params prm{};
+  prm.gamma = cfg->gamma;
+  prm.min_c2 = cfg->min_c2;
+  hipStream_t s = (hipStream_t)stream;
+  // the previous call's contents stay in the array: the skip flag the
+  // kernels test must read 0; the block maxima and the maximum are written
+  // before they are read
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, skip);
+  switch (nd) {
+    case 1:
+      hipLaunchKernelGGL((k_wave_max<1>), dim3((unsigned)nb), dim3(256), 0, s, g, prm, (int)NL,
+                         W_cv, partial);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_wave_max<2>), dim3((unsigned)nb), dim3(256), 0, s, g, prm, (int)NL,
+                         W_cv, partial);
+      break;
+    default:
+      hipLaunchKernelGGL((k_wave_max<3>), dim3((unsigned)nb), dim3(256), 0, s, g, prm, (int)NL,
+                         W_cv, partial);
+  }
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, partial, nb, cmax, skip);
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, s, cmax, hmin / (cfg->p + 1),
+                     cfg->cfl_coeff, dt_dev);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
 }
