@@ -7,12 +7,16 @@ fallback faces, minmod, CFL 0.8; rho = 2, vx = 0.5 for |y-0.5| < 0.25 else
 rho = 1, vx = -0.5, P = 2.5, vy = 0.01 sin(4 pi x), plus seeded uniform noise
 (amplitude 1e-3, numpy default_rng(0)) on the vx, vy subcell means.
 
-A step is one pass of the hot path over the whole state: compute_dt on the
-device (its scalar read back to the host, as in the reference's loop) and
-one spd_step (predictor + FV/FB corrector).  The state is resident in HBM
-before the timed region (10 warm-up steps, then 200 timed steps from t = 0),
-and the timed steps run without instrumentation; the per-kernel durations
-come from 20 further steps with HIP events around every launch.  The JSON line carries:
+A step is one pass of the hot path over the whole state: dt from the
+maximum wave speed of the state (compute_dt) and one spd_step (predictor +
+FV/FB corrector).  The state is resident in HBM before the timed region (10
+warm-up steps, then 200 timed steps from t = 0).  The timed steps run the
+device-side time loop (dt, step and time advance on the device, the
+reference's perform_iterations loop) replayed from one HIP graph captured
+beforehand, without instrumentation; the per-kernel durations come from 20
+further steps with HIP events around every launch, and the same number of
+steps with the host-driven loop (dt read back to the host every step) is
+reported as host_loop_ms_per_step.  The JSON line carries:
   * value: DOF-updates/s of the FP64 form (DOF = solution points; one
     DOF-update = one point advanced one full step), with the roofline of the
     dominant kernel (HIP-event durations over the timed region on the
@@ -466,19 +470,29 @@ def main():
         s.perturb_initial_condition(noise)
         s.advance(args.warmup)
         early = s.dm.device_value("U_sp").clone()
+        # the timed steps: the device-side loop (dt, step and time on the
+        # device), captured in one HIP graph beforehand (capture runs nothing)
+        s.prepare_graph(args.steps)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        s.advance(args.steps)
+        s.advance_device(args.steps)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         s.profile_begin(max_launches=200 * max(args.profile_steps, 1))
         s.advance(args.profile_steps)
         prof = s.profile_end()
+        # the same number of steps with the host-driven loop of the reference
+        # (dt read back to the host every step, kernels launched one by one)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.advance(args.steps)
+        torch.cuda.synchronize()
+        host_loop = time.perf_counter() - t0
         frac = s.troubled_fraction_next_step()
-        return s, elapsed, prof, frac, early
+        return s, elapsed, prof, frac, early, host_loop
 
-    s, elapsed, prof, frac, early64 = run_form("fp64")
-    s32, elapsed32, prof32, frac32, early32 = run_form("fp32")
+    s, elapsed, prof, frac, early64, host64 = run_form("fp64")
+    s32, elapsed32, prof32, frac32, early32, host32 = run_form("fp32")
 
     # the two forms after the same number of steps from the same state
     def l1_rel(r32, r64):
@@ -547,14 +561,16 @@ def main():
             "step_flop_per_dof": step_flops / dof if step_flops else None,
             "flops_source": os.path.relpath(FLOPS_SUMMARY, HERE)},
         "copy_GBps": copy_gbps,
+        "host_loop_ms_per_step": host64 / args.steps * 1e3,
         "troubled_fraction": frac,
         "kernels": kernels,
         "fp32_bulk": {
             "value": value32, "unit": "DOF-updates/s", "ms_per_step": elapsed32 / args.steps * 1e3,
+            "host_loop_ms_per_step": host32 / args.steps * 1e3,
             "speedup_vs_fp64": value32 / value,
             "step_hbm_roofline_frac": value32 * 64.0 / (HBM_PEAK_GBPS * 1e9),
             "l1_rel_vs_fp64": l1,
-            "steps_compared": args.warmup + args.steps + args.profile_steps + 1,
+            "steps_compared": args.warmup + 2 * args.steps + args.profile_steps + 1,
             "l1_rel_vs_fp64_after_warmup": l1_early, "steps_compared_after_warmup": args.warmup,
             "troubled_fraction": frac32,
             "kernels_ms_per_step": {k: v[0] / args.profile_steps

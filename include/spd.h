@@ -179,11 +179,15 @@ int spd_run_status(spd_ctx* ctx, double* time, long long* n_step, int* done, dou
 /* Launch shapes that the kernels take at run time (an extension with no
  * reference counterpart; the autotuner of spd_amd.ensemble sets them):
  * elements per block of the predictor update and of the 2-D high-order
- * face-flux kernel, and the form of the 2-D fallback correction (1: cell
- * tiles staged in LDS, 2: column strips marched in registers, the default;
- * the two agree to round-off, not bit for bit, so the autotuner leaves the
- * form alone).  value 0 restores the default; values above the maximum
- * (256 threads / 48 KiB of LDS per block; 2 forms) are rejected.
+ * face-flux kernel of form 1, and the form of the 2-D FV substep
+ * (1: k_ho2d writes the face fluxes, k_apply the candidate, the fallback
+ * corrects cell tiles staged in LDS; 2, the default: one kernel forms the
+ * face fluxes and the candidate, the fallback marches column strips in
+ * registers; 3: the same kernel without the face-flux arrays, the fallback
+ * runs on the compacted list of active tiles and forms their face fluxes
+ * again; the forms agree to round-off, not bit for bit, so the autotuner
+ * leaves the form alone).  value 0 restores the default; values above the
+ * maximum (256 threads / 48 KiB of LDS per block; 3 forms) are rejected.
  * spd_get_launch reports the value in effect and the maximum. */
 enum spd_launch_key {
   SPD_LAUNCH_PRED_EPB = 0,

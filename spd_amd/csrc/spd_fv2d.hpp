@@ -1007,5 +1007,564 @@ __device__ __forceinline__ void ho2d_items(const Ops<P, B>& op, const Geom& g, c
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Flux at the interior flux point f (1..p) of line lp of one element along
+// dimension A: Lagrange interpolation of the line's n solution points, then
+// primitives and the Euler flux (src/sd_simulator.py:190-198,
+// src/hydro.py:127-170); the arithmetic of ho2d_items.  ld(v, l) is the
+// element's value of variable v at solution point l (x fastest).
+// ---------------------------------------------------------------------------
+template <int P, int A, typename B, typename Ld>
+__device__ __forceinline__ void ho_fp_flux(const Ops<P, B>& op, B gamma, int f, int lp, Ld&& ld,
+                                           B* Fl) {
+  constexpr int V = 4, n = P + 1;
+  B M[V], W[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    B acc = B(0);
+#pragma unroll
+    for (int s = 0; s < n; ++s) {
+      const int l = (A == 0) ? (lp * n + s) : (s * n + lp);
+      acc += op.s2f[f][s] * ld(v, l);
+    }
+    M[v] = acc;
+  }
+  prims<2>(M, W, A, gamma);
+  flux<2>(W, Fl, A, gamma);
+}
+
+// ---------------------------------------------------------------------------
+// High-order FV face fluxes of time node t and the candidate state in one
+// pass (src/sdader_simulator.py:288-323,335-348, src/fv_simulator.py:189-205).
+// A block of 256 threads owns HocTile<P>::E elements (about 256 cells): the
+// state of its cells and the element-face Riemann fluxes it needs are loaded
+// into registers first, then the predictor slab of node t is staged in LDS;
+// one work item per (dimension, element, face column, line) evaluates the
+// flux on the line (element faces: the Riemann flux of the face grid), one
+// per (dimension, element, face column, transverse subcell) averages the
+// lines, and every cell of the block's elements gets
+// U_out = U_cv - dt_i sum(dF/dx), the expression of k_apply on the same face
+// values.  The FV face fluxes stay in LDS: each element forms its own n + 1
+// face columns per dimension (a face shared by two elements is formed by
+// both, identically).  U_out may be U_cv (the update in place without the
+// fallback): a cell is read and written by the same thread only.
+//   LDS: slab [V][E][ho_nlp], face columns [2][E][nf][n][V] (lines, then
+//   their transverse averages).
+// ---------------------------------------------------------------------------
+template <int P> struct HocTile {
+  static constexpr int n = P + 1, nf = P + 2, NL = n * n;
+  static constexpr int E = (256 / NL > 0) ? 256 / NL : 1;
+  static constexpr int NC = E * NL;               // cells of a block
+  static constexpr int NI = E * nf * n;           // face-column items per dimension
+  static constexpr int CPT = (NC + 255) / 256;    // cells per thread
+  static constexpr int IPT = (2 * NI + 255) / 256;  // items per thread (both dimensions)
+  template <typename B> static constexpr int lds_bytes() {
+    return (4 * E * ho_nlp(P) + 2 * NI * 4) * (int)sizeof(B);
+  }
+};
+
+template <int P, typename B>
+__global__ void __launch_bounds__(256)
+k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
+           const B* __restrict__ Ffx, const B* __restrict__ Ffy, const double* Ucv,
+           const double* __restrict__ dt_dev, double w_i, double* Uout, B* __restrict__ Fhx,
+           B* __restrict__ Fhy) {
+  SPD_SKIP(g);
+  using HT = HocTile<P>;
+  constexpr int V = 4, n = P + 1, nf = P + 2, NL = n * n, E = HT::E, NI = HT::NI;
+  B* slab = reinterpret_cast<B*>(spd_dyn_lds);  // [V][E][ho_nlp]
+  B* col = slab + V * E * ho_nlp(P);            // [2][E][nf][n][V]
+  const long long e0 = (long long)blockIdx.x * E;
+  const int ne = (int)((g.NE - e0 < E) ? (g.NE - e0) : E);
+  const int Nx = g.N[0], Ny = g.N[1], Nsx = g.Ns[0];
+  const int tid = threadIdx.x;
+  const B gamma = (B)prm.gamma;
+  const double dti = (*dt_dev) * w_i;
+  // the block's cells row by row across its elements (consecutive threads
+  // take consecutive x when the elements are consecutive in a row)
+  auto cell_of = [&](int item, int& el, int& i, int& qy, long long& ci, int& x, int& y) {
+    qy = item / (ne * n);
+    const int xl = item % (ne * n);
+    el = xl / n;
+    i = xl % n;
+    const long long e = e0 + el;
+    x = (int)(e % Nx) * n + i;
+    y = (int)(e / Nx) * n + qy;
+    ci = (long long)y * Nsx + x;
+  };
+  // 1. loads into registers: the cells' state, the element-face fluxes
+  double u[HT::CPT][V];
+#pragma unroll
+  for (int c = 0; c < HT::CPT; ++c) {
+    const int item = tid + c * 256;
+    if (item < ne * NL) {
+      int el, i, qy, x, y;
+      long long ci;
+      cell_of(item, el, i, qy, ci, x, y);
+#pragma unroll
+      for (int v = 0; v < V; ++v) u[c][v] = Ucv[v * g.Nc + ci];
+    }
+  }
+  B fl[HT::IPT][V];
+#pragma unroll
+  for (int it = 0; it < HT::IPT; ++it) {
+    const int item = tid + it * 256;
+    const int a = item / NI, r = item % NI;
+    const int lp = r % n, f = (r / n) % nf, el = r / (n * nf);
+    if (item >= 2 * NI || el >= ne || (f != 0 && f != nf - 1)) continue;
+    const long long e = e0 + el;
+    const int cx = (int)(e % Nx), cy = (int)(e / Nx);
+    if (a == 0) {
+      const long long i0 = ((long long)cy * n + lp) * (Nx + 1) + cx + (f == 0 ? 0 : 1);
+      const long long plane = (long long)(Nx + 1) * Ny * n;
+#pragma unroll
+      for (int v = 0; v < V; ++v) fl[it][v] = Ffx[(long long)(t * V + v) * plane + i0];
+    } else {
+      const long long i0 = (long long)(cy + (f == 0 ? 0 : 1)) * Nx * n + (long long)cx * n + lp;
+      const long long plane = (long long)Nx * n * (Ny + 1);
+#pragma unroll
+      for (int v = 0; v < V; ++v) fl[it][v] = Ffy[(long long)(t * V + v) * plane + i0];
+    }
+  }
+  // 2. the predictor slab of node t
+  for (int k = tid; k < V * ne * NL; k += 256) {
+    const int v = k / (ne * NL), r = k % (ne * NL);
+    slab[(v * E + r / NL) * ho_nlp(P) + r % NL] = Uad[((long long)(t * V + v) * g.NE + e0) * NL + r];
+  }
+  __syncthreads();
+  // 3. line fluxes of every face column, both dimensions
+#pragma unroll
+  for (int it = 0; it < HT::IPT; ++it) {
+    const int item = tid + it * 256;
+    const int a = item / NI, r = item % NI;
+    const int lp = r % n, f = (r / n) % nf, el = r / (n * nf);
+    if (item >= 2 * NI || el >= ne) continue;
+    B F[V];
+    if (f == 0 || f == nf - 1) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) F[v] = fl[it][v];
+    } else {
+      auto ld = [&](int v, int l) { return slab[(v * E + el) * ho_nlp(P) + l]; };
+      if (a == 0)
+        ho_fp_flux<P, 0, B>(op, gamma, f, lp, ld, F);
+      else
+        ho_fp_flux<P, 1, B>(op, gamma, f, lp, ld, F);
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) col[item * V + v] = F[v];
+  }
+  __syncthreads();
+  // 4. transverse averages (through registers: they replace the lines)
+  B av[HT::IPT][V];
+#pragma unroll
+  for (int it = 0; it < HT::IPT; ++it) {
+    const int item = tid + it * 256;
+    if (item >= 2 * NI || (item % NI) / (n * nf) >= ne) continue;
+    const int q = item % n, base = item - q;  // lines of the same (a, el, f)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      B acc = B(0);
+#pragma unroll
+      for (int s2 = 0; s2 < n; ++s2) acc += op.s2c[q][s2] * col[(base + s2) * V + v];
+      av[it][v] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < HT::IPT; ++it) {
+    const int item = tid + it * 256;
+    if (item >= 2 * NI || (item % NI) / (n * nf) >= ne) continue;
+#pragma unroll
+    for (int v = 0; v < V; ++v) col[item * V + v] = av[it][v];
+    if (Fhx) {
+      // the FV face-flux arrays for a fallback that reads them: an element
+      // writes its first n face columns, the last element of a row also its
+      // last (the layout of k_ho2d)
+      const int a = item / NI, r = item % NI;
+      const int q = r % n, f = (r / n) % nf, el = r / (n * nf);
+      const long long e = e0 + el;
+      const int cx = (int)(e % Nx), cy = (int)(e / Nx);
+      const int ca = (a == 0) ? cx : cy, Na = (a == 0) ? Nx : Ny;
+      if (f == nf - 1 && ca != Na - 1) continue;
+      const long long nxf = (a == 0) ? (long long)Nx * n + 1 : (long long)Nx * n;
+      const long long nface = nxf * ((a == 0) ? (long long)Ny * n : (long long)Ny * n + 1);
+      const long long fx = (a == 0) ? (long long)cx * n + f : (long long)cx * n + q;
+      const long long fy = (a == 0) ? (long long)cy * n + q : (long long)cy * n + f;
+      B* Fh = (a == 0) ? Fhx : Fhy;
+#pragma unroll
+      for (int v = 0; v < V; ++v) Fh[v * nface + fy * nxf + fx] = av[it][v];
+    }
+  }
+  __syncthreads();
+  // 5. the candidate
+#pragma unroll
+  for (int c = 0; c < HT::CPT; ++c) {
+    const int item = tid + c * 256;
+    if (item >= ne * NL) continue;
+    int el, i, qy, x, y;
+    long long ci;
+    cell_of(item, el, i, qy, ci, x, y);
+    const double rdx = g.rhfp[0][x + NGH], rdy = g.rhfp[1][y + NGH];
+    const B* fx0 = col + ((el * nf + i) * n + qy) * V;         // x-face i, subcell row qy
+    const B* fx1 = fx0 + n * V;                                // x-face i + 1
+    const B* fy0 = col + (NI + (el * nf + qy) * n + i) * V;    // y-face qy, subcell column i
+    const B* fy1 = fy0 + n * V;                                // y-face qy + 1
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double d = u[c][v] * 0.0;
+      d = d + ((double)fx1[v] - (double)fx0[v]) * rdx;
+      d = d + ((double)fy1[v] - (double)fy0[v]) * rdy;
+      Uout[v * g.Nc + ci] = u[c][v] - d * dti;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Compacted fallback.  Element-aligned cell tiles of FbTile<P> (EX x EY
+// elements, at most 32 x 8 cells): k_fbtile_list appends to `list` every tile
+// one of whose cells borders a face with a non-zero blending coefficient
+// (theta of the tile and of its one-cell ring without the corners; every
+// tile with godunov), and k_fbtile2d runs the MUSCL-Hancock + Riemann
+// fallback on the listed tiles only: as k_fbcorrect2d, with the high-order
+// face fluxes formed in LDS from the predictor slab of node t and the
+// element-face Riemann fluxes (the arithmetic of k_hocand2d) instead of read
+// from HBM, and only the cells that border a blended face written (the other
+// cells keep the candidate).  counters[0] counts the list, counters[1] hands
+// out its entries to the blocks (both zeroed before k_fbtile_list).
+// ---------------------------------------------------------------------------
+template <int P> struct FbTile {
+  static constexpr int n = P + 1;
+  static constexpr int EX = (32 / n > 0) ? 32 / n : 1;
+  static constexpr int EY = (8 / n > 0) ? 8 / n : 1;
+  static constexpr int TXc = EX * n, TYc = EY * n;
+  static constexpr int CXc = TXc + 2, CYc = TYc + 2;   // tile + 1 halo
+  static constexpr int WXc = TXc + 4, WYc = TYc + 4;   // tile + 2 halo
+  static constexpr int NFX = TYc * (TXc + 1), NFY = (TYc + 1) * TXc;  // faces of the tile
+};
+template <int P> inline long long fbtile_count(const Geom& g) {
+  using FT = FbTile<P>;
+  return (long long)((g.N[0] + FT::EX - 1) / FT::EX) * ((g.N[1] + FT::EY - 1) / FT::EY);
+}
+
+__global__ void k_zero2(int* c) {
+  if (threadIdx.x < 2 && blockIdx.x == 0) c[threadIdx.x] = 0;
+}
+
+// One wave per tile, FBL_WAVES tiles (consecutive in x) per block; one
+// atomic per block reserves the block's entries (same-address atomics
+// serialize in L2).
+constexpr int FBL_WAVES = 8;
+
+template <int P>
+__global__ void __launch_bounds__(64 * FBL_WAVES)
+k_fbtile_list(Geom g, Params prm, const double* __restrict__ theta, int* __restrict__ counters,
+              int* __restrict__ list, long long ntiles) {
+  SPD_SKIP(g);
+  using FT = FbTile<P>;
+  __shared__ int s_live[FBL_WAVES];
+  __shared__ int s_base;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int ntx = (g.N[0] + FT::EX - 1) / FT::EX;
+  const long long tile = (long long)blockIdx.x * FBL_WAVES + wave;
+  const int Nx = g.Ns[0], Ny = g.Ns[1], NGx = g.Ng[0];
+  bool live = false;
+  if (tile < ntiles) {
+    live = prm.godunov != 0;
+    const int x0 = (int)(tile % ntx) * FT::TXc, y0 = (int)(tile / ntx) * FT::TYc;
+    constexpr int NIT = (FT::CXc * FT::CYc + 63) / 64;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int k = lane + it * 64;
+      const int lx = k % FT::CXc, ly = k / FT::CXc;
+      const int x = x0 + lx - 1, y = y0 + ly - 1;
+      const bool corner = (lx == 0 || lx == FT::CXc - 1) && (ly == 0 || ly == FT::CYc - 1);
+      if (k < FT::CXc * FT::CYc && !corner && x >= -1 && x <= Nx && y >= -1 && y <= Ny)
+        live |= !(theta[(long long)(y + NGH) * NGx + (x + NGH)] == 0.0);
+    }
+  }
+  const int any = __any(live) ? 1 : 0;
+  if (lane == 0) s_live[wave] = any;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int cnt = 0;
+    for (int w = 0; w < FBL_WAVES; ++w) cnt += s_live[w];
+    s_base = cnt ? atomicAdd(&counters[0], cnt) : 0;
+  }
+  __syncthreads();
+  if (lane == 0 && any) {
+    int off = 0;
+    for (int w = 0; w < wave; ++w) off += s_live[w];
+    list[s_base + off] = (int)tile;
+  }
+}
+
+template <int P, int RS, typename B>
+__global__ void __launch_bounds__(256)
+k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
+           const B* __restrict__ Ffx, const B* __restrict__ Ffy, const double* __restrict__ Ucv,
+           const double* __restrict__ theta, int* __restrict__ counters,
+           const int* __restrict__ list, const double* __restrict__ dt_dev, double w_i,
+           double* __restrict__ Uout) {
+  SPD_SKIP(g);
+  using FT = FbTile<P>;
+  constexpr int V = 4, n = P + 1, NL = n * n;
+  constexpr int TXc = FT::TXc, TYc = FT::TYc, CXc = FT::CXc, CYc = FT::CYc;
+  constexpr int WXc = FT::WXc, WYc = FT::WYc;
+  constexpr int NFX = FT::NFX, NFY = FT::NFY;
+  constexpr int SW = (4 * WYc * WXc > 4 * (NFX + NFY)) ? 4 * WYc * WXc : 4 * (NFX + NFY);
+  static_assert(NFY >= NFX, "the x-face lines are staged in the y-face buffer");
+  __shared__ B sW[SW];                 // W (4 x WY x WX), then the face fluxes
+  __shared__ B sH[12][CYc][CXc];       // Mp(4), Sx(4), Sy(4)
+  __shared__ double sT[CYc][CXc];
+  __shared__ int s_idx;
+  B* bFx = sW;                          // [TYc][TXc+1][4]
+  B* bFy = sW + 4 * NFX;                // [TYc+1][TXc][4]
+  const int Nx = g.Ns[0], Ny = g.Ns[1], NGx = g.Ng[0];
+  const int NEx = g.N[0], NEy = g.N[1];
+  const int ntx = (NEx + FT::EX - 1) / FT::EX;
+  const B gamma = (B)prm.gamma;
+  const double dti = (*dt_dev) * w_i;
+  const B dtib = (B)dti;
+  const bool godunov = prm.godunov != 0;
+  const int count = counters[0];
+  B(*W)[WYc][WXc] = reinterpret_cast<B(*)[WYc][WXc]>(sW);
+  const long long planex = (long long)(NEx + 1) * NEy * n;
+  const long long planey = (long long)NEx * n * (NEy + 1);
+  auto U_at = [&](long long e, int v, int l) -> B {
+    return Uad[((long long)(t * V + v) * g.NE + e) * NL + l];
+  };
+
+  for (;;) {
+    __syncthreads();  // the previous tile's LDS is no longer read
+    if (threadIdx.x == 0) s_idx = atomicAdd(&counters[1], 1);
+    __syncthreads();
+    const int li = s_idx;
+    if (li >= count) break;
+    const int tile = list[li];
+    const int ex0 = (tile % ntx) * FT::EX, ey0 = (tile / ntx) * FT::EY;
+    const int x0 = ex0 * n, y0 = ey0 * n;
+
+    // 0. theta on tile + 1 halo
+    for (int k = threadIdx.x; k < CXc * CYc; k += blockDim.x) {
+      const int lx = k % CXc, ly = k / CXc;
+      const int x = x0 + lx - 1, y = y0 + ly - 1;
+      double th = 0.0;
+      if (x >= -1 && x <= Nx && y >= -1 && y <= Ny)
+        th = theta[(long long)(y + NGH) * NGx + (x + NGH)];
+      sT[ly][lx] = th;
+    }
+    // 1. old primitives on tile + 2 halo
+    for (int k = threadIdx.x; k < WXc * WYc; k += blockDim.x) {
+      const int lx = k % WXc, ly = k / WXc;
+      const int x = x0 + lx - 2, y = y0 + ly - 2;
+      double Wv[4] = {qnan(), qnan(), qnan(), qnan()};
+      if (in_ghost_range(g, x, y)) {
+        int c[3] = {x, y, 0};
+        load_W<2>(Ucv, g, c, Wv, prm.gamma);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) W[v][ly][lx] = (B)Wv[v];
+    }
+    __syncthreads();
+
+    // 2. Hancock state and half slopes of the cells that border an active face
+    for (int k = threadIdx.x; k < CXc * CYc; k += blockDim.x) {
+      const int lx = k % CXc, ly = k / CXc;
+      const int x = x0 + lx - 1, y = y0 + ly - 1;
+      bool need = false;
+      if (ly >= 1 && ly <= TYc && y < Ny) {
+        if (lx <= TXc && x + 1 <= Nx) need |= godunov || !(nmax(sT[ly][lx], sT[ly][lx + 1]) == 0.0);
+        if (lx >= 1 && x <= Nx) need |= godunov || !(nmax(sT[ly][lx - 1], sT[ly][lx]) == 0.0);
+      }
+      if (lx >= 1 && lx <= TXc && x < Nx) {
+        if (ly <= TYc && y + 1 <= Ny) need |= godunov || !(nmax(sT[ly][lx], sT[ly + 1][lx]) == 0.0);
+        if (ly >= 1 && y <= Ny) need |= godunov || !(nmax(sT[ly - 1][lx], sT[ly][lx]) == 0.0);
+      }
+      if (!need) continue;
+      const int wx = lx + 1, wy = ly + 1;
+      B Wc[4], dMh[2][4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Wc[v] = W[v][wy][wx];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int cb = (b == 0) ? x : y;
+        const int G = cb + NGH;
+        const B rhl = (B)g.rhcv[b][G - 1], rhr = (B)g.rhcv[b][G];
+        const B hf = (B)g.hfp[b][G];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const B Wm = (b == 0) ? W[v][wy][wx - 1] : W[v][wy - 1][wx];
+          const B Wp = (b == 0) ? W[v][wy][wx + 1] : W[v][wy + 1][wx];
+          const B L = (Wc[v] - Wm) * rhl;
+          const B R = (Wp - Wc[v]) * rhr;
+          dMh[b][v] = prm.moncen ? moncen<B>(L, R, (B)g.hcv[b][G - 1], (B)g.hcv[b][G], hf)
+                                 : minmod<B>(L, R);
+          sH[4 + 4 * b + v][ly][lx] = B(0.5) * dMh[b][v] * hf;
+        }
+      }
+      B dtW[4] = {B(0), B(0), B(0), B(0)};
+      const B rrho = qrcp<B>(Wc[0]);
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int vel = b + 1;
+        const B* dW = dMh[b];
+        dtW[0] = dtW[0] - (Wc[vel] * dW[0] + Wc[0] * dW[vel]);
+        dtW[3] = dtW[3] - (Wc[vel] * dW[3] + gamma * Wc[3] * dW[vel]);
+        dtW[vel] = dtW[vel] - (Wc[vel] * dW[vel] + dW[3] * rrho);
+        dtW[2] = dtW[2] - Wc[vel] * dW[2];
+      }
+      dtW[3] = dtW[3] * B(0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sH[v][ly][lx] = Wc[v] + B(0.5) * dtW[v] * dtib;
+    }
+    __syncthreads();
+
+    // 3. high-order face fluxes in the (now free) W buffer.  x: the line
+    // fluxes of every face column (staged in the y-face part), then the
+    // transverse averages into bFx.
+    for (int k = threadIdx.x; k < NFX; k += blockDim.x) {
+      const int fx = k % (TXc + 1), line = k / (TXc + 1);  // line = element row * n + lp
+      const int x = x0 + fx, el_x = fx / n, f = fx % n;
+      const int ey = ey0 + line / n, lp = line % n;
+      if (x > Nx || ey >= NEy) continue;
+      B Fl[V];
+      if (f == 0) {  // element face fa = ex0 + el_x
+        const long long i0 = ((long long)ey * n + lp) * (NEx + 1) + ex0 + el_x;
+#pragma unroll
+        for (int v = 0; v < V; ++v) Fl[v] = Ffx[(long long)(t * V + v) * planex + i0];
+      } else {
+        const long long e = (long long)ey * NEx + ex0 + el_x;
+        ho_fp_flux<P, 0, B>(op, gamma, f, lp, [&](int v, int l) { return U_at(e, v, l); }, Fl);
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) bFy[k * 4 + v] = Fl[v];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < NFX; k += blockDim.x) {
+      const int fx = k % (TXc + 1), ly = k / (TXc + 1);
+      const int q = ly % n, row0 = ly - q;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        B acc = B(0);
+#pragma unroll
+        for (int s = 0; s < n; ++s) acc += op.s2c[q][s] * bFy[((row0 + s) * (TXc + 1) + fx) * 4 + v];
+        bFx[k * 4 + v] = acc;
+      }
+    }
+    __syncthreads();
+    // y: the line fluxes of every face row into bFy, the averages through
+    // registers (they overwrite the lines)
+    for (int k = threadIdx.x; k < NFY; k += blockDim.x) {
+      const int col = k % TXc, fy = k / TXc;  // col = element column * n + lp
+      const int y = y0 + fy, el_y = fy / n, f = fy % n;
+      const int ex = ex0 + col / n, lp = col % n;
+      if (y > Ny || ex >= NEx) continue;
+      B Fl[V];
+      if (f == 0) {  // element face fa = ey0 + el_y
+        const long long i0 = (long long)(ey0 + el_y) * NEx * n + (long long)ex * n + lp;
+#pragma unroll
+        for (int v = 0; v < V; ++v) Fl[v] = Ffy[(long long)(t * V + v) * planey + i0];
+      } else {
+        const long long e = (long long)(ey0 + el_y) * NEx + ex;
+        ho_fp_flux<P, 1, B>(op, gamma, f, lp, [&](int v, int l) { return U_at(e, v, l); }, Fl);
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) bFy[k * 4 + v] = Fl[v];
+    }
+    __syncthreads();
+    constexpr int YIT = (NFY + 255) / 256;
+    B avg[YIT][4];
+#pragma unroll
+    for (int it = 0; it < YIT; ++it) {
+      const int k = threadIdx.x + it * 256;
+      if (k >= NFY) continue;
+      const int lx = k % TXc, fy = k / TXc;
+      const int q = lx % n, col0 = lx - q;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        B acc = B(0);
+#pragma unroll
+        for (int s = 0; s < n; ++s) acc += op.s2c[q][s] * bFy[(fy * TXc + col0 + s) * 4 + v];
+        avg[it][v] = acc;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < YIT; ++it) {
+      const int k = threadIdx.x + it * 256;
+      if (k >= NFY) continue;
+#pragma unroll
+      for (int v = 0; v < V; ++v) bFy[k * 4 + v] = avg[it][v];
+    }
+    __syncthreads();
+
+    // 4. fallback fluxes where the coefficient is non-zero, blended in place
+    for (int k = threadIdx.x; k < NFX; k += blockDim.x) {
+      const int fx = k % (TXc + 1), ly = k / (TXc + 1);
+      const int x = x0 + fx, y = y0 + ly;
+      if (x > Nx || y >= Ny) continue;
+      const double th = godunov ? 1.0 : nmax(sT[ly + 1][fx], sT[ly + 1][fx + 1]);
+      if (th == 0.0) continue;
+      B WL[4], WR[4], UL[4], UR[4], Ff[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        WL[v] = sH[v][ly + 1][fx] + sH[4 + v][ly + 1][fx];
+        WR[v] = sH[v][ly + 1][fx + 1] - sH[4 + v][ly + 1][fx + 1];
+      }
+      cons<2>(WL, UL, 0, gamma);
+      cons<2>(WR, UR, 0, gamma);
+      riemann<2, RS>(WL, WR, UL, UR, Ff, 0, gamma, (B)prm.min_c2, prm.lowmach_fv);
+      const B tb = (B)th;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) bFx[k * 4 + v] = tb * Ff[v] + (B(1) - tb) * bFx[k * 4 + v];
+    }
+    for (int k = threadIdx.x; k < NFY; k += blockDim.x) {
+      const int lx = k % TXc, fy = k / TXc;
+      const int x = x0 + lx, y = y0 + fy;
+      if (x >= Nx || y > Ny) continue;
+      const double th = godunov ? 1.0 : nmax(sT[fy][lx + 1], sT[fy + 1][lx + 1]);
+      if (th == 0.0) continue;
+      B WL[4], WR[4], UL[4], UR[4], Ff[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        WL[v] = sH[v][fy][lx + 1] + sH[8 + v][fy][lx + 1];
+        WR[v] = sH[v][fy + 1][lx + 1] - sH[8 + v][fy + 1][lx + 1];
+      }
+      cons<2>(WL, UL, 1, gamma);
+      cons<2>(WR, UR, 1, gamma);
+      riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, (B)prm.min_c2, prm.lowmach_fv);
+      const B tb = (B)th;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) bFy[k * 4 + v] = tb * Ff[v] + (B(1) - tb) * bFy[k * 4 + v];
+    }
+    __syncthreads();
+
+    // 5. the cells that border a blended face
+    for (int k = threadIdx.x; k < TXc * TYc; k += blockDim.x) {
+      const int tx = k % TXc, ty = k / TXc;
+      const int x = x0 + tx, y = y0 + ty;
+      if (x >= Nx || y >= Ny) continue;
+      const bool touched = godunov || !(nmax(sT[ty + 1][tx], sT[ty + 1][tx + 1]) == 0.0) ||
+                           !(nmax(sT[ty + 1][tx + 1], sT[ty + 1][tx + 2]) == 0.0) ||
+                           !(nmax(sT[ty][tx + 1], sT[ty + 1][tx + 1]) == 0.0) ||
+                           !(nmax(sT[ty + 1][tx + 1], sT[ty + 2][tx + 1]) == 0.0);
+      if (!touched) continue;
+      const double rdx = g.rhfp[0][x + NGH], rdy = g.rhfp[1][y + NGH];
+      const long long ci = (long long)y * Nx + x;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double u = Ucv[v * g.Nc + ci];
+        double d = u * 0.0;
+        d = d + ((double)bFx[(ty * (TXc + 1) + tx + 1) * 4 + v] -
+                 (double)bFx[(ty * (TXc + 1) + tx) * 4 + v]) * rdx;
+        d = d + ((double)bFy[((ty + 1) * TXc + tx) * 4 + v] - (double)bFy[(ty * TXc + tx) * 4 + v]) *
+                    rdy;
+        Uout[v * g.Nc + ci] = u - d * dti;
+      }
+    }
+  }
+}
+
 }  // namespace fv2d
 }  // namespace spd

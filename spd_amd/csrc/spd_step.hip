@@ -2282,11 +2282,29 @@ template <int NDIM, int P, typename B> struct Stepper {
     });
   }
 
-  // Elements per block of the 2-D high-order flux kernel (LDS slab of the
-  // block's elements, at most 48 KiB).
-  static int ho_epb_max() { return std::max(1, 49152 / (V * fv2d::ho_nlp(P) * (int)sizeof(B))); }
+  // Form of the 2-D fallback correction: 1 cell tiles and 2 (default) column
+  // strips, both over the high-order face-flux arrays written by k_ho2d and
+  // the candidate of k_apply; 3 the fused high-order flux + candidate kernel
+  // (no face-flux arrays) and the fallback on the compacted list of active
+  // tiles, which forms the high-order fluxes of its tiles again.  At
+  // Kelvin-Helmholtz 1024^2 p=3 in the first steps (70-100% of the tiles
+  // active) form 3 costs 17.8 against 13.2 ms per step: the tile kernel's
+  // halo and flux re-evaluation outweigh the saved face-flux traffic.
+  static int fb_form(const spd_ctx* c) {
+    return c->launch[SPD_LAUNCH_FB_FORM] > 0 ? c->launch[SPD_LAUNCH_FB_FORM] : 2;
+  }
+  // The fused k_hocand2d (fixed blocks of HocTile<P>::E elements) runs in
+  // every form but 1 (which keeps k_ho2d + k_apply), writing the face-flux
+  // arrays in form 2; the launch key of the high-order flux kernel sets the
+  // elements per block of k_ho2d.
+  static bool fused_ho(const spd_ctx* c) { return !c->cfg.FB || fb_form(c) != 1; }
+  // Elements per block of k_ho2d (LDS slab of the block's elements, at most
+  // 48 KiB).
+  static int ho_epb_max(const spd_ctx*) {
+    return std::max(1, 49152 / (V * fv2d::ho_nlp(P) * (int)sizeof(B)));
+  }
   static int ho_epb(const spd_ctx* c) {
-    const int mx = ho_epb_max();
+    const int mx = ho_epb_max(c);
     // default: the element count of a 24 KiB FP64 slab in both forms; p = 3:
     // 64 elements (a sweep at Kelvin-Helmholtz 1024^2 p=3 over 16..80 in both
     // precisions: 64 is the fastest, its blocks cover whole 64-element row
@@ -2295,15 +2313,11 @@ template <int NDIM, int P, typename B> struct Stepper {
     return c->launch[SPD_LAUNCH_HO_EPB] > 0 ? std::min(mx, c->launch[SPD_LAUNCH_HO_EPB])
                                             : std::min(mx, dflt);
   }
-  // Form of the 2-D fallback correction: 1 cell tiles, 2 column strips (default).
-  static int fb_form(const spd_ctx* c) {
-    return c->launch[SPD_LAUNCH_FB_FORM] > 0 ? c->launch[SPD_LAUNCH_FB_FORM] : 2;
-  }
   static int launch_value(const spd_ctx* c, int key) {
     return key == SPD_LAUNCH_PRED_EPB ? epb(c) : key == SPD_LAUNCH_HO_EPB ? ho_epb(c) : fb_form(c);
   }
-  static int launch_max(int key) {
-    return key == SPD_LAUNCH_PRED_EPB ? epb_max() : key == SPD_LAUNCH_HO_EPB ? ho_epb_max() : 2;
+  static int launch_max(const spd_ctx* c, int key) {
+    return key == SPD_LAUNCH_PRED_EPB ? epb_max() : key == SPD_LAUNCH_HO_EPB ? ho_epb_max(c) : 3;
   }
 
   // One FV substep with the 2-D tiled kernels; the corrected state goes to the
@@ -2311,18 +2325,30 @@ template <int NDIM, int P, typename B> struct Stepper {
   static void substep2d(spd_ctx* c, const OpsB& op, int i, double w, const double* dt,
                         hipStream_t s) {
     tape_state(c, i, s);
-    const int E = ho_epb(c);
-    const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
-    c->timed(KC_HOFLUX, s, [&] {
-      hipLaunchKernelGGL((fv2d::k_ho2d<P, B>), dim3(blocks), dim3(256),
-                         (size_t)V * E * fv2d::ho_nlp(P) * sizeof(B), s, op, c->g, c->prm, i,
-                         Uad(c), Ff(c, 0), Ff(c, 1), Fh(c, 0), Fh(c, 1), E);
-    });
-    if (!c->cfg.FB) {
-      apply(c, dt, w, c->Ucv, s);
-      return;
+    const int form = fb_form(c);
+    if (fused_ho(c)) {
+      // high-order fluxes and candidate in one pass (in place without the fallback)
+      using HT = fv2d::HocTile<P>;
+      double* out = c->cfg.FB ? c->Unew : c->Ucv;
+      const size_t lds = (size_t)HT::template lds_bytes<B>();
+      const unsigned hblocks = (unsigned)((c->g.NE + HT::E - 1) / HT::E);
+      c->timed(KC_HOFLUX, s, [&] {
+        B* fx = (c->cfg.FB && form == 2) ? Fh(c, 0) : nullptr;
+        B* fy = (c->cfg.FB && form == 2) ? Fh(c, 1) : nullptr;
+        hipLaunchKernelGGL((fv2d::k_hocand2d<P, B>), dim3(hblocks), dim3(256), lds, s, op, c->g,
+                           c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, dt, w, out, fx, fy);
+      });
+      if (!c->cfg.FB) return;
+    } else {
+      const int E = ho_epb(c);
+      const unsigned blocks = (unsigned)((c->g.NE + E - 1) / E);
+      c->timed(KC_HOFLUX, s, [&] {
+        hipLaunchKernelGGL((fv2d::k_ho2d<P, B>), dim3(blocks), dim3(256),
+                           (size_t)V * E * fv2d::ho_nlp(P) * sizeof(B), s, op, c->g, c->prm, i,
+                           Uad(c), Ff(c, 0), Ff(c, 1), Fh(c, 0), Fh(c, 1), E);
+      });
+      apply(c, dt, w, c->Unew, s);
     }
-    apply(c, dt, w, c->Unew, s);
     const dim3 tiles((c->g.Ns[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ns[1] + fv2d::TY - 1) / fv2d::TY);
     if (!c->tape_replay) {
       if (c->need_gate)
@@ -2343,9 +2369,33 @@ template <int NDIM, int P, typename B> struct Stepper {
     tape_theta(c, i, s);
     tap(c, i, s);
     // the corrected state replaces the candidate in place
+    if (form == 3) {
+      const long long ntiles = fv2d::fbtile_count<P>(c->g);
+      hipLaunchKernelGGL(fv2d::k_zero2, dim3(1), dim3(64), 0, s, c->fbt_counters);
+      c->timed(KC_THETA1, s, [&] {
+        hipLaunchKernelGGL((fv2d::k_fbtile_list<P>),
+                           dim3((unsigned)((ntiles + fv2d::FBL_WAVES - 1) / fv2d::FBL_WAVES)),
+                           dim3(64 * fv2d::FBL_WAVES), 0, s, c->g, c->prm, c->theta,
+                           c->fbt_counters, c->fbt_list, ntiles);
+      });
+      // blocks take listed tiles from a counter until the list is done
+      const unsigned nb = (unsigned)std::min<long long>(ntiles, 256 * 12);
+      c->timed(KC_FBFLUX, s, [&] {
+        if (c->cfg.riemann_fv != SPD_RS_LLF)
+          hipLaunchKernelGGL((fv2d::k_fbtile2d<P, RS_HLLC, B>), dim3(nb), dim3(256), 0, s, op,
+                             c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, c->theta,
+                             c->fbt_counters, c->fbt_list, dt, w, c->Unew);
+        else
+          hipLaunchKernelGGL((fv2d::k_fbtile2d<P, RS_LLF, B>), dim3(nb), dim3(256), 0, s, op,
+                             c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, c->theta,
+                             c->fbt_counters, c->fbt_list, dt, w, c->Unew);
+      });
+      std::swap(c->Ucv, c->Unew);
+      return;
+    }
     c->timed(KC_FBFLUX, s, [&] {
       const bool hllc = c->cfg.riemann_fv != SPD_RS_LLF;
-      if (fb_form(c) == 1) {
+      if (form == 1) {
         if (hllc)
           hipLaunchKernelGGL((fv2d::k_fbcorrect2d<RS_HLLC, B>), tiles, dim3(fv2d::NT), 0, s,
                              c->g, c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, c->Unew);
@@ -2556,7 +2606,7 @@ template <int NDIM, int P, typename B> struct Stepper {
 using StepFn = int (*)(spd_ctx*, double*, double*, double*, const double*, int, hipStream_t);
 using TensorFn = int (*)(spd_ctx*, int, const double*, double*, hipStream_t);
 using LaunchFn = int (*)(const spd_ctx*, int);
-using LaunchMaxFn = int (*)(int);
+using LaunchMaxFn = int (*)(const spd_ctx*, int);
 
 struct Entry {
   int ndim, p, precision;
@@ -2821,6 +2871,14 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
       if ((rc = c->alloc(&c->theta, (size_t)g.Ngc)) != SPD_OK) return rc;
       long long n1 = 1;
       for (int a = 0; a < nd; ++a) n1 *= g.Ns[a] + 2;
+      if (nd == 2) {
+        // compacted fallback: tiles of 32/(p+1) x 8/(p+1) elements (at least 1)
+        const int ex = std::max(1, 32 / c->n), ey = std::max(1, 8 / c->n);
+        const long long ntiles =
+            (long long)((g.N[0] + ex - 1) / ex) * ((g.N[1] + ey - 1) / ey);
+        if ((rc = c->alloc(&c->fbt_counters, 2)) != SPD_OK) return rc;
+        if ((rc = c->alloc(&c->fbt_list, (size_t)ntiles)) != SPD_OK) return rc;
+      }
       if (nd != 2 && (rc = alloc_bulk(&c->HS, (1 + nd) * V * n1)) != SPD_OK) return rc;
       if (nd != 2 && (rc = c->alloc(&c->rp, (size_t)(2 * g.Nc))) != SPD_OK) return rc;
       if (nd != 2 && (rc = c->alloc(&c->Wold, (size_t)(V * g.Nc))) != SPD_OK) return rc;
@@ -2972,9 +3030,9 @@ int spd_set_launch(spd_ctx* ctx, int key, int value) {
   if (!ctx || key < 0 || key >= SPD_LAUNCH_COUNT || value < 0)
     return fail(SPD_E_INVALID, "invalid launch key or value");
   const Entry* en = find_entry(ctx->ndim, ctx->p, ctx->cfg.precision);
-  if (value > en->launch_max(key))
+  if (value > en->launch_max(ctx, key))
     return fail(SPD_E_INVALID, "launch value above its maximum " +
-                                   std::to_string(en->launch_max(key)));
+                                   std::to_string(en->launch_max(ctx, key)));
   ctx->launch[key] = value;
   return SPD_OK;
 }
@@ -2983,7 +3041,7 @@ int spd_get_launch(const spd_ctx* ctx, int key, int* value, int* max_value) {
   if (!ctx || key < 0 || key >= SPD_LAUNCH_COUNT) return fail(SPD_E_INVALID, "invalid launch key");
   const Entry* en = find_entry(ctx->ndim, ctx->p, ctx->cfg.precision);
   if (value) *value = en->launch_value(ctx, key);
-  if (max_value) *max_value = en->launch_max(key);
+  if (max_value) *max_value = en->launch_max(ctx, key);
   return SPD_OK;
 }
 

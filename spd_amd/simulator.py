@@ -511,6 +511,15 @@ class SDADER_Simulator:
             self.compute_dt()
             self._step(snapshot=False)
 
+    def advance_device(self, nsteps=1):
+        """``nsteps`` steps of the device-side loop (dt from the recorded wave
+        speed, step, time += dt on the device; no t_end clamp), in chunks of
+        ``graph_steps`` replayed from a HIP graph, one host synchronisation
+        per chunk -- ``advance`` without a host round trip per step, and
+        without ``init_sim``/``end_sim``."""
+        if nsteps > 0:
+            self._run_on_device(0.0, nsteps, 0)
+
     def perform_update(self) -> bool:
         """One step; ``dm.U_sp``, ``dm.U_cv``, ``dm.W_cv``, ``time`` and ``n_step``
         are updated as in ``src/sdader_simulator.py:361-378``."""
@@ -532,20 +541,31 @@ class SDADER_Simulator:
         if self.rank == 0:
             print(f"t={self.time}, steps taken {self.n_step}, time taken {self.execution_time}")
 
+    def prepare_graph(self, k=None):
+        """Capture the device loop's chunk of ``k`` (default ``graph_steps``)
+        steps in a HIP graph without running it, and make ``k`` the chunk
+        size; the first chunk of a run otherwise captures its graph first."""
+        k = self.graph_steps if k is None else int(k)
+        if k <= 0:
+            raise ValueError("the chunk of a graph holds at least one step")
+        self.graph_steps = k
+        U_sp = self.dm.device_value("U_sp")
+        key = (U_sp.data_ptr(), k)
+        if self._graph is None or self._graph[0] != key:
+            self._graph = None
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.device(self.device), torch.cuda.graph(g):
+                _lib.check(self._lib.spd_run_steps(self._ctx, _lib.ptr(U_sp), k,
+                                                   self._stream), "spd_run_steps")
+            self._graph = (key, g)
+        return self._graph[1]
+
     def _run_chunk(self, U_sp, k):
         """Enqueue ``k`` device-loop steps: replay of a captured graph of
         ``graph_steps`` steps when ``k`` equals it, else direct launches."""
         s = self._stream
         if k == self.graph_steps and k > 0 and not self._profiling:
-            key = (U_sp.data_ptr(), k)
-            if self._graph is None or self._graph[0] != key:
-                self._graph = None
-                g = torch.cuda.CUDAGraph()
-                with torch.cuda.device(self.device), torch.cuda.graph(g):
-                    _lib.check(self._lib.spd_run_steps(self._ctx, _lib.ptr(U_sp), k,
-                                                       self._stream), "spd_run_steps")
-                self._graph = (key, g)
-            self._graph[1].replay()
+            self.prepare_graph(k).replay()
         else:
             _lib.check(self._lib.spd_run_steps(self._ctx, _lib.ptr(U_sp), k, s), "spd_run_steps")
 

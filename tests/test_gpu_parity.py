@@ -81,17 +81,21 @@ def _fb2d(name):
     return len(kw["N"]) == 2 and kw.get("update", "SD") == "FV" and kw.get("FB", False)
 
 
+@pytest.mark.parametrize("form", [1, 3])
 @pytest.mark.parametrize("name", [n for n in NAMES if _fb2d(n)])
-def test_fallback_tile_form_vs_oracle(name):
-    """The 2-D fallback correction has two kernel forms (cell tiles in LDS,
-    column strips in registers; the strips are the default): the tile form
-    meets the same bar against the oracle."""
+def test_fallback_forms_vs_oracle(name, form):
+    """The 2-D fallback correction has three kernel forms: 1 cell tiles in
+    LDS and 2 (the default, which every other test runs) column strips in
+    registers, both over the high-order face-flux arrays in HBM and the
+    separate candidate kernel, and 3 the fused high-order flux + candidate
+    kernel with the fallback on the compacted list of active tiles.  Forms 1
+    and 3 meet the same bar against the oracle."""
     from spd_amd import _lib
-    print(f"[gpu parity tiles] {name}", flush=True)
+    print(f"[gpu parity fallback form {form}] {name}", flush=True)
     fx = load(name)
     s = gpu_sim(fx)
-    _lib.check(s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], 1), "spd_set_launch")
-    assert s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], 3) < 0
+    _lib.check(s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], form), "spd_set_launch")
+    assert s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], 4) < 0
     o = oracle_sim(fx)
     for k in range(1, int(fx["nsteps"]) + 1):
         s.compute_dt()
@@ -187,8 +191,10 @@ def test_fallback_quiet_rows_vs_oracle(limiter):
     velocity noise the Kelvin-Helmholtz layers on 128^2 cells trouble cells
     only along the two density jumps, so most chunks (4 rows at this size) and
     rows take those paths: states and flags against the oracle, 3 steps."""
+    from spd_amd import _lib
     print(f"[gpu fallback quiet rows] {limiter}", flush=True)
     s, mk = _kh_sims(32, slope_limiter=limiter)
+    _lib.check(s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], 2), "spd_set_launch")
     o = mk()
     n_trouble = 0
     for k in range(3):
@@ -229,15 +235,18 @@ def test_xcd_band_tile_orders_vs_oracle():
     assert relerr_fields(s.dm.U_sp, o.U_sp) < TOL, relerr_fields(s.dm.U_sp, o.U_sp)
 
 
-def test_fallback_forms_agree_on_large_grid():
-    """Column strips (with quiet-row skipping) against the cell-tile form on
-    a Kelvin-Helmholtz grid of 256^2 cells without noise, one step at a time
+@pytest.mark.parametrize("forms", [(2, 1), (3, 1), (3, 2)])
+def test_fallback_forms_agree_on_large_grid(forms):
+    """The fallback forms pairwise (column strips with quiet-row skipping,
+    cell tiles, compacted tiles with the fused candidate) on a
+    Kelvin-Helmholtz grid of 256^2 cells without noise, one step at a time
     from the same state: same flags, states within 1e-12."""
     from spd_amd import _lib
-    print("[gpu fallback forms 256^2]", flush=True)
+    print(f"[gpu fallback forms 256^2] {forms}", flush=True)
     a, _ = _kh_sims(64)
     b, _ = _kh_sims(64)
-    _lib.check(b._lib.spd_set_launch(b._ctx, _lib.LAUNCH["fb_form"], 1), "spd_set_launch")
+    _lib.check(a._lib.spd_set_launch(a._ctx, _lib.LAUNCH["fb_form"], forms[0]), "spd_set_launch")
+    _lib.check(b._lib.spd_set_launch(b._ctx, _lib.LAUNCH["fb_form"], forms[1]), "spd_set_launch")
     for k in range(3):
         b.dm.U_sp = np.asarray(a.dm.U_sp)
         a.compute_dt()
