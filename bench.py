@@ -8,15 +8,14 @@ rho = 1, vx = -0.5, P = 2.5, vy = 0.01 sin(4 pi x), plus seeded uniform noise
 (amplitude 1e-3, numpy default_rng(0)) on the vx, vy subcell means.
 
 A step is one pass of the hot path over the whole state: dt from the
-maximum wave speed of the state (compute_dt) and one spd_step (predictor +
-FV/FB corrector).  The state is resident in HBM before the timed region (10
-warm-up steps, then 200 timed steps from t = 0).  The timed steps run the
-device-side time loop (dt, step and time advance on the device, the
-reference's perform_iterations loop) replayed from one HIP graph captured
-beforehand, without instrumentation; the per-kernel durations come from 20
-further steps with HIP events around every launch, and the same number of
-steps with the host-driven loop (dt read back to the host every step) is
-reported as host_loop_ms_per_step.  The JSON line carries:
+maximum wave speed of the state (compute_dt, its scalar read back to the host
+as in the reference's loop) and one spd_step (predictor + FV/FB corrector).
+The state is resident in HBM before the timed region (10 warm-up steps, then
+200 timed steps from t = 0), and the timed steps run without
+instrumentation; the per-kernel durations come from 20 further steps with
+HIP events around every launch, and the same number of steps with the
+device-side time loop (dt, step and time on the device) replayed from one
+HIP graph is reported as device_loop_ms_per_step.  The JSON line carries:
   * value: DOF-updates/s of the FP64 form (DOF = solution points; one
     DOF-update = one point advanced one full step), with the roofline of the
     dominant kernel (HIP-event durations over the timed region on the
@@ -77,10 +76,10 @@ def algorithmic_bytes(N, p, launches):
                  reads U_sp and, when another follows, writes its increment
                  (V C), which the second reads with U_sp (2 V C) in place of
                  the state (the one-element halo reads are not counted)
-      k_hoflux   T launches (k_ho2d): read the node's state slice (V C) and its
-                 element-face fluxes (2 V (N+1) N n), write both FV-face arrays
+      k_hoflux   T launches (k_hocand2d): read the node's state slice (V C),
+                 its element-face fluxes (2 V (N+1) N n) and U_cv (V C), write
+                 both FV-face arrays (for the fallback) and the candidate (V C)
       k_tensor   once: read + write V C
-      k_apply    T launches: read U_cv + both face arrays, write the candidate
       k_detect   T launches: read U_cv and the candidate (2 V C), write C flags
       k_theta1   T launches (k_theta2d): read C flags, write G
       k_fbface   T launches (k_fbmarch2d): read U_cv, theta, both face arrays,
@@ -102,9 +101,8 @@ def algorithmic_bytes(N, p, launches):
         "k_face_sd": T * V * C + 2 * Fe,
         "k_pred": ((2 * V * C) + (2 * V * C + T * V * C) + (m - 2) * (2 * T * V * C + V * C)
                    if m >= 2 else V * C + T * V * C),
-        "k_hoflux": T * (V * C + 2 * V * (N + 1) * N * n + 2 * Fd),
+        "k_hoflux": T * (3 * V * C + 2 * V * (N + 1) * N * n + 2 * Fd),
         "k_tensor": 2 * V * C,
-        "k_apply": T * (2 * V * C + 2 * Fd),
         "k_detect": T * (2 * V * C),
         "k_theta1": T * G,
         "k_fbface": T * (2 * V * C + G + 2 * Fd),
@@ -130,8 +128,7 @@ CLASS_KERNELS = {
                ("k_pred2d<3, 1, double, double, false, true>", 1),
                ("k_pred2d<3, 1, double, double, false, false>", 1)],
     "k_detect": [("k_detect2d", 4)],
-    "k_hoflux": [("k_ho2d<3, double>", 4)],
-    "k_apply": [("k_apply<2, double>", 4)],
+    "k_hoflux": [("k_hocand2d<3, double>", 4)],
     "k_theta1": [("k_theta2d", 4)],
     "k_cv2sp": [("k_tensor2d<3, 0>", 1)],
     "k_tensor": [("k_tensor2d<3, 1>", 1)],
@@ -163,6 +160,19 @@ def measured_traffic(cls):
     return sum(vals[k] * w for k, w in names.items()) / sum(names.values())
 
 
+def measured_step_traffic(prof, steps):
+    """HBM bytes per step summed over the kernel classes of the event window
+    (each class's counter bytes per launch times its launches per step);
+    None when a class is missing from the counter summary."""
+    total = 0.0
+    for k, (_, cnt) in prof.items():
+        t = measured_traffic(k)
+        if t is None:
+            return None
+        total += t * cnt / steps
+    return total
+
+
 def measured_fp64_flops(cls):
     """FP64 flop per launch of a kernel class from the committed counter
     summary (64 x (ADD + MUL + 2 FMA + TRANS) FP64 instructions per dispatch),
@@ -187,6 +197,17 @@ def measured_fp64_flops(cls):
     if len(vals) != len(names):
         return None
     return sum(vals[k] * w for k, w in names.items()) / sum(names.values())
+
+
+def step_traffic_summary(step_bytes, ms_step, alg_bytes):
+    """Measured HBM bytes per step (counter summary) against the step time,
+    beside the metric's algorithmic bytes per step (64 B per DOF)."""
+    if step_bytes is None:
+        return None
+    gbps = step_bytes / (ms_step * 1e-3) / 1e9
+    return {"bytes_per_step": step_bytes, "GBps": gbps, "frac": gbps / HBM_PEAK_GBPS,
+            "algorithmic_bytes_per_step": alg_bytes,
+            "traffic_over_algorithmic": step_bytes / alg_bytes, "source": os.path.relpath(TRAFFIC_SUMMARY, HERE)}
 
 
 def copy_bandwidth(torch, nbytes=2 << 30, reps=10):
@@ -257,32 +278,77 @@ def run_cpu_baseline(N=256, steps=1, warmup=1):
 
 def run_ensemble(torch, n=64, steps=10, sizes=(64, 128, 256, 384), seed=0):
     """The batched ensemble of 2-D four-quadrant Riemann problems (outflow,
-    FV+FB, HLLC, p and N drawn from a seed) on a bounded sample: ``steps``
-    steps of every member, batched (one multi-branch HIP graph launch per
-    chunk of steps, autotuned launch shapes) and then the same steps with the
-    members run one after the other by the host-driven loop."""
+    FV+FB, HLLC, p and N drawn from a seed) on a bounded sample of ``steps``
+    steps per member.
+
+    A probe run of the ``steps + 1`` steps finds the members the scheme drives
+    to non-finite values (for the small-member draw the reference's census,
+    tests/golden/ensemble_small_nonfinite.json, names the same members; it is
+    reported beside the count); only the members that stay finite are timed.
+    Both legs start from the same saved state (after one step that builds the
+    batched graph): the batched leg (one multi-branch HIP graph launch per
+    chunk of steps, autotuned launch shapes) and the members one after the
+    other with the host-driven loop (compute_dt + step, no snapshot writes)."""
     import contextlib
     from spd_amd.ensemble import Ensemble, riemann_member_kwargs, riemann_members, timed
     specs = riemann_members(n, seed=seed, sizes=sizes)
+    out = {"members": n, "sizes": list(sizes), "orders": [2, 3, 4, 5], "seed": seed,
+           "steps_per_member": steps}
     with contextlib.redirect_stdout(sys.stderr):
+        probe = Ensemble([riemann_member_kwargs(m) for m in specs], graph_steps=4, branches=8,
+                         tune=False)
+        probe.run(steps=steps + 1, finish=False)
+        torch.cuda.synchronize()
+        bad = [i for i, s_ in enumerate(probe.sims)
+               if not bool(torch.isfinite(s_.dm.device_value("U_sp")).all())]
+        del probe
+        torch.cuda.empty_cache()
+        out["members_nonfinite"] = len(bad)
+        census = os.path.join(HERE, "tests", "golden", "ensemble_small_nonfinite.json")
+        if os.path.exists(census):
+            with open(census) as f:
+                c = json.load(f)
+            d = c["draw"]
+            if (d["n"], d["seed"], tuple(d["sizes"])) == (n, seed, tuple(sizes)):
+                ref_bad = [i for i, k in enumerate(c["first_nonfinite_step"])
+                           if k is not None and k <= steps + 1]
+                out["members_nonfinite_reference"] = len(ref_bad)
+                out["nonfinite_same_members_as_reference"] = sorted(ref_bad) == sorted(bad)
+        keep = [m for i, m in enumerate(specs) if i not in set(bad)]
+        out["members_timed"] = len(keep)
         t0 = time.perf_counter()
-        ens = Ensemble([riemann_member_kwargs(m) for m in specs], graph_steps=4, branches=8,
+        ens = Ensemble([riemann_member_kwargs(m) for m in keep], graph_steps=4, branches=8,
                        tune=True)
         setup = time.perf_counter() - t0
         dof = sum(ens.dof)
         ens.run(steps=1, finish=False)                      # builds the graph once
+        saved = [(s_.dm.device_value("U_sp").clone(), s_.time, s_.n_step, s_.dm.dt)
+                 for s_ in ens.sims]
+
+        def restore():
+            for s_, (U, t, k, dt) in zip(ens.sims, saved):
+                s_.dm.device_value("U_sp").copy_(U)
+                s_.time, s_.n_step, s_.dm.dt = t, k, dt
+                s_._dt_from_step = True
+
+        def sequential():
+            for s_ in ens.sims:
+                s_.advance(steps)
+
         tb = timed(lambda: ens.run(steps=steps, finish=False))
-        ts = timed(lambda: ens.run_sequential(steps=steps))
-    import numpy as np
-    bad = sum(1 for s_ in ens.sims
-              if not bool(torch.isfinite(s_.dm.device_value("U_sp")).all()))
-    out = {"members": n, "sizes": list(sizes), "orders": [2, 3, 4, 5], "seed": seed,
-           "steps_per_member": steps, "dof_total": dof,
-           "batched_DOF_updates_per_s": dof * steps / tb,
-           "sequential_DOF_updates_per_s": dof * steps / ts,
-           "speedup_vs_sequential": ts / tb, "graph_builds": ens.graph_builds,
-           "graph_build_s": ens.build_seconds, "setup_and_tune_s": setup,
-           "members_nonfinite": bad}
+        restore()
+        # the recorded wave speeds belong to the batched leg's last states:
+        # take each member's first dt from its saved state instead
+        for s_ in ens.sims:
+            s_._dt_from_step = False
+            s_.dm.set_device_value("W_cv", s_._pointwise(
+                "spd_cons_to_prims", s_._tensor_op("spd_sp_to_cv", s_.dm.device_value("U_sp"))))
+        ts = timed(sequential)
+    out.update({"dof_total": dof,
+                "batched_DOF_updates_per_s": dof * steps / tb,
+                "sequential_DOF_updates_per_s": dof * steps / ts,
+                "speedup_vs_sequential": ts / tb, "graph_builds": ens.graph_builds,
+                "graph_build_s": ens.build_seconds, "setup_and_tune_s": setup})
     del ens
     torch.cuda.empty_cache()
     return out
@@ -433,6 +499,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=20)
     ap.add_argument("--adjoint-k", type=int, default=5)
     ap.add_argument("--adjoint-fd", type=int, default=2)
+    ap.add_argument("--t1-window", action="store_true",
+                    help="also run the FP64 form to t = 1.0 (about 23k steps, 5 minutes) and "
+                         "time the same number of steps there")
     args = ap.parse_args()
 
     if args.cpu_baseline_child:
@@ -470,29 +539,30 @@ def main():
         s.perturb_initial_condition(noise)
         s.advance(args.warmup)
         early = s.dm.device_value("U_sp").clone()
-        # the timed steps: the device-side loop (dt, step and time on the
-        # device), captured in one HIP graph beforehand (capture runs nothing)
+        # the timed steps: the host-driven loop of the reference (dt read back
+        # to the host every step, kernels launched one by one)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.advance(args.steps)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        timed_end = s.dm.device_value("U_sp").clone()
+        s.profile_begin(max_launches=200 * max(args.profile_steps, 1))
+        s.advance(args.profile_steps)
+        prof = s.profile_end()
+        # the same number of steps with the device-side loop (dt, step and
+        # time on the device) replayed from one HIP graph captured beforehand
         s.prepare_graph(args.steps)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         s.advance_device(args.steps)
         torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        s.profile_begin(max_launches=200 * max(args.profile_steps, 1))
-        s.advance(args.profile_steps)
-        prof = s.profile_end()
-        # the same number of steps with the host-driven loop of the reference
-        # (dt read back to the host every step, kernels launched one by one)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        s.advance(args.steps)
-        torch.cuda.synchronize()
-        host_loop = time.perf_counter() - t0
+        device_loop = time.perf_counter() - t0
         frac = s.troubled_fraction_next_step()
-        return s, elapsed, prof, frac, early, host_loop
+        return s, elapsed, prof, frac, early, device_loop, timed_end
 
-    s, elapsed, prof, frac, early64, host64 = run_form("fp64")
-    s32, elapsed32, prof32, frac32, early32, host32 = run_form("fp32")
+    s, elapsed, prof, frac, early64, dloop64, end64 = run_form("fp64")
+    s32, elapsed32, prof32, frac32, early32, dloop32, end32 = run_form("fp32")
 
     # the two forms after the same number of steps from the same state
     def l1_rel(r32, r64):
@@ -503,7 +573,8 @@ def main():
     torch.cuda.synchronize()
     l1 = l1_rel(s32.dm.device_value("U_sp"), s.dm.device_value("U_sp"))
     l1_early = l1_rel(early32, early64)
-    del s32, early32, early64
+    l1_timed = l1_rel(end32, end64)
+    del s32, early32, early64, end32, end64
 
     ms_step = elapsed / args.steps * 1e3
     value = dof * args.steps / elapsed
@@ -551,6 +622,8 @@ def main():
                      "alg_bytes_per_launch": bpl[dom],
                      "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, HERE)},
         "step_hbm_roofline_frac": value * 64.0 / (HBM_PEAK_GBPS * 1e9),
+        "step_traffic": step_traffic_summary(measured_step_traffic(prof, args.profile_steps),
+                                             ms_step, dof * 64.0),
         "compute_roofline": {
             "bound": "fp64 valu", "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "kernel": dom,
@@ -561,22 +634,46 @@ def main():
             "step_flop_per_dof": step_flops / dof if step_flops else None,
             "flops_source": os.path.relpath(FLOPS_SUMMARY, HERE)},
         "copy_GBps": copy_gbps,
-        "host_loop_ms_per_step": host64 / args.steps * 1e3,
+        "device_loop_ms_per_step": dloop64 / args.steps * 1e3,
         "troubled_fraction": frac,
         "kernels": kernels,
         "fp32_bulk": {
             "value": value32, "unit": "DOF-updates/s", "ms_per_step": elapsed32 / args.steps * 1e3,
-            "host_loop_ms_per_step": host32 / args.steps * 1e3,
+            "device_loop_ms_per_step": dloop32 / args.steps * 1e3,
             "speedup_vs_fp64": value32 / value,
             "step_hbm_roofline_frac": value32 * 64.0 / (HBM_PEAK_GBPS * 1e9),
             "l1_rel_vs_fp64": l1,
             "steps_compared": args.warmup + 2 * args.steps + args.profile_steps + 1,
             "l1_rel_vs_fp64_after_warmup": l1_early, "steps_compared_after_warmup": args.warmup,
+            "l1_rel_vs_fp64_after_timed": l1_timed,
+            "steps_compared_after_timed": args.warmup + args.steps,
             "troubled_fraction": frac32,
             "kernels_ms_per_step": {k: v[0] / args.profile_steps
                                     for k, v in sorted(prof32.items(), key=lambda kv: -kv[1][0])},
         },
     }
+    if args.t1_window:
+        # the device loop with the clamp to t_end, in chunks, with progress
+        s.init_sim()
+        t_start = time.perf_counter()
+        while s.time < 1.0:
+            s._run_on_device(1.0, 2000, 1)
+            print(f"t1 window: t = {s.time:.4f} after {s.n_step} steps, "
+                  f"{time.perf_counter() - t_start:.0f} s", file=sys.stderr, flush=True)
+        t1, n1 = s.time, s.n_step
+        s.prepare_graph(args.steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.advance_device(args.steps)
+        torch.cuda.synchronize()
+        el1 = time.perf_counter() - t0
+        U = s.dm.device_value("U_sp")
+        out["t1_window"] = {
+            "t_start": t1, "step_start": n1,
+            "value": dof * args.steps / el1, "ms_per_step": el1 / args.steps * 1e3,
+            "troubled_fraction": s.troubled_fraction_next_step(),
+            "finite": bool(torch.isfinite(U).all()),
+            "rho_min": float(U[0].min()), "rho_max": float(U[0].max())}
     if not args.no_ensemble:
         del s
         torch.cuda.empty_cache()
