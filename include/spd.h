@@ -39,7 +39,20 @@ enum spd_status {
 
 enum spd_riemann { SPD_RS_LLF = 0, SPD_RS_HLLC = 1, SPD_RS_LHLLC = 2 };
 enum spd_limiter { SPD_LIMITER_MINMOD = 0, SPD_LIMITER_MONCEN = 1 };
-enum spd_bc { SPD_BC_PERIODIC = 0, SPD_BC_GRADFREE = 1, SPD_BC_REFLECTIVE = 2 };
+/* Boundary kinds (src/sd_boundary.py:32-58, src/fv_simulator.py:224-253):
+ * "ic" and "pressure" sides keep the values stored at construction (the
+ * element traces of the initial state and the initial condition's ghost
+ * layers, spd_set_boundary_values); a pressure side also replaces the boundary
+ * element's own trace with the stored one.  A pressure side opposite a
+ * periodic side is rejected (the reference's result there depends on the
+ * order in which it stores the two sides). */
+enum spd_bc {
+  SPD_BC_PERIODIC = 0,
+  SPD_BC_GRADFREE = 1,
+  SPD_BC_REFLECTIVE = 2,
+  SPD_BC_IC = 3,
+  SPD_BC_PRESSURE = 4
+};
 enum spd_update { SPD_UPDATE_SD = 0, SPD_UPDATE_FV = 1 };
 enum spd_nad { SPD_NAD_RELATIVE = 0, SPD_NAD_DELTA = 1 };
 /* Arithmetic of the step.  FP64: the reference's arithmetic (parity form).
@@ -52,8 +65,7 @@ enum spd_precision { SPD_PRECISION_FP64 = 0, SPD_PRECISION_FP32 = 1 };
 /* Constructor keywords of SDADER_Simulator (src/sdader_simulator.py:15-27,
  * src/sd_simulator.py:20-26, src/fv_simulator.py:11-17, src/simulator.py:10-36)
  * as a plain struct.  Unsupported reference options (viscosity, potential,
- * well-balancing, isothermal, the "ic", "eq" and "pressure" sides,
- * predictor=False) have no field;
+ * well-balancing, isothermal, the "eq" side, predictor=False) have no field;
  * the Python facade rejects them.  The slope limiters (minmod, moncen) are
  * `slope_limiter`; the Riemann solvers (llf, hllc, lhllc) are `riemann_sd` /
  * `riemann_fv`. */
@@ -149,6 +161,20 @@ int spd_compute_dt_static(const spd_config* cfg, const double* W_cv, double* dt_
  * unchanged.  An extension with no reference counterpart (the reference
  * recomputes dt from dm.W_cv, src/sd_simulator.py:214-227). */
 int spd_swap_wave_max(spd_ctx* ctx, double* slot, void* stream);
+
+/* The stored boundary values of "ic" / "pressure" sides, set once before the
+ * first step (init_sd_Boundaries / init_fv_Boundaries,
+ * src/sdader_simulator.py:415-423, src/fv_simulator.py:213-222): the library
+ * takes the element traces of U_sp (the initial state, element layout) at
+ * the domain's first and last flux points of every dimension, and copies
+ * W_ghosted, the primitive subcell means of the initial condition on the
+ * ghosted FV grid (V, Ns_z+4, Ns_y+4, Ns_x+4) (x fastest; two ghost layers per
+ * side with the corners, from the reference's element mesh with one ghost
+ * element per side; only ghost cells are read).  spd_step and spd_run_steps
+ * refuse a context with such a side until this is called; the adjoint does
+ * not support them. */
+int spd_set_boundary_values(spd_ctx* ctx, const double* U_sp, const double* W_ghosted,
+                            void* stream);
 
 /* One step: ADER predictor + SD or FV(/FB) corrector
  * (SDADER_Simulator.perform_update, src/sdader_simulator.py:361-378).

@@ -301,7 +301,9 @@ def _contract(A, M, axis):
 # The simulator restatement
 # ---------------------------------------------------------------------------
 
-BC_KINDS = ("periodic", "gradfree", "reflective")
+BC_KINDS = ("periodic", "gradfree", "reflective", "ic", "pressure")
+# sides whose boundary values are the ones stored at construction
+BC_STORED = ("ic", "pressure")
 
 
 class OracleSimulator:
@@ -343,6 +345,11 @@ class OracleSimulator:
             for side in self.BC[a]:
                 if side not in BC_KINDS:
                     raise NotImplementedError(side)
+            if "pressure" in self.BC[a] and "periodic" in self.BC[a]:
+                # the reference's store_BC then reads the periodic trace from
+                # an element trace the pressure side has just overwritten (or
+                # not), depending on the order of the two sides
+                raise NotImplementedError("a pressure side opposite a periodic side")
         self.lim = [tuple(map(float, (xlim, ylim, zlim)[a])) for a in range(nd)]
         self.len = [self.lim[a][1] - self.lim[a][0] for a in range(nd)]
         self.h = [self.len[a] / self.N[a] for a in range(nd)]
@@ -441,19 +448,64 @@ class OracleSimulator:
         (``src/sd_simulator.py:59-90``, ``src/polynomials.py:124-148``, Q11)."""
         nd = self.ndim
         g = self.Nghe
+        stored = any(k in BC_STORED for a in range(nd) for k in self.BC[a])
+        W = self._quadrature_means(init_fct, ghosts=False)
+        if stored:
+            # the boundary values of "ic" / "pressure" sides, taken from the
+            # initial condition before any perturbation
+            # (src/sdader_simulator.py:50-55,415-423, src/fv_simulator.py:213-222)
+            Wg = self.to_fv(self._quadrature_means(init_fct, ghosts=True))
+            n, ngh = self.n, self.ngh
+            if n > 2:
+                Wg = Wg[(slice(None),) + (slice(n - ngh, -(n - ngh)),) * nd]
+            self.W_gh_fv = Wg
+            self.bc_sd = self._stored_traces(self.cons(self.cv_to_sp(W)))
+        if noise is not None:
+            W = W + noise
+        self.W_init_cv = W.copy()
+        self.W_cv = W.copy()
+        W_sp = self.cv_to_sp(self.W_cv)
+        self.U_sp = self.cons(W_sp)
+        self.U_cv = self.cons(self.W_cv)
+
+    def _stored_traces(self, U_sp):
+        """Element traces of U_sp at the domain's first and last flux points of
+        every dimension, with a time axis of length 1 (``init_sd_Boundaries``,
+        ``src/sdader_simulator.py:415-423``)."""
+        out = []
+        M = U_sp[:, None]
+        for a in range(self.ndim):
+            la = self._local_axis(a, 2)
+            ea = self._elem_axis(a, 2)
+            Mfp = _contract(M, self.ops.sp_to_fp, la)
+            left = np.take(np.take(Mfp, 0, axis=la), [0], axis=ea)
+            right = np.take(np.take(Mfp, self.nf - 1, axis=la), [self.N[a] - 1], axis=ea)
+            out.append((left, right))
+        return out
+
+    def _quadrature_means(self, init_fct, ghosts):
+        """Subcell means of the primitive IC with 2-point GL per dimension on
+        the active elements, or with ``ghosts`` on the element grid with Nghe
+        ghost elements per side (``src/sd_simulator.py:59-90``,
+        ``src/polynomials.py:124-148``)."""
+        nd = self.ndim
+        g = self.Nghe
         xq, wq = gl_nodes(2)
         # per-dimension subcell lower bounds and widths on the element grid
         lows, wids = [], []
+        NE = []
         for a in range(nd):
             Ng = self.N[a] + 2 * g
             length = self.len[a] + 2 * g * self.h[a]
             mesh = self.lim[a][0] + (np.arange(Ng)[:, None] + self.ops.fp[None, :]) \
                 * length / Ng - self.h[a]
-            mesh = mesh[g:-g]                       # active elements
+            if not ghosts:
+                mesh = mesh[g:-g]                   # active elements
+            NE.append(mesh.shape[0])
             lows.append(mesh[:, :-1])
             wids.append(np.diff(mesh, axis=1))
         # coordinates at quadrature points: shape (nd, E..., L..., Q...)
-        shapeE = [self.N[a] for a in range(nd)][::-1]
+        shapeE = [NE[a] for a in range(nd)][::-1]
         full_shape = shapeE + [self.n] * nd + [2] * nd
         pts = np.empty([nd] + full_shape)
         for a in range(nd):
@@ -461,7 +513,7 @@ class OracleSimulator:
             lo = lows[a]
             wd = wids[a]
             shp = [1] * (3 * nd)
-            shp[k] = self.N[a]
+            shp[k] = NE[a]
             shp[nd + k] = self.n
             qs = [1] * (3 * nd)
             qs[2 * nd + k] = 2
@@ -476,13 +528,7 @@ class OracleSimulator:
                 args += [wq, [2 * nd + k]]
             args.append(list(range(2 * nd)))
             W[var] = np.einsum(*args)
-        if noise is not None:
-            W = W + noise
-        self.W_init_cv = W.copy()
-        self.W_cv = W.copy()
-        W_sp = self.cv_to_sp(self.W_cv)
-        self.U_sp = self.cons(W_sp)
-        self.U_cv = self.cons(self.W_cv)
+        return W
 
     # -- dt -------------------------------------------------------------------
     def compute_dt(self):
@@ -521,13 +567,31 @@ class OracleSimulator:
             eal = ea                     # element axis is unchanged by the take (la > ea)
             Na = self.N[a]
             bl, br = self.BC[a]
-            if bl == "periodic":
+            shp = list(left.shape)
+            shp[eal] = 1
+            if bl in BC_STORED:
+                # ic: the stored trace outside; pressure: also inside, where
+                # it overwrites the element's own trace (src/sd_boundary.py:50-56)
+                gL = np.broadcast_to(self.bc_sd[a][0], shp)
+                if bl == "pressure":
+                    left = left.copy()
+                    idx = [slice(None)] * left.ndim
+                    idx[eal] = slice(0, 1)
+                    left[tuple(idx)] = gL
+            elif bl == "periodic":
                 gL = np.take(right, [Na - 1], axis=eal)
             else:
                 gL = np.take(left, [0], axis=eal).copy()
                 if bl == "reflective":
                     gL[a + 1] *= -1
-            if br == "periodic":
+            if br in BC_STORED:
+                gR = np.broadcast_to(self.bc_sd[a][1], shp)
+                if br == "pressure":
+                    right = right.copy()
+                    idx = [slice(None)] * right.ndim
+                    idx[eal] = slice(Na - 1, Na)
+                    right[tuple(idx)] = gR
+            elif br == "periodic":
                 gR = np.take(left, [0], axis=eal)
             else:
                 gR = np.take(right, [Na - 1], axis=eal).copy()
@@ -579,6 +643,13 @@ class OracleSimulator:
         for a in range(nd):
             shp = [2, self.V] + [(Ns[b] + 2 * g if b != a else g) for b in range(nd)][::-1]
             self.BC_fv.append(np.zeros(shp))
+            if hasattr(self, "W_gh_fv"):
+                # init_fv_Boundaries (src/fv_simulator.py:213-222): the ghost
+                # layers of the initial condition; "ic" / "pressure" sides
+                # keep them
+                ax = 1 + nd - 1 - a
+                self.BC_fv[a][0] = self.W_gh_fv[_sl(1 + nd, ax, slice(None, g))]
+                self.BC_fv[a][1] = self.W_gh_fv[_sl(1 + nd, ax, slice(-g, None))]
 
     def _active(self):
         g = self.ngh
@@ -598,6 +669,8 @@ class OracleSimulator:
             store = self.BC_fv[a]
             for side in (0, 1):
                 kind = self.BC[a][side]
+                if kind in BC_STORED:
+                    continue                      # src/fv_simulator.py:247-250
                 if kind == "periodic":
                     store[side] = Mfv[inner[side]]
                 elif all_sides:

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libspd_hip.so")
 
 RS = {"llf": 0, "hllc": 1, "lhllc": 2}
 LIMITER = {"minmod": 0, "moncen": 1}
-BC = {"periodic": 0, "gradfree": 1, "reflective": 2}
+BC = {"periodic": 0, "gradfree": 1, "reflective": 2, "ic": 3, "pressure": 4}
 UPDATE = {"SD": 0, "FV": 1}
 PRECISION = {"fp64": 0, "fp32": 1}
 TAP_ALL = -2
@@ -31,7 +31,7 @@ EXPORTS = [
     "spd_run_begin", "spd_run_steps", "spd_run_status", "spd_set_launch", "spd_get_launch",
     "spd_ic_points", "spd_ic_mean", "spd_sp_to_regular",
     "spd_adj_step", "spd_adj_backward", "spd_adj_theta_count", "spd_swap_wave_max",
-    "spd_compute_dt_static",
+    "spd_compute_dt_static", "spd_set_boundary_values",
 ]
 
 
@@ -98,6 +98,8 @@ def load():
     L.spd_compute_dt.restype = i
     L.spd_compute_dt_static.argtypes = [ctypes.POINTER(SpdConfig), dp, dp, vp]
     L.spd_compute_dt_static.restype = i
+    L.spd_set_boundary_values.argtypes = [vp, dp, dp, vp]
+    L.spd_set_boundary_values.restype = i
     L.spd_swap_wave_max.argtypes = [vp, dp, vp]
     L.spd_swap_wave_max.restype = i
     L.spd_step.argtypes = [vp, dp, dp, dp, dp, i, vp]

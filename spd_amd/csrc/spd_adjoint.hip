@@ -1274,6 +1274,8 @@ const AdjEntry kAdj[] = {
 
 int adj_check(const spd_ctx* c) {
   if (!c) return fail(SPD_E_INVALID, "null context");
+  if (c->g.stored)
+    return fail(SPD_E_UNSUPPORTED, "the adjoint is not built for \"ic\" / \"pressure\" sides");
   return SPD_OK;
 }
 

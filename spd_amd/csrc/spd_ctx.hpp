@@ -91,6 +91,7 @@ struct spd_ctx {
   double* partial = nullptr;  // per-block partial maxima
   long long npartial = 0;
   bool need_gate = false;     // some side is not periodic (global NAD gate needed)
+  bool bc_ready = false;      // the stored boundary values of ic / pressure sides are set
   int launch[SPD_LAUNCH_COUNT] = {0, 0, 0};  // tuned launch shapes (0 = default)
   // debug tap
   int dbg_sub = -1;

@@ -15,7 +15,9 @@
 namespace spd {
 
 enum : int { RS_LLF = 0, RS_HLLC = 1 };
-enum : int { BC_PERIODIC = 0, BC_GRADFREE = 1, BC_REFLECTIVE = 2 };
+enum : int { BC_PERIODIC = 0, BC_GRADFREE = 1, BC_REFLECTIVE = 2, BC_IC = 3, BC_PRESSURE = 4 };
+// Sides whose boundary values are stored at construction ("ic", "pressure").
+__host__ __device__ __forceinline__ bool bc_stored(int kind) { return kind >= BC_IC; }
 
 constexpr int NGH = 2;  // finite-volume ghost layers (Nghc in the reference)
 
@@ -273,6 +275,12 @@ struct Geom {
   const double* rdc[3];      // 1 / (centers[k+1] - centers[k-1]), k in [1, Ns+2]
   double rh[3];              // 1 / h
   const int* skip;  // device flag: when non-zero every step kernel returns at once
+  // stored boundary values of "ic" / "pressure" sides (null otherwise):
+  // element traces per dimension and side, (V, points across the side), and
+  // the initial primitives on the ghosted FV grid, (V, Ngc)
+  const double* bct[3][2];
+  const double* wgh;
+  int stored;     // some side keeps stored values
   long long NE;   // number of elements
   long long Nc;   // number of active subcells
   long long Ngc;  // number of ghosted subcells
@@ -338,6 +346,78 @@ __device__ __forceinline__ long long cell_index(const Geom& g, const int* c) {
   return idx;
 }
 
+// Solution point across dimension a of the stored element traces: the
+// transverse coordinates (element ce[b], line cl[b]) of the other
+// dimensions, higher dimensions slowest (the face grids' order).
+template <int NDIM>
+__device__ __forceinline__ long long bc_point(const Geom& g, int a, int n, const int* ce,
+                                              const int* cl) {
+  long long t = 0;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b)
+    if (b != a) t = t * g.Ns[b] + (long long)ce[b] * n + cl[b];
+  return t;
+}
+
+// Element-face traces at a domain face of dimension a whose side (0: the
+// first face, 1: the last) keeps stored values: the outer state becomes the
+// trace stored at construction, and a "pressure" side also overwrites the
+// boundary element's own trace with it (src/sd_boundary.py:50-56).
+template <int V, typename B>
+__device__ __forceinline__ void bc_stored_traces(const Geom& g, int a, int side, long long tp,
+                                                 B* outer, B* inner) {
+  const double* st = g.bct[a][side];
+  const long long P = g.Nc / g.Ns[a];
+#pragma unroll
+  for (int v = 0; v < V; ++v) outer[v] = (B)st[v * P + tp];
+  if (g.bc[a][side] == BC_PRESSURE) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) inner[v] = outer[v];
+  }
+}
+
+// Ghosted-grid index of coordinate c (each in [-NGH, Ns + NGH)).
+template <int NDIM>
+__device__ __forceinline__ long long ghosted_index(const Geom& g, const int* c) {
+  long long idx = 0;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b) idx = idx * g.Ng[b] + c[b] + NGH;
+  return idx;
+}
+
+// FV ghost value at a ghosted coordinate c that the stored ghost layers of
+// an "ic" / "pressure" side provide (the reference never refreshes them,
+// src/fv_simulator.py:213-253): the ghost fill runs x first, so the
+// dimensions are resolved from the last: a ghost coordinate of a side that
+// copies active cells is mapped (periodic wrap, non-mirrored copy, reflective
+// flip), and the first stored side met takes the value of its layers at the
+// coordinates reached, the lower dimensions unmapped.  Returns false (W
+// untouched) when no stored side is met.
+template <int NDIM>
+__device__ __forceinline__ bool stored_ghost(const Geom& g, const int* c, double* W) {
+  if (!g.stored) return false;
+  constexpr int V = NDIM + 2;
+  int m[3] = {c[0], c[1], c[2]};
+  bool flip[3] = {false, false, false};
+#pragma unroll
+  for (int a = NDIM - 1; a >= 0; --a) {
+    if (c[a] < 0 || c[a] >= g.Ns[a]) {
+      const int side = (c[a] < 0) ? 0 : 1;
+      if (bc_stored(g.bc[a][side])) {
+        const long long idx = ghosted_index<NDIM>(g, m);
+#pragma unroll
+        for (int v = 0; v < V; ++v) W[v] = g.wgh[v * g.Ngc + idx];
+#pragma unroll
+        for (int b = a + 1; b < NDIM; ++b)
+          if (flip[b]) W[b + 1] = -W[b + 1];
+        return true;
+      }
+      m[a] = ghost_map(c[a], g.Ns[a], g.bc[a], &flip[a]);
+    }
+  }
+  return false;
+}
+
 // Primitive state of the ghost-filled array at a ghosted coordinate `c`
 // (active range [0, Ns) per dimension; ghosts in [-NGH, 0) and [Ns, Ns+NGH)).
 // The value is the one left by the fill x first, then y, then z: every ghost
@@ -347,6 +427,7 @@ template <int NDIM>
 __device__ __forceinline__ void load_W(const double* __restrict__ U, const Geom& g, const int* c,
                                        double* W, double gamma) {
   constexpr int V = Dims<NDIM>::V;
+  if (stored_ghost<NDIM>(g, c, W)) return;
   int m[3];
   bool flip[3] = {false, false, false};
 #pragma unroll
@@ -366,6 +447,7 @@ template <int NDIM>
 __device__ __forceinline__ void load_prim(const double* __restrict__ Wp, const Geom& g,
                                           const int* c, double* W) {
   constexpr int V = Dims<NDIM>::V;
+  if (stored_ghost<NDIM>(g, c, W)) return;
   int m[3];
   bool flip[3] = {false, false, false};
 #pragma unroll

@@ -132,6 +132,8 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
       if (ok[j]) {
         double W[4];
         prims<2>(u[j], W, 0, prm.gamma);
+        int c[3] = {x0 + k % DNX - DH, y0 + k / DNX - DH, 0};
+        stored_ghost<2>(g, c, W);  // ghosts of "ic" / "pressure" sides
         r = W[0];
         p = W[3];
       }
@@ -157,6 +159,8 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
         prims<2>(u[j], W, 0, prm.gamma);
         if (fl[j][0]) W[1] = -W[1];
         if (fl[j][1]) W[2] = -W[2];
+        int c[3] = {x0 + k % ONX - 1, y0 + k / ONX - 1, 0};
+        stored_ghost<2>(g, c, W);  // ghosts of "ic" / "pressure" sides
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) wo[v][k / ONX][k % ONX] = W[v];
@@ -644,7 +648,7 @@ __device__ __forceinline__ B from_left(B v) {  // value of lane - 1 (lane 0: its
   return __shfl_up(v, 1, MW);
 }
 
-template <int RS, typename B>
+template <int RS, typename B, bool STORED>
 __global__ void __launch_bounds__(MW * MWAVES)
 k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
             const double* __restrict__ theta, const B* __restrict__ Fx,
@@ -688,12 +692,16 @@ k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
       for (int v = 0; v < 4; ++v) un[v] = Ucv[v * g.Nc + idx];
     }
   };
-  auto to_W = [&](B* W) {
+  auto to_W = [&](B* W, int y) {
     double Wv[4] = {qnan(), qnan(), qnan(), qnan()};
     if (okn) {
       prims<2>(un, Wv, 0, prm.gamma);
       if (flxn) Wv[1] = -Wv[1];
       if (flyn) Wv[2] = -Wv[2];
+      if constexpr (STORED) {
+        int c[3] = {x, y, 0};
+        stored_ghost<2>(g, c, Wv);  // ghosts of "ic" / "pressure" sides
+      }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) W[v] = (B)Wv[v];
@@ -735,9 +743,9 @@ k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
     constexpr bool SKIP = decltype(skip)::value;
     B Wm[4], W0[4], Wp[4];
     fetch(y0 - 2);
-    to_W(Wm);
+    to_W(Wm, y0 - 2);
     fetch(y0 - 1);
-    to_W(W0);
+    to_W(W0, y0 - 1);
     fetch(y0);
     double th = theta_at(y0 - 1), thn = theta_at(y0);
     double thm = 0.0;                  // theta of row k - 1
@@ -758,15 +766,15 @@ k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
       }
       if (SKIP && !primed) {
         fetch(k - 1);
-        to_W(Wm);
+        to_W(Wm, k - 1);
         fetch(k);
-        to_W(W0);
+        to_W(W0, k);
         fetch(k + 1);
         th = theta_at(k);
         thn = theta_at(k + 1);
         primed = true;
       }
-      to_W(Wp);  // row k + 1
+      to_W(Wp, k + 1);  // row k + 1
       fetch(k + 2);
       const double thk = th;
       th = thn;

@@ -220,7 +220,11 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
     B F[nf][V];
     if (active) {
       B qa[V];
-      if (ca == 0 && g.bc[a][0] != BC_PERIODIC) {
+      // transverse solution point of the thread's line (for stored sides)
+      const long long tp = (long long)((a == 0) ? gy : gx) * n + lp;
+      if (ca == 0 && bc_stored(g.bc[a][0])) {
+        bc_stored_traces<V, B>(g, a, 0, tp, qa, qL);
+      } else if (ca == 0 && g.bc[a][0] != BC_PERIODIC) {
 #pragma unroll
         for (int v = 0; v < V; ++v) qa[v] = qL[v];
         if (g.bc[a][0] == BC_REFLECTIVE) qa[a + 1] = -qa[a + 1];
@@ -233,7 +237,9 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
       solve(qa, qL, a, F[0]);
       if (ca == Na - 1 || la == TA - 1) {
         B qb[V];
-        if (ca == Na - 1 && g.bc[a][1] != BC_PERIODIC) {
+        if (ca == Na - 1 && bc_stored(g.bc[a][1])) {
+          bc_stored_traces<V, B>(g, a, 1, tp, qb, qR);
+        } else if (ca == Na - 1 && g.bc[a][1] != BC_PERIODIC) {
 #pragma unroll
           for (int v = 0; v < V; ++v) qb[v] = qR[v];
           if (g.bc[a][1] == BC_REFLECTIVE) qb[a + 1] = -qb[a + 1];

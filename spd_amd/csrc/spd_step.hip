@@ -125,6 +125,8 @@ k_face_sd_x(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsr
       const int k = g.bc[0][0];
       if (k == BC_PERIODIC) {
         trace(Na - 1, nf - 1, qL);
+      } else if (bc_stored(k)) {
+        bc_stored_traces<V, B>(g, 0, 0, bc_point<NDIM>(g, 0, n, ce, cl), qL, qR);
       } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) qL[v] = qR[v];
@@ -135,6 +137,8 @@ k_face_sd_x(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsr
       const int k = g.bc[0][1];
       if (k == BC_PERIODIC) {
         trace(0, 0, qR);
+      } else if (bc_stored(k)) {
+        bc_stored_traces<V, B>(g, 0, 1, bc_point<NDIM>(g, 0, n, ce, cl), qR, qL);
       } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) qR[v] = qL[v];
@@ -249,6 +253,8 @@ k_face_sd(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
     const int k = g.bc[a][0];
     if (k == BC_PERIODIC) {
       trace(Na - 1, nf - 1, qL);
+    } else if (bc_stored(k)) {
+      bc_stored_traces<V, B>(g, a, 0, bc_point<NDIM>(g, a, n, ce, cl), qL, qR);
     } else {
 #pragma unroll
       for (int v = 0; v < V; ++v) qL[v] = qR[v];
@@ -259,6 +265,8 @@ k_face_sd(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
     const int k = g.bc[a][1];
     if (k == BC_PERIODIC) {
       trace(0, 0, qR);
+    } else if (bc_stored(k)) {
+      bc_stored_traces<V, B>(g, a, 1, bc_point<NDIM>(g, a, n, ce, cl), qR, qL);
     } else {
 #pragma unroll
       for (int v = 0; v < V; ++v) qR[v] = qL[v];
@@ -384,6 +392,8 @@ k_face_sd2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __re
       const int b = g.bc[0][0];
       if (b == BC_PERIODIC) {
         gtrace(Nx - 1, ey, nf - 1, 0, k, qL);
+      } else if (bc_stored(b)) {
+        bc_stored_traces<V, B>(g, 0, 0, (long long)ey * n + k, qL, qR);
       } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) qL[v] = qR[v];
@@ -394,6 +404,8 @@ k_face_sd2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __re
       const int b = g.bc[0][1];
       if (b == BC_PERIODIC) {
         gtrace(0, ey, 0, 0, k, qR);
+      } else if (bc_stored(b)) {
+        bc_stored_traces<V, B>(g, 0, 1, (long long)ey * n + k, qR, qL);
       } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) qR[v] = qL[v];
@@ -430,6 +442,8 @@ k_face_sd2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __re
       const int b = g.bc[1][0];
       if (b == BC_PERIODIC) {
         gtrace(ex, Ny - 1, nf - 1, 1, k, qL);
+      } else if (bc_stored(b)) {
+        bc_stored_traces<V, B>(g, 1, 0, (long long)ex * n + k, qL, qR);
       } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) qL[v] = qR[v];
@@ -440,6 +454,8 @@ k_face_sd2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __re
       const int b = g.bc[1][1];
       if (b == BC_PERIODIC) {
         gtrace(ex, 0, 0, 1, k, qR);
+      } else if (bc_stored(b)) {
+        bc_stored_traces<V, B>(g, 1, 1, (long long)ex * n + k, qR, qL);
       } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) qR[v] = qL[v];
@@ -1376,6 +1392,11 @@ __device__ __forceinline__ double sed_alpha(const double* __restrict__ rp, const
     if (x >= -NGH && x < Ns + NGH) {
       int cc[3] = {c[0], c[1], c[2]};
       bool fl = false;
+      if ((x < 0 || x >= Ns) && bc_stored(g.bc[a][x < 0 ? 0 : 1])) {
+        cc[a] = x;  // the stored layers' density / pressure
+        w[k2 + 3] = g.wgh[(k == 0 ? 0 : NDIM + 1) * g.Ngc + ghosted_index<NDIM>(g, cc)];
+        continue;
+      }
       cc[a] = ghost_map(x, Ns, g.bc[a], &fl);  // rho and P are even under the flips
       w[k2 + 3] = r[cell_index<NDIM>(g, cc)];
     } else {
@@ -1514,10 +1535,16 @@ __device__ __forceinline__ double trouble_at(const Geom& g, const uint8_t* __res
     const int Ns = g.Ns[a];
     if (m[a] < 0 || m[a] >= Ns) {
       const int side = (m[a] < 0) ? 0 : 1;
+      if (bc_stored(g.bc[a][side]))  // the stored layers' density
+        return g.wgh[ghosted_index<NDIM>(g, m)];
       if (g.bc[a][side] != BC_PERIODIC) {
         m[a] = (side == 0) ? m[a] + NGH : m[a] - NGH;
 #pragma unroll
         for (int b = a - 1; b >= 0; --b) {
+          // the stale layers were copied from a state whose lower-dimension
+          // ghosts a stored side had filled with its own layers
+          if ((m[b] < 0 || m[b] >= g.Ns[b]) && bc_stored(g.bc[b][m[b] < 0 ? 0 : 1]))
+            return g.wgh[ghosted_index<NDIM>(g, m)];
           bool fl = false;
           m[b] = ghost_map(m[b], g.Ns[b], g.bc[b], &fl);
         }
@@ -2113,6 +2140,45 @@ __global__ void k_tap(Geom g, const uint8_t* __restrict__ flags, const double* _
   }
 }
 
+// Element traces of U_sp at the first (side 0) or last (side 1) flux point
+// of dimension a, at every solution point across that side
+// (init_sd_Boundaries, src/sdader_simulator.py:415-423): the stored boundary
+// values of "ic" / "pressure" sides, layout (V, points) in bc_point's order.
+template <int NDIM, int P>
+__global__ void __launch_bounds__(256)
+k_bc_traces(Ops<P> op, Geom g, int a, int side, const double* __restrict__ U,
+            double* __restrict__ out) {
+  constexpr int V = Dims<NDIM>::V, n = P + 1, nf = P + 2;
+  constexpr int NL = IPow<n, NDIM>::v;
+  const long long Pa = g.Nc / g.Ns[a];
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= V * Pa) return;
+  const int v = (int)(tid / Pa);
+  const long long tp = tid % Pa;
+  int ce[3] = {0, 0, 0}, cl[3] = {0, 0, 0};
+  long long x = tp;
+#pragma unroll
+  for (int b = 0; b < NDIM; ++b) {  // the inverse of bc_point: lower dimensions fastest
+    if (b == a) continue;
+    const int q = (int)(x % g.Ns[b]);
+    x /= g.Ns[b];
+    ce[b] = q / n;
+    cl[b] = q % n;
+  }
+  ce[a] = (side == 0) ? 0 : g.N[a] - 1;
+  const long long e = elem_index<NDIM>(g, ce);
+  int lb = 0;
+#pragma unroll
+  for (int b = NDIM - 1; b >= 0; --b) lb = lb * n + ((b == a) ? 0 : cl[b]);
+  const int sa = stride_of<n>(a);
+  const double* u = U + ((long long)v * g.NE + e) * NL + lb;
+  const int fp = (side == 0) ? 0 : nf - 1;
+  double acc = 0.0;
+#pragma unroll
+  for (int s = 0; s < n; ++s) acc += op.s2f[fp][s] * u[s * sa];
+  out[tid] = acc;
+}
+
 }  // namespace spd
 
 // ===========================================================================
@@ -2406,14 +2472,18 @@ template <int NDIM, int P, typename B> struct Stepper {
         const int rows = fv2d::march_rows(c->g.Ns[0], c->g.Ns[1]);
         const dim3 strips((c->g.Ns[0] + fv2d::MOUT - 1) / fv2d::MOUT,
                           (c->g.Ns[1] + rows * fv2d::MWAVES - 1) / (rows * fv2d::MWAVES));
-        if (hllc)
-          hipLaunchKernelGGL((fv2d::k_fbmarch2d<RS_HLLC, B>), strips, dim3(fv2d::MW * fv2d::MWAVES),
-                             0, s, c->g, c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, rows,
-                             c->Unew);
-        else
-          hipLaunchKernelGGL((fv2d::k_fbmarch2d<RS_LLF, B>), strips, dim3(fv2d::MW * fv2d::MWAVES),
-                             0, s, c->g, c->prm, c->Ucv, c->theta, Fh(c, 0), Fh(c, 1), dt, w, rows,
-                             c->Unew);
+        auto go = [&](auto rs, auto st) {
+          hipLaunchKernelGGL((fv2d::k_fbmarch2d<decltype(rs)::value, B, decltype(st)::value>),
+                             strips, dim3(fv2d::MW * fv2d::MWAVES), 0, s, c->g, c->prm, c->Ucv,
+                             c->theta, Fh(c, 0), Fh(c, 1), dt, w, rows, c->Unew);
+        };
+        using HL = std::integral_constant<int, RS_HLLC>;
+        using LF = std::integral_constant<int, RS_LLF>;
+        if (c->g.stored) {
+          if (hllc) go(HL{}, std::true_type{}); else go(LF{}, std::true_type{});
+        } else {
+          if (hllc) go(HL{}, std::false_type{}); else go(LF{}, std::false_type{});
+        }
       }
     });
     std::swap(c->Ucv, c->Unew);
@@ -2590,6 +2660,20 @@ template <int NDIM, int P, typename B> struct Stepper {
     return SPD_OK;
   }
 
+  // the stored element traces of every "ic" / "pressure" side from U_sp
+  static int bc_traces(spd_ctx* c, const double* U_sp, hipStream_t s) {
+    const Ops<P> op = make_ops<P>(*c->ops);
+    for (int a = 0; a < NDIM; ++a)
+      for (int sd = 0; sd < 2; ++sd) {
+        if (!bc_stored(c->g.bc[a][sd])) continue;
+        const long long cnt = (long long)V * (c->g.Nc / c->g.Ns[a]);
+        hipLaunchKernelGGL((k_bc_traces<NDIM, P>), dim3(grid_for(cnt)), dim3(256), 0, s, op, c->g,
+                           a, sd, U_sp, const_cast<double*>(c->g.bct[a][sd]));
+      }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+  }
+
   static int tensor(spd_ctx* c, int to_cv, const double* in, double* out, hipStream_t s) {
     const Ops<P> op = make_ops<P>(*c->ops);
     if constexpr (NDIM <= 2)
@@ -2605,6 +2689,7 @@ template <int NDIM, int P, typename B> struct Stepper {
 
 using StepFn = int (*)(spd_ctx*, double*, double*, double*, const double*, int, hipStream_t);
 using TensorFn = int (*)(spd_ctx*, int, const double*, double*, hipStream_t);
+using TracesFn = int (*)(spd_ctx*, const double*, hipStream_t);
 using LaunchFn = int (*)(const spd_ctx*, int);
 using LaunchMaxFn = int (*)(const spd_ctx*, int);
 
@@ -2614,11 +2699,12 @@ struct Entry {
   TensorFn tensor;
   LaunchFn launch_value;
   LaunchMaxFn launch_max;
+  TracesFn bc_traces;
 };
 
 #define SPD_ENTRY(D, P, PR, B)                                              \
   {D, P, PR, &Stepper<D, P, B>::step, &Stepper<D, P, B>::tensor,            \
-   &Stepper<D, P, B>::launch_value, &Stepper<D, P, B>::launch_max}
+   &Stepper<D, P, B>::launch_value, &Stepper<D, P, B>::launch_max, &Stepper<D, P, B>::bc_traces}
 #define SPD_ENTRIES(D, P) \
   SPD_ENTRY(D, P, SPD_PRECISION_FP64, double), SPD_ENTRY(D, P, SPD_PRECISION_FP32, float)
 const Entry kEntries[] = {
@@ -2741,8 +2827,13 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
     if (cfg->N[a] * (cfg->p + 1) < NGH)
       return fail(SPD_E_INVALID, "fewer subcells than ghost layers");
     for (int s = 0; s < 2; ++s)
-      if (cfg->bc[a][s] < SPD_BC_PERIODIC || cfg->bc[a][s] > SPD_BC_REFLECTIVE)
+      if (cfg->bc[a][s] < SPD_BC_PERIODIC || cfg->bc[a][s] > SPD_BC_PRESSURE)
         return fail(SPD_E_UNSUPPORTED, "boundary kind");
+    if ((cfg->bc[a][0] == SPD_BC_PRESSURE && cfg->bc[a][1] == SPD_BC_PERIODIC) ||
+        (cfg->bc[a][1] == SPD_BC_PRESSURE && cfg->bc[a][0] == SPD_BC_PERIODIC))
+      return fail(SPD_E_UNSUPPORTED,
+                  "a pressure side opposite a periodic side (the reference's result depends "
+                  "on the order in which it stores the two sides)");
     if (!(cfg->lim[a][1] > cfg->lim[a][0])) return fail(SPD_E_INVALID, "empty domain");
   }
   std::unique_ptr<spd_ctx> c(new spd_ctx());
@@ -2884,6 +2975,21 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
       if (nd != 2 && (rc = c->alloc(&c->Wold, (size_t)(V * g.Nc))) != SPD_OK) return rc;
     }
   }
+  // stored boundary values of "ic" / "pressure" sides
+  for (int a = 0; a < nd; ++a)
+    for (int sd = 0; sd < 2; ++sd)
+      if (bc_stored(cfg->bc[a][sd])) c->g.stored = 1;
+  if (c->g.stored) {
+    for (int a = 0; a < nd; ++a)
+      for (int sd = 0; sd < 2; ++sd) {
+        double* q = nullptr;
+        if ((rc = c->alloc(&q, (size_t)(V * (g.Nc / g.Ns[a])))) != SPD_OK) return rc;
+        c->g.bct[a][sd] = q;
+      }
+    double* w = nullptr;
+    if ((rc = c->alloc(&w, (size_t)(V * g.Ngc))) != SPD_OK) return rc;
+    c->g.wgh = w;
+  }
   if ((rc = c->alloc(&c->any_flag, 1)) != SPD_OK) return rc;
   if ((rc = c->alloc(&c->zero_flag, 1)) != SPD_OK) return rc;
   if ((rc = c->alloc(&c->run_skip, 1)) != SPD_OK) return rc;
@@ -3017,9 +3123,31 @@ int spd_swap_wave_max(spd_ctx* ctx, double* slot, void* stream) {
   return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
 }
 
+int spd_set_boundary_values(spd_ctx* ctx, const double* U_sp, const double* W_ghosted,
+                            void* stream) {
+  if (!ctx || !U_sp || !W_ghosted) return fail(SPD_E_INVALID, "null argument");
+  if (!ctx->g.stored)
+    return fail(SPD_E_INVALID, "no side of this context keeps stored values (ic / pressure)");
+  hipStream_t s = (hipStream_t)stream;
+  const int rc = find_entry(ctx->ndim, ctx->p, ctx->cfg.precision)->bc_traces(ctx, U_sp, s);
+  if (rc != SPD_OK) return rc;
+  SPD_HIP_CHECK(hipMemcpyAsync(const_cast<double*>(ctx->g.wgh), W_ghosted,
+                               (size_t)ctx->V * ctx->g.Ngc * sizeof(double),
+                               hipMemcpyDeviceToDevice, s));
+  ctx->bc_ready = true;
+  return SPD_OK;
+}
+
+static int need_boundary_values(const spd_ctx* ctx) {
+  if (ctx->g.stored && !ctx->bc_ready)
+    return fail(SPD_E_INVALID, "an ic / pressure side needs spd_set_boundary_values first");
+  return SPD_OK;
+}
+
 int spd_step(spd_ctx* ctx, double* U_sp, double* U_cv, double* W_cv, const double* dt_dev,
              int snapshot, void* stream) {
   if (!ctx || !U_sp || !dt_dev) return fail(SPD_E_INVALID, "null argument");
+  if (need_boundary_values(ctx) != SPD_OK) return SPD_E_INVALID;
   if (snapshot && ctx->cfg.update == SPD_UPDATE_FV && (!U_cv || !W_cv))
     return fail(SPD_E_INVALID, "snapshot needs U_cv and W_cv");
   const Entry* e = find_entry(ctx->ndim, ctx->p, ctx->cfg.precision);
@@ -3056,6 +3184,7 @@ int spd_run_begin(spd_ctx* ctx, double time, double t_end, long long n_step, lon
 
 int spd_run_steps(spd_ctx* ctx, double* U_sp, int nsteps, void* stream) {
   if (!ctx || !U_sp || nsteps < 0) return fail(SPD_E_INVALID, "invalid argument");
+  if (need_boundary_values(ctx) != SPD_OK) return SPD_E_INVALID;
   hipStream_t s = (hipStream_t)stream;
   const Entry* en = find_entry(ctx->ndim, ctx->p, ctx->cfg.precision);
   for (int k = 0; k < nsteps; ++k) {

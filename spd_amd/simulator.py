@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from . import initial_conditions as _ic
-from .mesh import ic_mesh, solution_points
+from .mesh import ghost_layer_means, ic_mesh, solution_points
 
 HOST, DEVICE = "host", "device"
 
@@ -99,7 +99,7 @@ def _bc_pair(side_pair):
     out = []
     for side in side_pair:
         if side not in _lib.BC:
-            if side in ("ic", "eq", "pressure"):
+            if side == "eq":
                 raise NotImplementedError(f"boundary type {side!r} is not implemented")
             raise ValueError(f"Undetermined boundary type {side!r}")
         out.append(_lib.BC[side])
@@ -113,9 +113,8 @@ class SDADER_Simulator:
     (``src/sdader_simulator.py:15-27``, ``src/sd_simulator.py:20-26``,
     ``src/fv_simulator.py:11-17``, ``src/simulator.py:10-36``).  Options the
     reference offers beyond this path (``viscosity``, ``potential``, ``WB``,
-    ``isothermal``, ``"ic"``/``"eq"``/``"pressure"`` sides,
-    ``predictor=False``, ``m != p``, ``Nghc != 2``) raise
-    ``NotImplementedError``.  ``device`` selects the GPU (default: current).
+    ``isothermal``, ``"eq"`` sides, ``predictor=False``, ``m != p``,
+    ``Nghc != 2``) raise ``NotImplementedError``.  ``device`` selects the GPU (default: current).
 
     ``perform_time_evolution`` and ``perform_iterations`` run the time loop on
     the device when ``device_loop`` is true: dt, the t_end clamp, the dt < 1e-14
@@ -277,6 +276,7 @@ class SDADER_Simulator:
         self.fp = {d: self.x_fp for d in self.dims}
 
         self.post_init()
+        self._store_boundary_values()
         self.compute_dt()
 
     # -- lifetime ---------------------------------------------------------------
@@ -337,6 +337,27 @@ class SDADER_Simulator:
             _lib.check(self._lib.spd_ic_mean(self._ctx, _lib.ptr(vals), _lib.ptr(W[var]),
                                              self._stream), "spd_ic_mean")
         self._set_initial_primitives(W)
+
+    def _store_boundary_values(self):
+        """The values "ic" / "pressure" sides keep, from the initial condition
+        as built at construction (init_sd_Boundaries / init_fv_Boundaries,
+        src/sdader_simulator.py:50-55,415-423, src/fv_simulator.py:213-222):
+        the library forms the element traces of the initial U_sp; the ghost
+        layers of the initial primitives come from ``mesh.ghost_layer_means``.
+        A later perturbation of the initial condition does not change them,
+        as in the reference."""
+        if not any(k in ("ic", "pressure") for d in self.dims for k in self.BC[d]):
+            return
+        dims = list(self.dims)
+        Wg = ghost_layer_means(self.init_fct, [self.N[d] for d in dims], self.p,
+                               [self.lim[d] for d in dims], [self.h[d] for d in dims], self.nvar,
+                               self.Nghe, self.Nghc)
+        Wd = torch.from_numpy(Wg).to(self.device)
+        _lib.check(self._lib.spd_set_boundary_values(self._ctx,
+                                                     _lib.ptr(self.dm.device_value("U_sp")),
+                                                     _lib.ptr(Wd), self._stream),
+                   "spd_set_boundary_values")
+        torch.cuda.synchronize(self.device)
 
     def _set_initial_primitives(self, W_cv):
         if isinstance(W_cv, torch.Tensor):

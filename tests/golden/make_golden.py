@@ -323,6 +323,34 @@ def main():
                                           BC=GF2 + (("gradfree", "gradfree"),), **LM, **MC),
                  {"kind": "step", "kw": {"vx": 0.5, "vy": -0.3, "vz": 0.2}},
                  steps=1, capture_steps=(1,))
+    # Sides whose boundary values are stored at construction: "ic" keeps the
+    # initial trace / ghost layers outside, "pressure" also overwrites the
+    # boundary element's own trace with it (src/sd_boundary.py:50-56,
+    # src/fv_simulator.py:213-253).
+    if want("sine1d_p3_ic"):
+        run_case("sine1d_p3_ic", dict(p=3, N=(12,), BC=(("ic", "ic"),), **HLLC, **FVFB),
+                 {"kind": "sine"}, steps=5, capture_steps=(1, 2), t_end=0.5)
+    if want("sod_p2_pressure_ic"):
+        run_case("sod_p2_pressure_ic", dict(p=2, N=(30,), BC=(("pressure", "ic"),), **HLLC,
+                                            **FVFB),
+                 {"kind": "sod"}, steps=5, capture_steps=(1, 2), t_end=0.4)
+    if want("quad10_p3_ic_pressure"):
+        states = ic.random_four_quadrant_states(np.random.default_rng(99))
+        run_case("quad10_p3_ic_pressure",
+                 dict(p=3, N=(10, 8), BC=(("ic", "pressure"), ("gradfree", "ic")), **HLLC, **FVFB),
+                 {"kind": "quad", "states": states.tolist()}, steps=3, capture_steps=(1, 2, 3))
+    if want("step12_p2_pressure"):
+        run_case("step12_p2_pressure",
+                 dict(p=2, N=(12, 10), BC=(("pressure", "ic"), ("pressure", "reflective")),
+                      riemann_solver_sd="llf", riemann_solver_fv="llf", **FVFB),
+                 {"kind": "step", "kw": {"vx": 1.0, "vy": 0.4, "P": 1.0}},
+                 steps=3, capture_steps=(1, 2, 3))
+    if want("step3d_p2_ic"):
+        run_case("step3d_p2_ic", dict(p=2, N=(5, 4, 4),
+                                      BC=(("ic", "gradfree"), ("gradfree", "pressure"),
+                                          ("ic", "ic")), **HLLC, **FVFB),
+                 {"kind": "step", "kw": {"vx": 0.5, "vy": -0.3, "vz": 0.2}},
+                 steps=2, capture_steps=(1, 2))
     # 3-D Taylor-Green, 4^3, p=3, periodic on [0, 2 pi]^3.
     if want("tg4_p3"):
         twopi = 2 * np.pi

@@ -75,3 +75,14 @@ def test_time_evolution(name):
     assert abs(s.time - float(fx["final_time"])) < 1e-14
     assert relerr_fields(s.U_sp, fx["final_U_sp"]) < TOL
     assert relerr_fields(s.W_cv, fx["final_W_cv"]) < TOL
+
+
+def test_pressure_opposite_periodic_is_rejected():
+    """A pressure side opposite a periodic side: the reference then reads the
+    periodic trace from an element trace the pressure side has just
+    overwritten, or not, depending on the order of the two sides; the oracle
+    (and the library) reject the combination."""
+    from spd_amd import initial_conditions as ic
+    with pytest.raises(NotImplementedError):
+        OracleSimulator(ic.sod_shock_tube, p=2, N=(10,), BC=(("pressure", "periodic"),),
+                        update="FV")
