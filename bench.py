@@ -76,16 +76,23 @@ def algorithmic_bytes(N, p, launches):
                  reads U_sp and, when another follows, writes its increment
                  (V C), which the second reads with U_sp (2 V C) in place of
                  the state (the one-element halo reads are not counted)
-      k_hoflux   T launches (k_hocand2d): read the node's state slice (V C),
-                 its element-face fluxes (2 V (N+1) N n) and U_cv (V C), write
-                 both FV-face arrays (for the fallback) and the candidate (V C)
+      k_hoflux   T launches (k_hocand2d with the first half of the
+                 detection): read the node's state slice (V C), its
+                 element-face fluxes (2 V (N+1) N n) and U_cv (V C), write the
+                 candidate (V C) and C flags (the FV-face arrays that the
+                 column-strip fallback reads are written on the substeps that
+                 take that path only, and are not counted)
       k_tensor   once: read + write V C
-      k_detect   T launches: read U_cv and the candidate (2 V C), write C flags
-      k_theta1   T launches (k_theta2d): read C flags, write G
-      k_fbface   T launches (k_fbmarch2d): read U_cv, theta, both face arrays,
-                 write V C (at most: cells with no blended face keep the
-                 candidate; rows without a blended face are skipped, so the
-                 launch reads less where few cells are troubled)
+      k_detect   T launches (k_sed2d): read C flags (tiles with smooth-extrema
+                 candidates also read the candidate's density and pressure;
+                 not counted)
+      k_theta1   T launches (k_theta2d, then the active-tile list): read C
+                 flags, write G
+      k_fbface   T launches (k_fbmarch2d or k_fbtile2d): read U_cv, theta, both
+                 face arrays, write V C (at most: cells with no blended face
+                 keep the candidate; rows and tiles without a blended face are
+                 skipped, so the launch reads less where few cells are
+                 troubled)
       k_cv2sp    once (k_tensor2d cv->sp with the wave-speed maximum of the
                  cells fused): read V C and write V C
     """
@@ -101,14 +108,14 @@ def algorithmic_bytes(N, p, launches):
         "k_face_sd": T * V * C + 2 * Fe,
         "k_pred": ((2 * V * C) + (2 * V * C + T * V * C) + (m - 2) * (2 * T * V * C + V * C)
                    if m >= 2 else V * C + T * V * C),
-        "k_hoflux": T * (3 * V * C + 2 * V * (N + 1) * N * n + 2 * Fd),
+        "k_hoflux": T * (3 * V * C + 2 * V * (N + 1) * N * n),
         "k_tensor": 2 * V * C,
-        "k_detect": T * (2 * V * C),
+        "k_detect": 0,
         "k_theta1": T * G,
         "k_fbface": T * (2 * V * C + G + 2 * Fd),
         "k_cv2sp": 2 * V * C,
     }
-    extra_bytes = {"k_detect": T * C * 1.0, "k_theta1": T * C * 1.0}
+    extra_bytes = {"k_hoflux": T * C * 1.0, "k_detect": T * C * 1.0, "k_theta1": T * C * 1.0}
     out = {}
     steps = launches["_steps"]
     for k, val in launches.items():
@@ -529,40 +536,49 @@ def main():
     dof = (N * (p + 1)) ** 2
     noise = ic.subcell_noise((N, N), p, [1, 2], 1e-3, seed=0)
 
-    def run_form(precision):
-        """Warm-up, then the timed steps (no instrumentation), then as many
-        steps with HIP events around every kernel (the per-kernel durations of
-        the roofline), then one more step that measures the troubled
-        fraction."""
+    def make_sim(precision):
         s = SDADER_Simulator(init_fct=ic.kelvin_helmholtz, device=dev, verbose=False,
                              precision=precision, **kh_kwargs(N, p))
         s.perturb_initial_condition(noise)
+        return s
+
+    def run_form(precision):
+        """Warm-up, then the timed steps: the device-side time loop of
+        perform_time_evolution (dt, step and time on the device, replayed from
+        one HIP graph captured beforehand); then the same steps from the same
+        state with the host-driven loop (dt read back every step, kernels
+        launched one by one) on a second simulator, which must end in the
+        same state bit for bit; then as many steps with HIP events around
+        every kernel (the per-kernel durations of the roofline), then one
+        more step that measures the troubled fraction."""
+        s = make_sim(precision)
         s.advance(args.warmup)
         early = s.dm.device_value("U_sp").clone()
-        # the timed steps: the host-driven loop of the reference (dt read back
-        # to the host every step, kernels launched one by one)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        s.advance(args.steps)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        timed_end = s.dm.device_value("U_sp").clone()
-        s.profile_begin(max_launches=200 * max(args.profile_steps, 1))
-        s.advance(args.profile_steps)
-        prof = s.profile_end()
-        # the same number of steps with the device-side loop (dt, step and
-        # time on the device) replayed from one HIP graph captured beforehand
         s.prepare_graph(args.steps)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         s.advance_device(args.steps)
         torch.cuda.synchronize()
-        device_loop = time.perf_counter() - t0
+        elapsed = time.perf_counter() - t0
+        timed_end = s.dm.device_value("U_sp").clone()
+        h = make_sim(precision)
+        h.advance(args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h.advance(args.steps)
+        torch.cuda.synchronize()
+        host_loop = time.perf_counter() - t0
+        same = bool(torch.equal(h.dm.device_value("U_sp"), timed_end))
+        del h
+        torch.cuda.empty_cache()
+        s.profile_begin(max_launches=200 * max(args.profile_steps, 1))
+        s.advance(args.profile_steps)
+        prof = s.profile_end()
         frac = s.troubled_fraction_next_step()
-        return s, elapsed, prof, frac, early, device_loop, timed_end
+        return s, elapsed, prof, frac, early, host_loop, timed_end, same
 
-    s, elapsed, prof, frac, early64, dloop64, end64 = run_form("fp64")
-    s32, elapsed32, prof32, frac32, early32, dloop32, end32 = run_form("fp32")
+    s, elapsed, prof, frac, early64, hloop64, end64, same64 = run_form("fp64")
+    s32, elapsed32, prof32, frac32, early32, hloop32, end32, same32 = run_form("fp32")
 
     # the two forms after the same number of steps from the same state
     def l1_rel(r32, r64):
@@ -634,16 +650,19 @@ def main():
             "step_flop_per_dof": step_flops / dof if step_flops else None,
             "flops_source": os.path.relpath(FLOPS_SUMMARY, HERE)},
         "copy_GBps": copy_gbps,
-        "device_loop_ms_per_step": dloop64 / args.steps * 1e3,
+        "timed_loop": "device-side time loop (one HIP graph of the timed steps)",
+        "host_loop_ms_per_step": hloop64 / args.steps * 1e3,
+        "host_loop_same_state": same64,
         "troubled_fraction": frac,
         "kernels": kernels,
         "fp32_bulk": {
             "value": value32, "unit": "DOF-updates/s", "ms_per_step": elapsed32 / args.steps * 1e3,
-            "device_loop_ms_per_step": dloop32 / args.steps * 1e3,
+            "host_loop_ms_per_step": hloop32 / args.steps * 1e3,
+            "host_loop_same_state": same32,
             "speedup_vs_fp64": value32 / value,
             "step_hbm_roofline_frac": value32 * 64.0 / (HBM_PEAK_GBPS * 1e9),
             "l1_rel_vs_fp64": l1,
-            "steps_compared": args.warmup + 2 * args.steps + args.profile_steps + 1,
+            "steps_compared": args.warmup + args.steps + args.profile_steps + 1,
             "l1_rel_vs_fp64_after_warmup": l1_early, "steps_compared_after_warmup": args.warmup,
             "l1_rel_vs_fp64_after_timed": l1_timed,
             "steps_compared_after_timed": args.warmup + args.steps,

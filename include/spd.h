@@ -206,15 +206,18 @@ int spd_run_status(spd_ctx* ctx, double* time, long long* n_step, int* done, dou
  * reference counterpart; the autotuner of spd_amd.ensemble sets them):
  * elements per block of the predictor update and of the 2-D high-order
  * face-flux kernel of form 1, and the form of the 2-D FV substep
- * (1: k_ho2d writes the face fluxes, k_apply the candidate, the fallback
- * corrects cell tiles staged in LDS; 2, the default: one kernel forms the
- * face fluxes and the candidate, the fallback marches column strips in
- * registers; 3: the same kernel without the face-flux arrays, the fallback
- * runs on the compacted list of active tiles and forms their face fluxes
- * again; the forms agree to round-off, not bit for bit, so the autotuner
- * leaves the form alone).  value 0 restores the default; values above the
- * maximum (256 threads / 48 KiB of LDS per block; 3 forms) are rejected.
- * spd_get_launch reports the value in effect and the maximum. */
+ * (1: k_ho2d writes the face fluxes, k_apply the candidate, k_detect2d the
+ * flags, the fallback corrects cell tiles staged in LDS; 2: one kernel forms
+ * the face fluxes, the candidate and the first half of the detection (NAD,
+ * PAD), k_sed2d the smooth-extrema half, the fallback marches column strips
+ * in registers over the face-flux arrays; 3: the same without the face-flux
+ * arrays, the fallback runs on the compacted list of active tiles and forms
+ * their face fluxes again; 4, the default: per substep the path of 2 or of
+ * 3, chosen on the device from the active tiles of the previous substep of
+ * the same step; the forms agree to round-off, not bit for bit, so the
+ * autotuner leaves the form alone).  value 0 restores the default; values
+ * above the maximum (256 threads / 48 KiB of LDS per block; 4 forms) are
+ * rejected.  spd_get_launch reports the value in effect and the maximum. */
 enum spd_launch_key {
   SPD_LAUNCH_PRED_EPB = 0,
   SPD_LAUNCH_HO_EPB = 1,

@@ -80,8 +80,9 @@ struct spd_ctx {
   void* HS = nullptr;         // Hancock states and half slopes, (1+ndim)*V per cell (bulk)
   double* rp = nullptr;       // density and pressure of the candidate (1-D/3-D detection)
   double* Wold = nullptr;     // primitives of the state entering the substep (1-D/3-D)
-  int* fbt_counters = nullptr;  // compacted 2-D fallback: list length, next entry
+  int* fbt_counters = nullptr;  // compacted 2-D fallback: list length, next entry, path
   int* fbt_list = nullptr;      // active tiles of the compacted 2-D fallback
+  uint8_t* fbt_marks = nullptr; // tiles theta marked active (cleared by the list kernel)
   int* any_flag = nullptr;
   int* zero_flag = nullptr;   // always 0: the skip flag of ordinary calls
   int* run_skip = nullptr;    // skip flag of device-loop steps

@@ -100,6 +100,97 @@ constexpr int DWO = 4 * ONY * ONX;
 constexpr int DAX = 2 * DTY * (DTX + 2), DAY = 2 * (DTY + 2) * DTX;
 static_assert(DAX + DAY <= DWO * 8, "smooth-extrema table must fit the old-state tile");
 
+// New density and pressure of the DTX x DTY tile at (x0, y0) and its DH halo
+// (ghost layers by the index maps, stored ghosts of "ic" / "pressure" sides;
+// NaN outside the ghosted grid).  Each thread issues the loads of all its
+// cells before it uses the first.
+__device__ __forceinline__ void stage_new_rho_p(const Geom& g, const Params& prm,
+                                                const double* __restrict__ Unew, int x0, int y0,
+                                                double (*wn)[DNY][DNX]) {
+  constexpr int NI = (DNX * DNY + DNT - 1) / DNT;
+  double u[NI][4];
+  bool ok[NI], fl[NI][2];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) {
+    const int k = threadIdx.x + j * DNT;
+    ok[j] = k < DNX * DNY && stage_cons(Unew, g, x0 + k % DNX - DH, y0 + k / DNX - DH, u[j], fl[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < NI; ++j) {
+    const int k = threadIdx.x + j * DNT;
+    if (k >= DNX * DNY) continue;
+    double r = qnan(), p = qnan();
+    if (ok[j]) {
+      double W[4];
+      prims<2>(u[j], W, 0, prm.gamma);
+      int c[3] = {x0 + k % DNX - DH, y0 + k / DNX - DH, 0};
+      stored_ghost<2>(g, c, W);  // ghosts of "ic" / "pressure" sides
+      r = W[0];
+      p = W[3];
+    }
+    wn[0][k / DNX][k % DNX] = r;
+    wn[1][k / DNX][k % DNX] = p;
+  }
+}
+
+// Smooth-extrema relaxation (src/trouble_detection.py:38-51,104-137) of the
+// density / pressure candidates pt[0], pt[3] of the calling thread's cell
+// (thread (tx, ty) of the DTX x DTY tile at (x0, y0)): the tile's bits table
+// (DAX + DAY bytes of LDS at Ax) is built from wn, the new density and
+// pressure on the tile + DH halo, and a candidate is kept where
+// alpha = min(1, alpha_x, alpha_y) < 1.  Every thread of the block calls it.
+__device__ __forceinline__ void sed_relax_tile(const Geom& g, double (*wn)[DNY][DNX],
+                                               uint8_t* Ax, int x0, int y0, bool inside,
+                                               bool* pt) {
+  const int Nx = g.Ns[0], Ny = g.Ns[1];
+  const int tx = threadIdx.x % DTX, ty = threadIdx.x / DTX;
+  const int x = x0 + tx, y = y0 + ty;
+  uint8_t* Ay = Ax + DAX;  // Ax: [2][DTY][DTX+2], x in [x0-1, x0+DTX]; Ay: [2][DTY+2][DTX]
+  const double* rdcx = g.rdc[0] + NGH;
+  const double* rdcy = g.rdc[1] + NGH;
+  const double* hfx = g.hfp[0] + NGH;
+  const double* hfy = g.hfp[1] + NGH;
+  for (int k = threadIdx.x; k < DAX + DAY; k += DNT) {
+    if (k < DAX) {
+      const int var = k / (DTY * (DTX + 2)), r = k % (DTY * (DTX + 2));
+      const int ly = r / (DTX + 2), lx = r % (DTX + 2);
+      const double* w = &wn[var][ly + DH][lx - 1 + DH];
+      Ax[k] = (uint8_t)sed_bits(w, 1, x0 + lx - 1, Nx, rdcx, hfx);
+    } else {
+      const int kk = k - DAX;
+      const int var = kk / ((DTY + 2) * DTX), r = kk % ((DTY + 2) * DTX);
+      const int ly = r / DTX, lx = r % DTX;
+      const double* w = &wn[var][ly - 1 + DH][lx + DH];
+      Ay[kk] = (uint8_t)sed_bits(w, DNX, y0 + ly - 1, Ny, rdcy, hfy);
+    }
+  }
+  __syncthreads();
+  if (!inside) return;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int var = (k == 0) ? 0 : 3;
+    if (!pt[var]) continue;
+    const double* w = &wn[k][ty + DH][tx + DH];
+    const uint8_t* ax = Ax + (k * DTY + ty) * (DTX + 2) + tx + 1;  // bits at x
+    const uint8_t* ay = Ay + (k * (DTY + 2) + ty + 1) * DTX + tx;  // bits at y
+    bool r1, r2;
+    if (Nx == 1 || x == Nx - 1)
+      r1 = sed_relax_lds(w, 1, x, Nx, rdcx, hfx);
+    else if (x == 0)
+      r1 = ((ax[0] | ax[1]) & 1) != 0;
+    else
+      r1 = ((ax[-1] | ax[0] | ax[1]) & 1) != 0;
+    if (Ny == 1 || y == Ny - 1)
+      r2 = sed_relax_lds(w, DNX, y, Ny, rdcy, hfy);
+    else if (y == 0)
+      r2 = ((ay[0] | ay[DTX]) & 1) != 0;
+    else
+      r2 = ((ay[-DTX] | ay[0] | ay[DTX]) & 1) != 0;
+    // alpha = min(1, alpha_x, alpha_y) < 1
+    pt[var] = pt[var] && (r1 || r2);
+  }
+}
+
 // FP64 in both precision forms: the smooth-extrema test takes second
 // differences of the new state, ~1e-8 of it at 4096 subcells per side.
 __global__ void __launch_bounds__(DNT)
@@ -115,32 +206,7 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
   const int Nx = g.Ns[0], Ny = g.Ns[1];
   // staging: each thread issues the loads of all its cells of a tile before
   // it uses the first (new rho, P on tile + 3, then old primitives on tile + 1)
-  {
-    constexpr int NI = (DNX * DNY + DNT - 1) / DNT;
-    double u[NI][4];
-    bool ok[NI], fl[NI][2];
-#pragma unroll
-    for (int j = 0; j < NI; ++j) {
-      const int k = threadIdx.x + j * DNT;
-      ok[j] = k < DNX * DNY && stage_cons(Unew, g, x0 + k % DNX - DH, y0 + k / DNX - DH, u[j], fl[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < NI; ++j) {
-      const int k = threadIdx.x + j * DNT;
-      if (k >= DNX * DNY) continue;
-      double r = qnan(), p = qnan();
-      if (ok[j]) {
-        double W[4];
-        prims<2>(u[j], W, 0, prm.gamma);
-        int c[3] = {x0 + k % DNX - DH, y0 + k / DNX - DH, 0};
-        stored_ghost<2>(g, c, W);  // ghosts of "ic" / "pressure" sides
-        r = W[0];
-        p = W[3];
-      }
-      wn[0][k / DNX][k % DNX] = r;
-      wn[1][k / DNX][k % DNX] = p;
-    }
-  }
+  stage_new_rho_p(g, prm, Unew, x0, y0, wn);
   {
     constexpr int NI = (ONX * ONY + DNT - 1) / DNT;
     double u[NI][4];
@@ -216,54 +282,7 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
     // smooth-extrema relaxation of the rho / P candidates; the table is built
     // only for tiles that hold one
     const int need = __syncthreads_or((pt[0] || pt[3]) ? 1 : 0);
-    if (need) {
-      uint8_t* Ax = reinterpret_cast<uint8_t*>(wbuf);  // [2][DTY][DTX+2], x in [x0-1, x0+DTX]
-      uint8_t* Ay = Ax + DAX;                          // [2][DTY+2][DTX], y in [y0-1, y0+DTY]
-      const double* rdcx = g.rdc[0] + NGH;
-      const double* rdcy = g.rdc[1] + NGH;
-      const double* hfx = g.hfp[0] + NGH;
-      const double* hfy = g.hfp[1] + NGH;
-      for (int k = threadIdx.x; k < DAX + DAY; k += DNT) {
-        if (k < DAX) {
-          const int var = k / (DTY * (DTX + 2)), r = k % (DTY * (DTX + 2));
-          const int ly = r / (DTX + 2), lx = r % (DTX + 2);
-          const double* w = &wn[var][ly + DH][lx - 1 + DH];
-          Ax[k] = (uint8_t)sed_bits(w, 1, x0 + lx - 1, Nx, rdcx, hfx);
-        } else {
-          const int kk = k - DAX;
-          const int var = kk / ((DTY + 2) * DTX), r = kk % ((DTY + 2) * DTX);
-          const int ly = r / DTX, lx = r % DTX;
-          const double* w = &wn[var][ly - 1 + DH][lx + DH];
-          Ay[kk] = (uint8_t)sed_bits(w, DNX, y0 + ly - 1, Ny, rdcy, hfy);
-        }
-      }
-      __syncthreads();
-      if (inside) {
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          const int var = (k == 0) ? 0 : 3;
-          if (!pt[var]) continue;
-          const double* w = &wn[k][ty + DH][tx + DH];
-          const uint8_t* ax = Ax + (k * DTY + ty) * (DTX + 2) + tx + 1;  // bits at x
-          const uint8_t* ay = Ay + (k * (DTY + 2) + ty + 1) * DTX + tx;  // bits at y
-          bool r1, r2;
-          if (Nx == 1 || x == Nx - 1)
-            r1 = sed_relax_lds(w, 1, x, Nx, rdcx, hfx);
-          else if (x == 0)
-            r1 = ((ax[0] | ax[1]) & 1) != 0;
-          else
-            r1 = ((ax[-1] | ax[0] | ax[1]) & 1) != 0;
-          if (Ny == 1 || y == Ny - 1)
-            r2 = sed_relax_lds(w, DNX, y, Ny, rdcy, hfy);
-          else if (y == 0)
-            r2 = ((ay[0] | ay[DTX]) & 1) != 0;
-          else
-            r2 = ((ay[-DTX] | ay[0] | ay[DTX]) & 1) != 0;
-          // alpha = min(1, alpha_x, alpha_y) < 1
-          pt[var] = pt[var] && (r1 || r2);
-        }
-      }
-    }
+    if (need) sed_relax_tile(g, wn, reinterpret_cast<uint8_t*>(wbuf), x0, y0, inside, pt);
   }
   if (inside) {
     bool tr = pt[0] || pt[3];
@@ -279,6 +298,36 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
 }
 
 // ---------------------------------------------------------------------------
+// Second half of the detection when the high-order flux kernel has run its
+// first half (k_hocand2d with DET; src/trouble_detection.py:38-55): that
+// kernel leaves per cell the code PAD (bit 0) | NAD density candidate (bit 1)
+// | NAD pressure candidate (bit 2).  A DTX x DTY tile without candidates
+// returns after one vote (its codes are already the final 0 / 1); a tile with
+// one stages the new density and pressure on the tile + 3 halo, applies the
+// smooth-extrema relaxation of k_detect2d to its candidates and writes their
+// final flags.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(DNT)
+k_sed2d(Geom g, Params prm, const double* __restrict__ Unew, uint8_t* __restrict__ flags) {
+  SPD_SKIP(g);
+  __shared__ double wn[2][DNY][DNX];
+  __shared__ uint8_t ab[DAX + DAY];
+  int bx, by;
+  xcd_band_tile(bx, by);
+  const int x0 = bx * DTX, y0 = by * DTY;
+  const int Nx = g.Ns[0], Ny = g.Ns[1];
+  const int x = x0 + threadIdx.x % DTX, y = y0 + threadIdx.x / DTX;
+  const bool inside = x < Nx && y < Ny;
+  const int code = inside ? (int)flags[(long long)y * Nx + x] : 0;
+  if (!__syncthreads_or(code & 6)) return;
+  stage_new_rho_p(g, prm, Unew, x0, y0, wn);
+  __syncthreads();
+  bool pt[4] = {(code & 2) != 0, false, false, (code & 4) != 0};
+  sed_relax_tile(g, wn, ab, x0, y0, inside, pt);
+  if (inside && (code & 6)) flags[(long long)y * Nx + x] = (pt[0] || pt[3] || (code & 1)) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
 // Blending coefficients theta on the ghosted grid (src/trouble_detection.py:
 // 73-87,139-169): trouble values on the tile + 2 halo, theta1 on tile + 1
 // halo, then the x and y 0.25 rings.  Tiles cover the ghosted grid; (gx, gy)
@@ -291,15 +340,71 @@ constexpr int HTX = 64, HTY = 16, HNT = 256;
 constexpr int TH = 2;
 constexpr int TNX = HTX + 2 * TH, TNY = HTY + 2 * TH;
 
+// Tiles of the compacted fallback that a block of k_theta2d marks: with
+// `marks` set, every cell whose theta is not 0 marks the fallback tiles
+// (cells of mk.tx x mk.ty, mk.ntx per row) holding it and its four
+// neighbours (the cells that border its faces), through a flag table of the
+// block in LDS; the list kernel reads and clears the marks.
+struct FbMarks {
+  uint8_t* marks;
+  int tx, ty, ntx, nty;
+};
+constexpr int MKT = 8;  // flag table of a block: MKT x MKT fallback tiles
+
 __global__ void __launch_bounds__(HNT)
 k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* __restrict__ Ucv,
           const double* __restrict__ Unew, const int* __restrict__ any_flag,
-          double* __restrict__ theta) {
+          double* __restrict__ theta, FbMarks mk) {
   SPD_SKIP(g);
   __shared__ double tr[TNY][TNX];
   __shared__ double t1[HTY + 2][HTX + 2];
+  // fallback-tile marks: the table bits (two words), and the table column /
+  // row of the cells x = gx0 - NGH - 1 + j, j < HTX + 2, and
+  // y = gy0 - NGH - 1 + j, j < HTY + 2 (-1 outside the grid or the table)
+  __shared__ unsigned smk[2];
+  __shared__ signed char mcol[HTX + 2], mrow[HTY + 2];
   const int gx0 = blockIdx.x * HTX, gy0 = blockIdx.y * HTY;
   const int NGx = g.Ng[0], NGy = g.Ng[1];
+  // first fallback tile of the block's table (cells from gx0 - NGH - 1)
+  const int mbx = mk.marks ? ((gx0 - NGH - 1 > 0) ? gx0 - NGH - 1 : 0) / mk.tx : 0;
+  const int mby = mk.marks ? ((gy0 - NGH - 1 > 0) ? gy0 - NGH - 1 : 0) / mk.ty : 0;
+  if (mk.marks) {
+    const int t = threadIdx.x;
+    if (t < 2) smk[t] = 0;
+    if (t < HTX + 2) {
+      const int x = gx0 - NGH - 1 + t;
+      const int c = (x >= 0 && x < g.Ns[0]) ? x / mk.tx - mbx : -1;
+      mcol[t] = (signed char)((c < MKT) ? c : -1);
+    } else if (t < HTX + 2 + HTY + 2) {
+      const int y = gy0 - NGH - 1 + (t - HTX - 2);
+      const int r = (y >= 0 && y < g.Ns[1]) ? y / mk.ty - mby : -1;
+      mrow[t - HTX - 2] = (signed char)((r < MKT) ? r : -1);
+    }
+  }
+  // theta of ghosted cell (gx, gy) of the block's outputs is not 0: the
+  // table bits of the tiles of the cell and of its four neighbours
+  unsigned long long mbits = 0;
+  auto mark = [&](int gx, int gy) {
+    const int jx = gx - gx0 + 1, jy = gy - gy0 + 1;
+    const int cj[5] = {mcol[jx], mcol[jx - 1], mcol[jx + 1], mcol[jx], mcol[jx]};
+    const int rj[5] = {mrow[jy], mrow[jy], mrow[jy], mrow[jy - 1], mrow[jy + 1]};
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (cj[q] >= 0 && rj[q] >= 0) mbits |= 1ull << (rj[q] * MKT + cj[q]);
+  };
+  // after the block's last barrier: write the flagged tiles' marks
+  auto flush_marks = [&]() {
+    if (mbits) {
+      if ((unsigned)mbits) atomicOr(&smk[0], (unsigned)mbits);
+      if ((unsigned)(mbits >> 32)) atomicOr(&smk[1], (unsigned)(mbits >> 32));
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < MKT * MKT && ((smk[t / 32] >> (t % 32)) & 1u)) {
+      const int tx = mbx + t % MKT, ty = mby + t / MKT;
+      if (tx < mk.ntx && ty < mk.nty) mk.marks[(long long)ty * mk.ntx + tx] = 1;
+    }
+  };
   // A tile whose staged cells are all active cells (most tiles) holds trouble
   // values 0 / 1 only, and with blending theta is then a function of the
   // distance to the nearest troubled cell (the 3 x 3 maximum of theta1 with
@@ -355,7 +460,9 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
       else if (r2 & 1)
         th = 0.25;
       theta[(long long)(gy0 + k / HTX) * NGx + gx0 + k % HTX] = th;
+      if (mk.marks && th != 0.0) mark(gx0 + k % HTX, gy0 + k / HTX);
     }
+    if (mk.marks) flush_marks();
     return;
   }
   {
@@ -377,8 +484,13 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
     for (int k = threadIdx.x; k < HTX * HTY; k += HNT) {
       const int tx = k % HTX, ty = k / HTX;
       const int gx = gx0 + tx, gy = gy0 + ty;
-      if (gx < NGx && gy < NGy) theta[(long long)gy * NGx + gx] = tr[ty + TH][tx + TH];
+      if (gx < NGx && gy < NGy) {
+        const double th = tr[ty + TH][tx + TH];
+        theta[(long long)gy * NGx + gx] = th;
+        if (mk.marks && !(th == 0.0)) mark(gx, gy);
+      }
     }
+    if (mk.marks) flush_marks();
     return;
   }
   for (int k = threadIdx.x; k < (HTX + 2) * (HTY + 2); k += HNT) {
@@ -413,8 +525,11 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
     };
     const double lo = (gy >= 1) ? rx(-1) : 0.0;
     const double hi = (gy <= NGy - 2) ? rx(+1) : 0.0;
-    theta[(long long)gy * NGx + gx] = ring1(gy, NGy, lo, rx(0), hi);
+    const double th = ring1(gy, NGy, lo, rx(0), hi);
+    theta[(long long)gy * NGx + gx] = th;
+    if (mk.marks && !(th == 0.0)) mark(gx, gy);
   }
+  if (mk.marks) flush_marks();
 }
 
 // ---------------------------------------------------------------------------
@@ -653,8 +768,9 @@ __global__ void __launch_bounds__(MW * MWAVES)
 k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
             const double* __restrict__ theta, const B* __restrict__ Fx,
             const B* __restrict__ Fy, const double* __restrict__ dt_dev, double w_i,
-            int MR, double* __restrict__ Uout) {
+            int MR, double* __restrict__ Uout, const int* __restrict__ gate) {
   SPD_SKIP(g);
+  if (gate && *gate == 0) return;  // the adaptive form took the compacted path
   const int lane = threadIdx.x % MW;
   const int Nx = g.Ns[0], Ny = g.Ns[1];
   const int NGx = g.Ng[0];
@@ -1056,7 +1172,14 @@ __device__ __forceinline__ void ho_fp_flux(const Ops<P, B>& op, B gamma, int f, 
 // values.  The FV face fluxes stay in LDS: each element forms its own n + 1
 // face columns per dimension (a face shared by two elements is formed by
 // both, identically).  U_out may be U_cv (the update in place without the
-// fallback): a cell is read and written by the same thread only.
+// fallback): a cell is read and written by the same thread only.  A block
+// owns a run of at most E elements of one element row.
+// With DET (and flags != null) the kernel also runs the first half of the
+// trouble detection (src/trouble_detection.py:5-55) on its cells: the NAD
+// bounds from the old primitives of the cells and their one-cell ring (staged
+// in LDS before the slab), the NAD and PAD tests of the candidate, the global
+// NAD gate (any_flag, when not null), and per cell the final flag, or the code
+// that k_sed2d finishes when the smooth-extrema relaxation follows.
 //   LDS: slab [V][E][ho_nlp], face columns [2][E][nf][n][V] (lines, then
 //   their transverse averages).
 // ---------------------------------------------------------------------------
@@ -1070,22 +1193,41 @@ template <int P> struct HocTile {
   template <typename B> static constexpr int lds_bytes() {
     return (4 * E * ho_nlp(P) + 2 * NI * 4) * (int)sizeof(B);
   }
+  // with the detection's first half: the old primitives of the block's cells
+  // and their ring, (nvar, n + 2, E n + 2) doubles (nvar = 2, or 4 with the
+  // global gate), go to the slab's LDS (free once the line fluxes are formed)
+  // when they fit there, else after the face columns
+  template <typename B> static constexpr int ring_bytes(int nvar) {
+    return nvar * (n + 2) * (E * n + 2) * 8;
+  }
+  template <typename B> static constexpr bool ring_in_slab(int nvar) {
+    return ring_bytes<B>(nvar) <= 4 * E * ho_nlp(P) * (int)sizeof(B);
+  }
+  template <typename B> static constexpr int lds_bytes_det(int nvar) {
+    return ring_in_slab<B>(nvar) ? lds_bytes<B>() : lds_bytes<B>() + ring_bytes<B>(nvar);
+  }
 };
 
-template <int P, typename B>
+template <int P, typename B, bool DET>
 __global__ void __launch_bounds__(256)
 k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
            const B* __restrict__ Ffx, const B* __restrict__ Ffy, const double* Ucv,
            const double* __restrict__ dt_dev, double w_i, double* Uout, B* __restrict__ Fhx,
-           B* __restrict__ Fhy) {
+           B* __restrict__ Fhy, uint8_t* __restrict__ flags, int* __restrict__ any_flag,
+           const int* __restrict__ fh_gate) {
   SPD_SKIP(g);
+  // the face-flux arrays are written when given and, with a gate, when it is set
+  if (fh_gate && *fh_gate == 0) Fhx = Fhy = nullptr;
   using HT = HocTile<P>;
   constexpr int V = 4, n = P + 1, nf = P + 2, NL = n * n, E = HT::E, NI = HT::NI;
   B* slab = reinterpret_cast<B*>(spd_dyn_lds);  // [V][E][ho_nlp]
   B* col = slab + V * E * ho_nlp(P);            // [2][E][nf][n][V]
-  const long long e0 = (long long)blockIdx.x * E;
-  const int ne = (int)((g.NE - e0 < E) ? (g.NE - e0) : E);
   const int Nx = g.N[0], Ny = g.N[1], Nsx = g.Ns[0];
+  // the block's elements: a run of at most E elements of one element row
+  const int nseg = (Nx + E - 1) / E;
+  const int ey = blockIdx.x / nseg, ex0 = (blockIdx.x % nseg) * E;
+  const long long e0 = (long long)ey * Nx + ex0;
+  const int ne = (Nx - ex0 < E) ? (Nx - ex0) : E;
   const int tid = threadIdx.x;
   const B gamma = (B)prm.gamma;
   const double dti = (*dt_dev) * w_i;
@@ -1135,6 +1277,8 @@ k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
       for (int v = 0; v < V; ++v) fl[it][v] = Ffy[(long long)(t * V + v) * plane + i0];
     }
   }
+  const bool det = DET && flags != nullptr;
+  const int nvar_nad = any_flag ? 4 : 2;  // velocities only feed the global gate
   // 2. the predictor slab of node t
   for (int k = tid; k < V * ne * NL; k += 256) {
     const int v = k / (ne * NL), r = k % (ne * NL);
@@ -1205,7 +1349,65 @@ k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
     }
   }
   __syncthreads();
-  // 5. the candidate
+  // 4b. (DET) the old primitives of the block's cells and their one-cell ring
+  // (density and pressure, all variables with the gate; the staging of
+  // k_detect2d), in the slab's LDS when it is large enough, else after the
+  // face columns
+  const int RX = ne * n + 2, RN = RX * (n + 2);
+  double* wo = reinterpret_cast<double*>(
+      spd_dyn_lds + (HT::template ring_in_slab<B>(nvar_nad) ? 0 : HT::template lds_bytes<B>()));
+  if (det) {
+    // the block's own cells from the registers, the ring from memory
+#pragma unroll
+    for (int c = 0; c < HT::CPT; ++c) {
+      const int item = tid + c * 256;
+      if (item >= ne * NL) continue;
+      const int o = (item / (ne * n) + 1) * RX + item % (ne * n) + 1;
+      double W[4];
+      prims<2>(u[c], W, 0, prm.gamma);
+      wo[o] = W[0];
+      wo[RN + o] = W[3];
+      if (nvar_nad == 4) {
+        wo[2 * RN + o] = W[1];
+        wo[3 * RN + o] = W[2];
+      }
+    }
+    const int X0 = ex0 * n - 1, Y0 = ey * n - 1;
+    const int RY = n + 2, NB = 2 * RX + 2 * (RY - 2);
+    for (int j = tid; j < NB; j += 256) {
+      int lx, ly;
+      if (j < 2 * RX) {
+        lx = j % RX;
+        ly = (j < RX) ? 0 : RY - 1;
+      } else {
+        lx = ((j - 2 * RX) % 2) * (RX - 1);
+        ly = 1 + (j - 2 * RX) / 2;
+      }
+      const int k = ly * RX + lx;
+      const int x = X0 + lx, y = Y0 + ly;
+      double uu[4], W[4] = {qnan(), qnan(), qnan(), qnan()};
+      bool fl2[2];
+      if (stage_cons(Ucv, g, x, y, uu, fl2)) {
+        prims<2>(uu, W, 0, prm.gamma);
+        if (fl2[0]) W[1] = -W[1];
+        if (fl2[1]) W[2] = -W[2];
+        int c3[3] = {x, y, 0};
+        stored_ghost<2>(g, c3, W);  // ghosts of "ic" / "pressure" sides
+      }
+      wo[k] = W[0];
+      wo[RN + k] = W[3];
+      if (nvar_nad == 4) {
+        wo[2 * RN + k] = W[1];
+        wo[3 * RN + k] = W[2];
+      }
+    }
+    __syncthreads();
+  }
+  // 5. the candidate; (DET) its NAD / PAD test: the code PAD (bit 0) | density
+  // candidate (bit 1) | pressure candidate (bit 2) for k_sed2d when the
+  // smooth-extrema relaxation follows, else the final flag
+  const bool sed = prm.p > 1 && prm.sed;
+  bool cand = false;
 #pragma unroll
   for (int c = 0; c < HT::CPT; ++c) {
     const int item = tid + c * 256;
@@ -1218,13 +1420,56 @@ k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
     const B* fx1 = fx0 + n * V;                                // x-face i + 1
     const B* fy0 = col + (NI + (el * nf + qy) * n + i) * V;    // y-face qy, subcell column i
     const B* fy1 = fy0 + n * V;                                // y-face qy + 1
+    double un[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       double d = u[c][v] * 0.0;
       d = d + ((double)fx1[v] - (double)fx0[v]) * rdx;
       d = d + ((double)fy1[v] - (double)fy0[v]) * rdy;
-      Uout[v * g.Nc + ci] = u[c][v] - d * dti;
+      un[v] = u[c][v] - d * dti;
+      Uout[v * g.Nc + ci] = un[v];
     }
+    if (det) {
+      double Wn[4];
+      prims<2>(un, Wn, 0, prm.gamma);
+      bool pt[4] = {false, false, false, false};
+      const int o = (qy + 1) * RX + el * n + i + 1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k >= nvar_nad) break;
+        const int v = (k == 0) ? 0 : (k == 1 ? 3 : k - 1);
+        const double* w = wo + k * RN + o;
+        double mx = w[0], mn = mx;
+        const double nb[4] = {w[-1], w[1], w[-RX], w[RX]};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          mx = nmax(mx, nb[q]);
+          mn = nmin(mn, nb[q]);
+        }
+        double lo = mn, hi = mx;
+        if (prm.p > 0) {
+          if (prm.nad_delta) {
+            const double eps = prm.tol * (hi - lo);
+            lo = lo - eps;
+            hi = hi + eps;
+          } else {
+            lo = lo - fabs(lo) * prm.tol;
+            hi = hi + fabs(hi) * prm.tol;
+          }
+        }
+        pt[v] = !(Wn[v] >= lo) || !(Wn[v] <= hi);
+        cand = cand || pt[v];
+      }
+      bool pad = false;
+      if (prm.pad)
+        pad = !(Wn[0] >= prm.min_rho) || !(Wn[0] <= prm.max_rho) || !(Wn[3] >= prm.min_P);
+      flags[ci] = sed ? (uint8_t)((pad ? 1 : 0) | (pt[0] ? 2 : 0) | (pt[3] ? 4 : 0))
+                      : (uint8_t)((pad || pt[0] || pt[3]) ? 1 : 0);
+    }
+  }
+  if (det && any_flag) {
+    const int blk = __syncthreads_or(cand ? 1 : 0);
+    if (blk && threadIdx.x == 0 && *(volatile int*)any_flag == 0) atomicOr(any_flag, 1);
   }
 }
 
@@ -1257,6 +1502,43 @@ template <int P> inline long long fbtile_count(const Geom& g) {
 
 __global__ void k_zero2(int* c) {
   if (threadIdx.x < 2 && blockIdx.x == 0) c[threadIdx.x] = 0;
+}
+
+// Path of the adaptive form for one substep, before its high-order flux
+// kernel: counters[2] = 1 (column strips over the face-flux arrays) when
+// `force` is set or the previous substep listed more than `thr` active
+// tiles, else 0 (compacted tiles, no face-flux arrays); the list counters are
+// cleared.  The first substep of a step forces the column strips, so a
+// step's path depends on its own input only.
+__global__ void k_fb_decide(Geom g, int* counters, long long thr, int force) {
+  SPD_SKIP(g);
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    counters[2] = (force || counters[0] > thr) ? 1 : 0;
+    counters[0] = 0;
+    counters[1] = 0;
+  }
+}
+
+// Active-tile list from the marks k_theta2d set (every tile with godunov):
+// one thread per tile, which clears its mark; one atomic per wave reserves
+// the wave's entries.
+__global__ void __launch_bounds__(256)
+k_fbmark_list(Geom g, Params prm, uint8_t* __restrict__ marks, int* __restrict__ counters,
+              int* __restrict__ list, long long ntiles) {
+  SPD_SKIP(g);
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = false;
+  if (t < ntiles) {
+    const bool m = marks[t] != 0;
+    if (m) marks[t] = 0;
+    live = m || prm.godunov != 0;
+  }
+  const unsigned long long b = __ballot(live);
+  const int lane = threadIdx.x % 64;
+  int base = 0;
+  if (lane == 0 && b) base = atomicAdd(&counters[0], __popcll(b));
+  base = __shfl(base, 0);
+  if (live) list[base + __popcll(b & ((1ull << lane) - 1ull))] = (int)t;
 }
 
 // One wave per tile, FBL_WAVES tiles (consecutive in x) per block; one
@@ -1313,8 +1595,9 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
            const B* __restrict__ Ffx, const B* __restrict__ Ffy, const double* __restrict__ Ucv,
            const double* __restrict__ theta, int* __restrict__ counters,
            const int* __restrict__ list, const double* __restrict__ dt_dev, double w_i,
-           double* __restrict__ Uout) {
+           double* __restrict__ Uout, bool adaptive) {
   SPD_SKIP(g);
+  if (adaptive && counters[2] != 0) return;  // the adaptive form took the column strips
   using FT = FbTile<P>;
   constexpr int V = 4, n = P + 1, NL = n * n;
   constexpr int TXc = FT::TXc, TYc = FT::TYc, CXc = FT::CXc, CYc = FT::CYc;

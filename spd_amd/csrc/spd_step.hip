@@ -2348,21 +2348,24 @@ template <int NDIM, int P, typename B> struct Stepper {
     });
   }
 
-  // Form of the 2-D fallback correction: 1 cell tiles and 2 (default) column
-  // strips, both over the high-order face-flux arrays written by k_ho2d and
-  // the candidate of k_apply; 3 the fused high-order flux + candidate kernel
-  // (no face-flux arrays) and the fallback on the compacted list of active
-  // tiles, which forms the high-order fluxes of its tiles again.  At
-  // Kelvin-Helmholtz 1024^2 p=3 in the first steps (70-100% of the tiles
-  // active) form 3 costs 17.8 against 13.2 ms per step: the tile kernel's
-  // halo and flux re-evaluation outweigh the saved face-flux traffic.
+  // Form of the 2-D FV substep: 1 k_ho2d + k_apply and the fallback on cell
+  // tiles, both over the high-order face-flux arrays; 2 the fused high-order
+  // flux + candidate kernel writing the face-flux arrays and the fallback
+  // marching column strips over them; 3 the fused kernel without face-flux
+  // arrays and the fallback on the compacted list of active tiles, which
+  // forms the high-order fluxes of its tiles again; 4 (default) adaptive:
+  // per substep the path of 2 or of 3, chosen on the device from the number
+  // of tiles the previous substep of the same step listed (the first substep
+  // takes the path of 2).  At Kelvin-Helmholtz 1024^2 p=3, FP64, form 3
+  // costs 15.4 against 12.9 ms per step in the first steps (99% of the tiles
+  // active: the tile kernel forms every flux of a tile again) and 9.7 against
+  // 10.6 ms after 220 steps (19% active: no face-flux arrays written).
   static int fb_form(const spd_ctx* c) {
-    return c->launch[SPD_LAUNCH_FB_FORM] > 0 ? c->launch[SPD_LAUNCH_FB_FORM] : 2;
+    return c->launch[SPD_LAUNCH_FB_FORM] > 0 ? c->launch[SPD_LAUNCH_FB_FORM] : 4;
   }
-  // The fused k_hocand2d (fixed blocks of HocTile<P>::E elements) runs in
-  // every form but 1 (which keeps k_ho2d + k_apply), writing the face-flux
-  // arrays in form 2; the launch key of the high-order flux kernel sets the
-  // elements per block of k_ho2d.
+  // Active-tile fraction above which the adaptive form takes the column
+  // strips (where the two paths cost the same at Kelvin-Helmholtz 1024^2).
+  static constexpr double kDenseTiles = 0.35;
   static bool fused_ho(const spd_ctx* c) { return !c->cfg.FB || fb_form(c) != 1; }
   // Elements per block of k_ho2d (LDS slab of the block's elements, at most
   // 48 KiB).
@@ -2383,7 +2386,7 @@ template <int NDIM, int P, typename B> struct Stepper {
     return key == SPD_LAUNCH_PRED_EPB ? epb(c) : key == SPD_LAUNCH_HO_EPB ? ho_epb(c) : fb_form(c);
   }
   static int launch_max(const spd_ctx* c, int key) {
-    return key == SPD_LAUNCH_PRED_EPB ? epb_max() : key == SPD_LAUNCH_HO_EPB ? ho_epb_max(c) : 3;
+    return key == SPD_LAUNCH_PRED_EPB ? epb_max() : key == SPD_LAUNCH_HO_EPB ? ho_epb_max(c) : 4;
   }
 
   // One FV substep with the 2-D tiled kernels; the corrected state goes to the
@@ -2392,17 +2395,39 @@ template <int NDIM, int P, typename B> struct Stepper {
                         hipStream_t s) {
     tape_state(c, i, s);
     const int form = fb_form(c);
-    if (fused_ho(c)) {
-      // high-order fluxes and candidate in one pass (in place without the fallback)
+    const bool fused = fused_ho(c);
+    const bool adaptive = c->cfg.FB && form == 4;
+    using FT = fv2d::FbTile<P>;
+    const long long ntiles = fv2d::fbtile_count<P>(c->g);
+    if (fused) {
+      // high-order fluxes and candidate in one pass (in place without the
+      // fallback), with the first half of the trouble detection
       using HT = fv2d::HocTile<P>;
       double* out = c->cfg.FB ? c->Unew : c->Ucv;
-      const size_t lds = (size_t)HT::template lds_bytes<B>();
-      const unsigned hblocks = (unsigned)((c->g.NE + HT::E - 1) / HT::E);
+      const bool det = c->cfg.FB && !c->tape_replay;
+      const size_t lds = det ? (size_t)HT::template lds_bytes_det<B>(c->need_gate ? 4 : 2)
+                             : (size_t)HT::template lds_bytes<B>();
+      const unsigned hblocks =
+          (unsigned)((long long)c->g.N[1] * ((c->g.N[0] + HT::E - 1) / HT::E));
+      if (det && c->need_gate)
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
+      if (adaptive)
+        hipLaunchKernelGGL(fv2d::k_fb_decide, dim3(1), dim3(64), 0, s, c->g, c->fbt_counters,
+                           (long long)(kDenseTiles * (double)ntiles),
+                           (i == 0 || c->tape_replay) ? 1 : 0);
+      const int* gate = adaptive ? c->fbt_counters + 2 : nullptr;
       c->timed(KC_HOFLUX, s, [&] {
-        B* fx = (c->cfg.FB && form == 2) ? Fh(c, 0) : nullptr;
-        B* fy = (c->cfg.FB && form == 2) ? Fh(c, 1) : nullptr;
-        hipLaunchKernelGGL((fv2d::k_hocand2d<P, B>), dim3(hblocks), dim3(256), lds, s, op, c->g,
-                           c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, dt, w, out, fx, fy);
+        const bool wf = c->cfg.FB && (form == 2 || form == 4);
+        B* fx = wf ? Fh(c, 0) : nullptr;
+        B* fy = wf ? Fh(c, 1) : nullptr;
+        if (det)
+          hipLaunchKernelGGL((fv2d::k_hocand2d<P, B, true>), dim3(hblocks), dim3(256), lds, s, op,
+                             c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, dt, w, out, fx,
+                             fy, c->flags, c->need_gate ? c->any_flag : nullptr, gate);
+        else
+          hipLaunchKernelGGL((fv2d::k_hocand2d<P, B, false>), dim3(hblocks), dim3(256), lds, s,
+                             op, c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, dt, w, out,
+                             fx, fy, nullptr, nullptr, gate);
       });
       if (!c->cfg.FB) return;
     } else {
@@ -2417,26 +2442,52 @@ template <int NDIM, int P, typename B> struct Stepper {
     }
     const dim3 tiles((c->g.Ns[0] + fv2d::TX - 1) / fv2d::TX, (c->g.Ns[1] + fv2d::TY - 1) / fv2d::TY);
     if (!c->tape_replay) {
-      if (c->need_gate)
-        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
       const dim3 dtiles((c->g.Ns[0] + fv2d::DTX - 1) / fv2d::DTX,
                         (c->g.Ns[1] + fv2d::DTY - 1) / fv2d::DTY);
-      c->timed(KC_DETECT, s, [&] {
-        hipLaunchKernelGGL(fv2d::k_detect2d, dtiles, dim3(fv2d::DNT), 0, s, c->g, c->prm, c->Ucv,
-                           c->Unew, c->flags, c->need_gate ? c->any_flag : nullptr);
-      });
+      if (fused) {
+        // the smooth-extrema half of the detection, on tiles with candidates
+        if (c->prm.p > 1 && c->prm.sed)
+          c->timed(KC_DETECT, s, [&] {
+            hipLaunchKernelGGL(fv2d::k_sed2d, dtiles, dim3(fv2d::DNT), 0, s, c->g, c->prm,
+                               c->Unew, c->flags);
+          });
+      } else {
+        if (c->need_gate)
+          hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
+        c->timed(KC_DETECT, s, [&] {
+          hipLaunchKernelGGL(fv2d::k_detect2d, dtiles, dim3(fv2d::DNT), 0, s, c->g, c->prm,
+                             c->Ucv, c->Unew, c->flags, c->need_gate ? c->any_flag : nullptr);
+        });
+      }
       const dim3 gtiles((c->g.Ng[0] + fv2d::HTX - 1) / fv2d::HTX,
                         (c->g.Ng[1] + fv2d::HTY - 1) / fv2d::HTY);
       c->timed(KC_THETA1, s, [&] {
+        fv2d::FbMarks mk{nullptr, FT::TXc, FT::TYc, (c->g.N[0] + FT::EX - 1) / FT::EX,
+                         (c->g.N[1] + FT::EY - 1) / FT::EY};
+        if (adaptive) mk.marks = c->fbt_marks;
         hipLaunchKernelGGL(fv2d::k_theta2d, gtiles, dim3(fv2d::HNT), 0, s, c->g, c->prm,
-                           c->flags, c->Ucv, c->Unew, c->any_flag, c->theta);
+                           c->flags, c->Ucv, c->Unew, c->any_flag, c->theta, mk);
       });
     }
     tape_theta(c, i, s);
     tap(c, i, s);
     // the corrected state replaces the candidate in place
+    // blocks of the compacted fallback take listed tiles from a counter until
+    // the list is done
+    auto fbtiles = [&]() {
+      const unsigned nb = (unsigned)std::min<long long>(ntiles, 256 * 12);
+      c->timed(KC_FBFLUX, s, [&] {
+        if (c->cfg.riemann_fv != SPD_RS_LLF)
+          hipLaunchKernelGGL((fv2d::k_fbtile2d<P, RS_HLLC, B>), dim3(nb), dim3(256), 0, s, op,
+                             c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, c->theta,
+                             c->fbt_counters, c->fbt_list, dt, w, c->Unew, adaptive);
+        else
+          hipLaunchKernelGGL((fv2d::k_fbtile2d<P, RS_LLF, B>), dim3(nb), dim3(256), 0, s, op,
+                             c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, c->theta,
+                             c->fbt_counters, c->fbt_list, dt, w, c->Unew, adaptive);
+      });
+    };
     if (form == 3) {
-      const long long ntiles = fv2d::fbtile_count<P>(c->g);
       hipLaunchKernelGGL(fv2d::k_zero2, dim3(1), dim3(64), 0, s, c->fbt_counters);
       c->timed(KC_THETA1, s, [&] {
         hipLaunchKernelGGL((fv2d::k_fbtile_list<P>),
@@ -2444,20 +2495,19 @@ template <int NDIM, int P, typename B> struct Stepper {
                            dim3(64 * fv2d::FBL_WAVES), 0, s, c->g, c->prm, c->theta,
                            c->fbt_counters, c->fbt_list, ntiles);
       });
-      // blocks take listed tiles from a counter until the list is done
-      const unsigned nb = (unsigned)std::min<long long>(ntiles, 256 * 12);
-      c->timed(KC_FBFLUX, s, [&] {
-        if (c->cfg.riemann_fv != SPD_RS_LLF)
-          hipLaunchKernelGGL((fv2d::k_fbtile2d<P, RS_HLLC, B>), dim3(nb), dim3(256), 0, s, op,
-                             c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, c->theta,
-                             c->fbt_counters, c->fbt_list, dt, w, c->Unew);
-        else
-          hipLaunchKernelGGL((fv2d::k_fbtile2d<P, RS_LLF, B>), dim3(nb), dim3(256), 0, s, op,
-                             c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, c->theta,
-                             c->fbt_counters, c->fbt_list, dt, w, c->Unew);
-      });
+      fbtiles();
       std::swap(c->Ucv, c->Unew);
       return;
+    }
+    if (adaptive && !c->tape_replay) {
+      // the list of the tiles theta marked (its length decides the next
+      // substep's path); the tile kernel runs on it when this substep took
+      // the compacted path
+      c->timed(KC_THETA1, s, [&] {
+        hipLaunchKernelGGL(fv2d::k_fbmark_list, dim3(grid_for(ntiles)), dim3(256), 0, s, c->g,
+                           c->prm, c->fbt_marks, c->fbt_counters, c->fbt_list, ntiles);
+      });
+      fbtiles();
     }
     c->timed(KC_FBFLUX, s, [&] {
       const bool hllc = c->cfg.riemann_fv != SPD_RS_LLF;
@@ -2475,7 +2525,8 @@ template <int NDIM, int P, typename B> struct Stepper {
         auto go = [&](auto rs, auto st) {
           hipLaunchKernelGGL((fv2d::k_fbmarch2d<decltype(rs)::value, B, decltype(st)::value>),
                              strips, dim3(fv2d::MW * fv2d::MWAVES), 0, s, c->g, c->prm, c->Ucv,
-                             c->theta, Fh(c, 0), Fh(c, 1), dt, w, rows, c->Unew);
+                             c->theta, Fh(c, 0), Fh(c, 1), dt, w, rows, c->Unew,
+                             adaptive ? c->fbt_counters + 2 : nullptr);
         };
         using HL = std::integral_constant<int, RS_HLLC>;
         using LF = std::integral_constant<int, RS_LLF>;
@@ -2967,8 +3018,11 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
         const int ex = std::max(1, 32 / c->n), ey = std::max(1, 8 / c->n);
         const long long ntiles =
             (long long)((g.N[0] + ex - 1) / ex) * ((g.N[1] + ey - 1) / ey);
-        if ((rc = c->alloc(&c->fbt_counters, 2)) != SPD_OK) return rc;
+        if ((rc = c->alloc(&c->fbt_counters, 4)) != SPD_OK) return rc;
         if ((rc = c->alloc(&c->fbt_list, (size_t)ntiles)) != SPD_OK) return rc;
+        if ((rc = c->alloc(&c->fbt_marks, (size_t)ntiles)) != SPD_OK) return rc;
+        SPD_HIP_CHECK(hipMemset(c->fbt_counters, 0, 4 * sizeof(int)));
+        SPD_HIP_CHECK(hipMemset(c->fbt_marks, 0, (size_t)ntiles));
       }
       if (nd != 2 && (rc = alloc_bulk(&c->HS, (1 + nd) * V * n1)) != SPD_OK) return rc;
       if (nd != 2 && (rc = c->alloc(&c->rp, (size_t)(2 * g.Nc))) != SPD_OK) return rc;

@@ -81,21 +81,24 @@ def _fb2d(name):
     return len(kw["N"]) == 2 and kw.get("update", "SD") == "FV" and kw.get("FB", False)
 
 
-@pytest.mark.parametrize("form", [1, 3])
+@pytest.mark.parametrize("form", [1, 2, 3])
 @pytest.mark.parametrize("name", [n for n in NAMES if _fb2d(n)])
 def test_fallback_forms_vs_oracle(name, form):
-    """The 2-D fallback correction has three kernel forms: 1 cell tiles in
-    LDS and 2 (the default, which every other test runs) column strips in
-    registers, both over the high-order face-flux arrays in HBM and the
-    separate candidate kernel, and 3 the fused high-order flux + candidate
-    kernel with the fallback on the compacted list of active tiles.  Forms 1
-    and 3 meet the same bar against the oracle."""
+    """The 2-D FV substep has four kernel forms: 1 separate high-order flux
+    and candidate kernels, separate detection and the fallback on cell tiles
+    in LDS, all over the high-order face-flux arrays in HBM; 2 the fused
+    high-order flux + candidate + first-half detection kernel writing the
+    face-flux arrays and the fallback marching column strips over them; 3 the
+    fused kernel without face-flux arrays and the fallback on the compacted
+    list of active tiles; 4 (the default, which every other test runs) the
+    path of 2 or of 3 chosen per substep on the device.  Forms 1 to 3 meet
+    the same bar against the oracle."""
     from spd_amd import _lib
     print(f"[gpu parity fallback form {form}] {name}", flush=True)
     fx = load(name)
     s = gpu_sim(fx)
     _lib.check(s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], form), "spd_set_launch")
-    assert s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], 4) < 0
+    assert s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], 5) < 0
     o = oracle_sim(fx)
     for k in range(1, int(fx["nsteps"]) + 1):
         s.compute_dt()
@@ -235,12 +238,12 @@ def test_xcd_band_tile_orders_vs_oracle():
     assert relerr_fields(s.dm.U_sp, o.U_sp) < TOL, relerr_fields(s.dm.U_sp, o.U_sp)
 
 
-@pytest.mark.parametrize("forms", [(2, 1), (3, 1), (3, 2)])
+@pytest.mark.parametrize("forms", [(2, 1), (3, 1), (3, 2), (4, 2), (4, 3)])
 def test_fallback_forms_agree_on_large_grid(forms):
-    """The fallback forms pairwise (column strips with quiet-row skipping,
-    cell tiles, compacted tiles with the fused candidate) on a
-    Kelvin-Helmholtz grid of 256^2 cells without noise, one step at a time
-    from the same state: same flags, states within 1e-12."""
+    """The substep forms pairwise (column strips with quiet-row skipping,
+    cell tiles, compacted tiles with the fused candidate, the adaptive
+    choice) on a Kelvin-Helmholtz grid of 256^2 cells without noise, one step
+    at a time from the same state: same flags, states within 1e-12."""
     from spd_amd import _lib
     print(f"[gpu fallback forms 256^2] {forms}", flush=True)
     a, _ = _kh_sims(64)

@@ -98,14 +98,14 @@ def test_quadrants_large_grid_vs_oracle(N, p, bc, seed):
     assert 0 < n_trouble < 0.5 * 4 * 3 * ncell, n_trouble
 
 
-@pytest.mark.parametrize("form", [1, 3])
+@pytest.mark.parametrize("form", [1, 2, 3])
 @pytest.mark.parametrize("bc", ["gradfree", "reflective"])
 def test_quadrants_other_fallback_forms_large_grid_vs_oracle(bc, form):
-    """The other forms of the 2-D fallback on the (256, 24)-element grid: 1
-    cell tiles over the face-flux arrays in HBM, 3 the fused high-order flux +
-    candidate kernel with the fallback on the compacted list of active tiles
-    (tiles of 8 x 2 elements: 32 columns, so the list holds tiles with and
-    without active faces on this grid)."""
+    """The fixed forms of the 2-D FV substep on the (256, 24)-element grid
+    (the default adaptive form runs in the tests above): 1 cell tiles over the
+    face-flux arrays in HBM, 2 column strips over them, 3 the fallback on the
+    compacted list of active tiles (tiles of 8 x 2 elements: 32 columns, so
+    the list holds tiles with and without active faces on this grid)."""
     from spd_amd import _lib
     print(f"[gpu parity large, fallback form {form}] {bc}", flush=True)
     s, o = _pair((256, 24), 3, bc, 15 if bc == "gradfree" else 16)
