@@ -293,9 +293,11 @@ def run_ensemble(torch, n=64, steps=10, sizes=(64, 128, 256, 384), seed=0):
     tests/golden/ensemble_small_nonfinite.json, names the same members; it is
     reported beside the count); only the members that stay finite are timed.
     Both legs start from the same saved state (after one step that builds the
-    batched graph): the batched leg (one multi-branch HIP graph launch per
-    chunk of steps, autotuned launch shapes) and the members one after the
-    other with the host-driven loop (compute_dt + step, no snapshot writes)."""
+    batched graph): the batched leg (one HIP graph launch for the steps; the
+    members of each order share one launch per stage of the step, the groups
+    on separate capture branches) and the members one after the other with
+    the host-driven loop (compute_dt + step, no snapshot writes) and the same
+    kernels (substep form 2)."""
     import contextlib
     from spd_amd.ensemble import Ensemble, riemann_member_kwargs, riemann_members, timed
     specs = riemann_members(n, seed=seed, sizes=sizes)
@@ -324,7 +326,7 @@ def run_ensemble(torch, n=64, steps=10, sizes=(64, 128, 256, 384), seed=0):
         keep = [m for i, m in enumerate(specs) if i not in set(bad)]
         out["members_timed"] = len(keep)
         t0 = time.perf_counter()
-        ens = Ensemble([riemann_member_kwargs(m) for m in keep], graph_steps=4, branches=8,
+        ens = Ensemble([riemann_member_kwargs(m) for m in keep], graph_steps=steps, branches=8,
                        tune=True)
         setup = time.perf_counter() - t0
         dof = sum(ens.dof)

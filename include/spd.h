@@ -202,6 +202,27 @@ int spd_run_steps(spd_ctx* ctx, double* U_sp, int nsteps, void* stream);
 int spd_run_status(spd_ctx* ctx, double* time, long long* n_step, int* done, double* dt,
                    void* stream);
 
+/* Batched steps of several 2-D simulations of one order (an extension with
+ * no reference counterpart, for ensembles of small problems, where one launch
+ * per kernel and simulation costs more than the work).  spd_batch_run_steps
+ * advances every member by nsteps steps of its device-side loop, as
+ * spd_run_steps does for one context, with one launch per stage of the step
+ * for all members: the grid is the members' grids one after the other, and
+ * a block runs the kernel for the member its index falls in.  Members: 2-D,
+ * update = FV with the fallback, the same order, precision, Riemann solvers
+ * and kinds of sides (sizes, domains, states and parameters may differ); each
+ * member's state U_sp (a device array in the element layout) is fixed when
+ * the batch is made, and its device loop is begun with spd_run_begin before
+ * the steps (spd_run_status reads each member's time and step count).  A
+ * member's results equal those of spd_run_steps on its context with the
+ * substep form 2 (SPD_LAUNCH_FB_FORM), bit for bit.  The batch holds device
+ * launch tables (a few hundred bytes per member and stage); its members must
+ * outlive it; calls that overlap on a batch or on one of its members race. */
+typedef struct spd_batch spd_batch;
+int spd_batch_create(spd_ctx** members, double** U_sp, int count, spd_batch** out);
+int spd_batch_run_steps(spd_batch* batch, int nsteps, void* stream);
+int spd_batch_destroy(spd_batch* batch);
+
 /* Launch shapes that the kernels take at run time (an extension with no
  * reference counterpart; the autotuner of spd_amd.ensemble sets them):
  * elements per block of the predictor update and of the 2-D high-order

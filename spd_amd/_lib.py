@@ -32,6 +32,7 @@ EXPORTS = [
     "spd_ic_points", "spd_ic_mean", "spd_sp_to_regular",
     "spd_adj_step", "spd_adj_backward", "spd_adj_theta_count", "spd_swap_wave_max",
     "spd_compute_dt_static", "spd_set_boundary_values",
+    "spd_batch_create", "spd_batch_run_steps", "spd_batch_destroy",
 ]
 
 
@@ -148,6 +149,12 @@ def load():
     L.spd_adj_backward.restype = i
     L.spd_adj_theta_count.argtypes = [vp]
     L.spd_adj_theta_count.restype = ctypes.c_size_t
+    L.spd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(dp), i, ctypes.POINTER(vp)]
+    L.spd_batch_create.restype = i
+    L.spd_batch_run_steps.argtypes = [vp, i, vp]
+    L.spd_batch_run_steps.restype = i
+    L.spd_batch_destroy.argtypes = [vp]
+    L.spd_batch_destroy.restype = i
     _lib = L
     return L
 

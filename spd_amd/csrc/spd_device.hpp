@@ -296,17 +296,30 @@ struct Geom {
 // multiple of 8, XCD k takes the tile columns [k W, (k + 1) W), W = gridDim.x
 // / 8, in row order: a tile's neighbours run on its XCD one and W blocks
 // apart.  Other grids keep the launch order.
-__device__ __forceinline__ void xcd_band_tile(int& bx, int& by) {
-  const int nbx = gridDim.x;
+// Block index of a launch (x, y) in a grid of nx x ny blocks: the hardware's,
+// or inside a batched launch (one launch for several simulations) the
+// block's place in the grid of the simulation it works for.
+struct BlockIx {
+  int x, y, nx, ny;
+};
+__device__ __forceinline__ BlockIx hw_block() {
+  return BlockIx{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y};
+}
+
+__device__ __forceinline__ void xcd_band_tile(const BlockIx& bi, int& bx, int& by) {
+  const int nbx = bi.nx;
   if (nbx % 8 != 0) {
-    bx = blockIdx.x;
-    by = blockIdx.y;
+    bx = bi.x;
+    by = bi.y;
     return;
   }
-  const int b = blockIdx.y * nbx + blockIdx.x;
+  const int b = bi.y * nbx + bi.x;
   const int W = nbx >> 3, j = b >> 3;
   bx = (b & 7) * W + j % W;
   by = j / W;
+}
+__device__ __forceinline__ void xcd_band_tile(int& bx, int& by) {
+  xcd_band_tile(hw_block(), bx, by);
 }
 
 struct Params {

@@ -307,13 +307,12 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
 // smooth-extrema relaxation of k_detect2d to its candidates and writes their
 // final flags.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(DNT)
-k_sed2d(Geom g, Params prm, const double* __restrict__ Unew, uint8_t* __restrict__ flags) {
+__device__ __forceinline__ void k_sed2d_body(const BlockIx bi, Geom g, Params prm, const double* __restrict__ Unew, uint8_t* __restrict__ flags) {
   SPD_SKIP(g);
   __shared__ double wn[2][DNY][DNX];
   __shared__ uint8_t ab[DAX + DAY];
   int bx, by;
-  xcd_band_tile(bx, by);
+  xcd_band_tile(bi, bx, by);
   const int x0 = bx * DTX, y0 = by * DTY;
   const int Nx = g.Ns[0], Ny = g.Ns[1];
   const int x = x0 + threadIdx.x % DTX, y = y0 + threadIdx.x / DTX;
@@ -325,6 +324,10 @@ k_sed2d(Geom g, Params prm, const double* __restrict__ Unew, uint8_t* __restrict
   bool pt[4] = {(code & 2) != 0, false, false, (code & 4) != 0};
   sed_relax_tile(g, wn, ab, x0, y0, inside, pt);
   if (inside && (code & 6)) flags[(long long)y * Nx + x] = (pt[0] || pt[3] || (code & 1)) ? 1 : 0;
+}
+__global__ void __launch_bounds__(DNT)
+k_sed2d(Geom g, Params prm, const double* __restrict__ Unew, uint8_t* __restrict__ flags) {
+  k_sed2d_body(hw_block(), g, prm, Unew, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,8 +354,7 @@ struct FbMarks {
 };
 constexpr int MKT = 8;  // flag table of a block: MKT x MKT fallback tiles
 
-__global__ void __launch_bounds__(HNT)
-k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* __restrict__ Ucv,
+__device__ __forceinline__ void k_theta2d_body(const BlockIx bi, Geom g, Params prm, const uint8_t* __restrict__ flags, const double* __restrict__ Ucv,
           const double* __restrict__ Unew, const int* __restrict__ any_flag,
           double* __restrict__ theta, FbMarks mk) {
   SPD_SKIP(g);
@@ -363,7 +365,7 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
   // y = gy0 - NGH - 1 + j, j < HTY + 2 (-1 outside the grid or the table)
   __shared__ unsigned smk[2];
   __shared__ signed char mcol[HTX + 2], mrow[HTY + 2];
-  const int gx0 = blockIdx.x * HTX, gy0 = blockIdx.y * HTY;
+  const int gx0 = bi.x * HTX, gy0 = bi.y * HTY;
   const int NGx = g.Ng[0], NGy = g.Ng[1];
   // first fallback tile of the block's table (cells from gx0 - NGH - 1)
   const int mbx = mk.marks ? ((gx0 - NGH - 1 > 0) ? gx0 - NGH - 1 : 0) / mk.tx : 0;
@@ -530,6 +532,12 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
     if (mk.marks && !(th == 0.0)) mark(gx, gy);
   }
   if (mk.marks) flush_marks();
+}
+__global__ void __launch_bounds__(HNT)
+k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* __restrict__ Ucv,
+          const double* __restrict__ Unew, const int* __restrict__ any_flag,
+          double* __restrict__ theta, FbMarks mk) {
+  k_theta2d_body(hw_block(), g, prm, flags, Ucv, Unew, any_flag, theta, mk);
 }
 
 // ---------------------------------------------------------------------------
@@ -764,8 +772,7 @@ __device__ __forceinline__ B from_left(B v) {  // value of lane - 1 (lane 0: its
 }
 
 template <int RS, typename B, bool STORED>
-__global__ void __launch_bounds__(MW * MWAVES)
-k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
+__device__ __forceinline__ void k_fbmarch2d_body(const BlockIx bi, Geom g, Params prm, const double* __restrict__ Ucv,
             const double* __restrict__ theta, const B* __restrict__ Fx,
             const B* __restrict__ Fy, const double* __restrict__ dt_dev, double w_i,
             int MR, double* __restrict__ Uout, const int* __restrict__ gate) {
@@ -774,8 +781,8 @@ k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
   const int lane = threadIdx.x % MW;
   const int Nx = g.Ns[0], Ny = g.Ns[1];
   const int NGx = g.Ng[0];
-  const int x = blockIdx.x * MOUT - 2 + lane;
-  const int y0 = (blockIdx.y * MWAVES + threadIdx.x / MW) * MR;
+  const int x = bi.x * MOUT - 2 + lane;
+  const int y0 = (bi.y * MWAVES + threadIdx.x / MW) * MR;
   if (y0 >= Ny) return;  // a whole wave
   const int y1 = (y0 + MR < Ny) ? y0 + MR : Ny;
   const B gamma = (B)prm.gamma, mc2 = (B)prm.min_c2;
@@ -1023,6 +1030,14 @@ k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
   else
     march(std::true_type{});
 }
+template <int RS, typename B, bool STORED>
+__global__ void __launch_bounds__(MW * MWAVES)
+k_fbmarch2d(Geom g, Params prm, const double* __restrict__ Ucv,
+            const double* __restrict__ theta, const B* __restrict__ Fx,
+            const B* __restrict__ Fy, const double* __restrict__ dt_dev, double w_i,
+            int MR, double* __restrict__ Uout, const int* __restrict__ gate) {
+  k_fbmarch2d_body<RS, B, STORED>(hw_block(), g, prm, Ucv, theta, Fx, Fy, dt_dev, w_i, MR, Uout, gate);
+}
 
 // ---------------------------------------------------------------------------
 // High-order fluxes of time node t on the FV faces of both dimensions
@@ -1209,8 +1224,7 @@ template <int P> struct HocTile {
 };
 
 template <int P, typename B, bool DET>
-__global__ void __launch_bounds__(256)
-k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
+__device__ __forceinline__ void k_hocand2d_body(const BlockIx bi, Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
            const B* __restrict__ Ffx, const B* __restrict__ Ffy, const double* Ucv,
            const double* __restrict__ dt_dev, double w_i, double* Uout, B* __restrict__ Fhx,
            B* __restrict__ Fhy, uint8_t* __restrict__ flags, int* __restrict__ any_flag,
@@ -1225,7 +1239,7 @@ k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
   const int Nx = g.N[0], Ny = g.N[1], Nsx = g.Ns[0];
   // the block's elements: a run of at most E elements of one element row
   const int nseg = (Nx + E - 1) / E;
-  const int ey = blockIdx.x / nseg, ex0 = (blockIdx.x % nseg) * E;
+  const int ey = bi.x / nseg, ex0 = (bi.x % nseg) * E;
   const long long e0 = (long long)ey * Nx + ex0;
   const int ne = (Nx - ex0 < E) ? (Nx - ex0) : E;
   const int tid = threadIdx.x;
@@ -1471,6 +1485,15 @@ k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
     const int blk = __syncthreads_or(cand ? 1 : 0);
     if (blk && threadIdx.x == 0 && *(volatile int*)any_flag == 0) atomicOr(any_flag, 1);
   }
+}
+template <int P, typename B, bool DET>
+__global__ void __launch_bounds__(256)
+k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
+           const B* __restrict__ Ffx, const B* __restrict__ Ffy, const double* Ucv,
+           const double* __restrict__ dt_dev, double w_i, double* Uout, B* __restrict__ Fhx,
+           B* __restrict__ Fhy, uint8_t* __restrict__ flags, int* __restrict__ any_flag,
+           const int* __restrict__ fh_gate) {
+  k_hocand2d_body<P, B, DET>(hw_block(), op, g, prm, t, Uad, Ffx, Ffy, Ucv, dt_dev, w_i, Uout, Fhx, Fhy, flags, any_flag, fh_gate);
 }
 
 // ---------------------------------------------------------------------------

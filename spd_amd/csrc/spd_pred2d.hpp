@@ -72,8 +72,7 @@ template <int P, bool FIRST> struct Layout {
 // it, with the same operations in the same order (the same values): one node
 // of writes and one of reads instead of T each.
 template <int P, int RS, typename S, typename B, bool FIRST, bool SRC_CMP>
-__global__ void __launch_bounds__(256)
-k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
+__device__ __forceinline__ void k_pred2d_body(const BlockIx bi, Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
          const double* __restrict__ Usp, const double* __restrict__ dt_dev, B* __restrict__ out,
          int out_cmp) {
   static_assert(!(FIRST && SRC_CMP), "the first iteration reads U_sp");
@@ -89,7 +88,7 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
   const int Nx = g.N[0], Ny = g.N[1];
   const long long NE = g.NE;
   const int ntx = (Nx + TX - 1) / TX;
-  const int bx = (int)(blockIdx.x % ntx), by = (int)(blockIdx.x / ntx);
+  const int bx = (int)(bi.x % ntx), by = (int)(bi.x / ntx);
   const int tid = threadIdx.x;
   const int el = tid / (T * n), rem = tid % (T * n);
   const int t = rem / n, lp = rem % n;
@@ -328,6 +327,13 @@ k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
       out[((long long)(tn * V + v) * NE + e) * NL + l] = (B)(u0 - (double)acc * dt);
     }
   }
+}
+template <int P, int RS, typename S, typename B, bool FIRST, bool SRC_CMP>
+__global__ void __launch_bounds__(256)
+k_pred2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
+         const double* __restrict__ Usp, const double* __restrict__ dt_dev, B* __restrict__ out,
+         int out_cmp) {
+  k_pred2d_body<P, RS, S, B, FIRST, SRC_CMP>(hw_block(), op, g, prm, src, Tsrc, Usp, dt_dev, out, out_cmp);
 }
 
 }  // namespace pred2d

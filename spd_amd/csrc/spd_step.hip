@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -298,8 +299,7 @@ template <int P> struct FaceTile2 {
 };
 
 template <int P, int RS, typename S, typename B>
-__global__ void __launch_bounds__(256)
-k_face_sd2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __restrict__ Fx,
+__device__ __forceinline__ void k_face_sd2d_body(const BlockIx bi, Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __restrict__ Fx,
             B* __restrict__ Fy) {
   SPD_SKIP(g);
   constexpr int V = 4, n = P + 1, nf = P + 2, NL = n * n;
@@ -313,13 +313,13 @@ k_face_sd2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __re
   // on its XCD one and W blocks before it); otherwise launch order
   int t, txb, tyb;
   if (ntx % 8 == 0) {
-    const int W = ntx >> 3, j = (int)(blockIdx.x >> 3), jj = j % (W * nty);
+    const int W = ntx >> 3, j = (int)(bi.x >> 3), jj = j % (W * nty);
     t = j / (W * nty);
-    txb = (int)(blockIdx.x & 7) * W + jj % W;
+    txb = (int)(bi.x & 7) * W + jj % W;
     tyb = jj / W;
   } else {
-    t = (int)(blockIdx.x / (unsigned)(ntx * nty));
-    const int rb = (int)(blockIdx.x % (unsigned)(ntx * nty));
+    t = (int)(bi.x / (unsigned)(ntx * nty));
+    const int rb = (int)(bi.x % (unsigned)(ntx * nty));
     txb = rb % ntx;
     tyb = rb / ntx;
   }
@@ -479,6 +479,12 @@ k_face_sd2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __re
   for (int it = threadIdx.x; it < EX * n * EY; it += 256) yface(it / (EX * n), (it % (EX * n)) / n, it % n);
   if (ey0 + EY == Ny)
     for (int it = threadIdx.x; it < EX * n; it += 256) yface(EY, it / n, it % n);
+}
+template <int P, int RS, typename S, typename B>
+__global__ void __launch_bounds__(256)
+k_face_sd2d(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, B* __restrict__ Fx,
+            B* __restrict__ Fy) {
+  k_face_sd2d_body<P, RS, S, B>(hw_block(), op, g, prm, src, Fx, Fy);
 }
 
 // ---------------------------------------------------------------------------
@@ -1130,7 +1136,7 @@ __device__ __forceinline__ void tensor2d_segment(const Ops<P>& op, const Geom& g
       cw = nmax(cw, c);
     }
     cw = block_max(cw);
-    if (threadIdx.x == 0) partial[blockIdx.x] = cw;
+    if (threadIdx.x == 0) *partial = cw;  // the block's own slot
     if (in_copy) {
       for (int k = threadIdx.x; k < V * n * wc; k += 256) {
         const int v = k / (n * wc), r = k % (n * wc), el = r / NL, l = r % NL;
@@ -1164,21 +1170,27 @@ __device__ __forceinline__ void tensor2d_segment(const Ops<P>& op, const Geom& g
 
 // MODE: 1 sp -> cv, 0 cv -> sp (with the wave speed).  Blocks of 256 threads.
 template <int P, int MODE>
-__global__ void __launch_bounds__(256)
-k_tensor2d(Ops<P> op, Geom g, Params prm, const double* __restrict__ in, double* __restrict__ out,
+__device__ __forceinline__ void k_tensor2d_body(const BlockIx bi, Ops<P> op, Geom g, Params prm, const double* __restrict__ in, double* __restrict__ out,
            double* __restrict__ in_copy, double* __restrict__ Wcv_sd, double* __restrict__ partial) {
   SPD_SKIP(g);
   constexpr int EB = T2D<P>::EB;
   __shared__ typename T2D<P>::Tile su, st;  // input; after the x pass, then the output
   const int nbx = (g.N[0] + EB - 1) / EB;
-  const int ey = (int)(blockIdx.x / nbx), ex0 = (int)(blockIdx.x % nbx) * EB;
+  const int ey = (int)(bi.x / nbx), ex0 = (int)(bi.x % nbx) * EB;
   const int ne = (g.N[0] - ex0 < EB) ? g.N[0] - ex0 : EB;
+  double* pblk = partial ? partial + bi.x : partial;  // the block's maximum (cv -> sp)
   if (ne == EB)
-    tensor2d_segment<P, MODE, true>(op, g, prm, in, out, in_copy, Wcv_sd, partial, ey, ex0, ne, su,
+    tensor2d_segment<P, MODE, true>(op, g, prm, in, out, in_copy, Wcv_sd, pblk, ey, ex0, ne, su,
                                     st);
   else
-    tensor2d_segment<P, MODE, false>(op, g, prm, in, out, in_copy, Wcv_sd, partial, ey, ex0, ne,
+    tensor2d_segment<P, MODE, false>(op, g, prm, in, out, in_copy, Wcv_sd, pblk, ey, ex0, ne,
                                      su, st);
+}
+template <int P, int MODE>
+__global__ void __launch_bounds__(256)
+k_tensor2d(Ops<P> op, Geom g, Params prm, const double* __restrict__ in, double* __restrict__ out,
+           double* __restrict__ in_copy, double* __restrict__ Wcv_sd, double* __restrict__ partial) {
+  k_tensor2d_body<P, MODE>(hw_block(), op, g, prm, in, out, in_copy, Wcv_sd, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -1937,13 +1949,16 @@ k_wave_max(Geom g, Params prm, int NL, const double* __restrict__ W, double* __r
 }
 
 // Reduce the block partial maxima into *cmax (one block of 1024 threads).
-__global__ void __launch_bounds__(1024)
-k_reduce_max(const double* __restrict__ partial, long long n, double* cmax, const int* skip) {
+__device__ __forceinline__ void k_reduce_max_body(const BlockIx bi, const double* __restrict__ partial, long long n, double* cmax, const int* skip) {
   if (*skip) return;
   double v = 0.0;
   for (long long i = threadIdx.x; i < n; i += blockDim.x) v = nmax(v, partial[i]);
   v = block_max(v);
   if (threadIdx.x == 0) *cmax = v;
+}
+__global__ void __launch_bounds__(1024)
+k_reduce_max(const double* __restrict__ partial, long long n, double* cmax, const int* skip) {
+  k_reduce_max_body(hw_block(), partial, n, cmax, skip);
 }
 
 __global__ void k_swap_scalar(double* a, double* b) {
@@ -1974,9 +1989,9 @@ __global__ void k_run_init(RunState* st, int* skip, double time, double t_end, l
 
 // dt of the next step from the recorded maximum wave speed, with the
 // reference's clamp to t_end and its dt < 1e-14 stop.
-__global__ void k_run_dt(const double* cmax, double h, double cfl, RunState* st, double* dt_out,
+__device__ __forceinline__ void k_run_dt_body(const BlockIx bi, const double* cmax, double h, double cfl, RunState* st, double* dt_out,
                          int* skip) {
-  if (threadIdx.x || blockIdx.x) return;
+  if (threadIdx.x || bi.x) return;
   if (st->done) {
     *skip = 1;
     return;
@@ -2000,9 +2015,14 @@ __global__ void k_run_dt(const double* cmax, double h, double cfl, RunState* st,
   *dt_out = dt;
   *skip = 0;
 }
+__global__ void 
+k_run_dt(const double* cmax, double h, double cfl, RunState* st, double* dt_out,
+                         int* skip) {
+  k_run_dt_body(hw_block(), cmax, h, cfl, st, dt_out, skip);
+}
 
-__global__ void k_run_advance(RunState* st, const int* skip) {
-  if (threadIdx.x || blockIdx.x) return;
+__device__ __forceinline__ void k_run_advance_body(const BlockIx bi, RunState* st, const int* skip) {
+  if (threadIdx.x || bi.x) return;
   if (*skip) return;
   st->time = st->time + st->dt;
   st->n_step += 1;
@@ -2010,6 +2030,10 @@ __global__ void k_run_advance(RunState* st, const int* skip) {
     st->n_left -= 1;
     if (st->n_left == 0) st->done = 1;
   }
+}
+__global__ void 
+k_run_advance(RunState* st, const int* skip) {
+  k_run_advance_body(hw_block(), st, skip);
 }
 
 // Quadrature points of the initial-condition subcell means
@@ -2072,8 +2096,12 @@ k_ic_mean(Geom g, const double* __restrict__ vals, double w0, double w1,
   out[tid] = acc;
 }
 
-__global__ void k_zero_counters(int* any_flag) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *any_flag = 0;
+__device__ __forceinline__ void k_zero_counters_body(const BlockIx bi, int* any_flag) {
+  if (threadIdx.x == 0 && bi.x == 0) *any_flag = 0;
+}
+__global__ void 
+k_zero_counters(int* any_flag) {
+  k_zero_counters_body(hw_block(), any_flag);
 }
 
 template <int NDIM>
@@ -2811,7 +2839,93 @@ int static_scratch(unsigned char** out) {
 
 }  // namespace
 
+#include "spd_batch.hpp"
+
+namespace {
+
+int batch_build(spd_batch* b, int par) {
+  using namespace spd::batch;
+  const bool f32 = b->precision == SPD_PRECISION_FP32;
+  switch (b->p) {
+    case 1: return f32 ? Plan2D<1, float>::build(b, par) : Plan2D<1, double>::build(b, par);
+    case 2: return f32 ? Plan2D<2, float>::build(b, par) : Plan2D<2, double>::build(b, par);
+    case 3: return f32 ? Plan2D<3, float>::build(b, par) : Plan2D<3, double>::build(b, par);
+    case 4: return f32 ? Plan2D<4, float>::build(b, par) : Plan2D<4, double>::build(b, par);
+    case 5: return f32 ? Plan2D<5, float>::build(b, par) : Plan2D<5, double>::build(b, par);
+    default: return fail(SPD_E_INVALID, "no batched kernels for this order");
+  }
+}
+
+}  // namespace
+
 extern "C" {
+
+static int need_boundary_values(const spd_ctx* ctx);
+
+int spd_batch_create(spd_ctx** members, double** U_sp, int count, spd_batch** out) {
+  if (!members || !U_sp || !out || count < 1) return fail(SPD_E_INVALID, "invalid argument");
+  *out = nullptr;
+  const spd_ctx* c0 = members[0];
+  for (int i = 0; i < count; ++i) {
+    const spd_ctx* c = members[i];
+    if (!c || !U_sp[i]) return fail(SPD_E_INVALID, "null member or state");
+    if (c->ndim != 2 || c->cfg.update != SPD_UPDATE_FV || !c->cfg.FB)
+      return fail(SPD_E_INVALID, "batched steps need 2-D members with update=FV and the fallback");
+    if (c->p != c0->p || c->cfg.m != c0->cfg.m || c->cfg.precision != c0->cfg.precision ||
+        c->cfg.riemann_sd != c0->cfg.riemann_sd || c->cfg.riemann_fv != c0->cfg.riemann_fv ||
+        (c->g.stored != 0) != (c0->g.stored != 0) || c->need_gate != c0->need_gate)
+      return fail(SPD_E_INVALID,
+                  "batch members differ in order, precision, Riemann solvers or side kinds");
+    if (c->tape_on || c->tape_replay || c->prof_on || c->dbg_sub != -1)
+      return fail(SPD_E_INVALID, "a batch member is taping, profiling or tapping");
+    if (need_boundary_values(c) != SPD_OK) return SPD_E_INVALID;
+    for (int j = 0; j < i; ++j)
+      if (members[j] == c) return fail(SPD_E_INVALID, "a context appears twice in the batch");
+  }
+  std::unique_ptr<spd_batch> b(new spd_batch());
+  b->p = c0->p;
+  b->precision = c0->cfg.precision;
+  b->T = c0->T;
+  for (int i = 0; i < count; ++i) {
+    b->mem.push_back(members[i]);
+    b->usp.push_back(U_sp[i]);
+    b->ucv0.push_back(members[i]->Ucv);
+    b->unew0.push_back(members[i]->Unew);
+  }
+  int rc = batch_build(b.get(), 0);
+  if (rc == SPD_OK && (b->T % 2)) rc = batch_build(b.get(), 1);
+  if (rc != SPD_OK) return rc;
+  *out = b.release();
+  return SPD_OK;
+}
+
+int spd_batch_run_steps(spd_batch* b, int nsteps, void* stream) {
+  if (!b || nsteps < 0) return fail(SPD_E_INVALID, "invalid argument");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t nm = b->mem.size();
+  int par = (b->mem[0]->Ucv == b->ucv0[0]) ? 0 : 1;
+  for (size_t i = 0; i < nm; ++i)
+    if ((b->mem[i]->Ucv == b->ucv0[i] ? 0 : 1) != par)
+      return fail(SPD_E_INVALID, "batch members were stepped apart since the batch was made");
+  for (int k = 0; k < nsteps; ++k) {
+    for (auto& f : b->plan[par]) f(s);
+    if (b->T % 2) par ^= 1;
+  }
+  const int m = b->mem[0]->cfg.m;
+  for (size_t i = 0; i < nm; ++i) {
+    spd_ctx* c = b->mem[i];
+    c->Ucv = par ? b->unew0[i] : b->ucv0[i];
+    c->Unew = par ? b->ucv0[i] : b->unew0[i];
+    c->Uad_cur = ((m - 1) & 1) ? c->Uad2 : c->Uad;
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPD_OK : fail(SPD_E_HIP, hipGetErrorString(e));
+}
+
+int spd_batch_destroy(spd_batch* b) {
+  delete b;
+  return SPD_OK;
+}
 
 const char* spd_last_error(void) { return g_last_error.c_str(); }
 

@@ -155,31 +155,95 @@ def _lib_device():
     return DEVICE
 
 
+def _batch_key(sim, max_dof):
+    """Members with equal keys can share batched launches (spd_batch_create):
+    2-D, FV with the fallback, and the same order, precision, Riemann solvers
+    and kinds of sides; None for a member that cannot be batched or whose
+    degrees of freedom reach ``max_dof`` (such a member's own launches
+    already fill the GPU)."""
+    if sim.ndim != 2 or sim.update != "FV" or not sim.FB or _dof(sim) >= max_dof:
+        return None
+    sides = [k for d in sim.dims for k in sim.BC[d]]
+    stored = any(k in ("ic", "pressure") for k in sides)
+    gate = any(k != "periodic" for k in sides)
+    return (sim.p, sim.precision, sim.riemann_solver_sd_name, sim.riemann_solver_fv_name,
+            stored, gate)
+
+
 class Ensemble:
     """A set of simulators advanced together (see the module docstring).
 
     ``members``: list of keyword dicts for :class:`SDADER_Simulator` (each with
     its ``init_fct``).  ``graph_steps``: steps of every member per graph launch.
-    ``branches``: capture branches (streams) the members are spread over.
-    ``tune``: run the autotuner on every member configuration first."""
+    ``branches``: capture branches (streams) the launch groups are spread over.
+    ``batched``: members that can share launches (2-D, FV with the fallback,
+    same order, precision, Riemann solvers and kinds of sides, fewer than
+    ``batch_max_dof`` degrees of freedom) are advanced by one launch per stage
+    of the step for the whole group (spd_batch_run_steps; the substep form is
+    set to 2 on those members, whose results then equal their own runs with
+    that form bit for bit); the others, or all members without ``batched``,
+    by their own launches.  ``tune``: run the autotuner on every member
+    configuration that keeps its own launches."""
 
     def __init__(self, members, device=None, graph_steps=8, branches=4, tune=True,
-                 tuner=None):
+                 tuner=None, batched=True, batch_max_dof=1 << 20):
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None \
             else torch.device(device)
         self.sims = [SDADER_Simulator(device=self.device, **kw) for kw in members]
         self.graph_steps = int(graph_steps)
         self.branches = max(1, int(branches))
+        self.batched = bool(batched)
+        keys = [_batch_key(s, batch_max_dof) if self.batched else None for s in self.sims]
+        groups = {}
+        for i, k in enumerate(keys):
+            if k is not None:
+                groups.setdefault(k, []).append(i)
+        self.groups = list(groups.values())
+        self._solo = [i for i, k in enumerate(keys) if k is None]
+        for g in self.groups:
+            for i in g:
+                s = self.sims[i]
+                _lib.check(s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], 2),
+                           "spd_set_launch")
         self.tuner = tuner if tuner is not None else (Autotuner() if tune else None)
         self.tuning = {}
         if self.tuner is not None:
-            for s in self.sims:
+            for i in self._solo:
+                s = self.sims[i]
                 self.tuning[Autotuner.key(s)] = self.tuner.tune(s)
         self._streams = [torch.cuda.Stream(device=self.device) for _ in range(self.branches)]
         self._graph = None
+        self._batches = {}   # member indices -> (state pointers, spd_batch handle)
         self.graph_builds = 0
         self.build_seconds = 0.0
         self.replays = 0
+
+    def __del__(self):
+        try:
+            for _, h in self._batches.values():
+                _lib.load().spd_batch_destroy(h)
+        except Exception:
+            pass
+
+    def _batch(self, members):
+        """The spd_batch of a group, made again when a member's state tensor
+        has moved."""
+        key = tuple(members)
+        U = [self.sims[i].dm.device_value("U_sp") for i in members]
+        ptrs = tuple(u.data_ptr() for u in U)
+        old = self._batches.get(key)
+        if old is not None and old[0] == ptrs:
+            return old[1]
+        lib = _lib.load()
+        if old is not None:
+            lib.spd_batch_destroy(old[1])
+        n = len(members)
+        ctxs = (ctypes.c_void_p * n)(*[self.sims[i]._ctx for i in members])
+        states = (ctypes.c_void_p * n)(*ptrs)
+        h = ctypes.c_void_p()
+        _lib.check(lib.spd_batch_create(ctxs, states, n, ctypes.byref(h)), "spd_batch_create")
+        self._batches[key] = (ptrs, h)
+        return h
 
     # -- helpers ------------------------------------------------------------------
     @property
@@ -203,32 +267,55 @@ class Ensemble:
             loads[b] += cost[i]
         return groups
 
+    def _units(self, active):
+        """Launch units of the active members: each batched group whose
+        members are all active as one unit (a finished member's blocks leave
+        at once), each other member alone."""
+        act = set(active)
+        units = []
+        for g in self.groups:
+            if any(i in act for i in g):
+                units.append(("batch", tuple(g)))
+        grouped = {i for g in self.groups for i in g}
+        units += [("solo", (i,)) for i in active if i not in grouped]
+        return units
+
     def _graph_key(self, active):
-        return (tuple(active), self.graph_steps, self.branches,
-                tuple(self.sims[i].dm.device_value("U_sp").data_ptr() for i in active))
+        units = self._units(active)
+        return (tuple(units), self.graph_steps, self.branches,
+                tuple(self.sims[i].dm.device_value("U_sp").data_ptr()
+                      for _, ms in units for i in ms))
 
     def _build_graph(self, active, cost):
         t0 = _time.perf_counter()
         key = self._graph_key(active)
         self._graph = None
-        groups = self._assign(active, cost)
+        units = self._units(active)
+        handles = {u: self._batch(u[1]) for u in units if u[0] == "batch"}
+        ucost = [sum(cost[i] for i in ms) for _, ms in units]
+        branches = self._assign(range(len(units)), ucost)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.device(self.device), torch.cuda.graph(g):
             cap = torch.cuda.current_stream()
-            for b, members in enumerate(groups):
-                if not members:
+            for b, idx in enumerate(branches):
+                if not idx:
                     continue
                 st = self._streams[b]
                 st.wait_stream(cap)
                 with torch.cuda.stream(st):
-                    for i in members:
-                        s = self.sims[i]
+                    for j in idx:
+                        kind, ms = units[j]
+                        sh = ctypes.c_void_p(st.cuda_stream)
+                        if kind == "batch":
+                            _lib.check(_lib.load().spd_batch_run_steps(
+                                handles[units[j]], self.graph_steps, sh), "spd_batch_run_steps")
+                            continue
+                        s = self.sims[ms[0]]
                         U = s.dm.device_value("U_sp")
-                        _lib.check(s._lib.spd_run_steps(s._ctx, _lib.ptr(U), self.graph_steps,
-                                                        ctypes.c_void_p(st.cuda_stream)),
+                        _lib.check(s._lib.spd_run_steps(s._ctx, _lib.ptr(U), self.graph_steps, sh),
                                    "spd_run_steps")
-            for b, members in enumerate(groups):
-                if members:
+            for b, idx in enumerate(branches):
+                if idx:
                     cap.wait_stream(self._streams[b])
         self.graph_builds += 1
         self._graph = (key, g)
