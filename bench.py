@@ -129,14 +129,14 @@ def algorithmic_bytes(N, p, launches):
 # rocprofv3 names of the kernels of each class in the FP64 2-D HLLC workload,
 # with their launches per step
 CLASS_KERNELS = {
-    "k_fbface": [("k_fbmarch2d<1, double>", 4)],
+    "k_fbface": [("k_fbmarch2d<1, double, false>", 4), ("k_fbtile2d<3, 1, double>", 4)],
     "k_face_sd": [("k_face_sd2d<3, 1, double, double>", 1)],
     "k_pred": [("k_pred2d<3, 1, double, double, true, false>", 1),
                ("k_pred2d<3, 1, double, double, false, true>", 1),
                ("k_pred2d<3, 1, double, double, false, false>", 1)],
-    "k_detect": [("k_detect2d", 4)],
-    "k_hoflux": [("k_hocand2d<3, double>", 4)],
-    "k_theta1": [("k_theta2d", 4)],
+    "k_detect": [("k_sed2d", 4)],
+    "k_hoflux": [("k_hocand2d<3, double, true>", 4)],
+    "k_theta1": [("k_theta2d", 4), ("k_fbmark_list", 4)],
     "k_cv2sp": [("k_tensor2d<3, 0>", 1)],
     "k_tensor": [("k_tensor2d<3, 1>", 1)],
 }
@@ -485,6 +485,11 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=N
     return out, g
 
 
+def progress(msg):
+    """One progress line on stderr (the JSON result goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -555,6 +560,7 @@ def main():
         more step that measures the troubled fraction."""
         s = make_sim(precision)
         s.advance(args.warmup)
+        progress(f"{precision}: {args.warmup} warm-up steps")
         early = s.dm.device_value("U_sp").clone()
         s.prepare_graph(args.steps)
         torch.cuda.synchronize()
@@ -562,6 +568,8 @@ def main():
         s.advance_device(args.steps)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+        progress(f"{precision}: {args.steps} timed steps (device loop), "
+                 f"{elapsed / args.steps * 1e3:.3f} ms per step")
         timed_end = s.dm.device_value("U_sp").clone()
         h = make_sim(precision)
         h.advance(args.warmup)
@@ -570,6 +578,8 @@ def main():
         h.advance(args.steps)
         torch.cuda.synchronize()
         host_loop = time.perf_counter() - t0
+        progress(f"{precision}: the same steps with the host loop, "
+                 f"{host_loop / args.steps * 1e3:.3f} ms per step")
         same = bool(torch.equal(h.dm.device_value("U_sp"), timed_end))
         del h
         torch.cuda.empty_cache()
