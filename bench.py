@@ -25,15 +25,18 @@ HIP graph is reported as device_loop_ms_per_step.  The JSON line carries:
     relative L1 difference from the FP64 form after the same steps;
   * ensemble / ensemble_small_members: 64 four-quadrant Riemann problems
     (N in {64,128,256,384} resp. {16,32,64}, p in {2..5}, outflow), a fixed
-    number of steps per member, batched (multi-branch HIP graph, autotuned
-    launch shapes) against the members run one after the other;
-  * adjoint: the reverse-mode derivative on the 3-D Taylor-Green flow
-    (p = 3, FV+FB, HLLC, periodic [0, 2 pi]^3, seeded density perturbation of
-    amplitude 1e-2): gradient of the final kinetic energy with respect to the
-    initial density field through the checkpointed torch.autograd.Function,
-    its time against the forward run, and central differences of the
-    frozen-flag map on random directions; adjoint_fp32_bulk: the same in the
-    FP32-bulk form, its gradient compared with the FP64 form's;
+    number of steps per member, batched (members below 2^20 DOF share one
+    launch per stage of the step per order, the others keep their own
+    launches; one HIP graph) against the members run one after the other;
+  * adjoint: the reverse-mode derivative on the 3-D Taylor-Green flow at
+    128^3 elements (p = 3, FV+FB, HLLC, periodic [0, 2 pi]^3, seeded density
+    perturbation of amplitude 1e-2), a short run (4 steps, a checkpoint every
+    2): gradient of the final kinetic energy with respect to the initial
+    density field through the checkpointed torch.autograd.Function, its time
+    against the forward run, and central differences of the frozen-flag map
+    on 16 random directions; adjoint_fp32_bulk: the FP32-bulk form's
+    gradient at 64^3 (20 steps) against the FP64 form's
+    (adjoint_fp64_reference);
   * cpu_baseline: the NumPy oracle on a bounded sample, one host core.
 
 Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10]
@@ -388,7 +391,8 @@ def smooth_direction(torch, s, rng, modes=6, kmax=2):
     return out.contiguous()
 
 
-def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=None):
+def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=None,
+                fd_steps=(1e-7, 1e-8, 1e-9)):
     """Gradient of the final kinetic energy of the 3-D Taylor-Green flow with
     respect to the initial density field (``steps`` steps, a checkpoint every
     ``k``), timed against the forward run; central differences (steps 1e-7, 1e-8
@@ -415,6 +419,7 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=N
     U0[0] *= 1.0 + 1e-2 * (2 * torch.rand(U0[0].shape, generator=gen, device="cuda",
                                           dtype=torch.float64) - 1)
     W0 = s._pointwise("spd_cons_to_prims", U0)
+    del U0
     gam = s.gamma
 
     def cons(rho):
@@ -434,6 +439,7 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=N
         UN, _, dts = adj._forward(cons(W0[0]), steps, k, None)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
+    del UN
     rho = W0[0].clone().requires_grad_(True)
     torch.cuda.reset_peak_memory_stats()
     torch.cuda.synchronize()
@@ -442,6 +448,7 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=N
     J.backward()
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+    progress(f"adjoint {N}^3 {precision}: forward {t_fwd:.2f} s, gradient {t_grad:.2f} s")
     peak = torch.cuda.max_memory_allocated()
     g = rho.grad
     errs = []
@@ -464,11 +471,13 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=N
             return float((ke_pts(final(W0[0] + eps * d)) - ke_pts(final(W0[0] - eps * d))).sum())
 
         rng = np.random.default_rng(5)
-        for _ in range(n_fd):
+        for i in range(n_fd):
             d = smooth_direction(torch, s, rng)
             ad = float((g * d).sum())
-            e = [abs(dJ(d, eps) / (2 * eps) - ad) / abs(ad) for eps in (1e-7, 1e-8, 1e-9)]
+            e = [abs(dJ(d, eps) / (2 * eps) - ad) / abs(ad) for eps in fd_steps]
             errs.append(min(e))
+            progress(f"adjoint {N}^3 {precision}: direction {i + 1} of {n_fd}, "
+                     f"relative difference {errs[-1]:.2e}")
     out = {"workload": f"Taylor-Green 3-D {N}^3 p=3 FV+FB HLLC periodic, "
                        f"{precision.upper()}, density perturbation 1e-2, "
                        f"d(final kinetic energy)/d(initial density)",
@@ -509,10 +518,16 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint leg (for profiling) and print its JSON")
-    ap.add_argument("--adjoint-N", type=int, default=64)
-    ap.add_argument("--adjoint-steps", type=int, default=20)
-    ap.add_argument("--adjoint-k", type=int, default=5)
-    ap.add_argument("--adjoint-fd", type=int, default=2)
+    ap.add_argument("--adjoint-N", type=int, default=128,
+                    help="elements per side of the FP64 adjoint leg (with the FD check)")
+    ap.add_argument("--adjoint-steps", type=int, default=4)
+    ap.add_argument("--adjoint-k", type=int, default=2)
+    ap.add_argument("--adjoint-fd", type=int, default=16)
+    ap.add_argument("--adjoint-fp32-N", type=int, default=64,
+                    help="elements per side of the FP32-bulk adjoint leg and of the FP64 "
+                         "gradient it is compared with")
+    ap.add_argument("--adjoint-fp32-steps", type=int, default=20)
+    ap.add_argument("--adjoint-fp32-k", type=int, default=5)
     ap.add_argument("--t1-window", action="store_true",
                     help="also run the FP64 form to t = 1.0 (about 23k steps, 5 minutes) and "
                          "time the same number of steps there")
@@ -715,10 +730,17 @@ def main():
         if "ensemble" not in out:
             del s
             torch.cuda.empty_cache()
-        out["adjoint"], g64 = run_adjoint(torch, args.adjoint_N, args.adjoint_steps,
-                                          args.adjoint_k, args.adjoint_fd)
-        out["adjoint_fp32_bulk"], _ = run_adjoint(torch, args.adjoint_N, args.adjoint_steps,
-                                                  args.adjoint_k, 0, precision="fp32",
+        # the differentiation configuration's grid, a short run: the FP64 form
+        # with central differences of its frozen-flag map on random directions
+        out["adjoint"], g = run_adjoint(torch, args.adjoint_N, args.adjoint_steps,
+                                        args.adjoint_k, args.adjoint_fd, fd_steps=(1e-7, 1e-8))
+        del g
+        torch.cuda.empty_cache()
+        # the FP32-bulk form against the FP64 form's gradient on a grid where both
+        # contexts' tapes and reverse-sweep buffers fit one after the other
+        nf, sf, kf = args.adjoint_fp32_N, args.adjoint_fp32_steps, args.adjoint_fp32_k
+        out["adjoint_fp64_reference"], g64 = run_adjoint(torch, nf, sf, kf, 0)
+        out["adjoint_fp32_bulk"], _ = run_adjoint(torch, nf, sf, kf, 0, precision="fp32",
                                                   ref_grad=g64)
         del g64
     if not args.no_cpu_baseline:

@@ -162,6 +162,9 @@ class Adjoint:
     @_own_wave_max
     def _backward(self, U_bar, nsteps, k, ckpt, dts):
         Ub = self._check(U_bar).clone()
+        # the reverse sweep's buffers are allocated on first use: hand torch's
+        # cached blocks back to the device first
+        torch.cuda.empty_cache()
         starts = sorted(ckpt)
         for j0 in reversed(starts):
             j1 = min(j0 + k, nsteps)

@@ -1064,33 +1064,81 @@ int ensure_tape(spd_ctx* c) {
   return SPD_OK;
 }
 
+// Adjoint scratch.  The reverse sweep reads the tape and never the forward
+// step's own buffers, and the two never run at the same time (the taped
+// forward of a step, then its reverse sweep), so in the FP64 form the
+// cotangent arrays of the same size and type live in the forward's buffers:
+// the iterate cotangent in the predictor state, the element-face cotangents
+// in the element-face fluxes, the FV-face cotangents in the FV face fluxes,
+// the Hancock states in the forward's (1-D / 3-D), and the two FV-state
+// cotangents in the two FV state buffers (every form).  The rest is one pool
+// shared by the two phases of the sweep: the FV substeps (fallback
+// face-state cotangents, and what did not alias) and then the predictor
+// (dudt cotangent, right-trace cotangents), which never hold values at the
+// same time.  Within the predictor phase the dudt cotangent (read by the
+// flux-point reverse kernels) and the right-trace cotangents (written by the
+// element-face reverse kernels that follow) share their space too.
 int ensure_scratch(spd_ctx* c) {
   if (!c->adj) c->adj = new AdjScratch();
   AdjScratch* A = c->adj;
   if (A->ready) return SPD_OK;
   const long long it = (long long)c->T * c->V * c->g.NE * c->NL;
+  const bool f64 = c->cfg.precision == SPD_PRECISION_FP64;
+  const bool fv = c->cfg.update == SPD_UPDATE_FV, fb = fv && c->cfg.FB;
   int rc;
-  if ((rc = c->alloc(&A->Uadb, (size_t)it)) != SPD_OK) return rc;
-  if ((rc = c->alloc(&A->Db, (size_t)it)) != SPD_OK) return rc;
+  // phase 1 (FV substeps) and phase 2 (predictor) parts of the pool, in doubles
+  long long p1 = 0, p2 = 0;
+  const bool uadb_own = !(f64 && c->Uad);
+  const bool ffb_own = !f64;
+  const bool fhob_own = !(f64 && fv);
+  const bool hs_own = !(f64 && c->HS);
+  if (uadb_own) p1 += it;  // lives through both phases: first in the pool
+  long long qr = 0;
   for (int a = 0; a < c->ndim; ++a) {
-    if ((rc = c->alloc(&A->Ffb[a], (size_t)(c->T * c->V * plane_of(c, a)))) != SPD_OK) return rc;
-    if ((rc = c->alloc(&A->Qr[a], (size_t)(c->T * c->V * plane_of(c, a)))) != SPD_OK) return rc;
+    if (ffb_own) p1 += (long long)c->T * c->V * plane_of(c, a);
+    qr += (long long)c->T * c->V * plane_of(c, a);
   }
-  if (c->cfg.update == SPD_UPDATE_FV) {
-    for (int k = 0; k < 2; ++k)
-      if ((rc = c->alloc(&A->Ucvb[k], (size_t)(c->V * c->g.Nc))) != SPD_OK) return rc;
+  const long long live = p1;  // Uadb and Ffb: both phases
+  p2 = std::max(it, qr);      // Db, then Qr in the same space
+  if (fv) {
     for (int a = 0; a < c->ndim; ++a) {
-      const size_t nfc = (size_t)(c->V * nface_of(c, a));
-      if ((rc = c->alloc(&A->Fhob[a], nfc)) != SPD_OK) return rc;
-      if (c->cfg.FB) {
-        if ((rc = c->alloc(&A->WbL[a], nfc)) != SPD_OK) return rc;
-        if ((rc = c->alloc(&A->WbR[a], nfc)) != SPD_OK) return rc;
+      const long long nfc = (long long)c->V * nface_of(c, a);
+      if (fhob_own) p1 += nfc;
+      if (fb) p1 += 2 * nfc;  // WbL, WbR
+    }
+    if (fb && hs_own) p1 += (long long)(1 + c->ndim) * c->V * n1_of(c);
+    if (!c->Ucv) p1 += c->V * c->g.Nc;
+    if (!c->Unew) p1 += c->V * c->g.Nc;
+  }
+  const long long pool_n = std::max(p1, live + p2);
+  double* pool = nullptr;
+  if ((rc = c->alloc(&pool, (size_t)pool_n)) != SPD_OK) return rc;
+  double* q = pool;
+  auto take = [&](long long n) {
+    double* r = q;
+    q += n;
+    return r;
+  };
+  A->Uadb = uadb_own ? take(it) : (double*)c->Uad;
+  for (int a = 0; a < c->ndim; ++a)
+    A->Ffb[a] = ffb_own ? take((long long)c->T * c->V * plane_of(c, a)) : (double*)c->Fface[a];
+  double* phase = q;  // both phases start here
+  if (fv) {
+    A->Ucvb[0] = c->Ucv ? c->Ucv : take(c->V * c->g.Nc);
+    A->Ucvb[1] = c->Unew ? c->Unew : take(c->V * c->g.Nc);
+    for (int a = 0; a < c->ndim; ++a) {
+      const long long nfc = (long long)c->V * nface_of(c, a);
+      A->Fhob[a] = fhob_own ? take(nfc) : (double*)c->Fho[a];
+      if (fb) {
+        A->WbL[a] = take(nfc);
+        A->WbR[a] = take(nfc);
       }
     }
-    if (c->cfg.FB &&
-        (rc = c->alloc(&A->HS, (size_t)((1 + c->ndim) * c->V * n1_of(c)))) != SPD_OK)
-      return rc;
+    if (fb) A->HS = hs_own ? take((long long)(1 + c->ndim) * c->V * n1_of(c)) : (double*)c->HS;
   }
+  q = phase;
+  A->Db = q;
+  for (int a = 0; a < c->ndim; ++a) A->Qr[a] = take((long long)c->T * c->V * plane_of(c, a));
   for (int a = c->ndim; a < 3; ++a) {
     A->Ffb[a] = A->Ffb[0];
     A->Qr[a] = A->Qr[0];
