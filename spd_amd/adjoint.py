@@ -15,7 +15,9 @@ of every step; the backward pass re-runs each interval from its checkpoint
 (the step is deterministic, so the recomputed flags are the forward pass's),
 then walks the interval backwards, recording one step's tape (Picard
 iterates, FV substep states, blending coefficients) and running its reverse
-sweep.  Memory: ``nsteps / k + k`` states plus one step's tape.
+sweep.  Memory: ``nsteps / k + k`` states plus one step's tape; with
+``offload`` the states live in pinned host memory and two at a time on the
+device.
 
 Both forms are differentiated; the reverse sweep is FP64 in both.  In the
 FP32-bulk form the tape holds the form's FP32 predictor iterates, so the sweep
@@ -25,6 +27,7 @@ from __future__ import annotations
 
 import contextlib
 import functools
+import sys
 
 import torch
 
@@ -51,8 +54,13 @@ class Adjoint:
     simulator's state is not touched: neither its arrays nor the maximum wave
     speed its context records for the simulator's next dt."""
 
-    def __init__(self, sim):
+    def __init__(self, sim, offload=False, verbose=False):
         self.sim = sim
+        self.verbose = bool(verbose)  # one line per checkpoint interval (stderr)
+        # keep checkpoints and the recomputed states of an interval in host
+        # memory (the differentiation configuration's 128^3 grid: 30 states of
+        # 5.4 GB beside a 204 GiB context)
+        self.offload = bool(offload)
         self.lib = _lib.load()
         self.device = sim.device
         self.fb = sim.update == "FV" and bool(sim.FB)
@@ -134,6 +142,25 @@ class Adjoint:
         _lib.check(self.lib.spd_step(self.sim._ctx, _lib.ptr(U), None, None, _lib.ptr(dt), 0,
                                      self._stream), "spd_step")
 
+    def _keep(self, U, slot=None):
+        """A copy of the state U for later: on the device, or with ``offload``
+        in host memory (pinned, so that the copies run at the bus's rate;
+        ``slot``, a host tensor of the same shape, is reused when given)."""
+        if not self.offload:
+            return U.clone()
+        if slot is None:
+            slot = torch.empty(U.shape, dtype=U.dtype, pin_memory=True)
+        slot.copy_(U)
+        return slot
+
+    def _fetch(self, V, out=None):
+        """The kept state V on the device (``out`` reused when given)."""
+        if not self.offload:
+            return V if out is None else out.copy_(V)
+        if out is None:
+            return V.to(self.device)
+        return out.copy_(V)
+
     @_own_wave_max
     def _forward(self, U0, nsteps, k, dt):
         """Run nsteps from U0; returns (U_N, checkpoints, dts)."""
@@ -142,7 +169,9 @@ class Adjoint:
         ckpt = {}
         for s in range(nsteps):
             if s % k == 0:
-                ckpt[s] = U.clone()
+                ckpt[s] = self._keep(U)
+                if self.verbose and s:
+                    print(f"adjoint: forward step {s} of {nsteps}", file=sys.stderr, flush=True)
             d = dts[s:s + 1]
             if dt is not None:
                 d.fill_(float(dt))
@@ -166,15 +195,28 @@ class Adjoint:
         # cached blocks back to the device first
         torch.cuda.empty_cache()
         starts = sorted(ckpt)
+        slots = []   # host copies of an interval's states (offload), reused
+        Ud = None    # device copy of a kept state (offload)
         for j0 in reversed(starts):
             j1 = min(j0 + k, nsteps)
             states = [ckpt[j0]]
-            U = ckpt[j0].clone()
+            U = self._fetch(ckpt[j0]).clone() if not self.offload else ckpt[j0].to(self.device)
             for s in range(j0, j1 - 1):
                 self._advance(U, dts[s:s + 1])
-                states.append(U.clone())
+                i = s - j0
+                states.append(self._keep(U, slots[i] if i < len(slots) else None))
+                if self.offload and i >= len(slots):
+                    slots.append(states[-1])
+            del U
             for s in range(j1 - 1, j0 - 1, -1):
-                Ub = self.step_vjp(states[s - j0], dts[s:s + 1], Ub)
+                if self.offload:
+                    Ud = self._fetch(states[s - j0], Ud)
+                    Ub = self.step_vjp(Ud, dts[s:s + 1], Ub)
+                else:
+                    Ub = self.step_vjp(states[s - j0], dts[s:s + 1], Ub)
+            if self.verbose:
+                torch.cuda.synchronize(self.device)
+                print(f"adjoint: steps {j0}..{j1 - 1} reversed", file=sys.stderr, flush=True)
         return Ub
 
     def run(self, U0, nsteps, checkpoint_every=20, dt=None):

@@ -139,6 +139,13 @@ def test_checkpointed_gradient_of_kinetic_energy(name, nsteps, k):
         grads.append(rho.grad.clone())
     g = grads[0]
     assert float((g - grads[1]).abs().max() / g.abs().max()) < 1e-13
+    # the same with the checkpoints and interval states in pinned host memory
+    from spd_amd.adjoint import Adjoint
+    adj_h = Adjoint(s, offload=True)
+    rho = W0[0].clone().requires_grad_(True)
+    UN = adj_h.run(cons_from_density(rho, W0, gam, nd).contiguous(), nsteps, checkpoint_every=k)
+    kinetic_energy(UN, nd).backward()
+    assert float((g - rho.grad).abs().max() / g.abs().max()) < 1e-13
     # central differences of the same frozen map: the forward's dt and
     # coefficients of every step replayed
     with torch.no_grad():
