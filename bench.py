@@ -392,7 +392,7 @@ def smooth_direction(torch, s, rng, modes=6, kmax=2):
 
 
 def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=None,
-                fd_steps=(1e-7, 1e-8, 1e-9)):
+                fd_steps=(1e-7, 1e-8, 1e-9), offload=False):
     """Gradient of the final kinetic energy of the 3-D Taylor-Green flow with
     respect to the initial density field (``steps`` steps, a checkpoint every
     ``k``), timed against the forward run; central differences (steps 1e-7, 1e-8
@@ -401,8 +401,9 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=N
     random smooth directions (white-noise directions put many limiter
     arguments within the difference step of a kink at this size).  With
     ``precision="fp32"`` the FP32-bulk form is differentiated and its gradient
-    compared with ``ref_grad`` (the FP64 form's) instead.  Returns (summary,
-    gradient)."""
+    compared with ``ref_grad`` (the FP64 form's) instead.  With ``offload`` the
+    checkpoints and recomputed states live in host memory (the configuration's
+    200 steps at 128^3).  Returns (summary, gradient)."""
     import math
     import numpy as np
     from spd_amd import initial_conditions as ic
@@ -413,7 +414,7 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=N
                          zlim=(0, L), update="FV", FB=True, riemann_solver_sd="hllc",
                          riemann_solver_fv="hllc", device="cuda:0", verbose=False,
                          precision=precision)
-    adj = Adjoint(s)
+    adj = Adjoint(s, offload=offload, verbose=steps > 20)
     U0 = s.dm.device_value("U_sp").clone().contiguous()
     gen = torch.Generator(device="cuda").manual_seed(0)
     U0[0] *= 1.0 + 1e-2 * (2 * torch.rand(U0[0].shape, generator=gen, device="cuda",
@@ -481,7 +482,8 @@ def run_adjoint(torch, N=64, steps=20, k=5, n_fd=2, precision="fp64", ref_grad=N
     out = {"workload": f"Taylor-Green 3-D {N}^3 p=3 FV+FB HLLC periodic, "
                        f"{precision.upper()}, density perturbation 1e-2, "
                        f"d(final kinetic energy)/d(initial density)",
-           "steps": steps, "checkpoint_every": k, "dof": (N * 4) ** 3,
+           "steps": steps, "checkpoint_every": k, "checkpoints_in_host_memory": offload,
+           "dof": (N * 4) ** 3,
            "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
            "peak_torch_GiB": peak / 2 ** 30,
            "scratch_GiB": float(s._lib.spd_ctx_scratch_bytes(s._ctx)) / 2 ** 30,
@@ -523,6 +525,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=4)
     ap.add_argument("--adjoint-k", type=int, default=2)
     ap.add_argument("--adjoint-fd", type=int, default=16)
+    ap.add_argument("--adjoint-offload", action="store_true",
+                    help="keep the adjoint's checkpoints in host memory (with --adjoint-only)")
     ap.add_argument("--adjoint-fp32-N", type=int, default=64,
                     help="elements per side of the FP32-bulk adjoint leg and of the FP64 "
                          "gradient it is compared with")
@@ -553,7 +557,8 @@ def main():
     torch.cuda.set_device(dev)
     if args.adjoint_only:
         print(json.dumps({"adjoint": run_adjoint(torch, args.adjoint_N, args.adjoint_steps,
-                                                 args.adjoint_k, args.adjoint_fd)[0]}))
+                                                 args.adjoint_k, args.adjoint_fd,
+                                                 offload=args.adjoint_offload)[0]}))
         return
     dof = (N * (p + 1)) ** 2
     noise = ic.subcell_noise((N, N), p, [1, 2], 1e-3, seed=0)

@@ -16,8 +16,7 @@ of every step; the backward pass re-runs each interval from its checkpoint
 then walks the interval backwards, recording one step's tape (Picard
 iterates, FV substep states, blending coefficients) and running its reverse
 sweep.  Memory: ``nsteps / k + k`` states plus one step's tape; with
-``offload`` the states live in pinned host memory and two at a time on the
-device.
+``offload`` the states live in host memory and two at a time on the device.
 
 Both forms are differentiated; the reverse sweep is FP64 in both.  In the
 FP32-bulk form the tape holds the form's FP32 predictor iterates, so the sweep
@@ -144,12 +143,14 @@ class Adjoint:
 
     def _keep(self, U, slot=None):
         """A copy of the state U for later: on the device, or with ``offload``
-        in host memory (pinned, so that the copies run at the bus's rate;
-        ``slot``, a host tensor of the same shape, is reused when given)."""
+        in host memory (``slot``, a host tensor of the same shape, is reused
+        when given).  Host memory that the GPU box lets a process pin is a few
+        tens of GB, less than the 30 states of 5.4 GB the differentiation
+        configuration keeps, so the copies go through pageable memory."""
         if not self.offload:
             return U.clone()
         if slot is None:
-            slot = torch.empty(U.shape, dtype=U.dtype, pin_memory=True)
+            slot = torch.empty(U.shape, dtype=U.dtype)
         slot.copy_(U)
         return slot
 

@@ -139,7 +139,7 @@ def test_checkpointed_gradient_of_kinetic_energy(name, nsteps, k):
         grads.append(rho.grad.clone())
     g = grads[0]
     assert float((g - grads[1]).abs().max() / g.abs().max()) < 1e-13
-    # the same with the checkpoints and interval states in pinned host memory
+    # the same with the checkpoints and interval states in host memory
     from spd_amd.adjoint import Adjoint
     adj_h = Adjoint(s, offload=True)
     rho = W0[0].clone().requires_grad_(True)

@@ -21,9 +21,9 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def members(n, seed, sizes, orders):
+def members(n, seed, sizes, orders, precision="fp64"):
     from spd_amd.ensemble import riemann_member_kwargs, riemann_members
-    return [riemann_member_kwargs(m) for m in riemann_members(n, seed, sizes, orders)]
+    return [riemann_member_kwargs(m, precision) for m in riemann_members(n, seed, sizes, orders)]
 
 
 def solo(kw, form=0, **run):
@@ -100,6 +100,24 @@ def test_ensemble_matches_solo_runs(run, batched):
     o = oracle_run(i)
     assert relerr_fields(ens.sims[i].dm.U_sp, o.U_sp) < 1e-12, relerr_fields(ens.sims[i].dm.U_sp,
                                                                               o.U_sp)
+
+
+def test_batched_fp32_members_match_solo_runs():
+    """The batched kernels of the FP32-bulk form: members of three orders,
+    7 steps, against the members run alone with the same substep form (bit
+    for bit)."""
+    from spd_amd.ensemble import Ensemble
+    kws = members(6, 3, (8, 12, 16), (2, 3, 5), precision="fp32")
+    print("[ensemble batched fp32]", flush=True)
+    ens = Ensemble(kws, device="cuda:0", graph_steps=4, branches=3, batched=True)
+    assert len(ens.groups) == 3 and not ens._solo
+    ens.run(steps=7)
+    torch.cuda.synchronize()
+    for i, kw in enumerate(kws):
+        ref = solo(kw, form=2, steps=7)
+        s = ens.sims[i]
+        assert s.n_step == ref.n_step and s.time == ref.time, i
+        np.testing.assert_array_equal(s.dm.U_sp, ref.dm.U_sp)
 
 
 def test_nonfinite_members_are_the_reference_schemes():
