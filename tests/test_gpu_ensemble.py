@@ -116,7 +116,10 @@ def test_batched_fp32_members_match_solo_runs():
     for i, kw in enumerate(kws):
         ref = solo(kw, form=2, steps=7)
         s = ens.sims[i]
-        assert s.n_step == ref.n_step and s.time == ref.time, i
+        # (a member that the FP32 rounding drives to non-finite values does so
+        # in both runs, at the same step: NaN positions and time included)
+        assert s.n_step == ref.n_step, i
+        np.testing.assert_array_equal(np.float64(s.time), np.float64(ref.time))
         np.testing.assert_array_equal(s.dm.U_sp, ref.dm.U_sp)
 
 
