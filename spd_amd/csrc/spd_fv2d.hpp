@@ -1502,14 +1502,11 @@ k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
 // one of whose cells borders a face with a non-zero blending coefficient
 // (theta of the tile and of its one-cell ring without the corners; every
 // tile with godunov), and k_fbtile2d runs the MUSCL-Hancock + Riemann
-// fallback on the listed tiles only: the Hancock states of the cells next to
-// an active face (non-zero coefficient), at each active face the fallback
-// flux and the high-order flux, formed in LDS from the predictor slab of node
-// t and the element-face Riemann fluxes (the arithmetic of k_hocand2d) for
-// the face-column segments that hold an active face only, and in each cell
-// that borders an active face the candidate (already in U_out) corrected by
-// the flux changes theta (F_FB - F_HO): U_cv - dt_i sum(dF/dx) with the
-// blended fluxes, up to round-off (the other cells keep the candidate).  counters[0] counts the list, counters[1] hands
+// fallback on the listed tiles only: as k_fbcorrect2d, with the high-order
+// face fluxes formed in LDS from the predictor slab of node t and the
+// element-face Riemann fluxes (the arithmetic of k_hocand2d) instead of read
+// from HBM, and only the cells that border a blended face written (the other
+// cells keep the candidate).  counters[0] counts the list, counters[1] hands
 // out its entries to the blocks (both zeroed before k_fbtile_list).
 // ---------------------------------------------------------------------------
 template <int P> struct FbTile {
@@ -1737,27 +1734,14 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
     }
     __syncthreads();
 
-    // 3. high-order face fluxes of the active faces (non-zero coefficient, or
-    // every face with godunov) in the (now free) W buffer: the line fluxes of
-    // every face-column segment (one element's n lines at one flux point)
-    // holding an active face, then the transverse averages at the active
-    // faces.  x: the lines are staged in the y-face part, the averages go to
-    // bFx.
-    auto xact = [&](int fx, int ly) {  // x-face left of tile cell fx, row ly
-      return godunov || !(nmax(sT[ly + 1][fx], sT[ly + 1][fx + 1]) == 0.0);
-    };
-    auto yact = [&](int lx, int fy) {  // y-face below tile cell column lx, row fy
-      return godunov || !(nmax(sT[fy][lx + 1], sT[fy + 1][lx + 1]) == 0.0);
-    };
+    // 3. high-order face fluxes in the (now free) W buffer.  x: the line
+    // fluxes of every face column (staged in the y-face part), then the
+    // transverse averages into bFx.
     for (int k = threadIdx.x; k < NFX; k += blockDim.x) {
       const int fx = k % (TXc + 1), line = k / (TXc + 1);  // line = element row * n + lp
       const int x = x0 + fx, el_x = fx / n, f = fx % n;
       const int ey = ey0 + line / n, lp = line % n;
       if (x > Nx || ey >= NEy) continue;
-      bool seg = false;
-#pragma unroll
-      for (int q = 0; q < n; ++q) seg = seg || xact(fx, line - lp + q);
-      if (!seg) continue;
       B Fl[V];
       if (f == 0) {  // element face fa = ex0 + el_x
         const long long i0 = ((long long)ey * n + lp) * (NEx + 1) + ex0 + el_x;
@@ -1773,7 +1757,6 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
     __syncthreads();
     for (int k = threadIdx.x; k < NFX; k += blockDim.x) {
       const int fx = k % (TXc + 1), ly = k / (TXc + 1);
-      if (!xact(fx, ly)) continue;
       const int q = ly % n, row0 = ly - q;
 #pragma unroll
       for (int v = 0; v < V; ++v) {
@@ -1784,17 +1767,13 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
       }
     }
     __syncthreads();
-    // y: the line fluxes of the segments into bFy, the averages through
+    // y: the line fluxes of every face row into bFy, the averages through
     // registers (they overwrite the lines)
     for (int k = threadIdx.x; k < NFY; k += blockDim.x) {
       const int col = k % TXc, fy = k / TXc;  // col = element column * n + lp
       const int y = y0 + fy, el_y = fy / n, f = fy % n;
       const int ex = ex0 + col / n, lp = col % n;
       if (y > Ny || ex >= NEx) continue;
-      bool seg = false;
-#pragma unroll
-      for (int q = 0; q < n; ++q) seg = seg || yact(col - lp + q, fy);
-      if (!seg) continue;
       B Fl[V];
       if (f == 0) {  // element face fa = ey0 + el_y
         const long long i0 = (long long)(ey0 + el_y) * NEx * n + (long long)ex * n + lp;
@@ -1813,7 +1792,7 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
 #pragma unroll
     for (int it = 0; it < YIT; ++it) {
       const int k = threadIdx.x + it * 256;
-      if (k >= NFY || !yact(k % TXc, k / TXc)) continue;
+      if (k >= NFY) continue;
       const int lx = k % TXc, fy = k / TXc;
       const int q = lx % n, col0 = lx - q;
 #pragma unroll
@@ -1828,23 +1807,19 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
 #pragma unroll
     for (int it = 0; it < YIT; ++it) {
       const int k = threadIdx.x + it * 256;
-      if (k >= NFY || !yact(k % TXc, k / TXc)) continue;
+      if (k >= NFY) continue;
 #pragma unroll
       for (int v = 0; v < V; ++v) bFy[k * 4 + v] = avg[it][v];
     }
     __syncthreads();
 
-    // 4. per face, the change the fallback makes to the high-order flux:
-    // theta (F_FB - F_HO) at the active faces, 0 at the others
+    // 4. fallback fluxes where the coefficient is non-zero, blended in place
     for (int k = threadIdx.x; k < NFX; k += blockDim.x) {
       const int fx = k % (TXc + 1), ly = k / (TXc + 1);
       const int x = x0 + fx, y = y0 + ly;
+      if (x > Nx || y >= Ny) continue;
       const double th = godunov ? 1.0 : nmax(sT[ly + 1][fx], sT[ly + 1][fx + 1]);
-      if (x > Nx || y >= Ny || th == 0.0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) bFx[k * 4 + v] = B(0);
-        continue;
-      }
+      if (th == 0.0) continue;
       B WL[4], WR[4], UL[4], UR[4], Ff[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -1856,17 +1831,14 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
       riemann<2, RS>(WL, WR, UL, UR, Ff, 0, gamma, (B)prm.min_c2, prm.lowmach_fv);
       const B tb = (B)th;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) bFx[k * 4 + v] = tb * (Ff[v] - bFx[k * 4 + v]);
+      for (int v = 0; v < 4; ++v) bFx[k * 4 + v] = tb * Ff[v] + (B(1) - tb) * bFx[k * 4 + v];
     }
     for (int k = threadIdx.x; k < NFY; k += blockDim.x) {
       const int lx = k % TXc, fy = k / TXc;
       const int x = x0 + lx, y = y0 + fy;
+      if (x >= Nx || y > Ny) continue;
       const double th = godunov ? 1.0 : nmax(sT[fy][lx + 1], sT[fy + 1][lx + 1]);
-      if (x >= Nx || y > Ny || th == 0.0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) bFy[k * 4 + v] = B(0);
-        continue;
-      }
+      if (th == 0.0) continue;
       B WL[4], WR[4], UL[4], UR[4], Ff[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -1878,25 +1850,26 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
       riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, (B)prm.min_c2, prm.lowmach_fv);
       const B tb = (B)th;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) bFy[k * 4 + v] = tb * (Ff[v] - bFy[k * 4 + v]);
+      for (int v = 0; v < 4; ++v) bFy[k * 4 + v] = tb * Ff[v] + (B(1) - tb) * bFy[k * 4 + v];
     }
     __syncthreads();
 
-    // 5. the cells that border a blended face: the candidate (in U_out, from
-    // the high-order fluxes) corrected by the flux changes,
-    // U_cv - dt_i sum(dF_blend / dx) = U_cand - dt_i sum(d[theta (F_FB - F_HO)] / dx)
+    // 5. the cells that border a blended face
     for (int k = threadIdx.x; k < TXc * TYc; k += blockDim.x) {
       const int tx = k % TXc, ty = k / TXc;
       const int x = x0 + tx, y = y0 + ty;
       if (x >= Nx || y >= Ny) continue;
-      const bool touched = xact(tx, ty) || xact(tx + 1, ty) || yact(tx, ty) || yact(tx, ty + 1);
+      const bool touched = godunov || !(nmax(sT[ty + 1][tx], sT[ty + 1][tx + 1]) == 0.0) ||
+                           !(nmax(sT[ty + 1][tx + 1], sT[ty + 1][tx + 2]) == 0.0) ||
+                           !(nmax(sT[ty][tx + 1], sT[ty + 1][tx + 1]) == 0.0) ||
+                           !(nmax(sT[ty + 1][tx + 1], sT[ty + 2][tx + 1]) == 0.0);
       if (!touched) continue;
       const double rdx = g.rhfp[0][x + NGH], rdy = g.rhfp[1][y + NGH];
       const long long ci = (long long)y * Nx + x;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const double u = Uout[v * g.Nc + ci];
-        double d = 0.0;
+        const double u = Ucv[v * g.Nc + ci];
+        double d = u * 0.0;
         d = d + ((double)bFx[(ty * (TXc + 1) + tx + 1) * 4 + v] -
                  (double)bFx[(ty * (TXc + 1) + tx) * 4 + v]) * rdx;
         d = d + ((double)bFy[((ty + 1) * TXc + tx) * 4 + v] - (double)bFy[(ty * TXc + tx) * 4 + v]) *
