@@ -236,14 +236,18 @@ int spd_batch_destroy(spd_batch* batch);
  * their face fluxes again; 4, the default: per substep the path of 2 or of
  * 3, chosen on the device from the active tiles of the previous substep of
  * the same step; the forms agree to round-off, not bit for bit, so the
- * autotuner leaves the form alone).  value 0 restores the default; values
- * above the maximum (256 threads / 48 KiB of LDS per block; 4 forms) are
- * rejected.  spd_get_launch reports the value in effect and the maximum. */
+ * autotuner leaves the form alone), and the elements per block of the 2-D
+ * fused high-order flux + candidate kernel (at most (256 / (p+1)^2), the
+ * default; fewer give small grids more blocks; the results do not depend on
+ * it).  value 0 restores the default; values above the maximum (256 threads /
+ * 48 KiB of LDS per block; 4 forms) are rejected.  spd_get_launch reports the
+ * value in effect and the maximum. */
 enum spd_launch_key {
   SPD_LAUNCH_PRED_EPB = 0,
   SPD_LAUNCH_HO_EPB = 1,
   SPD_LAUNCH_FB_FORM = 2,
-  SPD_LAUNCH_COUNT = 3
+  SPD_LAUNCH_HOC_EPB = 3,
+  SPD_LAUNCH_COUNT = 4
 };
 int spd_set_launch(spd_ctx* ctx, int key, int value);
 int spd_get_launch(const spd_ctx* ctx, int key, int* value, int* max_value);

@@ -20,7 +20,7 @@ BC = {"periodic": 0, "gradfree": 1, "reflective": 2, "ic": 3, "pressure": 4}
 UPDATE = {"SD": 0, "FV": 1}
 PRECISION = {"fp64": 0, "fp32": 1}
 TAP_ALL = -2
-LAUNCH = {"pred_epb": 0, "ho_epb": 1, "fb_form": 2}
+LAUNCH = {"pred_epb": 0, "ho_epb": 1, "fb_form": 2, "hoc_epb": 3}
 
 EXPORTS = [
     "spd_last_error", "spd_abi_version", "spd_config_size", "spd_ctx_create", "spd_ctx_destroy",

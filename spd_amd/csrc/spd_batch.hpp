@@ -99,7 +99,7 @@ template <typename B> struct AHoc {
   B* Fhy;
   uint8_t* flags;
   int* any_flag;
-  int gx;
+  int Er, gx;
 };
 struct ASed {
   Geom g;
@@ -180,7 +180,7 @@ kb_hocand2d(Ops<P, B> op, int t, double w_i, const AHoc<B>* __restrict__ a,
   const AHoc<B>& A = a[m];
   fv2d::k_hocand2d_body<P, B, true>(place(start, m, A.gx), op, A.g, A.prm, t, A.Uad, A.Ffx, A.Ffy,
                                     A.Ucv, A.dt, w_i, A.Uout, A.Fhx, A.Fhy, A.flags, A.any_flag,
-                                    nullptr);
+                                    nullptr, A.Er);
 }
 __global__ void __launch_bounds__(fv2d::DNT)
 kb_sed2d(const ASed* __restrict__ a, const int* __restrict__ start, int nm) {
@@ -404,10 +404,12 @@ template <int P, typename B> struct Plan2D {
         std::vector<long long> nb(nm);
         for (int i = 0; i < nm; ++i) {
           spd_ctx* c = b->mem[i];
-          nb[i] = (long long)c->g.N[1] * ((c->g.N[0] + HT::E - 1) / HT::E);
+          const int v = c->launch[SPD_LAUNCH_HOC_EPB];
+          const int Eb = (v > 0 && v < HT::E) ? v : HT::E;
+          nb[i] = (long long)c->g.N[1] * ((c->g.N[0] + Eb - 1) / Eb);
           a[i] = AHoc<B>{geom(i), c->prm, dst(i, m - 1), (B*)c->Fface[0], (B*)c->Fface[1],
                          ucv(i, k), c->run_dt, unew(i, k), (B*)c->Fho[0], (B*)c->Fho[1],
-                         c->flags, c->need_gate ? c->any_flag : nullptr, (int)nb[i]};
+                         c->flags, c->need_gate ? c->any_flag : nullptr, Eb, (int)nb[i]};
         }
         long long tot = 0;
         const AHoc<B>* da = b->upload(a, &rc);

@@ -1228,7 +1228,7 @@ __device__ __forceinline__ void k_hocand2d_body(const BlockIx bi, Ops<P, B> op, 
            const B* __restrict__ Ffx, const B* __restrict__ Ffy, const double* Ucv,
            const double* __restrict__ dt_dev, double w_i, double* Uout, B* __restrict__ Fhx,
            B* __restrict__ Fhy, uint8_t* __restrict__ flags, int* __restrict__ any_flag,
-           const int* __restrict__ fh_gate) {
+           const int* __restrict__ fh_gate, int Er) {
   SPD_SKIP(g);
   // the face-flux arrays are written when given and, with a gate, when it is set
   if (fh_gate && *fh_gate == 0) Fhx = Fhy = nullptr;
@@ -1237,11 +1237,13 @@ __device__ __forceinline__ void k_hocand2d_body(const BlockIx bi, Ops<P, B> op, 
   B* slab = reinterpret_cast<B*>(spd_dyn_lds);  // [V][E][ho_nlp]
   B* col = slab + V * E * ho_nlp(P);            // [2][E][nf][n][V]
   const int Nx = g.N[0], Ny = g.N[1], Nsx = g.Ns[0];
-  // the block's elements: a run of at most E elements of one element row
-  const int nseg = (Nx + E - 1) / E;
-  const int ey = bi.x / nseg, ex0 = (bi.x % nseg) * E;
+  // the block's elements: a run of at most Eb <= E elements of one element
+  // row (Er: the elements per block chosen at run time, 0 = E)
+  const int Eb = (Er > 0 && Er < E) ? Er : E;
+  const int nseg = (Nx + Eb - 1) / Eb;
+  const int ey = bi.x / nseg, ex0 = (bi.x % nseg) * Eb;
   const long long e0 = (long long)ey * Nx + ex0;
-  const int ne = (Nx - ex0 < E) ? (Nx - ex0) : E;
+  const int ne = (Nx - ex0 < Eb) ? (Nx - ex0) : Eb;
   const int tid = threadIdx.x;
   const B gamma = (B)prm.gamma;
   const double dti = (*dt_dev) * w_i;
@@ -1492,8 +1494,8 @@ k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
            const B* __restrict__ Ffx, const B* __restrict__ Ffy, const double* Ucv,
            const double* __restrict__ dt_dev, double w_i, double* Uout, B* __restrict__ Fhx,
            B* __restrict__ Fhy, uint8_t* __restrict__ flags, int* __restrict__ any_flag,
-           const int* __restrict__ fh_gate) {
-  k_hocand2d_body<P, B, DET>(hw_block(), op, g, prm, t, Uad, Ffx, Ffy, Ucv, dt_dev, w_i, Uout, Fhx, Fhy, flags, any_flag, fh_gate);
+           const int* __restrict__ fh_gate, int Er) {
+  k_hocand2d_body<P, B, DET>(hw_block(), op, g, prm, t, Uad, Ffx, Ffy, Ucv, dt_dev, w_i, Uout, Fhx, Fhy, flags, any_flag, fh_gate, Er);
 }
 
 // ---------------------------------------------------------------------------

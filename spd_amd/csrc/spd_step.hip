@@ -2410,11 +2410,22 @@ template <int NDIM, int P, typename B> struct Stepper {
     return c->launch[SPD_LAUNCH_HO_EPB] > 0 ? std::min(mx, c->launch[SPD_LAUNCH_HO_EPB])
                                             : std::min(mx, dflt);
   }
+  // Elements per block of the fused k_hocand2d (at most HocTile<P>::E).
+  static int hoc_epb(const spd_ctx* c) {
+    constexpr int mx = fv2d::HocTile<P>::E;
+    return c->launch[SPD_LAUNCH_HOC_EPB] > 0 ? std::min(mx, c->launch[SPD_LAUNCH_HOC_EPB]) : mx;
+  }
   static int launch_value(const spd_ctx* c, int key) {
-    return key == SPD_LAUNCH_PRED_EPB ? epb(c) : key == SPD_LAUNCH_HO_EPB ? ho_epb(c) : fb_form(c);
+    return key == SPD_LAUNCH_PRED_EPB ? epb(c)
+           : key == SPD_LAUNCH_HO_EPB ? ho_epb(c)
+           : key == SPD_LAUNCH_HOC_EPB ? hoc_epb(c)
+                                       : fb_form(c);
   }
   static int launch_max(const spd_ctx* c, int key) {
-    return key == SPD_LAUNCH_PRED_EPB ? epb_max() : key == SPD_LAUNCH_HO_EPB ? ho_epb_max(c) : 4;
+    return key == SPD_LAUNCH_PRED_EPB ? epb_max()
+           : key == SPD_LAUNCH_HO_EPB ? ho_epb_max(c)
+           : key == SPD_LAUNCH_HOC_EPB ? fv2d::HocTile<P>::E
+                                       : 4;
   }
 
   // One FV substep with the 2-D tiled kernels; the corrected state goes to the
@@ -2435,8 +2446,8 @@ template <int NDIM, int P, typename B> struct Stepper {
       const bool det = c->cfg.FB && !c->tape_replay;
       const size_t lds = det ? (size_t)HT::template lds_bytes_det<B>(c->need_gate ? 4 : 2)
                              : (size_t)HT::template lds_bytes<B>();
-      const unsigned hblocks =
-          (unsigned)((long long)c->g.N[1] * ((c->g.N[0] + HT::E - 1) / HT::E));
+      const int Eb = hoc_epb(c);
+      const unsigned hblocks = (unsigned)((long long)c->g.N[1] * ((c->g.N[0] + Eb - 1) / Eb));
       if (det && c->need_gate)
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
       if (adaptive)
@@ -2451,11 +2462,11 @@ template <int NDIM, int P, typename B> struct Stepper {
         if (det)
           hipLaunchKernelGGL((fv2d::k_hocand2d<P, B, true>), dim3(hblocks), dim3(256), lds, s, op,
                              c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, dt, w, out, fx,
-                             fy, c->flags, c->need_gate ? c->any_flag : nullptr, gate);
+                             fy, c->flags, c->need_gate ? c->any_flag : nullptr, gate, Eb);
         else
           hipLaunchKernelGGL((fv2d::k_hocand2d<P, B, false>), dim3(hblocks), dim3(256), lds, s,
                              op, c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, dt, w, out,
-                             fx, fy, nullptr, nullptr, gate);
+                             fx, fy, nullptr, nullptr, gate, Eb);
       });
       if (!c->cfg.FB) return;
     } else {

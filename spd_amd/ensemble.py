@@ -12,10 +12,11 @@ synchronisation inside.  Members that reach their end keep replaying as no-ops
 rebuilt without them once the work left in the graph has shrunk.
 
 Launch shapes the kernels take at run time (elements per block of the
-predictor update and of the 2-D high-order flux kernel) are chosen per member
-configuration by :class:`Autotuner`, which times the candidate shapes on the
-member itself (state saved and restored around the trial steps) and caches the
-winner per (ndim, N, p, precision).
+predictor update, of the 2-D high-order flux kernel and of the 2-D fused
+high-order flux + candidate kernel) are chosen per member configuration by
+:class:`Autotuner`, which times the candidate shapes on the member itself
+(state saved and restored around the trial steps) and caches the winner per
+(ndim, N, p, precision); batched launches take each member's own shape.
 
 The members' numbers are those of the members run alone: the same kernels run
 on the same data in the same order within each member.
@@ -79,7 +80,7 @@ class _Saved:
 # Launch keys the autotuner sets: the ones that only regroup independent work
 # into blocks (bit-identical results).  The form of the 2-D fallback kernel
 # ("fb_form") changes the evaluation order of a few expressions and stays fixed.
-TUNED = ("pred_epb", "ho_epb")
+TUNED = ("pred_epb", "ho_epb", "hoc_epb")
 
 
 class Autotuner:
@@ -118,7 +119,7 @@ class Autotuner:
         sim.dm.switch_to(_lib_device())
         choice = {}
         for name, key in _lib.LAUNCH.items():
-            if name not in TUNED or (name == "ho_epb" and sim.ndim != 2):
+            if name not in TUNED or (name in ("ho_epb", "hoc_epb") and sim.ndim != 2):
                 continue
             cur, mx = ctypes.c_int(), ctypes.c_int()
             _lib.check(lib.spd_get_launch(ctx, key, ctypes.byref(cur), ctypes.byref(mx)))
@@ -183,7 +184,8 @@ class Ensemble:
     set to 2 on those members, whose results then equal their own runs with
     that form bit for bit); the others, or all members without ``batched``,
     by their own launches.  ``tune``: run the autotuner on every member
-    configuration that keeps its own launches."""
+    configuration that keeps its own launches (batched launches honour each
+    member's shapes, set with spd_set_launch)."""
 
     def __init__(self, members, device=None, graph_steps=8, branches=4, tune=True,
                  tuner=None, batched=True, batch_max_dof=1 << 20):
@@ -208,6 +210,11 @@ class Ensemble:
         self.tuner = tuner if tuner is not None else (Autotuner() if tune else None)
         self.tuning = {}
         if self.tuner is not None:
+            # members in batched groups keep the default shapes: a shape timed
+            # on a member alone (more, smaller blocks for a small grid) is not
+            # the best inside a batched launch whose members fill the GPU
+            # together (small-member ensemble: 4.9x with tuned members, 5.5x
+            # without)
             for i in self._solo:
                 s = self.sims[i]
                 self.tuning[Autotuner.key(s)] = self.tuner.tune(s)
