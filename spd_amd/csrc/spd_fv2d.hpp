@@ -396,9 +396,16 @@ __device__ __forceinline__ void k_theta2d_body(const BlockIx bi, Geom g, Params 
   };
   // after the block's last barrier: write the flagged tiles' marks
   auto flush_marks = [&]() {
-    if (mbits) {
-      if ((unsigned)mbits) atomicOr(&smk[0], (unsigned)mbits);
-      if ((unsigned)(mbits >> 32)) atomicOr(&smk[1], (unsigned)(mbits >> 32));
+    // OR over the wave first: one LDS atomic per wave and word
+    unsigned lo = (unsigned)mbits, hi = (unsigned)(mbits >> 32);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      lo |= __shfl_xor(lo, off);
+      hi |= __shfl_xor(hi, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      if (lo) atomicOr(&smk[0], lo);
+      if (hi) atomicOr(&smk[1], hi);
     }
     __syncthreads();
     const int t = threadIdx.x;
