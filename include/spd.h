@@ -234,9 +234,9 @@ int spd_batch_destroy(spd_batch* batch);
  * in registers over the face-flux arrays; 3: the same without the face-flux
  * arrays, the fallback runs on the compacted list of active tiles and forms
  * their face fluxes again; 4, the default: per substep the path of 2 or of
- * 3, chosen on the device from the active tiles of the previous substep of
- * the same step; the forms agree to round-off, not bit for bit, so the
- * autotuner leaves the form alone), and the elements per block of the 2-D
+ * 3, chosen on the device from the active tiles of the previous substep;
+ * forms 2, 3 and 4 give the same bits, form 1 agrees with them to round-off,
+ * so the autotuner leaves the form alone), and the elements per block of the 2-D
  * fused high-order flux + candidate kernel (at most (256 / (p+1)^2), the
  * default; fewer give small grids more blocks; the results do not depend on
  * it).  value 0 restores the default; values above the maximum (256 threads /

@@ -78,10 +78,18 @@ template <int NDIM> __device__ __forceinline__ constexpr int rolled(int a, int k
   return 1 + (a + k) % NDIM;
 }
 
+// The pointwise formulas below (primitives, conservatives, fluxes, sound
+// speed, Riemann solvers, the moncen limiter) are evaluated without
+// contraction into fused multiply-adds: each operation is rounded on its own,
+// so every kernel that inlines them forms the same bits from the same inputs
+// (the 2-D fallback's column-strip and compacted-tile paths rely on it), and
+// the rounding is that of the reference's NumPy expressions.
+
 // conservative -> primitive (src/hydro.py:48-86), rolled list of dim a.  The
 // FP32 form recovers the pressure from the total energy in FP64.
 template <int NDIM, typename R>
 __device__ __forceinline__ void prims(const R* U, R* W, int a, R gamma) {
+#pragma clang fp contract(off)
   constexpr int P = Dims<NDIM>::P;
   W[0] = U[0];
   if constexpr (IsF32<R>::v) {
@@ -113,6 +121,7 @@ __device__ __forceinline__ void prims(const R* U, R* W, int a, R gamma) {
 // primitive -> conservative (src/hydro.py:88-125), rolled list of dim a.
 template <int NDIM, typename R>
 __device__ __forceinline__ void cons(const R* W, R* U, int a, R gamma) {
+#pragma clang fp contract(off)
   constexpr int P = Dims<NDIM>::P;
   U[0] = W[0];
   R K = W[0] * R(0);
@@ -130,6 +139,7 @@ __device__ __forceinline__ void cons(const R* W, R* U, int a, R gamma) {
 // sum runs over the rolled list in reverse, so the normal component is last.
 template <int NDIM, typename R>
 __device__ __forceinline__ void flux(const R* W, R* F, int a, R gamma) {
+#pragma clang fp contract(off)
   constexpr int P = Dims<NDIM>::P;
   const int v1 = a + 1;
   R K = W[0] * R(0);
@@ -149,6 +159,7 @@ __device__ __forceinline__ void flux(const R* W, R* F, int a, R gamma) {
 
 // Sound speed with the where-clamp of src/hydro.py:3-46 (NaN -> min_c2).
 template <typename R> __device__ __forceinline__ R sound(R P, R rho, R gamma, R min_c2) {
+#pragma clang fp contract(off)
   R c2 = qdiv<R>(gamma * P, rho);
   c2 = (c2 > min_c2) ? c2 : min_c2;
   return qsqrt<R>(c2);
@@ -158,6 +169,7 @@ template <typename R> __device__ __forceinline__ R sound(R P, R rho, R gamma, R 
 template <int NDIM, typename R>
 __device__ __forceinline__ void llf(const R* WL, const R* WR, const R* UL, const R* UR, R* F,
                                     int a, R gamma, R min_c2) {
+#pragma clang fp contract(off)
   constexpr int V = Dims<NDIM>::V;
   constexpr int P = Dims<NDIM>::P;
   const int v1 = a + 1;
@@ -180,6 +192,7 @@ __device__ __forceinline__ void llf(const R* WL, const R* WR, const R* UL, const
 template <int NDIM, typename R>
 __device__ __forceinline__ void hllc(const R* WL, const R* WR, const R* UL, const R* UR, R* F,
                                      int a, R gamma, R min_c2, int low_mach) {
+#pragma clang fp contract(off)
   constexpr int P = Dims<NDIM>::P;
   const int v1 = a + 1;
   const R cL = sound<R>(WL[P], WL[0], gamma, min_c2) + fabs(WL[v1]);
@@ -249,6 +262,7 @@ template <typename R> __device__ __forceinline__ R minmod(R dL, R dR) {
   return (fabs(dR) < fabs(dL)) ? dR : dL;
 }
 template <typename R> __device__ __forceinline__ R moncen(R dL, R dR, R xL, R xR, R xM) {
+#pragma clang fp contract(off)
   const R dC = qdiv<R>(xL * dL + xR * dR, xL + xR);
   R slope = nmin(fabs(qdiv<R>(R(2) * dL * xL, xM)), fabs(qdiv<R>(R(2) * dR * xR, xM)));
   const R sg = (dC > R(0)) ? R(1) : ((dC < R(0)) ? R(-1) : dC);  // numpy.sign

@@ -548,6 +548,66 @@ k_theta2d(Geom g, Params prm, const uint8_t* __restrict__ flags, const double* _
 }
 
 // ---------------------------------------------------------------------------
+// The fallback's arithmetic per cell and per face (src/muscl.py:12-75,
+// 129-221, src/sdader_simulator.py:325-331, src/fv_simulator.py:189-205),
+// shared by every 2-D fallback kernel.  Nothing here is contracted into fused
+// multiply-adds, so each operation is rounded on its own whatever the kernel
+// around it keeps in registers, LDS or other lanes: the column-strip and the
+// compacted-tile fallbacks give the same bits, and a substep may take either.
+// ---------------------------------------------------------------------------
+// Limited slope of one variable along one axis from its two neighbours.
+template <typename B>
+__device__ __forceinline__ B fb_slope(B Wm, B Wc, B Wp, B rhl, B rhr, B hcl, B hcr, B hf,
+                                      int mc) {
+#pragma clang fp contract(off)
+  const B L = (Wc - Wm) * rhl, R = (Wp - Wc) * rhr;
+  return mc ? moncen<B>(L, R, hcl, hcr, hf) : minmod<B>(L, R);
+}
+// Offset of the face states from the Hancock state: half the slope times the
+// cell width.
+template <typename B>
+__device__ __forceinline__ B fb_half(B dM, B hf) {
+#pragma clang fp contract(off)
+  return B(0.5) * dM * hf;
+}
+// Hancock half-step state of a cell from its primitives and limited slopes.
+template <typename B>
+__device__ __forceinline__ void fb_hancock(const B* Wc, const B* dMx, const B* dMy, B gamma,
+                                           B dtib, B* Mp) {
+#pragma clang fp contract(off)
+  B dtW[4] = {B(0), B(0), B(0), B(0)};
+  const B rrho = qrcp<B>(Wc[0]);  // shared by both directions (within an ulp)
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int vel = b + 1;
+    const B* dW = (b == 0) ? dMx : dMy;
+    dtW[0] = dtW[0] - (Wc[vel] * dW[0] + Wc[0] * dW[vel]);
+    dtW[3] = dtW[3] - (Wc[vel] * dW[3] + gamma * Wc[3] * dW[vel]);
+    dtW[vel] = dtW[vel] - (Wc[vel] * dW[vel] + dW[3] * rrho);
+    dtW[2] = dtW[2] - Wc[vel] * dW[2];
+  }
+  dtW[3] = dtW[3] * B(0);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Mp[v] = Wc[v] + B(0.5) * dtW[v] * dtib;
+}
+// Blended face flux: theta times the fallback flux plus (1 - theta) times the
+// high-order flux.
+template <typename B>
+__device__ __forceinline__ B fb_blend(B tb, B Ff, B Fh) {
+#pragma clang fp contract(off)
+  return tb * Ff + (B(1) - tb) * Fh;
+}
+// Corrected state of a cell from its face fluxes (the expression of k_apply).
+__device__ __forceinline__ double fb_apply(double u, double FxL, double FxR, double FyB,
+                                           double FyT, double rdx, double rdy, double dti) {
+#pragma clang fp contract(off)
+  double d = u * 0.0;
+  d = d + (FxR - FxL) * rdx;
+  d = d + (FyT - FyB) * rdy;
+  return u - d * dti;
+}
+
+// ---------------------------------------------------------------------------
 // MUSCL-Hancock correction on a TX x TY tile of cells (src/muscl.py:61-75,
 // 129-221, src/fv_simulator.py:88-175,189-205, src/sdader_simulator.py:325-331):
 //   0. theta on tile + 1 halo; a tile none of whose faces has a non-zero
@@ -630,7 +690,7 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
     }
     if (!need) continue;
     const int wx = lx + 1, wy = ly + 1;  // position in W
-    B Wc[4], dMh[2][4];
+    B Wc[4], dMh[2][4], Mp[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) Wc[v] = W[v][wy][wx];
 #pragma unroll
@@ -643,27 +703,14 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
       for (int v = 0; v < 4; ++v) {
         const B Wm = (b == 0) ? W[v][wy][wx - 1] : W[v][wy - 1][wx];
         const B Wp = (b == 0) ? W[v][wy][wx + 1] : W[v][wy + 1][wx];
-        const B L = (Wc[v] - Wm) * rhl;
-        const B R = (Wp - Wc[v]) * rhr;
-        dMh[b][v] = prm.moncen ? moncen<B>(L, R, (B)g.hcv[b][G - 1], (B)g.hcv[b][G], hf)
-                               : minmod<B>(L, R);
-        sH[4 + 4 * b + v][ly][lx] = B(0.5) * dMh[b][v] * hf;
+        dMh[b][v] = fb_slope<B>(Wm, Wc[v], Wp, rhl, rhr, (B)g.hcv[b][G - 1], (B)g.hcv[b][G], hf,
+                                prm.moncen);
+        sH[4 + 4 * b + v][ly][lx] = fb_half<B>(dMh[b][v], hf);
       }
     }
-    B dtW[4] = {B(0), B(0), B(0), B(0)};
-    const B rrho = qrcp<B>(Wc[0]);  // shared by both directions (within an ulp)
+    fb_hancock<B>(Wc, dMh[0], dMh[1], gamma, dtib, Mp);
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int vel = b + 1;
-      const B* dW = dMh[b];
-      dtW[0] = dtW[0] - (Wc[vel] * dW[0] + Wc[0] * dW[vel]);
-      dtW[3] = dtW[3] - (Wc[vel] * dW[3] + gamma * Wc[3] * dW[vel]);
-      dtW[vel] = dtW[vel] - (Wc[vel] * dW[vel] + dW[3] * rrho);
-      dtW[2] = dtW[2] - Wc[vel] * dW[2];
-    }
-    dtW[3] = dtW[3] * B(0);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) sH[v][ly][lx] = Wc[v] + B(0.5) * dtW[v] * dtib;
+    for (int v = 0; v < 4; ++v) sH[v][ly][lx] = Mp[v];
   }
   __syncthreads();
 
@@ -693,7 +740,7 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
       riemann<2, RS>(WL, WR, UL, UR, Ff, 0, gamma, (B)prm.min_c2, prm.lowmach_fv);
       const B tb = (B)th;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) F[v] = tb * Ff[v] + (B(1) - tb) * F[v];
+      for (int v = 0; v < 4; ++v) F[v] = fb_blend<B>(tb, Ff[v], F[v]);
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) bFx[(ly * (TX + 1) + fx) * 4 + v] = F[v];
@@ -720,7 +767,7 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
       riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, (B)prm.min_c2, prm.lowmach_fv);
       const B tb = (B)th;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) F[v] = tb * Ff[v] + (B(1) - tb) * F[v];
+      for (int v = 0; v < 4; ++v) F[v] = fb_blend<B>(tb, Ff[v], F[v]);
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) bFy[(fy * TX + lx) * 4 + v] = F[v];
@@ -734,14 +781,11 @@ k_fbcorrect2d(Geom g, Params prm, const double* __restrict__ Ucv,
   const double rdx = g.rhfp[0][x + NGH], rdy = g.rhfp[1][y + NGH];
   const long long ci = (long long)y * Nx + x;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const double u = Ucv[v * g.Nc + ci];
-    double d = u * 0.0;
-    d = d + ((double)bFx[(ty * (TX + 1) + tx + 1) * 4 + v] -
-             (double)bFx[(ty * (TX + 1) + tx) * 4 + v]) * rdx;
-    d = d + ((double)bFy[((ty + 1) * TX + tx) * 4 + v] - (double)bFy[(ty * TX + tx) * 4 + v]) * rdy;
-    Uout[v * g.Nc + ci] = u - d * dti;
-  }
+  for (int v = 0; v < 4; ++v)
+    Uout[v * g.Nc + ci] = fb_apply(
+        Ucv[v * g.Nc + ci], (double)bFx[(ty * (TX + 1) + tx) * 4 + v],
+        (double)bFx[(ty * (TX + 1) + tx + 1) * 4 + v], (double)bFy[(ty * TX + tx) * 4 + v],
+        (double)bFy[((ty + 1) * TX + tx) * 4 + v], rdx, rdy, dti);
 }
 
 // ---------------------------------------------------------------------------
@@ -921,37 +965,16 @@ __device__ __forceinline__ void k_fbmarch2d_body(const BlockIx bi, Geom g, Param
       const int Gy = k + NGH;  // k in [-1, Ny]
       const B rhly = (B)g.rhcv[1][Gy - 1], rhry = (B)g.rhcv[1][Gy], hfy = (B)g.hfp[1][Gy];
       B Mp[4], Sx[4], Sy[4], dMx[4], dMy[4];
+      const B hcly = (B)g.hcv[1][Gy - 1], hcry = (B)g.hcv[1][Gy];
   #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const B Wl = from_left(W0[v]), Wr = from_right(W0[v]);
-        const B Lx = (W0[v] - Wl) * rhlx, Rx = (Wr - W0[v]) * rhrx;
-        const B Ly = (W0[v] - Wm[v]) * rhly, Ry = (Wp[v] - W0[v]) * rhry;
-        if (prm.moncen) {
-          dMx[v] = moncen<B>(Lx, Rx, hclx, hcrx, hfx);
-          dMy[v] = moncen<B>(Ly, Ry, (B)g.hcv[1][Gy - 1], (B)g.hcv[1][Gy], hfy);
-        } else {
-          dMx[v] = minmod<B>(Lx, Rx);
-          dMy[v] = minmod<B>(Ly, Ry);
-        }
-        Sx[v] = B(0.5) * dMx[v] * hfx;
-        Sy[v] = B(0.5) * dMy[v] * hfy;
+        dMx[v] = fb_slope<B>(Wl, W0[v], Wr, rhlx, rhrx, hclx, hcrx, hfx, prm.moncen);
+        dMy[v] = fb_slope<B>(Wm[v], W0[v], Wp[v], rhly, rhry, hcly, hcry, hfy, prm.moncen);
+        Sx[v] = fb_half<B>(dMx[v], hfx);
+        Sy[v] = fb_half<B>(dMy[v], hfy);
       }
-      {
-        B dtW[4] = {B(0), B(0), B(0), B(0)};
-        const B rrho = qrcp<B>(W0[0]);
-  #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          const int vel = b + 1;
-          const B* dW = (b == 0) ? dMx : dMy;
-          dtW[0] = dtW[0] - (W0[vel] * dW[0] + W0[0] * dW[vel]);
-          dtW[3] = dtW[3] - (W0[vel] * dW[3] + gamma * W0[3] * dW[vel]);
-          dtW[vel] = dtW[vel] - (W0[vel] * dW[vel] + dW[3] * rrho);
-          dtW[2] = dtW[2] - W0[vel] * dW[2];
-        }
-        dtW[3] = dtW[3] * B(0);
-  #pragma unroll
-        for (int v = 0; v < 4; ++v) Mp[v] = W0[v] + B(0.5) * dtW[v] * dtib;
-      }
+      fb_hancock<B>(W0, dMx, dMy, gamma, dtib, Mp);
       // lane + 1's lower x-face state, its theta
       B WRx[4];
   #pragma unroll
@@ -973,7 +996,7 @@ __device__ __forceinline__ void k_fbmarch2d_body(const BlockIx bi, Geom g, Param
           riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, mc2, prm.lowmach_fv);
           const B tb = (B)tf;
   #pragma unroll
-          for (int v = 0; v < 4; ++v) Fyk[v] = tb * Ff[v] + (B(1) - tb) * Fyk[v];
+          for (int v = 0; v < 4; ++v) Fyk[v] = fb_blend<B>(tb, Ff[v], Fyk[v]);
         }
         // update of row k - 1
         B FxL[4];
@@ -988,13 +1011,9 @@ __device__ __forceinline__ void k_fbmarch2d_body(const BlockIx bi, Geom g, Param
             const double rdy = g.rhfp[1][y + NGH];
             const long long ci = (long long)y * Nx + x;
   #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const double u = Ucv[v * g.Nc + ci];
-              double d = u * 0.0;
-              d = d + ((double)FxR[v] - (double)FxL[v]) * rdx;
-              d = d + ((double)Fyk[v] - (double)FyB[v]) * rdy;
-              Uout[v * g.Nc + ci] = u - d * dti;
-            }
+            for (int v = 0; v < 4; ++v)
+              Uout[v * g.Nc + ci] = fb_apply(Ucv[v * g.Nc + ci], (double)FxL[v], (double)FxR[v],
+                                             (double)FyB[v], (double)Fyk[v], rdx, rdy, dti);
           }
         }
   #pragma unroll
@@ -1013,7 +1032,7 @@ __device__ __forceinline__ void k_fbmarch2d_body(const BlockIx bi, Geom g, Param
           riemann<2, RS>(WL, WRx, UL, UR, Ff, 0, gamma, mc2, prm.lowmach_fv);
           const B tb = (B)tf;
   #pragma unroll
-          for (int v = 0; v < 4; ++v) Fxk[v] = tb * Ff[v] + (B(1) - tb) * Fxk[v];
+          for (int v = 0; v < 4; ++v) Fxk[v] = fb_blend<B>(tb, Ff[v], Fxk[v]);
         }
   #pragma unroll
         for (int v = 0; v < 4; ++v) FxR[v] = Fxk[v];
@@ -1540,8 +1559,7 @@ __global__ void k_zero2(int* c) {
 // kernel: counters[2] = 1 (column strips over the face-flux arrays) when
 // `force` is set or the previous substep listed more than `thr` active
 // tiles, else 0 (compacted tiles, no face-flux arrays); the list counters are
-// cleared.  The first substep of a step forces the column strips, so a
-// step's path depends on its own input only.
+// cleared.  A replayed tape (no tile marks) forces the column strips.
 __global__ void k_fb_decide(Geom g, int* counters, long long thr, int force) {
   SPD_SKIP(g);
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -1706,7 +1724,7 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
       }
       if (!need) continue;
       const int wx = lx + 1, wy = ly + 1;
-      B Wc[4], dMh[2][4];
+      B Wc[4], dMh[2][4], Mp[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) Wc[v] = W[v][wy][wx];
 #pragma unroll
@@ -1719,27 +1737,14 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
         for (int v = 0; v < 4; ++v) {
           const B Wm = (b == 0) ? W[v][wy][wx - 1] : W[v][wy - 1][wx];
           const B Wp = (b == 0) ? W[v][wy][wx + 1] : W[v][wy + 1][wx];
-          const B L = (Wc[v] - Wm) * rhl;
-          const B R = (Wp - Wc[v]) * rhr;
-          dMh[b][v] = prm.moncen ? moncen<B>(L, R, (B)g.hcv[b][G - 1], (B)g.hcv[b][G], hf)
-                                 : minmod<B>(L, R);
-          sH[4 + 4 * b + v][ly][lx] = B(0.5) * dMh[b][v] * hf;
+          dMh[b][v] = fb_slope<B>(Wm, Wc[v], Wp, rhl, rhr, (B)g.hcv[b][G - 1], (B)g.hcv[b][G],
+                                  hf, prm.moncen);
+          sH[4 + 4 * b + v][ly][lx] = fb_half<B>(dMh[b][v], hf);
         }
       }
-      B dtW[4] = {B(0), B(0), B(0), B(0)};
-      const B rrho = qrcp<B>(Wc[0]);
+      fb_hancock<B>(Wc, dMh[0], dMh[1], gamma, dtib, Mp);
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int vel = b + 1;
-        const B* dW = dMh[b];
-        dtW[0] = dtW[0] - (Wc[vel] * dW[0] + Wc[0] * dW[vel]);
-        dtW[3] = dtW[3] - (Wc[vel] * dW[3] + gamma * Wc[3] * dW[vel]);
-        dtW[vel] = dtW[vel] - (Wc[vel] * dW[vel] + dW[3] * rrho);
-        dtW[2] = dtW[2] - Wc[vel] * dW[2];
-      }
-      dtW[3] = dtW[3] * B(0);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) sH[v][ly][lx] = Wc[v] + B(0.5) * dtW[v] * dtib;
+      for (int v = 0; v < 4; ++v) sH[v][ly][lx] = Mp[v];
     }
     __syncthreads();
 
@@ -1840,7 +1845,7 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
       riemann<2, RS>(WL, WR, UL, UR, Ff, 0, gamma, (B)prm.min_c2, prm.lowmach_fv);
       const B tb = (B)th;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) bFx[k * 4 + v] = tb * Ff[v] + (B(1) - tb) * bFx[k * 4 + v];
+      for (int v = 0; v < 4; ++v) bFx[k * 4 + v] = fb_blend<B>(tb, Ff[v], bFx[k * 4 + v]);
     }
     for (int k = threadIdx.x; k < NFY; k += blockDim.x) {
       const int lx = k % TXc, fy = k / TXc;
@@ -1859,7 +1864,7 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
       riemann<2, RS>(WL, WR, UL, UR, Ff, 1, gamma, (B)prm.min_c2, prm.lowmach_fv);
       const B tb = (B)th;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) bFy[k * 4 + v] = tb * Ff[v] + (B(1) - tb) * bFy[k * 4 + v];
+      for (int v = 0; v < 4; ++v) bFy[k * 4 + v] = fb_blend<B>(tb, Ff[v], bFy[k * 4 + v]);
     }
     __syncthreads();
 
@@ -1876,15 +1881,11 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
       const double rdx = g.rhfp[0][x + NGH], rdy = g.rhfp[1][y + NGH];
       const long long ci = (long long)y * Nx + x;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const double u = Ucv[v * g.Nc + ci];
-        double d = u * 0.0;
-        d = d + ((double)bFx[(ty * (TXc + 1) + tx + 1) * 4 + v] -
-                 (double)bFx[(ty * (TXc + 1) + tx) * 4 + v]) * rdx;
-        d = d + ((double)bFy[((ty + 1) * TXc + tx) * 4 + v] - (double)bFy[(ty * TXc + tx) * 4 + v]) *
-                    rdy;
-        Uout[v * g.Nc + ci] = u - d * dti;
-      }
+      for (int v = 0; v < 4; ++v)
+        Uout[v * g.Nc + ci] = fb_apply(
+            Ucv[v * g.Nc + ci], (double)bFx[(ty * (TXc + 1) + tx) * 4 + v],
+            (double)bFx[(ty * (TXc + 1) + tx + 1) * 4 + v], (double)bFy[(ty * TXc + tx) * 4 + v],
+            (double)bFy[((ty + 1) * TXc + tx) * 4 + v], rdx, rdy, dti);
     }
   }
 }

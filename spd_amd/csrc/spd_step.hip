@@ -2383,8 +2383,11 @@ template <int NDIM, int P, typename B> struct Stepper {
   // arrays and the fallback on the compacted list of active tiles, which
   // forms the high-order fluxes of its tiles again; 4 (default) adaptive:
   // per substep the path of 2 or of 3, chosen on the device from the number
-  // of tiles the previous substep of the same step listed (the first substep
-  // takes the path of 2).  At Kelvin-Helmholtz 1024^2 p=3, FP64, form 3
+  // of tiles the previous substep listed (that of the previous step for a
+  // step's first substep; none before the first step).  Forms 2 and 3 give
+  // the same bits (one arithmetic for the fallback, no contraction into
+  // fused multiply-adds there), so the choice never changes a result.  At
+  // Kelvin-Helmholtz 1024^2 p=3, FP64, form 3
   // costs 15.4 against 12.9 ms per step in the first steps (99% of the tiles
   // active: the tile kernel forms every flux of a tile again) and 9.7 against
   // 10.6 ms after 220 steps (19% active: no face-flux arrays written).
@@ -2452,8 +2455,7 @@ template <int NDIM, int P, typename B> struct Stepper {
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c->any_flag);
       if (adaptive)
         hipLaunchKernelGGL(fv2d::k_fb_decide, dim3(1), dim3(64), 0, s, c->g, c->fbt_counters,
-                           (long long)(kDenseTiles * (double)ntiles),
-                           (i == 0 || c->tape_replay) ? 1 : 0);
+                           (long long)(kDenseTiles * (double)ntiles), c->tape_replay ? 1 : 0);
       const int* gate = adaptive ? c->fbt_counters + 2 : nullptr;
       c->timed(KC_HOFLUX, s, [&] {
         const bool wf = c->cfg.FB && (form == 2 || form == 4);

@@ -243,7 +243,8 @@ def test_fallback_forms_agree_on_large_grid(forms):
     """The substep forms pairwise (column strips with quiet-row skipping,
     cell tiles, compacted tiles with the fused candidate, the adaptive
     choice) on a Kelvin-Helmholtz grid of 256^2 cells without noise, one step
-    at a time from the same state: same flags, states within 1e-12."""
+    at a time from the same state: same flags, states within 1e-12 (forms
+    2, 3 and 4: the same bits)."""
     from spd_amd import _lib
     print(f"[gpu fallback forms 256^2] {forms}", flush=True)
     a, _ = _kh_sims(64)
@@ -259,3 +260,52 @@ def test_fallback_forms_agree_on_large_grid(forms):
         for x, y in zip(ra, rb):
             assert np.array_equal(x["troubles"], y["troubles"]), k
         assert relerr_fields(a.dm.U_sp, b.dm.U_sp) < TOL, (k, relerr_fields(a.dm.U_sp, b.dm.U_sp))
+        if 1 not in forms:
+            assert np.array_equal(np.asarray(a.dm.U_sp), np.asarray(b.dm.U_sp)), k
+
+
+def _forms_sim(kind, N, p, precision, limiter):
+    from spd_amd import initial_conditions as ic
+    from spd_amd.ensemble import riemann_member_kwargs, riemann_members
+    from spd_amd.simulator import SDADER_Simulator
+    if kind == "kh":
+        kw = dict(p=p, N=(N, N), BC=(("periodic", "periodic"),) * 2, update="FV", FB=True,
+                  riemann_solver_sd="hllc", riemann_solver_fv="hllc", slope_limiter=limiter,
+                  cfl_coeff=0.8, precision=precision)
+        s = SDADER_Simulator(init_fct=ic.kelvin_helmholtz, device="cuda:0", verbose=False, **kw)
+        s.perturb_initial_condition(ic.subcell_noise((N, N), p, [1, 2], 1e-3, seed=0))
+        return s
+    kw = riemann_member_kwargs(riemann_members(1, 7, (N,), (p,))[0], precision)
+    kw["slope_limiter"] = limiter
+    if kind == "quadrants-walls":
+        kw["BC"] = (("gradfree", "reflective"), ("reflective", "gradfree"))
+    return SDADER_Simulator(device="cuda:0", **kw)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("kind,N,p,limiter", [("kh", 64, 3, "minmod"), ("kh", 128, 3, "moncen"),
+                                              ("quadrants", 32, 5, "minmod"),
+                                              ("quadrants-walls", 48, 2, "moncen")])
+def test_fallback_forms_bit_identical(kind, N, p, limiter, precision):
+    """Forms 2 (column strips over stored high-order face fluxes), 3
+    (compacted tiles that form them again) and 4 (either, per substep, from
+    the previous substep's count of active tiles, across steps too) give the
+    same bits: the fallback's arithmetic is one set of uncontracted helpers.
+    20 free-running steps of a noise-seeded Kelvin-Helmholtz grid (dense
+    fallback first, sparse later) and of a four-quadrant Riemann problem with
+    gradient-free and reflective sides: identical states, times and step
+    counts."""
+    from spd_amd import _lib
+    print(f"[gpu fallback forms bit-identical] {kind} N={N} p={p} {limiter} {precision}",
+          flush=True)
+    out = []
+    for form in (2, 3, 4):
+        s = _forms_sim(kind, N, p, precision, limiter)
+        _lib.check(s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["fb_form"], form), "spd_set_launch")
+        s.perform_iterations(20)
+        U = np.asarray(s.dm.U_sp)
+        assert np.isfinite(U).all()
+        out.append((U, s.time, s.n_step))
+    for U, t, n in out[1:]:
+        assert np.array_equal(U, out[0][0])
+        assert t == out[0][1] and n == out[0][2]
