@@ -2395,8 +2395,10 @@ template <int NDIM, int P, typename B> struct Stepper {
     return c->launch[SPD_LAUNCH_FB_FORM] > 0 ? c->launch[SPD_LAUNCH_FB_FORM] : 4;
   }
   // Active-tile fraction above which the adaptive form takes the column
-  // strips (where the two paths cost the same at Kelvin-Helmholtz 1024^2).
-  static constexpr double kDenseTiles = 0.35;
+  // strips.  Kelvin-Helmholtz 1024^2 p=3 FP64, steps 10-210 from the seeded
+  // noise, ms per step: 0.35 11.03, 0.5 10.92, 0.65 10.89, 0.8 10.95,
+  // 0.9 10.98.
+  static constexpr double kDenseTiles = 0.65;
   static bool fused_ho(const spd_ctx* c) { return !c->cfg.FB || fb_form(c) != 1; }
   // Elements per block of k_ho2d (LDS slab of the block's elements, at most
   // 48 KiB).
