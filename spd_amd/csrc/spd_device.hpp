@@ -78,45 +78,54 @@ template <int NDIM> __device__ __forceinline__ constexpr int rolled(int a, int k
   return 1 + (a + k) % NDIM;
 }
 
-// The pointwise formulas below (primitives, conservatives, fluxes, sound
-// speed, Riemann solvers, the moncen limiter) are evaluated without
-// contraction into fused multiply-adds: each operation is rounded on its own,
-// so every kernel that inlines them forms the same bits from the same inputs
-// (the 2-D fallback's column-strip and compacted-tile paths rely on it), and
-// the rounding is that of the reference's NumPy expressions.
+// Evaluation without contraction into fused multiply-adds.  The conservative
+// form, sound speed, Riemann solvers and moncen limiter below always round
+// each operation on its own; the primitives and the Euler flux come in two
+// forms, prims / flux (contracted: the predictor's volume terms, the faces,
+// the detection) and prims_rn / flux_rn (each operation rounded).  Every
+// kernel that inlines an uncontracted formula forms the same bits from the
+// same inputs: the 2-D fallback's column-strip and compacted-tile paths take
+// the high-order face fluxes, the old primitives and the fallback fluxes from
+// these forms and give the same results.  Their rounding is that of the
+// reference's NumPy expressions.
 
 // conservative -> primitive (src/hydro.py:48-86), rolled list of dim a.  The
-// FP32 form recovers the pressure from the total energy in FP64.
+// FP32 form recovers the pressure from the total energy in FP64.  The FP64
+// form takes one reciprocal of the density for all velocity components
+// (within an ulp of the per-component divisions of src/hydro.py:79-81).
+#define SPD_PRIMS_BODY                                                   \
+  constexpr int P = Dims<NDIM>::P;                                       \
+  W[0] = U[0];                                                           \
+  if constexpr (IsF32<R>::v) {                                           \
+    const float r = __builtin_amdgcn_rcpf(U[0]);                         \
+    double K = 0.0;                                                      \
+    _Pragma("unroll") for (int k = 0; k < NDIM; ++k) {                   \
+      const int v = rolled<NDIM>(a, k);                                  \
+      W[v] = U[v] * r;                                                   \
+      K = K + (double)U[v] * (double)W[v];                               \
+    }                                                                    \
+    W[P] = (float)(((double)gamma - 1.0) * ((double)U[P] - 0.5 * K));    \
+  } else {                                                               \
+    const R rr = R(1.0) / U[0];                                          \
+    R K = U[0] * R(0.0);                                                 \
+    _Pragma("unroll") for (int k = 0; k < NDIM; ++k) {                   \
+      const int v = rolled<NDIM>(a, k);                                  \
+      W[v] = U[v] * rr;                                                  \
+      K = K + W[v] * W[v];                                               \
+    }                                                                    \
+    K = K * (R(0.5) * U[0]);                                             \
+    W[P] = (gamma - R(1.0)) * (U[P] - K);                                \
+  }
 template <int NDIM, typename R>
 __device__ __forceinline__ void prims(const R* U, R* W, int a, R gamma) {
-#pragma clang fp contract(off)
-  constexpr int P = Dims<NDIM>::P;
-  W[0] = U[0];
-  if constexpr (IsF32<R>::v) {
-    const float r = __builtin_amdgcn_rcpf(U[0]);
-    double K = 0.0;
-#pragma unroll
-    for (int k = 0; k < NDIM; ++k) {
-      const int v = rolled<NDIM>(a, k);
-      W[v] = U[v] * r;
-      K = K + (double)U[v] * (double)W[v];
-    }
-    W[P] = (float)(((double)gamma - 1.0) * ((double)U[P] - 0.5 * K));
-  } else {
-    // one reciprocal of the density for all velocity components (within an
-    // ulp of the per-component divisions of src/hydro.py:79-81)
-    const R rr = R(1.0) / U[0];
-    R K = U[0] * R(0.0);
-#pragma unroll
-    for (int k = 0; k < NDIM; ++k) {
-      const int v = rolled<NDIM>(a, k);
-      W[v] = U[v] * rr;
-      K = K + W[v] * W[v];
-    }
-    K = K * (R(0.5) * U[0]);
-    W[P] = (gamma - R(1.0)) * (U[P] - K);
-  }
+  SPD_PRIMS_BODY
 }
+template <int NDIM, typename R>
+__device__ __forceinline__ void prims_rn(const R* U, R* W, int a, R gamma) {
+#pragma clang fp contract(off)
+  SPD_PRIMS_BODY
+}
+#undef SPD_PRIMS_BODY
 
 // primitive -> conservative (src/hydro.py:88-125), rolled list of dim a.
 template <int NDIM, typename R>
@@ -137,25 +146,31 @@ __device__ __forceinline__ void cons(const R* W, R* U, int a, R gamma) {
 
 // Euler flux along dim a from primitives (src/hydro.py:127-170): the kinetic
 // sum runs over the rolled list in reverse, so the normal component is last.
+#define SPD_FLUX_BODY                                                    \
+  constexpr int P = Dims<NDIM>::P;                                       \
+  const int v1 = a + 1;                                                  \
+  R K = W[0] * R(0);                                                     \
+  R m = R(0);                                                            \
+  _Pragma("unroll") for (int k = NDIM - 1; k >= 0; --k) {                \
+    const int v = rolled<NDIM>(a, k);                                    \
+    m = W[0] * W[v];                                                     \
+    K = K + m * W[v];                                                    \
+    F[v] = m * W[v1];                                                    \
+  }                                                                      \
+  const R E = W[P] * inv_gm1<R>(gamma) + R(0.5) * K;                     \
+  F[0] = m;                                                              \
+  F[v1] = m * W[v1] + W[P];                                              \
+  F[P] = W[v1] * (E + W[P]);
 template <int NDIM, typename R>
 __device__ __forceinline__ void flux(const R* W, R* F, int a, R gamma) {
-#pragma clang fp contract(off)
-  constexpr int P = Dims<NDIM>::P;
-  const int v1 = a + 1;
-  R K = W[0] * R(0);
-  R m = R(0);
-#pragma unroll
-  for (int k = NDIM - 1; k >= 0; --k) {
-    const int v = rolled<NDIM>(a, k);
-    m = W[0] * W[v];
-    K = K + m * W[v];
-    F[v] = m * W[v1];
-  }
-  const R E = W[P] * inv_gm1<R>(gamma) + R(0.5) * K;
-  F[0] = m;
-  F[v1] = m * W[v1] + W[P];
-  F[P] = W[v1] * (E + W[P]);
+  SPD_FLUX_BODY
 }
+template <int NDIM, typename R>
+__device__ __forceinline__ void flux_rn(const R* W, R* F, int a, R gamma) {
+#pragma clang fp contract(off)
+  SPD_FLUX_BODY
+}
+#undef SPD_FLUX_BODY
 
 // Sound speed with the where-clamp of src/hydro.py:3-46 (NaN -> min_c2).
 template <typename R> __device__ __forceinline__ R sound(R P, R rho, R gamma, R min_c2) {
@@ -174,8 +189,8 @@ __device__ __forceinline__ void llf(const R* WL, const R* WR, const R* UL, const
   constexpr int P = Dims<NDIM>::P;
   const int v1 = a + 1;
   R FL[V], FR[V];
-  flux<NDIM, R>(WL, FL, a, gamma);
-  flux<NDIM, R>(WR, FR, a, gamma);
+  flux_rn<NDIM, R>(WL, FL, a, gamma);
+  flux_rn<NDIM, R>(WR, FR, a, gamma);
   const R cL = sound<R>(WL[P], WL[0], gamma, min_c2) + fabs(WL[v1]);
   const R cR = sound<R>(WR[P], WR[0], gamma, min_c2) + fabs(WR[v1]);
   const R cmax = (cL > cR) ? cL : cR;
@@ -463,7 +478,7 @@ __device__ __forceinline__ void load_W(const double* __restrict__ U, const Geom&
   double u[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) u[v] = U[v * g.Nc + idx];
-  prims<NDIM>(u, W, 0, gamma);
+  prims_rn<NDIM>(u, W, 0, gamma);
 #pragma unroll
   for (int a = 0; a < NDIM; ++a)
     if (flip[a]) W[a + 1] = -W[a + 1];

@@ -869,7 +869,7 @@ __device__ __forceinline__ void k_fbmarch2d_body(const BlockIx bi, Geom g, Param
   auto to_W = [&](B* W, int y) {
     double Wv[4] = {qnan(), qnan(), qnan(), qnan()};
     if (okn) {
-      prims<2>(un, Wv, 0, prm.gamma);
+      prims_rn<2>(un, Wv, 0, prm.gamma);
       if (flxn) Wv[1] = -Wv[1];
       if (flyn) Wv[2] = -Wv[2];
       if constexpr (STORED) {
@@ -1191,12 +1191,12 @@ __device__ __forceinline__ void ho_fp_flux(const Ops<P, B>& op, B gamma, int f, 
 #pragma unroll
     for (int s = 0; s < n; ++s) {
       const int l = (A == 0) ? (lp * n + s) : (s * n + lp);
-      acc += op.s2f[f][s] * ld(v, l);
+      acc = fma(op.s2f[f][s], ld(v, l), acc);
     }
     M[v] = acc;
   }
-  prims<2>(M, W, A, gamma);
-  flux<2>(W, Fl, A, gamma);
+  prims_rn<2>(M, W, A, gamma);
+  flux_rn<2>(W, Fl, A, gamma);
 }
 
 // ---------------------------------------------------------------------------
@@ -1360,7 +1360,7 @@ __device__ __forceinline__ void k_hocand2d_body(const BlockIx bi, Ops<P, B> op, 
     for (int v = 0; v < V; ++v) {
       B acc = B(0);
 #pragma unroll
-      for (int s2 = 0; s2 < n; ++s2) acc += op.s2c[q][s2] * col[(base + s2) * V + v];
+      for (int s2 = 0; s2 < n; ++s2) acc = fma(op.s2c[q][s2], col[(base + s2) * V + v], acc);
       av[it][v] = acc;
     }
   }
@@ -1776,7 +1776,8 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
       for (int v = 0; v < V; ++v) {
         B acc = B(0);
 #pragma unroll
-        for (int s = 0; s < n; ++s) acc += op.s2c[q][s] * bFy[((row0 + s) * (TXc + 1) + fx) * 4 + v];
+        for (int s = 0; s < n; ++s)
+          acc = fma(op.s2c[q][s], bFy[((row0 + s) * (TXc + 1) + fx) * 4 + v], acc);
         bFx[k * 4 + v] = acc;
       }
     }
@@ -1813,7 +1814,7 @@ k_fbtile2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
       for (int v = 0; v < V; ++v) {
         B acc = B(0);
 #pragma unroll
-        for (int s = 0; s < n; ++s) acc += op.s2c[q][s] * bFy[(fy * TXc + col0 + s) * 4 + v];
+        for (int s = 0; s < n; ++s) acc = fma(op.s2c[q][s], bFy[(fy * TXc + col0 + s) * 4 + v], acc);
         avg[it][v] = acc;
       }
     }

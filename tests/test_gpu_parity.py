@@ -268,8 +268,10 @@ def _forms_sim(kind, N, p, precision, limiter):
     from spd_amd import initial_conditions as ic
     from spd_amd.ensemble import riemann_member_kwargs, riemann_members
     from spd_amd.simulator import SDADER_Simulator
-    if kind == "kh":
-        kw = dict(p=p, N=(N, N), BC=(("periodic", "periodic"),) * 2, update="FV", FB=True,
+    if kind in ("kh", "kh-walls"):
+        bc = (("periodic", "periodic"), ("reflective", "reflective")) if kind == "kh-walls" \
+            else (("periodic", "periodic"),) * 2
+        kw = dict(p=p, N=(N, N), BC=bc, update="FV", FB=True,
                   riemann_solver_sd="hllc", riemann_solver_fv="hllc", slope_limiter=limiter,
                   cfl_coeff=0.8, precision=precision)
         s = SDADER_Simulator(init_fct=ic.kelvin_helmholtz, device="cuda:0", verbose=False, **kw)
@@ -277,24 +279,22 @@ def _forms_sim(kind, N, p, precision, limiter):
         return s
     kw = riemann_member_kwargs(riemann_members(1, 7, (N,), (p,))[0], precision)
     kw["slope_limiter"] = limiter
-    if kind == "quadrants-walls":
-        kw["BC"] = (("gradfree", "reflective"), ("reflective", "gradfree"))
     return SDADER_Simulator(device="cuda:0", **kw)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("kind,N,p,limiter", [("kh", 64, 3, "minmod"), ("kh", 128, 3, "moncen"),
                                               ("quadrants", 32, 5, "minmod"),
-                                              ("quadrants-walls", 48, 2, "moncen")])
+                                              ("kh-walls", 48, 2, "moncen")])
 def test_fallback_forms_bit_identical(kind, N, p, limiter, precision):
     """Forms 2 (column strips over stored high-order face fluxes), 3
     (compacted tiles that form them again) and 4 (either, per substep, from
     the previous substep's count of active tiles, across steps too) give the
     same bits: the fallback's arithmetic is one set of uncontracted helpers.
     20 free-running steps of a noise-seeded Kelvin-Helmholtz grid (dense
-    fallback first, sparse later) and of a four-quadrant Riemann problem with
-    gradient-free and reflective sides: identical states, times and step
-    counts."""
+    fallback first, sparse later; periodic, or reflective walls across y)
+    and of a four-quadrant Riemann problem with gradient-free sides:
+    identical states, times and step counts."""
     from spd_amd import _lib
     print(f"[gpu fallback forms bit-identical] {kind} N={N} p={p} {limiter} {precision}",
           flush=True)
