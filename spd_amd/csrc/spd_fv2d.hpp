@@ -1249,7 +1249,9 @@ template <int P> struct HocTile {
   }
 };
 
-template <int P, typename B, bool DET>
+// FULL: every block holds E elements (E per block and a row length that is a
+// multiple of E); the index arithmetic then divides by constants only.
+template <int P, typename B, bool DET, bool FULL = false>
 __device__ __forceinline__ void k_hocand2d_body(const BlockIx bi, Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
            const B* __restrict__ Ffx, const B* __restrict__ Ffy, const double* Ucv,
            const double* __restrict__ dt_dev, double w_i, double* Uout, B* __restrict__ Fhx,
@@ -1265,11 +1267,11 @@ __device__ __forceinline__ void k_hocand2d_body(const BlockIx bi, Ops<P, B> op, 
   const int Nx = g.N[0], Ny = g.N[1], Nsx = g.Ns[0];
   // the block's elements: a run of at most Eb <= E elements of one element
   // row (Er: the elements per block chosen at run time, 0 = E)
-  const int Eb = (Er > 0 && Er < E) ? Er : E;
-  const int nseg = (Nx + Eb - 1) / Eb;
+  const int Eb = FULL ? E : ((Er > 0 && Er < E) ? Er : E);
+  const int nseg = FULL ? Nx / E : (Nx + Eb - 1) / Eb;
   const int ey = bi.x / nseg, ex0 = (bi.x % nseg) * Eb;
   const long long e0 = (long long)ey * Nx + ex0;
-  const int ne = (Nx - ex0 < Eb) ? (Nx - ex0) : Eb;
+  const int ne = FULL ? E : ((Nx - ex0 < Eb) ? (Nx - ex0) : Eb);
   const int tid = threadIdx.x;
   const B gamma = (B)prm.gamma;
   const double dti = (*dt_dev) * w_i;
@@ -1280,9 +1282,8 @@ __device__ __forceinline__ void k_hocand2d_body(const BlockIx bi, Ops<P, B> op, 
     const int xl = item % (ne * n);
     el = xl / n;
     i = xl % n;
-    const long long e = e0 + el;
-    x = (int)(e % Nx) * n + i;
-    y = (int)(e / Nx) * n + qy;
+    x = (ex0 + el) * n + i;
+    y = ey * n + qy;
     ci = (long long)y * Nsx + x;
   };
   // 1. loads into registers: the cells' state, the element-face fluxes
@@ -1305,8 +1306,7 @@ __device__ __forceinline__ void k_hocand2d_body(const BlockIx bi, Ops<P, B> op, 
     const int a = item / NI, r = item % NI;
     const int lp = r % n, f = (r / n) % nf, el = r / (n * nf);
     if (item >= 2 * NI || el >= ne || (f != 0 && f != nf - 1)) continue;
-    const long long e = e0 + el;
-    const int cx = (int)(e % Nx), cy = (int)(e / Nx);
+    const int cx = ex0 + el, cy = ey;
     if (a == 0) {
       const long long i0 = ((long long)cy * n + lp) * (Nx + 1) + cx + (f == 0 ? 0 : 1);
       const long long plane = (long long)(Nx + 1) * Ny * n;
@@ -1377,8 +1377,7 @@ __device__ __forceinline__ void k_hocand2d_body(const BlockIx bi, Ops<P, B> op, 
       // last (the layout of k_ho2d)
       const int a = item / NI, r = item % NI;
       const int q = r % n, f = (r / n) % nf, el = r / (n * nf);
-      const long long e = e0 + el;
-      const int cx = (int)(e % Nx), cy = (int)(e / Nx);
+      const int cx = ex0 + el, cy = ey;
       const int ca = (a == 0) ? cx : cy, Na = (a == 0) ? Nx : Ny;
       if (f == nf - 1 && ca != Na - 1) continue;
       const long long nxf = (a == 0) ? (long long)Nx * n + 1 : (long long)Nx * n;
@@ -1514,14 +1513,14 @@ __device__ __forceinline__ void k_hocand2d_body(const BlockIx bi, Ops<P, B> op, 
     if (blk && threadIdx.x == 0 && *(volatile int*)any_flag == 0) atomicOr(any_flag, 1);
   }
 }
-template <int P, typename B, bool DET>
+template <int P, typename B, bool DET, bool FULL>
 __global__ void __launch_bounds__(256)
 k_hocand2d(Ops<P, B> op, Geom g, Params prm, int t, const B* __restrict__ Uad,
            const B* __restrict__ Ffx, const B* __restrict__ Ffy, const double* Ucv,
            const double* __restrict__ dt_dev, double w_i, double* Uout, B* __restrict__ Fhx,
            B* __restrict__ Fhy, uint8_t* __restrict__ flags, int* __restrict__ any_flag,
            const int* __restrict__ fh_gate, int Er) {
-  k_hocand2d_body<P, B, DET>(hw_block(), op, g, prm, t, Uad, Ffx, Ffy, Ucv, dt_dev, w_i, Uout, Fhx, Fhy, flags, any_flag, fh_gate, Er);
+  k_hocand2d_body<P, B, DET, FULL>(hw_block(), op, g, prm, t, Uad, Ffx, Ffy, Ucv, dt_dev, w_i, Uout, Fhx, Fhy, flags, any_flag, fh_gate, Er);
 }
 
 // ---------------------------------------------------------------------------

@@ -2463,14 +2463,22 @@ template <int NDIM, int P, typename B> struct Stepper {
         const bool wf = c->cfg.FB && (form == 2 || form == 4);
         B* fx = wf ? Fh(c, 0) : nullptr;
         B* fy = wf ? Fh(c, 1) : nullptr;
-        if (det)
-          hipLaunchKernelGGL((fv2d::k_hocand2d<P, B, true>), dim3(hblocks), dim3(256), lds, s, op,
+        // blocks all full: constant index arithmetic
+        const bool full = Eb == HT::E && c->g.N[0] % HT::E == 0;
+        auto go = [&](auto det_c, auto full_c) {
+          constexpr bool D = decltype(det_c)::value, F = decltype(full_c)::value;
+          hipLaunchKernelGGL((fv2d::k_hocand2d<P, B, D, F>), dim3(hblocks), dim3(256), lds, s, op,
                              c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, dt, w, out, fx,
-                             fy, c->flags, c->need_gate ? c->any_flag : nullptr, gate, Eb);
-        else
-          hipLaunchKernelGGL((fv2d::k_hocand2d<P, B, false>), dim3(hblocks), dim3(256), lds, s,
-                             op, c->g, c->prm, i, Uad(c), Ff(c, 0), Ff(c, 1), c->Ucv, dt, w, out,
-                             fx, fy, nullptr, nullptr, gate, Eb);
+                             fy, D ? c->flags : nullptr,
+                             (D && c->need_gate) ? c->any_flag : nullptr, gate, Eb);
+        };
+        using T_ = std::true_type;
+        using F_ = std::false_type;
+        if (det) {
+          if (full) go(T_{}, T_{}); else go(T_{}, F_{});
+        } else {
+          if (full) go(F_{}, T_{}); else go(F_{}, F_{});
+        }
       });
       if (!c->cfg.FB) return;
     } else {
