@@ -334,8 +334,7 @@ k_adj_fvface(Geom g, Params prm, int fb, const double* __restrict__ Ub1,
     long long x = tid;
 #pragma unroll
     for (int b = 0; b < NDIM; ++b) {
-      c[b] = (int)(x % ext[b]);
-      x /= ext[b];
+      c[b] = split_index(x, ext[b]);
     }
   }
   const double dti = (*dt_dev) * w_i;
@@ -710,8 +709,7 @@ k_adj_face(Ops<P> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
 #pragma unroll
     for (int k = 0; k < NDIM; ++k) {
       const int b = (k < NDIM - 1) ? (k < a ? k : k + 1) : a;
-      q[b] = (int)(x % ext[b]);
-      x /= ext[b];
+      q[b] = split_index(x, ext[b]);
     }
 #pragma unroll
     for (int b = NDIM - 1; b >= 0; --b) pidx = pidx * ext[b] + q[b];

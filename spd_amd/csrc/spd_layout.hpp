@@ -21,8 +21,34 @@ template <int P, typename R = double> struct Ops {
 // ---------------------------------------------------------------------------
 // index helpers
 // ---------------------------------------------------------------------------
+// Remainder of the non-negative index x by d > 0, with x replaced by the
+// quotient: 32-bit arithmetic when x fits (a 64-bit division by a run-time
+// divisor costs several times as much).
+__device__ __forceinline__ int split_index(long long& x, int d) {
+  if ((unsigned long long)x <= 0xffffffffull) {
+    const unsigned u = (unsigned)x, q = u / (unsigned)d;
+    x = (long long)q;
+    return (int)(u - q * (unsigned)d);
+  }
+  const long long q = x / d;
+  const int r = (int)(x - q * d);
+  x = q;
+  return r;
+}
+
+// Coordinates of element e; 32-bit division when the element count allows
+// (a 64-bit division by a run-time divisor costs several times more).
 template <int NDIM>
 __device__ __forceinline__ void elem_coords(const Geom& g, long long e, int* c) {
+  if (g.NE <= 0xffffffffll) {
+    unsigned u = (unsigned)e;
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+      c[a] = (int)(u % (unsigned)g.N[a]);
+      u /= (unsigned)g.N[a];
+    }
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < NDIM; ++a) {
     c[a] = (int)(e % g.N[a]);

@@ -91,8 +91,7 @@ k_face_sd_x(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsr
     long long x = ep;
 #pragma unroll
     for (int d = 1; d < NDIM; ++d) {
-      ce[d] = (int)(x % g.N[d]);
-      x /= g.N[d];
+      ce[d] = split_index(x, g.N[d]);
     }
     int r = lp;
 #pragma unroll
@@ -204,22 +203,18 @@ k_face_sd(Ops<P, B> op, Geom g, Params prm, const S* __restrict__ src, int Tsrc,
     // that the lanes of a wave read whole planes of the elements (its lines
     // along z are n*n values apart) instead of 4-value pieces of rows
     long long x = pidx;
-    cl[0] = (int)(x % n);
-    x /= n;
-    cl[1] = (int)(x % n);
-    x /= n;
-    ce[0] = (int)(x % g.N[0]);
-    x /= g.N[0];
-    ce[1] = (int)(x % g.N[1]);
-    f = (int)(x / g.N[1]);
+    cl[0] = split_index(x, n);
+    cl[1] = split_index(x, n);
+    ce[0] = split_index(x, g.N[0]);
+    ce[1] = split_index(x, g.N[1]);
+    f = (int)x;
     widx = ((long long)f * ext[1] + (long long)ce[1] * n + cl[1]) * ext[0] + (long long)ce[0] * n +
            cl[0];
   } else {
     long long x = pidx;
 #pragma unroll
     for (int b = 0; b < NDIM; ++b) {
-      const int q = (int)(x % ext[b]);
-      x /= ext[b];
+      const int q = split_index(x, ext[b]);
       if (b == a) {
         f = q;
       } else {
@@ -824,11 +819,12 @@ k_tensor_el(Ops<P> op, Geom g, int mode, const double* __restrict__ in, int in_f
   constexpr int NY = (NDIM == 2) ? n : 1;
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= (long long)V * g.NE) return;
-  const long long e = tid % g.NE;
-  const int v = (int)(tid / g.NE);
+  long long x = tid;
+  const long long e = (g.NE <= 0x7fffffffll) ? split_index(x, (int)g.NE) : tid % g.NE;
+  const int v = (g.NE <= 0x7fffffffll) ? (int)x : (int)(tid / g.NE);
   const double(*M)[n] = (mode == 1) ? op.s2c : ((mode == 2) ? op.s2r : op.c2s);
-  const int ex = (int)(e % g.N[0]);
-  const long long ey = e / g.N[0];
+  long long ey = e;
+  const int ex = split_index(ey, g.N[0]);
   // point (j, i) of the element: element layout e*NL + j*n + i; FV layout
   // (ey*n + j) * Ns_x + ex*n + i
   auto idx = [&](int fv, int j, int i) -> long long {
@@ -893,8 +889,7 @@ k_wave_cells(Geom g, Params prm, int n, const double* __restrict__ Ucv,
       long long x = ci;
 #pragma unroll
       for (int a = 0; a < NDIM; ++a) {
-        c[a] = (int)(x % g.Ns[a]);
-        x /= g.Ns[a];
+        c[a] = split_index(x, g.Ns[a]);
       }
       long long e = 0, l = 0;
 #pragma unroll
@@ -1309,8 +1304,7 @@ k_apply(Geom g, const double* Uin, const B* __restrict__ F0, const B* __restrict
     long long x = cell;
 #pragma unroll
     for (int a = 0; a < NDIM; ++a) {
-      c[a] = (int)(x % g.Ns[a]);
-      x /= g.Ns[a];
+      c[a] = split_index(x, g.Ns[a]);
     }
   }
   const B* F[3] = {F0, F1, F2};
@@ -1466,8 +1460,7 @@ k_detect(Geom g, Params prm, const double* __restrict__ Wold, const double* __re
       long long x = cell;
 #pragma unroll
       for (int a = 0; a < NDIM; ++a) {
-        c[a] = (int)(x % g.Ns[a]);
-        x /= g.Ns[a];
+        c[a] = split_index(x, g.Ns[a]);
       }
     }
     double Wn[V], Wo[V], mx[V], mn[V];
@@ -1584,8 +1577,7 @@ __device__ __forceinline__ void grid_coords(long long x, const int* ext, long lo
   } else {
 #pragma unroll
     for (int a = 0; a < NDIM; ++a) {
-      c[a] = (int)(x % ext[a]);
-      x /= ext[a];
+      c[a] = split_index(x, ext[a]);
     }
   }
 }
@@ -1894,8 +1886,7 @@ k_fbface(Geom g, Params prm, const B* __restrict__ HS, const double* __restrict_
     long long x = tid;
 #pragma unroll
     for (int b = 0; b < NDIM; ++b) {
-      c[b] = (int)(x % ext[b]);
-      x /= ext[b];
+      c[b] = split_index(x, ext[b]);
     }
   }
   long long gl = 0, gr = 0, hl = 0, hr = 0, n1 = 1;
@@ -2054,18 +2045,15 @@ k_ic_points(Geom g, int n, const double* __restrict__ lows, const double* __rest
   int q[3] = {0, 0, 0}, l[3] = {0, 0, 0}, e[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < NDIM; ++a) {
-    q[a] = (int)(x % 2);
-    x /= 2;
+    q[a] = split_index(x, 2);
   }
 #pragma unroll
   for (int a = 0; a < NDIM; ++a) {
-    l[a] = (int)(x % n);
-    x /= n;
+    l[a] = split_index(x, n);
   }
 #pragma unroll
   for (int a = 0; a < NDIM; ++a) {
-    e[a] = (int)(x % g.N[a]);
-    x /= g.N[a];
+    e[a] = split_index(x, g.N[a]);
   }
   long long off = 0;
 #pragma unroll
@@ -2155,8 +2143,7 @@ __global__ void k_tap(Geom g, const uint8_t* __restrict__ flags, const double* _
     long long x = tid;
 #pragma unroll
     for (int b = 0; b < NDIM; ++b) {
-      c[b] = (int)(x % ext[b]);
-      x /= ext[b];
+      c[b] = split_index(x, ext[b]);
     }
     long long gl = 0, gr = 0;
 #pragma unroll
@@ -2188,8 +2175,7 @@ k_bc_traces(Ops<P> op, Geom g, int a, int side, const double* __restrict__ U,
 #pragma unroll
   for (int b = 0; b < NDIM; ++b) {  // the inverse of bc_point: lower dimensions fastest
     if (b == a) continue;
-    const int q = (int)(x % g.Ns[b]);
-    x /= g.Ns[b];
+    const int q = split_index(x, g.Ns[b]);
     ce[b] = q / n;
     cl[b] = q % n;
   }
