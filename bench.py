@@ -8,19 +8,24 @@ rho = 1, vx = -0.5, P = 2.5, vy = 0.01 sin(4 pi x), plus seeded uniform noise
 (amplitude 1e-3, numpy default_rng(0)) on the vx, vy subcell means.
 
 A step is one pass of the hot path over the whole state: dt from the
-maximum wave speed of the state (compute_dt, its scalar read back to the host
-as in the reference's loop) and one spd_step (predictor + FV/FB corrector).
-The state is resident in HBM before the timed region (10 warm-up steps, then
-200 timed steps from t = 0), and the timed steps run without
-instrumentation; the per-kernel durations come from 20 further steps with
-HIP events around every launch, and the same number of steps with the
-device-side time loop (dt, step and time on the device) replayed from one
-HIP graph is reported as device_loop_ms_per_step.  The JSON line carries:
+maximum wave speed of the state and one spd_step (predictor + FV/FB
+corrector).  The state is resident in HBM before the timed region (10
+warm-up steps, then 200 timed steps from t = 0); the timed steps run the
+device-side time loop of perform_time_evolution (dt, step and time on the
+device) replayed from one HIP graph, without instrumentation; the same steps
+with the host-driven loop (dt read back every step, as in the reference's
+loop) are reported as host_loop_ms_per_step and must end in the same state
+bit for bit; the per-kernel durations come from 20 further steps with HIP
+events around every launch.  The JSON line carries:
   * value: DOF-updates/s of the FP64 form (DOF = solution points; one
     DOF-update = one point advanced one full step), with the roofline of the
-    dominant kernel (HIP-event durations over the timed region on the
-    kernel's own stream; HBM traffic per launch from the committed counter
-    summary) and per-kernel times;
+    dominant kernel (HIP-event durations on the kernel's own stream; HBM
+    traffic per launch from the committed counter summary), the step's HBM
+    traffic and FP64 rate from the committed counter summaries
+    (step_traffic, compute_roofline) and per-kernel times;
+  * t1_window_recorded: the same workload timed from t = 1 (200 steps after
+    the ~26k that reach it), as recorded by --t1-window in the committed
+    summary (--t1-window measures it in the run, several minutes more);
   * fp32_bulk: the same workload in the FP32-bulk form, its speedup and its
     relative L1 difference from the FP64 form after the same steps;
   * ensemble / ensemble_small_members: 64 four-quadrant Riemann problems
@@ -145,6 +150,31 @@ CLASS_KERNELS = {
 }
 TRAFFIC_SUMMARY = os.path.join(HERE, "profiles", "kh1024_p3_hbm_traffic.md")
 FLOPS_SUMMARY = os.path.join(HERE, "profiles", "kh1024_p3_fp64_flops.md")
+T1_SUMMARY = os.path.join(HERE, "profiles", "kh1024_p3_t1_window.md")
+
+
+def recorded_t1_window():
+    """The t = 1 window of the same workload as recorded by `--t1-window`
+    (reaching t = 1 takes ~26k steps, several minutes, so the default run
+    reports the committed record instead of measuring it); None when absent."""
+    if not os.path.exists(T1_SUMMARY):
+        return None
+    rows = {}
+    with open(T1_SUMMARY) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 2:
+                rows[cells[0]] = cells[1]
+    try:
+        out = {"ms_per_step": float(rows["ms per step"]),
+               "value": float(rows["DOF-updates/s"]),
+               "steps_to_t1": int(rows["steps to t = 1.0"].split()[0]),
+               "troubled_fraction": float(rows["troubled fraction (substep average of the next step)"]
+                                          .rstrip("%")) / 100.0}
+    except (KeyError, ValueError):
+        return None
+    out["source"] = os.path.relpath(T1_SUMMARY, HERE)
+    return out
 
 
 def measured_traffic(cls):
@@ -711,6 +741,10 @@ def main():
     }
     if fp32 is not None:
         out["fp32_bulk"] = fp32
+    if not args.t1_window:
+        rec = recorded_t1_window()
+        if rec is not None:
+            out["t1_window_recorded"] = rec
     if args.t1_window:
         # the device loop with the clamp to t_end, in chunks, with progress
         s.init_sim()
