@@ -309,3 +309,41 @@ def test_fallback_forms_bit_identical(kind, N, p, limiter, precision):
     for U, t, n in out[1:]:
         assert np.array_equal(U, out[0][0])
         assert t == out[0][1] and n == out[0][2]
+
+
+@pytest.mark.parametrize("p,N", [(1, (64, 12)), (2, (56, 10)), (3, (32, 12)), (4, (20, 8)),
+                                 (5, (14, 8))])
+def test_candidate_kernel_block_runs_bit_identical(p, N):
+    """The fused candidate kernel has a variant for rows of whole E-element
+    blocks (E = 256 / (p+1)^2: 64, 28, 16, 10, 7; constant index arithmetic)
+    and the general one (any elements per block, a partial last block).  The
+    grids here are whole blocks wide, so the default takes the first; E - 1
+    elements per block forces the second.  Both run the same arithmetic:
+    3 steps of a noise-seeded Kelvin-Helmholtz grid give the same bits, and
+    the default's first step stays within the bar of the oracle."""
+    from oracle.spd_oracle import OracleSimulator
+    from spd_amd import _lib
+    from spd_amd import initial_conditions as ic
+    from spd_amd.simulator import SDADER_Simulator
+    print(f"[gpu candidate kernel block runs] p={p} N={N}", flush=True)
+    E = 256 // (p + 1) ** 2
+    assert N[0] % E == 0
+    kw = dict(p=p, N=N, BC=(("periodic", "periodic"), ("periodic", "periodic")), update="FV",
+              FB=True, riemann_solver_sd="hllc", riemann_solver_fv="hllc", cfl_coeff=0.8)
+    noise = ic.subcell_noise(N, p, [1, 2], 1e-3, seed=0)
+    out = []
+    for epb in (0, E - 1):
+        s = SDADER_Simulator(init_fct=ic.kelvin_helmholtz, device="cuda:0", verbose=False, **kw)
+        s.perturb_initial_condition(noise)
+        _lib.check(s._lib.spd_set_launch(s._ctx, _lib.LAUNCH["hoc_epb"], epb), "spd_set_launch")
+        if epb == 0:
+            o = OracleSimulator(ic.kelvin_helmholtz, noise=noise, **kw)
+            o.perform_update()
+            s.perform_update()
+            assert relerr_fields(s.dm.U_sp, o.U_sp) < TOL, relerr_fields(s.dm.U_sp, o.U_sp)
+            s.perform_iterations(2)
+        else:
+            s.perform_iterations(3)
+        out.append(np.asarray(s.dm.U_sp))
+    assert np.isfinite(out[0]).all()
+    assert np.array_equal(out[0], out[1])
