@@ -2798,7 +2798,7 @@ struct Entry {
 const Entry kEntries[] = {
     SPD_ENTRIES(1, 1), SPD_ENTRIES(1, 2), SPD_ENTRIES(1, 3), SPD_ENTRIES(1, 4), SPD_ENTRIES(1, 5),
     SPD_ENTRIES(2, 1), SPD_ENTRIES(2, 2), SPD_ENTRIES(2, 3), SPD_ENTRIES(2, 4), SPD_ENTRIES(2, 5),
-    SPD_ENTRIES(3, 1), SPD_ENTRIES(3, 2), SPD_ENTRIES(3, 3),
+    SPD_ENTRIES(3, 1), SPD_ENTRIES(3, 2), SPD_ENTRIES(3, 3), SPD_ENTRIES(3, 4), SPD_ENTRIES(3, 5),
 };
 #undef SPD_ENTRIES
 #undef SPD_ENTRY

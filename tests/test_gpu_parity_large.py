@@ -148,3 +148,37 @@ def test_taylor_green_3d_multiblock_vs_oracle():
     assert relerr_fields(s.dm.U_sp, o.U_sp) < TOL
     n_trouble = _lockstep(s, o, 2)
     assert n_trouble > 0
+
+
+@pytest.mark.parametrize("update", ["FV", "SD"])
+@pytest.mark.parametrize("p", [4, 5])
+def test_taylor_green_3d_high_orders_vs_oracle(p, update):
+    """3-D Taylor-Green at p = 4 and 5 on a (4, 3, 2)-element grid (unequal
+    sides, so a transposed index shows), density perturbation 1e-2: two steps
+    against the oracle (flags, theta and state with the fallback; state with
+    the SD update)."""
+    import math
+    from oracle.spd_oracle import OracleSimulator
+    from spd_amd import initial_conditions as ic
+    from spd_amd.simulator import SDADER_Simulator
+    print(f"[gpu parity 3-D Taylor-Green (4,3,2) p={p} {update}]", flush=True)
+    L = 2 * math.pi
+    N = (4, 3, 2)
+    kw = dict(p=p, N=N, BC=(("periodic", "periodic"),) * 3, xlim=(0, L), ylim=(0, L),
+              zlim=(0, L), update=update, FB=(update == "FV"), riemann_solver_sd="hllc",
+              riemann_solver_fv="hllc")
+    noise = ic.subcell_noise(N, p, [0], 1e-2, seed=1)
+    s = SDADER_Simulator(init_fct=ic.taylor_green, device="cuda:0", verbose=False, **kw)
+    s.perturb_initial_condition(noise)
+    o = OracleSimulator(ic.taylor_green, noise=noise, **kw)
+    assert relerr_fields(s.dm.U_sp, o.U_sp) < TOL
+    if update == "FV":
+        assert _lockstep(s, o, 2) > 0
+    else:
+        for k in range(2):
+            s.compute_dt()
+            o.compute_dt()
+            assert abs(s.dm.dt - o.dt) <= TOL * o.dt, (k, s.dm.dt, o.dt)
+            s.perform_update()
+            o.perform_update()
+            assert relerr_fields(s.dm.U_sp, o.U_sp) < TOL, (k, relerr_fields(s.dm.U_sp, o.U_sp))
