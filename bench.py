@@ -44,6 +44,11 @@ events around every launch.  The JSON line carries:
     (adjoint_fp64_reference);
   * cpu_baseline: the NumPy oracle on a bounded sample, one host core.
 
+--precision-run fp32 (or perturbed) runs only the precision configuration's
+Kelvin-Helmholtz leg instead: the FP64 form and the FP32-bulk form (or the
+FP64 form from an initial state perturbed by 1e-14) to t = 2.0, with the
+relative L1 difference per field at five times (about 15 minutes).
+
 Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10]
 """
 from __future__ import annotations
@@ -531,6 +536,42 @@ def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+def l1_rel_fields(Ua, Ub):
+    """Relative L1 difference of the 2-D states Ua and Ub per field (over all
+    solution points, relative to Ub's L1 norm): rho, momenta (x and y
+    together), E."""
+    out = {name: float((Ua[v] - Ub[v]).abs().sum() / Ub[v].abs().sum())
+           for name, v in (("rho", 0), ("E", 3))}
+    out["momenta"] = float((Ua[1:3] - Ub[1:3]).abs().sum() / Ub[1:3].abs().sum())
+    return out
+
+
+def run_precision(torch, make_ref, make_other, other, marks, chunk=2000):
+    """The Kelvin-Helmholtz run of the precision configuration: the FP64 form
+    (make_ref) and a second simulation (make_other: the FP32-bulk form, or the
+    FP64 form from a perturbed initial state) each advanced to every time in
+    `marks` by the device-side loop with its clamp to that time (their dt
+    sequences differ), and the relative L1 difference per field at each mark."""
+    a, b = make_ref(), make_other()
+    a.init_sim()
+    b.init_sim()
+    t0 = time.perf_counter()
+    rows = []
+    for tm in marks:
+        for name, s in (("fp64", a), (other, b)):
+            while s.time < tm:
+                s._run_on_device(tm, chunk, 1)
+                progress(f"precision run: {name} t = {s.time:.4f} after {s.n_step} steps, "
+                         f"{time.perf_counter() - t0:.0f} s")
+        torch.cuda.synchronize()
+        Ua, Ub = a.dm.device_value("U_sp"), b.dm.device_value("U_sp")
+        rows.append({"t": tm, "steps_fp64": a.n_step, f"steps_{other}": b.n_step,
+                     "finite": bool(torch.isfinite(Ua).all() and torch.isfinite(Ub).all()),
+                     "l1_rel": l1_rel_fields(Ub, Ua)})
+        progress(f"precision run: t = {tm}: {rows[-1]['l1_rel']}")
+    return {"other": other, "marks": rows, "seconds": time.perf_counter() - t0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -565,6 +606,12 @@ def main():
                          "gradient it is compared with")
     ap.add_argument("--adjoint-fp32-steps", type=int, default=20)
     ap.add_argument("--adjoint-fp32-k", type=int, default=5)
+    ap.add_argument("--precision-run", choices=["fp32", "perturbed"], default=None,
+                    help="run only the precision configuration's Kelvin-Helmholtz leg: the FP64 "
+                         "form and the FP32-bulk form (or the FP64 form from an initial state "
+                         "perturbed by 1e-14 on vx) to --precision-t-end, with the relative L1 "
+                         "difference per field at t_end/8, t_end/4, t_end/2, 3 t_end/4, t_end")
+    ap.add_argument("--precision-t-end", type=float, default=2.0)
     ap.add_argument("--t1-window", action="store_true",
                     help="also run the FP64 form to t = 1.0 (about 23k steps, 5 minutes) and "
                          "time the same number of steps there")
@@ -601,6 +648,23 @@ def main():
                              precision=precision, **kh_kwargs(N, p))
         s.perturb_initial_condition(noise)
         return s
+
+    if args.precision_run:
+        te = args.precision_t_end
+        marks = [te / 8, te / 4, te / 2, 3 * te / 4, te]
+        if args.precision_run == "fp32":
+            other = lambda: make_sim("fp32")  # noqa: E731
+        else:
+            def other():
+                s = SDADER_Simulator(init_fct=ic.kelvin_helmholtz, device=dev, verbose=False,
+                                     precision="fp64", **kh_kwargs(N, p))
+                s.perturb_initial_condition(noise + ic.subcell_noise((N, N), p, [1], 1e-14, seed=7))
+                return s
+        res = run_precision(torch, lambda: make_sim("fp64"), other, args.precision_run, marks)
+        res["workload"] = (f"Kelvin-Helmholtz {N}^2 p={p} FV+FB HLLC minmod CFL 0.8 periodic, "
+                           f"seeded noise 1e-3 on vx, vy, to t = {te}")
+        print(json.dumps({"precision_run": res}))
+        return
 
     def run_form(precision):
         """Warm-up, then the timed steps: the device-side time loop of
@@ -648,11 +712,7 @@ def main():
         s32, elapsed32, prof32, frac32, early32, hloop32, end32, same32 = run_form("fp32")
 
         # the two forms after the same number of steps from the same state
-        def l1_rel(r32, r64):
-            out = {name: float((r32[v] - r64[v]).abs().sum() / r64[v].abs().sum())
-                   for name, v in (("rho", 0), ("E", 3))}
-            out["momenta"] = float((r32[1:3] - r64[1:3]).abs().sum() / r64[1:3].abs().sum())
-            return out
+        l1_rel = l1_rel_fields
         torch.cuda.synchronize()
         l1 = l1_rel(s32.dm.device_value("U_sp"), s.dm.device_value("U_sp"))
         l1_early = l1_rel(early32, early64)
