@@ -62,15 +62,12 @@ __device__ __forceinline__ double sed_dU(const double* w, int stride, int x, int
 // in [0, 1], and a correctly rounded |c| / |dv| is below 1 exactly when
 // |c| < |dv| (NaN and dv == 0 give 1 there and false here), so the test needs
 // no division.  Returns bit 0: A < 1 (true outside [0, Ns), where the
-// reference's A is 0), bit 1: alpha_L < 1.
-__device__ __forceinline__ int sed_bits(const double* w, int stride, int x, int Ns, const double* rdc,
-                                        const double* hf) {
-  if (x < 0 || x >= Ns) return 1;
-  const double dUm = sed_dU(w - stride, stride, x - 1, Ns, rdc);
-  const double dU0 = sed_dU(w, stride, x, Ns, rdc);
-  const double dUp = sed_dU(w + stride, stride, x + 1, Ns, rdc);
-  const double d2U = (dUp - dUm) * rdc[x];
-  const double dv = 0.5 * hf[x] * d2U;
+// reference's A is 0), bit 1: alpha_L < 1.  sed_bits_core takes the first
+// differences at x - 1, x, x + 1 and rdc, hf at x.
+__device__ __forceinline__ int sed_bits_core(double dUm, double dU0, double dUp, double rdc_x,
+                                             double hf_x) {
+  const double d2U = (dUp - dUm) * rdc_x;
+  const double dv = 0.5 * hf_x * d2U;
   const double vL = dUm - dU0;
   const double vR = dUp - dU0;
   const double cL = (dv < 0.0) ? ((vL > 0.0) ? vL : 0.0) : ((vL < 0.0) ? vL : 0.0);
@@ -78,6 +75,29 @@ __device__ __forceinline__ int sed_bits(const double* w, int stride, int x, int 
   const double adv = fabs(dv);
   const bool l = fabs(cL) < adv, r = fabs(cR) < adv;
   return ((l || r) ? 1 : 0) | (l ? 2 : 0);
+}
+__device__ __forceinline__ int sed_bits(const double* w, int stride, int x, int Ns, const double* rdc,
+                                        const double* hf) {
+  if (x < 0 || x >= Ns) return 1;
+  const double dUm = sed_dU(w - stride, stride, x - 1, Ns, rdc);
+  const double dU0 = sed_dU(w, stride, x, Ns, rdc);
+  const double dUp = sed_dU(w + stride, stride, x + 1, Ns, rdc);
+  return sed_bits_core(dUm, dU0, dUp, rdc[x], hf[x]);
+}
+
+// The same with rdc and hf from a block's LDS copy: rl, hl hold them at the
+// local index xl of coordinate x.
+__device__ __forceinline__ double sed_dU_l(const double* w, int stride, int x, int Ns,
+                                          const double* rl, int xl) {
+  return (x >= -1 && x <= Ns) ? (w[stride] - w[-stride]) * rl[xl] : 0.0;
+}
+__device__ __forceinline__ int sed_bits_l(const double* w, int stride, int x, int Ns,
+                                          const double* rl, const double* hl, int xl) {
+  if (x < 0 || x >= Ns) return 1;
+  const double dUm = sed_dU_l(w - stride, stride, x - 1, Ns, rl, xl - 1);
+  const double dU0 = sed_dU_l(w, stride, x, Ns, rl, xl);
+  const double dUp = sed_dU_l(w + stride, stride, x + 1, Ns, rl, xl + 1);
+  return sed_bits_core(dUm, dU0, dUp, rl[xl], hl[xl]);
 }
 
 // alpha < 1 along one axis for the cell at ca, with the compute_min edge rules
@@ -133,15 +153,42 @@ __device__ __forceinline__ void stage_new_rho_p(const Geom& g, const Params& prm
   }
 }
 
+// Subcell geometry of a DTX x DTY tile at (x0, y0) in LDS: rdc and hf along
+// x at x = x0 - DH + j (j < DNX) and along y at y = y0 - DH + j (j < DNY);
+// zero where the smooth-extrema test does not read them.
+struct SedGeo {
+  double rdcx[DNX], hfx[DNX], rdcy[DNY], hfy[DNY];
+};
+__device__ __forceinline__ void stage_sed_geo(const Geom& g, int x0, int y0, SedGeo* sg) {
+  const int Nx = g.Ns[0], Ny = g.Ns[1];
+  for (int j = threadIdx.x; j < 2 * DNX + 2 * DNY; j += DNT) {
+    if (j < DNX) {
+      const int x = x0 - DH + j;
+      sg->rdcx[j] = (x >= -1 && x <= Nx) ? g.rdc[0][x + NGH] : 0.0;
+    } else if (j < 2 * DNX) {
+      const int x = x0 - DH + (j - DNX);
+      sg->hfx[j - DNX] = (x >= 0 && x < Nx) ? g.hfp[0][x + NGH] : 0.0;
+    } else if (j < 2 * DNX + DNY) {
+      const int y = y0 - DH + (j - 2 * DNX);
+      sg->rdcy[j - 2 * DNX] = (y >= -1 && y <= Ny) ? g.rdc[1][y + NGH] : 0.0;
+    } else {
+      const int y = y0 - DH + (j - 2 * DNX - DNY);
+      sg->hfy[j - 2 * DNX - DNY] = (y >= 0 && y < Ny) ? g.hfp[1][y + NGH] : 0.0;
+    }
+  }
+}
+
 // Smooth-extrema relaxation (src/trouble_detection.py:38-51,104-137) of the
 // density / pressure candidates pt[0], pt[3] of the calling thread's cell
 // (thread (tx, ty) of the DTX x DTY tile at (x0, y0)): the tile's bits table
 // (DAX + DAY bytes of LDS at Ax) is built from wn, the new density and
 // pressure on the tile + DH halo, and a candidate is kept where
-// alpha = min(1, alpha_x, alpha_y) < 1.  Every thread of the block calls it.
+// alpha = min(1, alpha_x, alpha_y) < 1.  With sg (the tile's geometry in
+// LDS, stage_sed_geo) each thread builds the entries of its own column and
+// row of the table.  Every thread of the block calls it.
 __device__ __forceinline__ void sed_relax_tile(const Geom& g, double (*wn)[DNY][DNX],
                                                uint8_t* Ax, int x0, int y0, bool inside,
-                                               bool* pt) {
+                                               bool* pt, const SedGeo* sg = nullptr) {
   const int Nx = g.Ns[0], Ny = g.Ns[1];
   const int tx = threadIdx.x % DTX, ty = threadIdx.x / DTX;
   const int x = x0 + tx, y = y0 + ty;
@@ -150,18 +197,36 @@ __device__ __forceinline__ void sed_relax_tile(const Geom& g, double (*wn)[DNY][
   const double* rdcy = g.rdc[1] + NGH;
   const double* hfx = g.hfp[0] + NGH;
   const double* hfy = g.hfp[1] + NGH;
-  for (int k = threadIdx.x; k < DAX + DAY; k += DNT) {
-    if (k < DAX) {
-      const int var = k / (DTY * (DTX + 2)), r = k % (DTY * (DTX + 2));
-      const int ly = r / (DTX + 2), lx = r % (DTX + 2);
-      const double* w = &wn[var][ly + DH][lx - 1 + DH];
-      Ax[k] = (uint8_t)sed_bits(w, 1, x0 + lx - 1, Nx, rdcx, hfx);
-    } else {
-      const int kk = k - DAX;
-      const int var = kk / ((DTY + 2) * DTX), r = kk % ((DTY + 2) * DTX);
-      const int ly = r / DTX, lx = r % DTX;
-      const double* w = &wn[var][ly - 1 + DH][lx + DH];
-      Ay[kk] = (uint8_t)sed_bits(w, DNX, y0 + ly - 1, Ny, rdcy, hfy);
+  if (sg) {
+#pragma unroll
+    for (int var = 0; var < 2; ++var) {
+      // x bits of row ty at lx = tx, tx + DTX (x = x0 + lx - 1)
+      for (int lx = tx; lx < DTX + 2; lx += DTX) {
+        const int xl = lx - 1 + DH;
+        Ax[(var * DTY + ty) * (DTX + 2) + lx] = (uint8_t)sed_bits_l(
+            &wn[var][ty + DH][xl], 1, x0 + lx - 1, Nx, sg->rdcx, sg->hfx, xl);
+      }
+      // y bits of column tx at ly = ty, ty + DTY (y = y0 + ly - 1)
+      for (int ly = ty; ly < DTY + 2; ly += DTY) {
+        const int yl = ly - 1 + DH;
+        Ay[(var * (DTY + 2) + ly) * DTX + tx] = (uint8_t)sed_bits_l(
+            &wn[var][yl][tx + DH], DNX, y0 + ly - 1, Ny, sg->rdcy, sg->hfy, yl);
+      }
+    }
+  } else {
+    for (int k = threadIdx.x; k < DAX + DAY; k += DNT) {
+      if (k < DAX) {
+        const int var = k / (DTY * (DTX + 2)), r = k % (DTY * (DTX + 2));
+        const int ly = r / (DTX + 2), lx = r % (DTX + 2);
+        const double* w = &wn[var][ly + DH][lx - 1 + DH];
+        Ax[k] = (uint8_t)sed_bits(w, 1, x0 + lx - 1, Nx, rdcx, hfx);
+      } else {
+        const int kk = k - DAX;
+        const int var = kk / ((DTY + 2) * DTX), r = kk % ((DTY + 2) * DTX);
+        const int ly = r / DTX, lx = r % DTX;
+        const double* w = &wn[var][ly - 1 + DH][lx + DH];
+        Ay[kk] = (uint8_t)sed_bits(w, DNX, y0 + ly - 1, Ny, rdcy, hfy);
+      }
     }
   }
   __syncthreads();
@@ -310,6 +375,7 @@ k_detect2d(Geom g, Params prm, const double* __restrict__ Ucv, const double* __r
 __device__ __forceinline__ void k_sed2d_body(const BlockIx bi, Geom g, Params prm, const double* __restrict__ Unew, uint8_t* __restrict__ flags) {
   SPD_SKIP(g);
   __shared__ double wn[2][DNY][DNX];
+  __shared__ SedGeo sg;
   __shared__ uint8_t ab[DAX + DAY];
   int bx, by;
   xcd_band_tile(bi, bx, by);
@@ -319,10 +385,11 @@ __device__ __forceinline__ void k_sed2d_body(const BlockIx bi, Geom g, Params pr
   const bool inside = x < Nx && y < Ny;
   const int code = inside ? (int)flags[(long long)y * Nx + x] : 0;
   if (!__syncthreads_or(code & 6)) return;
+  stage_sed_geo(g, x0, y0, &sg);
   stage_new_rho_p(g, prm, Unew, x0, y0, wn);
   __syncthreads();
   bool pt[4] = {(code & 2) != 0, false, false, (code & 4) != 0};
-  sed_relax_tile(g, wn, ab, x0, y0, inside, pt);
+  sed_relax_tile(g, wn, ab, x0, y0, inside, pt, &sg);
   if (inside && (code & 6)) flags[(long long)y * Nx + x] = (pt[0] || pt[3] || (code & 1)) ? 1 : 0;
 }
 __global__ void __launch_bounds__(DNT)
