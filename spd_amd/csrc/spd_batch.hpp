@@ -117,6 +117,7 @@ struct ATheta {
   const int* any_flag;
   double* theta;
   int gx;
+  uint8_t* dirty;  // the member's theta tile marks (FbMarks::dirty)
 };
 template <typename B> struct AMarch {
   Geom g;
@@ -193,7 +194,7 @@ kb_theta2d(const ATheta* __restrict__ a, const int* __restrict__ start, int nm) 
   const int m = member_of(start, nm);
   const ATheta& A = a[m];
   fv2d::k_theta2d_body(place(start, m, A.gx), A.g, A.prm, A.flags, A.Ucv, A.Unew, A.any_flag,
-                       A.theta, fv2d::FbMarks{nullptr, 1, 1, 0, 0});
+                       A.theta, fv2d::FbMarks{nullptr, 1, 1, 0, 0, A.dirty});
 }
 template <int RS, typename B, bool STORED>
 __global__ void __launch_bounds__(fv2d::MW * fv2d::MWAVES)
@@ -445,7 +446,8 @@ template <int P, typename B> struct Plan2D {
           spd_ctx* c = b->mem[i];
           const int gx = (c->g.Ng[0] + fv2d::HTX - 1) / fv2d::HTX;
           nb[i] = (long long)gx * ((c->g.Ng[1] + fv2d::HTY - 1) / fv2d::HTY);
-          a[i] = ATheta{geom(i), c->prm, c->flags, ucv(i, k), unew(i, k), c->any_flag, c->theta, gx};
+          a[i] = ATheta{geom(i), c->prm, c->flags, ucv(i, k), unew(i, k),
+                        c->any_flag, c->theta, gx, c->theta_dirty};
         }
         long long tot = 0;
         const ATheta* da = b->upload(a, &rc);

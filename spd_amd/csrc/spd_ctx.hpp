@@ -83,6 +83,8 @@ struct spd_ctx {
   int* fbt_counters = nullptr;  // compacted 2-D fallback: list length, next entry, path
   int* fbt_list = nullptr;      // active tiles of the compacted 2-D fallback
   uint8_t* fbt_marks = nullptr; // tiles theta marked active (cleared by the list kernel)
+  uint8_t* theta_dirty = nullptr;  // 2-D theta tiles that may hold non-zero values (k_theta2d)
+  long long theta_tiles = 0;
   int* any_flag = nullptr;
   int* zero_flag = nullptr;   // always 0: the skip flag of ordinary calls
   int* run_skip = nullptr;    // skip flag of device-loop steps

@@ -418,6 +418,11 @@ constexpr int TNX = HTX + 2 * TH, TNY = HTY + 2 * TH;
 struct FbMarks {
   uint8_t* marks;
   int tx, ty, ntx, nty;
+  // per theta tile (HTX x HTY, row-major over the launch grid): 1 where the
+  // tile may hold non-zero theta; a tile of the byte form without troubled
+  // cells that is not marked writes nothing (its theta is 0 already).  Null:
+  // every tile writes.
+  uint8_t* dirty = nullptr;
 };
 constexpr int MKT = 8;  // flag table of a block: MKT x MKT fallback tiles
 
@@ -493,6 +498,9 @@ __device__ __forceinline__ void k_theta2d_body(const BlockIx bi, Geom g, Params 
   // and 5-wide horizontal ORs, then the rows' codes are combined per output.
   const int cx0 = gx0 - TH - NGH, cy0 = gy0 - TH - NGH;  // cell coordinates of tr[0][0]
   if (prm.blending && cx0 >= 0 && cy0 >= 0 && cx0 + TNX <= g.Ns[0] && cy0 + TNY <= g.Ns[1]) {
+    // read by every thread before the barrier below, cleared or set after it
+    const long long tile = (long long)bi.y * ((NGx + HTX - 1) / HTX) + bi.x;
+    const bool was_dirty = mk.dirty ? mk.dirty[tile] != 0 : true;
     uint8_t* fb = reinterpret_cast<uint8_t*>(&tr[0][0]);  // [TNY][TNX] flags
     uint8_t* hc = reinterpret_cast<uint8_t*>(&t1[0][0]);  // [TNY][HTX] row codes
     constexpr int NIT = (TNX * TNY + HNT - 1) / HNT;
@@ -512,10 +520,13 @@ __device__ __forceinline__ void k_theta2d_body(const BlockIx bi, Geom g, Params 
       any |= f[j];
     }
     if (!__syncthreads_or(any)) {
+      if (!was_dirty) return;
       for (int k = threadIdx.x; k < HTX * HTY; k += HNT)
         theta[(long long)(gy0 + k / HTX) * NGx + gx0 + k % HTX] = 0.0;
+      if (mk.dirty && threadIdx.x == 0) mk.dirty[tile] = 0;
       return;
     }
+    if (mk.dirty && !was_dirty && threadIdx.x == 0) mk.dirty[tile] = 1;
     for (int k = threadIdx.x; k < TNY * HTX; k += HNT) {
       const uint8_t* r = fb + (k / HTX) * TNX + k % HTX + TH;
       const int c = r[0], h1 = r[-1] | c | r[1], h2 = r[-2] | h1 | r[2];
@@ -1617,6 +1628,10 @@ template <int P> inline long long fbtile_count(const Geom& g) {
   return (long long)((g.N[0] + FT::EX - 1) / FT::EX) * ((g.N[1] + FT::EY - 1) / FT::EY);
 }
 
+__global__ void __launch_bounds__(256) k_fill_u8(uint8_t* p, long long n, uint8_t v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 __global__ void k_zero2(int* c) {
   if (threadIdx.x < 2 && blockIdx.x == 0) c[threadIdx.x] = 0;
 }

@@ -2502,6 +2502,7 @@ template <int NDIM, int P, typename B> struct Stepper {
         fv2d::FbMarks mk{nullptr, FT::TXc, FT::TYc, (c->g.N[0] + FT::EX - 1) / FT::EX,
                          (c->g.N[1] + FT::EY - 1) / FT::EY};
         if (adaptive) mk.marks = c->fbt_marks;
+        mk.dirty = c->theta_dirty;
         hipLaunchKernelGGL(fv2d::k_theta2d, gtiles, dim3(fv2d::HNT), 0, s, c->g, c->prm,
                            c->flags, c->Ucv, c->Unew, c->any_flag, c->theta, mk);
       });
@@ -2590,9 +2591,12 @@ template <int NDIM, int P, typename B> struct Stepper {
     if (!c->tape_on && !c->tape_replay) return;
     const size_t bytes = (size_t)c->g.Ngc * sizeof(double);
     double* slot = c->tape_theta + (long long)i * c->g.Ngc;
-    if (c->tape_replay)
+    if (c->tape_replay) {
       (void)hipMemcpyAsync(c->theta, slot, bytes, hipMemcpyDeviceToDevice, s);
-    else
+      if (c->theta_dirty)  // theta no longer matches the tiles' marks
+        hipLaunchKernelGGL(fv2d::k_fill_u8, dim3((unsigned)((c->theta_tiles + 255) / 256)),
+                           dim3(256), 0, s, c->theta_dirty, c->theta_tiles, (uint8_t)1);
+    } else
       (void)hipMemcpyAsync(slot, c->theta, bytes, hipMemcpyDeviceToDevice, s);
   }
 
@@ -3146,6 +3150,11 @@ int spd_ctx_create(const spd_config* cfg, spd_ctx** out) {
         if ((rc = c->alloc(&c->fbt_marks, (size_t)ntiles)) != SPD_OK) return rc;
         SPD_HIP_CHECK(hipMemset(c->fbt_counters, 0, 4 * sizeof(int)));
         SPD_HIP_CHECK(hipMemset(c->fbt_marks, 0, (size_t)ntiles));
+        // theta tiles of k_theta2d, all marked as possibly non-zero
+        c->theta_tiles = (long long)((g.Ng[0] + fv2d::HTX - 1) / fv2d::HTX) *
+                         ((g.Ng[1] + fv2d::HTY - 1) / fv2d::HTY);
+        if ((rc = c->alloc(&c->theta_dirty, (size_t)c->theta_tiles)) != SPD_OK) return rc;
+        SPD_HIP_CHECK(hipMemset(c->theta_dirty, 1, (size_t)c->theta_tiles));
       }
       if (nd != 2 && (rc = alloc_bulk(&c->HS, (1 + nd) * V * n1)) != SPD_OK) return rc;
       if (nd != 2 && (rc = c->alloc(&c->rp, (size_t)(2 * g.Nc))) != SPD_OK) return rc;
