@@ -1942,8 +1942,20 @@ k_wave_max(Geom g, Params prm, int NL, const double* __restrict__ W, double* __r
 // Reduce the block partial maxima into *cmax (one block of 1024 threads).
 __device__ __forceinline__ void k_reduce_max_body(const BlockIx bi, const double* __restrict__ partial, long long n, double* cmax, const int* skip) {
   if (*skip) return;
-  double v = 0.0;
-  for (long long i = threadIdx.x; i < n; i += blockDim.x) v = nmax(v, partial[i]);
+  // eight independent loads in flight per thread (the maximum does not depend
+  // on the order in which the partials are combined)
+  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const long long st = blockDim.x;
+  long long i = threadIdx.x;
+  for (; i + 7 * st < n; i += 8 * st) {
+    double q[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) q[j] = partial[i + j * st];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = nmax(a[j], q[j]);
+  }
+  for (; i < n; i += st) a[0] = nmax(a[0], partial[i]);
+  double v = nmax(nmax(nmax(a[0], a[1]), nmax(a[2], a[3])), nmax(nmax(a[4], a[5]), nmax(a[6], a[7])));
   v = block_max(v);
   if (threadIdx.x == 0) *cmax = v;
 }
