@@ -72,7 +72,7 @@ enum spd_precision { SPD_PRECISION_FP64 = 0, SPD_PRECISION_FP32 = 1 };
 typedef struct spd_config {
   int ndim;               /* 1, 2 or 3 */
   int N[3];               /* elements per dimension (x, y, z) */
-  int p;                  /* polynomial degree (solution points p+1) */
+  int p;                  /* polynomial degree (solution points p+1), 1..5 */
   int m;                  /* ADER order - 1; must equal p in this build */
   double lim[3][2];       /* domain bounds per dimension */
   double gamma;
