@@ -1651,8 +1651,7 @@ __global__ void k_fb_decide(Geom g, int* counters, long long thr, int force) {
 }
 
 // Active-tile list from the marks k_theta2d set (every tile with godunov):
-// one thread per tile, which clears its mark; one atomic per wave reserves
-// the wave's entries.
+// one thread per tile, which clears its mark; blocks of 256 threads.
 __global__ void __launch_bounds__(256)
 k_fbmark_list(Geom g, Params prm, uint8_t* __restrict__ marks, int* __restrict__ counters,
               int* __restrict__ list, long long ntiles) {
@@ -1664,11 +1663,21 @@ k_fbmark_list(Geom g, Params prm, uint8_t* __restrict__ marks, int* __restrict__
     if (m) marks[t] = 0;
     live = m || prm.godunov != 0;
   }
+  // one atomic per block reserves its entries (same-address atomics
+  // serialize in L2); the waves take consecutive ranges
+  __shared__ int s_cnt[4];
+  __shared__ int s_base;
   const unsigned long long b = __ballot(live);
-  const int lane = threadIdx.x % 64;
-  int base = 0;
-  if (lane == 0 && b) base = atomicAdd(&counters[0], __popcll(b));
-  base = __shfl(base, 0);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) s_cnt[wave] = __popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    s_base = tot ? atomicAdd(&counters[0], tot) : 0;
+  }
+  __syncthreads();
+  int base = s_base;
+  for (int w = 0; w < wave; ++w) base += s_cnt[w];
   if (live) list[base + __popcll(b & ((1ull << lane) - 1ull))] = (int)t;
 }
 
